@@ -1,0 +1,311 @@
+"""Seeded synthetic meshed transmission grids: the workloads the N-1 path is measured on.
+
+The reference measures on real networks (case9241pegase, rte6515); the workloads here are synthetic grids built
+from a seed, with every distribution fixed by the workload definition:
+
+* buses at uniform random 2-D positions in the unit square;
+* a random spanning tree: buses are visited in a random order and each one links to the nearest bus already
+  placed (a geometric tree, so that the grid is planar-like, as a transmission grid is);
+* chords to the 3 nearest neighbours of each bus (buses visited in a second random order, neighbours by
+  increasing distance) until the branch count is reached;
+* per branch x ~ U(0.01, 0.2) pu, r = x / U(3, 10), total charging b ~ U(0, 0.05) pu split b/2 per side;
+  exactly 10 % of the branches are transformers with ratio U(0.95, 1.05) on side 1;
+* bus 0 is the slack; exactly 20 % of the other buses are PV with V setpoint U(0.98, 1.05) and symmetric
+  reactive limits +-U(50, 300) MVar (the slack also draws a setpoint and a capacity);
+* every bus has a load P ~ lognormal(mu=3, sigma=0.8) MW and Q = P * U(0.2, 0.4);
+* total generation is 1.02 x total load, split over the slack and PV generators in proportion to their
+  capacity draw (the same U(50, 300) draw as the reactive limit).
+
+Per-unit base 400 MVA. The workload definition gives impedances in pu and loads in MW but no base; at OLF's
+100 MVA (``PerUnit.SB``) every generated grid is past its loadability (the flat-start 118-bus case reaches its
+15-iteration budget at ||F|| = 23 pu), and at 1000 MVA line charging drives most PV buses to their reactive
+limits; at 400 MVA all three base cases converge.
+
+The packing into the solver's arrays follows the reference's extractor layout
+(``GpuAcFamilyPacker.java:100-147``, ``StructureNative.java:39-46``): closed branches as 8 doubles
+``(y, g1, g2, b1, b2, ksi, r1, a1)`` and 12 ints ``(P1 Q1 P2 Q2 ph1 v1 ph2 v2 a1col r1col altQ1 altQ2)`` over
+stable rows (bus i owns row/col ``2i`` = phi / P and ``2i+1`` = V / Q or V target, ``docs/gpu-vs-olf.md:44-58``).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+SB = 400.0  # MVA per-unit base: at 100 MVA every generated grid is past its loadability
+
+# bus types
+PQ, PV, SLACK = 0, 1, 2
+
+# row modes (reference GpuRowMode, docs/gpu-vs-olf.md:60-76)
+FIXED_VARIABLE, EQUATION, BUS_TARGET_Q = 0, 1, 4
+# variable types (reference GpuAcVariableType, OLF AcVariableType ordinals)
+BUS_V, BUS_PHI = 0, 1
+
+
+@dataclass
+class Network:
+    """A synthetic grid in physical units (MW, MVar, pu impedances)."""
+
+    n_bus: int
+    bus1: np.ndarray          # int32 [E]
+    bus2: np.ndarray          # int32 [E]
+    r: np.ndarray             # [E] pu
+    x: np.ndarray             # [E] pu
+    b: np.ndarray             # [E] pu, total charging
+    ratio: np.ndarray         # [E] side-1 ratio (1 for lines)
+    bus_type: np.ndarray      # int8 [N]
+    vset: np.ndarray          # [N] pu (PV and slack)
+    qmin: np.ndarray          # [N] MVar (PV)
+    qmax: np.ndarray          # [N] MVar (PV)
+    pload: np.ndarray         # [N] MW
+    qload: np.ndarray         # [N] MVar
+    pgen: np.ndarray          # [N] MW
+    pos: np.ndarray = field(repr=False, default=None)
+
+    @property
+    def n_branch(self) -> int:
+        return int(self.bus1.shape[0])
+
+    @property
+    def n(self) -> int:
+        return 2 * self.n_bus
+
+    # ---- packing (the solver's inputs) ----
+    def pi_model(self) -> np.ndarray:
+        """Closed-branch PiModel pack, 8 doubles per branch: y, g1, g2, b1, b2, ksi, r1, a1."""
+        z = np.hypot(self.r, self.x)
+        p = np.zeros((self.n_branch, 8))
+        p[:, 0] = 1.0 / z
+        p[:, 3] = self.b / 2
+        p[:, 4] = self.b / 2
+        p[:, 5] = np.arctan2(self.r, self.x)
+        p[:, 6] = self.ratio
+        return p
+
+    def branch_indices(self) -> np.ndarray:
+        """Closed-branch index pack, 12 ints per branch, on stable rows."""
+        a = 2 * self.bus1.astype(np.int32)
+        c = 2 * self.bus2.astype(np.int32)
+        ix = np.full((self.n_branch, 12), -1, dtype=np.int32)
+        ix[:, 0] = a
+        ix[:, 1] = a + 1
+        ix[:, 2] = c
+        ix[:, 3] = c + 1
+        ix[:, 4] = a
+        ix[:, 5] = a + 1
+        ix[:, 6] = c
+        ix[:, 7] = c + 1
+        return ix
+
+    def system(self):
+        """Flat-start state, targets, row modes and variable types over the 2N stable rows."""
+        n = self.n
+        x0 = np.zeros(n)
+        x0[1::2] = 1.0
+        target = np.zeros(n)
+        mode = np.full(n, EQUATION, dtype=np.int32)
+        var_type = np.empty(n, dtype=np.int32)
+        var_type[0::2] = BUS_PHI
+        var_type[1::2] = BUS_V
+        target[0::2] = (self.pgen - self.pload) / SB
+        target[1::2] = -self.qload / SB
+        voltage = self.bus_type != PQ
+        target[1::2][voltage] = self.vset[voltage]
+        mode[1::2][voltage] = FIXED_VARIABLE
+        slack = np.nonzero(self.bus_type == SLACK)[0]
+        target[2 * slack] = 0.0
+        mode[2 * slack] = FIXED_VARIABLE
+        return x0, target, mode, var_type
+
+    def reactive_limits(self):
+        """Reactive-limit entries of the PV buses (pu): voltage row, minQ, maxQ, loadQ, targetV, bus."""
+        pv = np.nonzero(self.bus_type == PV)[0].astype(np.int32)
+        return dict(
+            voltage_row=(2 * pv + 1).astype(np.int32),
+            min_q=self.qmin[pv] / SB,
+            max_q=self.qmax[pv] / SB,
+            load_q=self.qload[pv] / SB,
+            target_v=self.vset[pv].copy(),
+            bus=pv,
+        )
+
+
+def generate(n_bus: int, n_branch: int, seed: int, load_scale: float = 1.0) -> Network:
+    """Build the seeded grid. Deterministic for a given (n_bus, n_branch, seed, load_scale)."""
+    if n_branch < n_bus - 1:
+        raise ValueError("a connected grid needs at least n_bus - 1 branches")
+    rng = np.random.default_rng(seed)
+    pos = rng.random((n_bus, 2))
+    order = rng.permutation(n_bus)
+    chord_order = rng.permutation(n_bus)
+
+    # geometric random spanning tree: each bus in `order` links to the nearest bus placed before it
+    best_d = np.full(n_bus, np.inf)
+    best_j = np.full(n_bus, -1, dtype=np.int64)
+    placed = np.zeros(n_bus, dtype=bool)
+    first = order[0]
+    placed[first] = True
+    d = ((pos - pos[first]) ** 2).sum(1)
+    upd = d < best_d
+    best_d[upd] = d[upd]
+    best_j[upd] = first
+    edges_a, edges_b = [], []
+    seen = set()
+    for t in range(1, n_bus):
+        i = int(order[t])
+        j = int(best_j[i])
+        edges_a.append(j)
+        edges_b.append(i)
+        seen.add((min(i, j), max(i, j)))
+        placed[i] = True
+        d = ((pos - pos[i]) ** 2).sum(1)
+        upd = (d < best_d) & ~placed
+        best_d[upd] = d[upd]
+        best_j[upd] = i
+
+    # chords to the k nearest neighbours until the branch count is reached
+    k = 3
+    while len(edges_a) < n_branch:
+        nn = _knn(pos, k)
+        for i in chord_order:
+            i = int(i)
+            for j in nn[i]:
+                j = int(j)
+                key = (min(i, j), max(i, j))
+                if key in seen:
+                    continue
+                seen.add(key)
+                edges_a.append(i)
+                edges_b.append(j)
+                if len(edges_a) >= n_branch:
+                    break
+            if len(edges_a) >= n_branch:
+                break
+        k += 1
+        if k > n_bus - 1:
+            raise ValueError("too many branches for a simple graph")
+
+    e = n_branch
+    bus1 = np.asarray(edges_a, dtype=np.int32)
+    bus2 = np.asarray(edges_b, dtype=np.int32)
+    x = rng.uniform(0.01, 0.2, e)
+    r = x / rng.uniform(3.0, 10.0, e)
+    b = rng.uniform(0.0, 0.05, e)
+    ratio = np.ones(e)
+    n_tr = int(round(0.1 * e))
+    tr = rng.permutation(e)[:n_tr]
+    ratio[tr] = rng.uniform(0.95, 1.05, n_tr)
+
+    n_pv = int(round(0.2 * n_bus))
+    pv = 1 + rng.permutation(n_bus - 1)[:n_pv]
+    gens = np.concatenate([[0], pv])
+    bus_type = np.zeros(n_bus, dtype=np.int8)
+    bus_type[pv] = PV
+    bus_type[0] = SLACK
+    vset = np.ones(n_bus)
+    vset[gens] = rng.uniform(0.98, 1.05, n_pv + 1)
+    cap = rng.uniform(50.0, 300.0, n_pv + 1)
+    qmin = np.zeros(n_bus)
+    qmax = np.zeros(n_bus)
+    qmin[pv] = -cap[1:]
+    qmax[pv] = cap[1:]
+
+    pload = rng.lognormal(3.0, 0.8, n_bus) * load_scale
+    qload = pload * rng.uniform(0.2, 0.4, n_bus)
+    pgen = np.zeros(n_bus)
+    pgen[gens] = 1.02 * pload.sum() * cap / cap.sum()
+    return Network(n_bus, bus1, bus2, r, x, b, ratio, bus_type, vset, qmin, qmax, pload, qload, pgen, pos)
+
+
+def _knn(pos: np.ndarray, k: int) -> np.ndarray:
+    """The k nearest other buses of every bus, by increasing distance (ties: lower index)."""
+    try:
+        from scipy.spatial import cKDTree
+        _, nn = cKDTree(pos).query(pos, k=k + 1)
+        return nn[:, 1:]
+    except ImportError:  # pragma: no cover - scipy ships with the image
+        out = np.empty((pos.shape[0], k), dtype=np.int64)
+        for i in range(pos.shape[0]):
+            d = ((pos - pos[i]) ** 2).sum(1)
+            d[i] = np.inf
+            out[i] = np.argsort(d, kind="stable")[:k]
+        return out
+
+
+def bridges(net: Network) -> np.ndarray:
+    """Boolean mask of bridge branches (their outage splits the grid), by an iterative Tarjan low-link walk."""
+    n, e = net.n_bus, net.n_branch
+    adj_ptr = np.zeros(n + 1, dtype=np.int64)
+    np.add.at(adj_ptr, net.bus1 + 1, 1)
+    np.add.at(adj_ptr, net.bus2 + 1, 1)
+    adj_ptr = np.cumsum(adj_ptr)
+    adj_to = np.empty(2 * e, dtype=np.int64)
+    adj_br = np.empty(2 * e, dtype=np.int64)
+    fill = adj_ptr[:-1].copy()
+    for br in range(e):
+        a, c = int(net.bus1[br]), int(net.bus2[br])
+        adj_to[fill[a]] = c
+        adj_br[fill[a]] = br
+        fill[a] += 1
+        adj_to[fill[c]] = a
+        adj_br[fill[c]] = br
+        fill[c] += 1
+    disc = np.full(n, -1, dtype=np.int64)
+    low = np.zeros(n, dtype=np.int64)
+    is_bridge = np.zeros(e, dtype=bool)
+    timer = 0
+    for root in range(n):
+        if disc[root] >= 0:
+            continue
+        disc[root] = low[root] = timer
+        timer += 1
+        stack = [(root, -1, int(adj_ptr[root]))]
+        while stack:
+            u, via, p = stack[-1]
+            if p < adj_ptr[u + 1]:
+                stack[-1] = (u, via, p + 1)
+                w, br = int(adj_to[p]), int(adj_br[p])
+                if br == via:
+                    continue
+                if disc[w] < 0:
+                    disc[w] = low[w] = timer
+                    timer += 1
+                    stack.append((w, br, int(adj_ptr[w])))
+                else:
+                    low[u] = min(low[u], disc[w])
+            else:
+                stack.pop()
+                if stack:
+                    parent = stack[-1][0]
+                    low[parent] = min(low[parent], low[u])
+                    if low[u] > disc[parent]:
+                        is_bridge[via] = True
+    return is_bridge
+
+
+def sample_contingencies(net: Network, count: int, bridge_fraction: float, seed: int) -> np.ndarray:
+    """Single-branch outages: round(count*bridge_fraction) bridges, the rest non-bridges, uniformly without
+    replacement, in one shuffled order."""
+    rng = np.random.default_rng(seed)
+    br = bridges(net)
+    nb = int(round(count * bridge_fraction))
+    bridge_ids = np.nonzero(br)[0]
+    plain_ids = np.nonzero(~br)[0]
+    if nb > bridge_ids.size or count - nb > plain_ids.size:
+        raise ValueError("not enough branches of the requested kind")
+    pick = np.concatenate([rng.permutation(bridge_ids)[:nb], rng.permutation(plain_ids)[:count - nb]])
+    return rng.permutation(pick).astype(np.int32)
+
+
+# The workloads, by name: (buses, branches, grid seed, load scale)
+WORKLOADS = {
+    "grid118": (118, 186, 118, 1.0),
+    "grid2000": (2000, 3100, 2000, 1.0),
+    "grid9241": (9241, 16000, 9241, 1.1),
+}
+
+
+def workload(name: str, load_scale: float | None = None) -> Network:
+    nb, ne, seed, scale = WORKLOADS[name]
+    return generate(nb, ne, seed, scale if load_scale is None else load_scale)

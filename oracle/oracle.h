@@ -1,0 +1,108 @@
+/*
+ * CPU oracle for the AC Newton-Raphson N-1 path — TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C FP64 restatement of the reference's algorithm, used by tests/, by the smoke check and by
+ * bench.py's CPU-baseline leg, always as the checker and never as the thing shipped. The product path
+ * (libolfhip.so) never links or calls it.
+ *
+ * Every function cites the reference file:line it follows. Parity pins live in tests/ (see the
+ * test names quoted next to each function). Functions without a pin say "parity unpinned".
+ */
+#ifndef OLF_ORACLE_H
+#define OLF_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes: reference GpuAcSolverStatus (docs/gpu-vs-olf.md:196). */
+enum { ORC_CONVERGED = 0, ORC_MAX_ITERATION_REACHED = 1, ORC_DIVERGED = -1, ORC_REJECTED = -2,
+       ORC_LINEAR_SOLVE_FAILED = -3 };
+/* Row modes: reference GpuRowMode. */
+enum { ORC_FIXED_VARIABLE = 0, ORC_EQUATION = 1 };
+/* Reactive-limit type: reference GpuQLimitType. */
+enum { ORC_QLIM_NONE = 0, ORC_QLIM_MIN = 1, ORC_QLIM_MAX = 2 };
+
+/* One closed branch, 28 outputs in the order of the reference scatter table
+ * (native/cuda/equations/generated/closed_branch_scatter.h:19-51):
+ * p1 dp1dv1 dp1dv2 dp1dph1 dp1dph2 dp1da1 dp1dr1, q1 ..., p2 ..., q2 ...
+ * p8 = (y, g1, g2, b1, b2, ksi, r1, a1).  Pinned by tests/test_oracle_formulas.py. */
+void oracle_closed_branch(const double *p8, double v1, double ph1, double v2, double ph2, double *out28);
+
+typedef struct oracle_sys oracle_sys;
+
+/* Stable-row system over n = 2*n_bus rows with the superset CSR pattern (every row carries its bus's
+ * and its neighbours' phi/V columns), reference structure/csr_pattern.cu:75-140. */
+oracle_sys *oracle_sys_create(int n_bus, int n_branch, const double *p8, const int *idx12);
+void oracle_sys_free(oracle_sys *s);
+int oracle_sys_nnz(const oracle_sys *s);
+void oracle_sys_pattern(const oracle_sys *s, int *rowptr, int *colidx);
+
+/* Mismatch F = sum(flows) - target on EQUATION rows, F = x - target and a unit row on FIXED_VARIABLE rows
+ * (reference equations/fill_launch.cu:82-126); qinj[bus] = sum of reactive flows leaving the bus (the
+ * reference's qBuf). Branches listed in out_br are skipped (the disabled-element mask).
+ * J may be NULL (mismatch only). Pinned by tests/test_oracle_formulas.py and tests/test_oracle_nr.py. */
+void oracle_fill(const oracle_sys *s, const double *x, const int *mode, const double *target,
+                 const int *out_br, int n_out, double *F, double *qinj, double *J);
+
+/* Sparse LU with threshold partial pivoting (Gilbert-Peierls left-looking) and a bus-level minimum degree
+ * column order; solves A x = b. Returns 0, or -1 on a structurally/numerically singular matrix.
+ * Replaces the reference's linear solvers (lu/batched_lu.cuh:66-75, cuDSS) on the CPU.
+ * Pinned by tests/test_oracle_lu.py (dense partial-pivot solves). */
+int oracle_lu_solve(int n, const int *rowptr, const int *colidx, const double *vals, const double *b,
+                    double *x);
+
+/* Minimum-degree order of a symmetric graph given as CSR adjacency (no self loops needed).
+ * Pinned by tests/test_oracle_lu.py (permutation + fill check). */
+void oracle_min_degree(int n, const int *adjptr, const int *adj, int *perm);
+
+typedef struct {
+    int max_iter;          /* Newton budget per solve (OLF maxNewtonRaphsonIterations) */
+    double tol;            /* ||F||inf stopping threshold (newton_iteration.cu:300) */
+    int reactive_limits;   /* run the local PV<->PQ outer loop */
+    double rl_tol;         /* reactive-limit tolerance, pu */
+    int max_outer;         /* productive outer-loop iterations (outer_loop_driver.cu:191) */
+    int max_switch;        /* maxPqPvSwitch (outer_loop_driver.cu:194) */
+    int scaling_mode;      /* 0 NONE, 2 MAX_VOLTAGE_CHANGE (state_vector_scaling.cu:57-97) */
+    double max_dv, max_dphi;
+    int batch_of_one;      /* a bad last solve is LINEAR_SOLVE_FAILED (1) or REJECTED (0) */
+} oracle_nr_params;
+
+typedef struct {
+    int n;                 /* entries */
+    const int *voltage_row, *bus;
+    const double *min_q, *max_q, *load_q, *target_v;
+} oracle_rl;
+
+typedef struct {
+    int status, iterations, total_iterations, solves, outer_switches;
+    double norm;
+} oracle_result;
+
+/* One AC load flow with the reactive-limit outer loop, in place on x/mode/target/limit_type/switch_count.
+ * islanded[bus] (may be NULL) marks buses whose entries leave the outer loop.
+ * Follows solver/newton_iteration.cu:93-203,276-397,580-660 and outerloop/reactive_limits.cu:321-371.
+ * Pinned by tests/test_oracle_nr.py (two-bus closed form, Y-bus mismatch, power balance). */
+void oracle_nr_solve(const oracle_sys *s, const oracle_nr_params *prm, const oracle_rl *rl,
+                     const int *out_br, int n_out, const unsigned char *islanded,
+                     double *x, int *mode, double *target, signed char *limit_type,
+                     signed char *switch_count, oracle_result *res);
+
+/* Buses cut off from the slack's component once the listed branches are out (1 = islanded).
+ * Reference batch/plan/IslandedBuses.java:50-60. Returns the islanded count. Pinned by
+ * tests/test_oracle_nr.py (bridge outages vs the generator's bridge list). */
+int oracle_islands(const oracle_sys *s, int slack_bus, const int *out_br, int n_out, unsigned char *islanded);
+
+/* Batched N-1 on the CPU: every contingency starts from the base state/modes/targets/limit types,
+ * islanded buses are frozen at 1<0 (identity rows), OpenMP over contingencies with n_threads threads.
+ * ctg is n_ctg x max_out branch ids, -1 padded. Outputs per contingency. Returns the thread count used. */
+int oracle_n1(const oracle_sys *s, const oracle_nr_params *prm, const oracle_rl *rl, int slack_bus,
+              const double *x_base, const int *mode_base, const double *target_base,
+              const signed char *limit_type_base, int n_ctg, int max_out, const int *ctg,
+              int n_threads, double *x_out, int *status, int *iterations, int *total_iterations,
+              double *norm);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
