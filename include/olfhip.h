@@ -1,0 +1,204 @@
+/*
+ * olfhip — C ABI of the MI355X AC Newton-Raphson / batched N-1 library (libolfhip.so).
+ *
+ * The drop-in boundary for the reference's JNI natives: one JNI function per Java native marshals its arrays into
+ * these entry points (the binding is shown in INTEGRATION.md). Plain pointers and sizes only. Every entry returns 0
+ * on success or a negative OLFHIP_E_* code; olfhip_last_error() then holds the message (per thread). No exception
+ * crosses this boundary. Caller-owned input arrays are borrowed read-only for the duration of the call; outputs go
+ * into caller-provided buffers.
+ *
+ * Threading: a context is not re-entrant (one context per thread, as the reference serialises the device under one
+ * lock, GpuBatchPrecomputer.java:65); structures may be shared by contexts of the same thread.
+ */
+#ifndef OLFHIP_H
+#define OLFHIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* error codes */
+#define OLFHIP_OK 0
+#define OLFHIP_E_ARGUMENT -1   /* a contract violation (reference: IllegalArgumentException/IllegalStateException) */
+#define OLFHIP_E_DEVICE -2     /* a HIP runtime failure (reference: RuntimeException "GPU batched Newton-Raphson failed") */
+#define OLFHIP_E_MEMORY -3     /* a device or host allocation failed (the caller shrinks its chunk) */
+#define OLFHIP_E_UNSUPPORTED -4 /* an input shape this build does not model (reference: refusal, handed back to OLF) */
+
+/* per-scenario statuses: reference GpuAcSolverStatus (docs/gpu-vs-olf.md:196) */
+#define OLFHIP_CONVERGED 0
+#define OLFHIP_MAX_ITERATION_REACHED 1
+#define OLFHIP_DIVERGED -1
+#define OLFHIP_REJECTED -2
+#define OLFHIP_LINEAR_SOLVE_FAILED -3
+
+/* row modes: reference GpuRowMode */
+#define OLFHIP_ROW_FIXED_VARIABLE 0
+#define OLFHIP_ROW_EQUATION 1
+
+/* linear-solve modes (reference GpuLinearSolveMode; the reference's UBATCH/ROBUST are cuDSS modes) */
+#define OLFHIP_LU_FP64 0            /* batched block LU in FP64 */
+#define OLFHIP_LU_FP32_REFINED 1    /* FP32 factors and solves, one FP64-residual refinement step */
+
+/* state vector scaling: reference GpuStateVectorScalingMode */
+#define OLFHIP_SCALING_NONE 0
+#define OLFHIP_SCALING_MAX_VOLTAGE_CHANGE 2
+
+typedef struct olfhip_structure olfhip_structure;
+typedef struct olfhip_context olfhip_context;
+
+/* Message of the last failed call on this thread ("" if none). Reference: the RuntimeException text thrown by the
+ * JNI layer (jni/batch_solve_jni.cu:296-306). */
+const char *olfhip_last_error(void);
+
+/* Number of visible MI355X devices (0 without one); never fails. */
+int olfhip_device_count(void);
+
+/*
+ * Shared structure of one extraction: stable rows, the superset pattern, the fill gather tables, the fill-reducing
+ * order and the symbolic block LU plan. Replaces StructureNative.createStructure
+ * (src/main/java/com/powsybl/openloadflow/ac/equations/vector/gpu/StructureNative.java:49-59, JNI
+ * native/cuda/jni/structure_jni.cu:8) for the families this path models: closed branches (8 doubles
+ * y g1 g2 b1 b2 ksi r1 a1 and 12 ints P1 Q1 P2 Q2 ph1 v1 ph2 v2 a1col r1col altQ1 altQ2 each, the reference's
+ * packing, StructureNative.java:39-46) and bus shunts (2 doubles g b, 3 ints Prow Qrow vcol). Rows must be stable
+ * bus rows (P on the bus's phi row 2i, Q/V on its V row 2i+1); a1col/r1col/altQ must be -1 (no controls):
+ * anything else returns OLFHIP_E_UNSUPPORTED. n = 2 * buses.
+ */
+int olfhip_create_structure(int n, int n_closed_branch, const double *closed_branch_parameters,
+                            const int *closed_branch_indices, int n_shunt, const double *shunt_parameters,
+                            const int *shunt_indices, olfhip_structure **out);
+
+/* Drops the caller's reference (contexts keep the structure alive). NULL is a no-op. Reference:
+ * StructureNative.destroyStructure (StructureNative.java:67). */
+void olfhip_destroy_structure(olfhip_structure *structure);
+
+/* Statistics of the symbolic plan: [0] buses, [1] LU block slots per lane, [2] elimination levels,
+ * [3] factor block multiply-adds per lane, [4] pattern nonzeros (scalar), [5] LU nonzeros (scalar).
+ * Reference: BatchedLuNative.supernodalPlanStatistics / eliminationLevels (BatchedLuNative.java:75,82). */
+int olfhip_structure_statistics(const olfhip_structure *structure, long long *stats6);
+
+/* The bus elimination order of the plan (n/2 ints). Reference: BatchedLuNative.minimumDegreeOrder
+ * (BatchedLuNative.java:72). */
+int olfhip_structure_order(const olfhip_structure *structure, int *bus_order);
+
+/* Scenarios that fit one chunk: 70 % of free device memory over the per-scenario cost, clamped to [1, max_cap].
+ * Reference: StructureNative.recommendBatchSize (StructureNative.java:69-81). */
+int olfhip_recommend_batch_size(const olfhip_structure *structure, int max_cap, int mode, int *out);
+
+/* Reactive-limit entries of the local PV -> PQ outer loop (the reactive-limit part of
+ * BatchSolveNative.createBatchContext, BatchSolveNative.java:41-48). Values in pu; voltage_row is the entry bus's
+ * V row, whose P row is voltage_row - 1. */
+typedef struct {
+    int n_entries;
+    const int *voltage_row;
+    const double *min_q;
+    const double *max_q;
+    const double *load_q;
+    const double *target_v;
+} olfhip_reactive_limits;
+
+/* Per-context tuning (the autotuner's choices; zeros take the defaults). */
+typedef struct {
+    int lanes_per_tile;   /* scenarios per workgroup tile: 1, 2, 4, 8, 16 or 32 (0: picked from n_scenarios) */
+    int threads_per_tile; /* workgroup size: 256, 512 or 1024 (0: 512) */
+} olfhip_tuning;
+
+/* A batch context of n_scenarios slots on a structure: per-scenario device state and LU storage, sized once.
+ * Reference: BatchSolveNative.createBatchContext (BatchSolveNative.java:41-48; native/cuda/jni/batch_solve_jni.cu:29-81).
+ * tuning may be NULL. */
+int olfhip_create_batch_context(olfhip_structure *structure, int n_scenarios, int mode,
+                                const olfhip_reactive_limits *reactive_limits, const olfhip_tuning *tuning,
+                                olfhip_context **out);
+
+/* Frees the context's device memory. NULL is a no-op. Reference: BatchSolveNative.destroyBatchContext
+ * (BatchSolveNative.java:51). */
+void olfhip_destroy_batch_context(olfhip_context *context);
+
+/* Inputs of one batched solve: the hot-path subset of BatchSolveNative.solveBatchContext
+ * (BatchSolveNative.java:106-125), same meaning and order conventions:
+ *   x0_base, target_base, base_row_mode: [n] shared by every scenario (tiled on the device);
+ *   row_mode_entries: (scenario, row, mode) triples where a scenario's mode differs from the base (islanded buses
+ *     frozen, ...), with target_overrides (scenario * n + row -> value) for the targets those rows hold;
+ *   outaged_branches: [n_scenarios][max_outages] closed-branch ids, -1 padded (the closed-branch part of the
+ *     disabled-element mask; at most 4 per scenario);
+ *   x0_per_scenario: NULL or [n_scenarios][n] (the warm-start hook, BatchSolveNative.java:119);
+ *   reactive_limits_seed_state: NULL or [n_scenarios][n_entries] reactive-limit types (0 none, 1 min, 2 max)
+ *     of entries that start switched to PQ (the reference's reactiveLimitsSeedState);
+ *   max_iter, tol (||F||inf), reactive_limit_tol, state_vector_scaling_mode, max_dv, max_dphi,
+ *   max_outer_iterations (<= 0: 20), max_pq_pv_switch (< 0: 3), skip_reactive_limits;
+ *   states_to_host: 0 keeps the states on the device (BatchSolveNative.setBatchStatesToHost), where
+ *     olfhip_read_states reads them later. */
+typedef struct {
+    int n_scenarios;
+    const double *x0_base;
+    const double *target_base;
+    const int *base_row_mode;
+    int n_row_mode_entries;
+    const int *row_mode_entries;
+    int n_target_overrides;
+    const long long *target_override_index;
+    const double *target_override_value;
+    int max_outages;
+    const int *outaged_branches;
+    const double *x0_per_scenario;
+    const signed char *reactive_limits_seed_state;
+    int max_iter;
+    double tol;
+    double reactive_limit_tol;
+    int state_vector_scaling_mode;
+    double max_dv;
+    double max_dphi;
+    int max_outer_iterations;
+    int max_pq_pv_switch;
+    int skip_reactive_limits;
+    int states_to_host;
+} olfhip_solve_inputs;
+
+/* Outputs (any may be NULL): results [n_scenarios][n+3] = (x, iterations of the last solve, ||F||inf, status) in
+ * install order, the reference's layout (jni/batch_solve_jni.cu:308-337); status, iterations (last solve),
+ * total_iterations (all solves of the outer loop) and norm per scenario. */
+typedef struct {
+    double *results;
+    int *status;
+    int *iterations;
+    int *total_iterations;
+    double *norm;
+} olfhip_solve_outputs;
+
+/* Runs the batched Newton-Raphson with per-scenario retirement and the reactive-limit outer loop.
+ * Reference: BatchSolveNative.solveBatchContext (BatchSolveNative.java:106-125; native/cuda/jni/batch_solve_jni.cu:84-393,
+ * native/cuda/solver/outer_loop_driver.cu:80-410, native/cuda/solver/newton_iteration.cu:93-203). */
+int olfhip_solve_batch_context(olfhip_context *context, const olfhip_solve_inputs *inputs,
+                               olfhip_solve_outputs *outputs);
+
+/* States of the last solve (states_to_host = 0): [n_scenarios][n] into x. Reference: the StateHandback readback of
+ * states left on the device (BatchSolveNative.setBatchStatesToHost). */
+int olfhip_read_states(olfhip_context *context, int n_scenarios, double *x);
+
+/* Buses cut off from slack_bus once the listed closed branches are out (1 = islanded); returns the count in
+ * *n_islanded. Reference: the islanding plan of GpuBatchedSecurityAnalysisSolver.solveBatchN1
+ * (GpuBatchedSecurityAnalysisSolver.java:196-238) and batch/plan/IslandedBuses.java:50-60. */
+int olfhip_islanded_buses(const olfhip_structure *structure, int slack_bus, int n_outaged, const int *outaged,
+                          unsigned char *islanded, int *n_islanded);
+
+/* Standalone batched LU on the structure's Jacobian pattern: factorizes the systems of lane_map and solves them.
+ * values: [capacity][nnz] CSR values over olfhip_structure_pattern's pattern, scenario-major; rhs, solutions:
+ * [capacity][n]. pivot_clamps: per lane_map entry, the diagonal blocks clamped as near-singular (may be NULL).
+ * Reference: BatchedLuNative.factorizeAndSolve (BatchedLuNative.java:96-98; jni/batched_lu_jni.cu:309). */
+int olfhip_lu_factorize_and_solve(olfhip_structure *structure, int mode, int capacity, const double *values,
+                                  const double *rhs, int n_lanes, const int *lane_map, double *solutions,
+                                  int *pivot_clamps);
+
+/* The superset Jacobian pattern of the structure in stable-row CSR (rowptr [n+1], colidx [nnz]); pass NULL arrays
+ * to read nnz only. */
+int olfhip_structure_pattern(const olfhip_structure *structure, int *nnz, int *rowptr, int *colidx);
+
+/* Picks lanes_per_tile and threads_per_tile for this workload by timing full solves of `inputs` on the device for
+ * every candidate tile shape (a project-written autotuner; the reference sizes its chunks from memory only,
+ * StructureNative.java:69-81). */
+int olfhip_autotune(olfhip_structure *structure, int mode, const olfhip_reactive_limits *reactive_limits,
+                    const olfhip_solve_inputs *inputs, olfhip_tuning *best, double *best_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,738 @@
+// Host driver and C ABI of libolfhip.so (declarations and reference citations in include/olfhip.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "../../include/olfhip.h"
+#include "errors.hpp"
+#include "nr_kernels.hpp"
+#include "plan.hpp"
+
+namespace olfhip {
+
+static thread_local std::string gLastError;
+
+#define HIP_OK(expr)                                                                                     \
+    do {                                                                                                 \
+        hipError_t e_ = (expr);                                                                          \
+        if (e_ != hipSuccess) {                                                                          \
+            if (e_ == hipErrorOutOfMemory) throw MemoryError(std::string(#expr) + ": out of device memory"); \
+            throw DeviceError(std::string(#expr) + ": " + hipGetErrorString(e_));                        \
+        }                                                                                                \
+    } while (0)
+
+static bool traceOn() {
+    static int on = -1;
+    if (on < 0) {
+        const char *e = std::getenv("OLFHIP_TRACE");
+        on = (e && *e && std::strcmp(e, "0") != 0) ? 1 : 0;
+    }
+    return on == 1;
+}
+
+// RAII device buffer
+struct DBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    ~DBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    void alloc(size_t b) {
+        if (b <= bytes && p) return;
+        release();
+        HIP_OK(hipMalloc(&p, b ? b : 16));
+        bytes = b;
+    }
+    template <typename X>
+    void upload(const std::vector<X> &v, hipStream_t s = nullptr) {
+        alloc(v.size() * sizeof(X));
+        if (!v.empty()) HIP_OK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(X), hipMemcpyHostToDevice, s));
+    }
+    template <typename X>
+    X *as() const { return reinterpret_cast<X *>(p); }
+};
+
+struct StructureImpl {
+    Plan plan;
+    std::vector<int> bus1, bus2;
+    std::vector<int> incPtr, inc;    // branches incident to each bus (island search)
+    int refs = 1;
+    bool onDevice = false;
+    int device = -1;
+    std::vector<std::unique_ptr<DBuf>> bufs;
+    DevPlan dp{};
+
+    template <typename X>
+    const X *put(const std::vector<X> &v) {
+        bufs.emplace_back(new DBuf());
+        bufs.back()->upload(v);
+        return bufs.back()->as<X>();
+    }
+    void ensureDevice() {
+        if (onDevice) return;
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess || count == 0) throw DeviceError("no HIP device is visible");
+        HIP_OK(hipGetDevice(&device));
+        const Plan &P = plan;
+        dp.nb = P.nb;
+        dp.nSlots = P.nSlots;
+        dp.nLevels = P.nLevels;
+        dp.levelNodePtr = put(P.levelNodePtr);
+        dp.levelNodes = put(P.levelNodes);
+        dp.diagContribPtr = put(P.diagContribPtr);
+        dp.diagContribA = put(P.diagContribA);
+        dp.diagContribB = put(P.diagContribB);
+        dp.levelPairPtr = put(P.levelPairPtr);
+        dp.pairSlot = put(P.pairSlot);
+        dp.pairNode = put(P.pairNode);
+        dp.pairContribPtr = put(P.pairContribPtr);
+        dp.pairContribA = put(P.pairContribA);
+        dp.pairContribB = put(P.pairContribB);
+        dp.nodeBase = put(P.nodeBase);
+        dp.sPtr = put(P.sPtr);
+        dp.sIdx = put(P.sIdx);
+        dp.hbPtr = put(P.hbPtr);
+        dp.hbBranch = put(P.hbBranch);
+        dp.hbCoef = reinterpret_cast<const double4 *>(put(P.hbCoef));
+        dp.grpPtr = put(P.grpPtr);
+        dp.grpEnd = put(P.grpEnd);
+        dp.grpFar = put(P.grpFar);
+        dp.grpSlot = put(P.grpSlot);
+        dp.fillPtr = put(P.fillPtr);
+        dp.fillSlot = put(P.fillSlot);
+        dp.shuntG = put(P.shuntG);
+        dp.shuntB = put(P.shuntB);
+        dp.perm = put(P.perm);
+        dp.pos = put(P.pos);
+        HIP_OK(hipDeviceSynchronize());
+        onDevice = true;
+    }
+};
+
+static void release(StructureImpl *s) {
+    if (s && --s->refs == 0) delete s;
+}
+
+// Template dispatch over the lane tile width.
+template <typename Fn>
+static void withT(int T, Fn &&fn) {
+    switch (T) {
+        case 1: fn(std::integral_constant<int, 1>{}); break;
+        case 2: fn(std::integral_constant<int, 2>{}); break;
+        case 4: fn(std::integral_constant<int, 4>{}); break;
+        case 8: fn(std::integral_constant<int, 8>{}); break;
+        case 16: fn(std::integral_constant<int, 16>{}); break;
+        case 32: fn(std::integral_constant<int, 32>{}); break;
+        default: throw ArgumentError("lanes_per_tile must be 1, 2, 4, 8, 16 or 32");
+    }
+}
+
+static int defaultLanesPerTile(int nScenarios) {
+    if (nScenarios <= 1) return 1;
+    if (nScenarios <= 16) return 2;
+    if (nScenarios <= 512) return 4;
+    return 8;
+}
+
+struct ContextImpl {
+    StructureImpl *st = nullptr;
+    int capacity = 0, mode = 0, T = 1, NT = 512, nTiles = 0;
+    hipStream_t stream = nullptr;
+    int *pinnedCount = nullptr;
+    // reactive limits
+    int nRL = 0;
+    DBuf rlNode, rlMinQ, rlMaxQ, rlLoadQ, rlTargetV;
+    // per-lane state
+    DBuf x, F, dx, r, target, qinj, mode_, lu, y, dx1;
+    DBuf it, status, solves, outer, total, clamps, norm, ratio, outages, tileRunning, count;
+    DBuf rlType, rlCount, rlPass;
+    // install inputs
+    DBuf x0Base, x0Per, targetBase, modeBase, modeEntries, targetIdx, targetVal, seed, states;
+    int lastInstalled = 0;
+    double lastSolveMs = 0.0;
+    int lastSteps = 0;
+
+    DevState devState() const {
+        DevState S{};
+        S.nTiles = nTiles;
+        S.nInstalled = lastInstalled;
+        S.x = x.as<double>();
+        S.F = F.as<double>();
+        S.dx = dx.as<double>();
+        S.r = r.as<double>();
+        S.target = target.as<double>();
+        S.qinj = qinj.as<double>();
+        S.mode = mode_.as<signed char>();
+        S.lu = lu.p;
+        S.y = y.p;
+        S.dx1 = dx1.p;
+        S.it = it.as<int>();
+        S.status = status.as<int>();
+        S.solves = solves.as<int>();
+        S.outer = outer.as<int>();
+        S.total = total.as<int>();
+        S.clamps = clamps.as<int>();
+        S.norm = norm.as<double>();
+        S.ratio = ratio.as<double>();
+        S.outages = outages.as<int>();
+        S.maxOut = MAX_OUTAGES;
+        S.rlType = rlType.as<signed char>();
+        S.rlCount = rlCount.as<signed char>();
+        S.rlPass = rlPass.as<signed char>();
+        S.tileRunning = tileRunning.as<int>();
+        return S;
+    }
+    DevRL devRL() const {
+        return DevRL{nRL, rlNode.as<int>(), rlMinQ.as<double>(), rlMaxQ.as<double>(), rlLoadQ.as<double>(),
+                     rlTargetV.as<double>()};
+    }
+    ~ContextImpl() {
+        if (pinnedCount) (void)hipHostFree(pinnedCount);
+        if (stream) (void)hipStreamDestroy(stream);
+        release(st);
+    }
+};
+
+static size_t luBytes(int mode) { return mode == OLFHIP_LU_FP32_REFINED ? 4 : 8; }
+
+static ContextImpl *createContext(StructureImpl *st, int nScenarios, int mode, const olfhip_reactive_limits *rl,
+                                  const olfhip_tuning *tuning) {
+    if (!st) throw ArgumentError("structure is null");
+    if (nScenarios <= 0) throw ArgumentError("n_scenarios must be positive");
+    if (mode != OLFHIP_LU_FP64 && mode != OLFHIP_LU_FP32_REFINED) throw ArgumentError("unknown linear-solve mode");
+    st->ensureDevice();
+    std::unique_ptr<ContextImpl> c(new ContextImpl());
+    st->refs++;
+    c->st = st;
+    c->capacity = nScenarios;
+    c->mode = mode;
+    c->T = (tuning && tuning->lanes_per_tile > 0) ? tuning->lanes_per_tile : defaultLanesPerTile(nScenarios);
+    c->NT = (tuning && tuning->threads_per_tile > 0) ? tuning->threads_per_tile : 512;
+    withT(c->T, [](auto) {});
+    if (c->NT != 256 && c->NT != 512 && c->NT != 1024) throw ArgumentError("threads_per_tile must be 256, 512 or 1024");
+    c->nTiles = (nScenarios + c->T - 1) / c->T;
+    HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIP_OK(hipHostMalloc((void **)&c->pinnedCount, sizeof(int), hipHostMallocDefault));
+    const Plan &P = st->plan;
+    const size_t lanes = (size_t)c->nTiles * c->T;
+    const size_t nv = lanes * P.nb * 2;
+    c->x.alloc(nv * 8);
+    c->F.alloc(nv * 8);
+    c->dx.alloc(nv * 8);
+    c->r.alloc(nv * 8);
+    c->target.alloc(nv * 8);
+    c->qinj.alloc(lanes * P.nb * 8);
+    c->mode_.alloc(nv);
+    c->lu.alloc(lanes * (size_t)P.nSlots * 4 * luBytes(mode));
+    c->y.alloc(nv * luBytes(mode));
+    c->dx1.alloc(nv * luBytes(mode));
+    for (DBuf *b : {&c->it, &c->status, &c->solves, &c->outer, &c->total, &c->clamps}) b->alloc(lanes * 4);
+    c->norm.alloc(lanes * 8);
+    c->ratio.alloc(lanes * 8);
+    c->outages.alloc(lanes * MAX_OUTAGES * 4);
+    c->tileRunning.alloc((size_t)c->nTiles * 4);
+    c->count.alloc(16);
+    // reactive limits, entry buses in node numbering
+    if (rl && rl->n_entries > 0) {
+        if (!rl->voltage_row || !rl->min_q || !rl->max_q || !rl->load_q || !rl->target_v)
+            throw ArgumentError("reactive-limit arrays are null");
+        c->nRL = rl->n_entries;
+        std::vector<int> node(c->nRL);
+        for (int e = 0; e < c->nRL; e++) {
+            const int vr = rl->voltage_row[e];
+            if (vr < 0 || vr >= P.n || (vr & 1) == 0) throw ArgumentError("reactive-limit voltage row is not a V row");
+            node[e] = P.pos[vr >> 1];
+        }
+        c->rlNode.upload(node);
+        c->rlMinQ.upload(std::vector<double>(rl->min_q, rl->min_q + c->nRL));
+        c->rlMaxQ.upload(std::vector<double>(rl->max_q, rl->max_q + c->nRL));
+        c->rlLoadQ.upload(std::vector<double>(rl->load_q, rl->load_q + c->nRL));
+        c->rlTargetV.upload(std::vector<double>(rl->target_v, rl->target_v + c->nRL));
+    }
+    const size_t ne = lanes * (size_t)std::max(c->nRL, 1);
+    c->rlType.alloc(ne);
+    c->rlCount.alloc(ne);
+    c->rlPass.alloc(ne);
+    HIP_OK(hipDeviceSynchronize());
+    return c.release();
+}
+
+static int gridFor(size_t work, int block) {
+    size_t g = (work + block - 1) / block;
+    return (int)std::max<size_t>(1, std::min<size_t>(g, 65536));
+}
+
+static void solve(ContextImpl *c, const olfhip_solve_inputs *in, olfhip_solve_outputs *out) {
+    if (!in) throw ArgumentError("inputs are null");
+    const Plan &P = c->st->plan;
+    const int n = P.n;
+    const int K = in->n_scenarios;
+    if (K <= 0 || K > c->capacity) throw ArgumentError("n_scenarios must be in [1, context capacity]");
+    if (!in->x0_base || !in->target_base || !in->base_row_mode) throw ArgumentError("base state arrays are null");
+    if (in->max_iter <= 0) throw ArgumentError("max_iter must be positive");
+    if (in->max_outages < 0 || in->max_outages > MAX_OUTAGES)
+        throw UnsupportedError("at most 4 outaged branches per scenario are modelled");
+    if (in->max_outages > 0 && !in->outaged_branches) throw ArgumentError("outaged_branches is null");
+    if (in->n_row_mode_entries > 0 && !in->row_mode_entries) throw ArgumentError("row_mode_entries is null");
+    if (in->n_target_overrides > 0 && (!in->target_override_index || !in->target_override_value))
+        throw ArgumentError("target override arrays are null");
+    if (in->state_vector_scaling_mode != OLFHIP_SCALING_NONE &&
+        in->state_vector_scaling_mode != OLFHIP_SCALING_MAX_VOLTAGE_CHANGE)
+        throw UnsupportedError("state vector scaling: only NONE and MAX_VOLTAGE_CHANGE are modelled");
+    for (int i = 0; i < n; i++)
+        if (in->base_row_mode[i] != OLFHIP_ROW_FIXED_VARIABLE && in->base_row_mode[i] != OLFHIP_ROW_EQUATION)
+            throw UnsupportedError("base row mode " + std::to_string(in->base_row_mode[i]) + " is not modelled");
+    for (int q = 0; q < in->n_row_mode_entries; q++) {
+        const int *e = in->row_mode_entries + 3 * q;
+        if (e[0] < 0 || e[0] >= K || e[1] < 0 || e[1] >= n || (e[2] != 0 && e[2] != 1))
+            throw ArgumentError("row_mode_entries[" + std::to_string(q) + "] out of range");
+    }
+    for (int q = 0; q < in->n_target_overrides; q++)
+        if (in->target_override_index[q] < 0 || in->target_override_index[q] >= (long long)K * n)
+            throw ArgumentError("target_override_index[" + std::to_string(q) + "] out of range");
+    hipStream_t s = c->stream;
+    auto t0 = std::chrono::steady_clock::now();
+    c->lastInstalled = K;
+    const size_t lanes = (size_t)c->nTiles * c->T;
+    // outage lists, padded to MAX_OUTAGES
+    std::vector<int> outs(lanes * MAX_OUTAGES, -1);
+    for (int k = 0; k < K; k++)
+        for (int q = 0; q < in->max_outages; q++) {
+            const int b = in->outaged_branches[(size_t)k * in->max_outages + q];
+            if (b >= P.nBranch) throw ArgumentError("outaged branch id out of range");
+            outs[(size_t)k * MAX_OUTAGES + q] = b < 0 ? -1 : b;
+        }
+    c->outages.upload(outs, s);
+    c->x0Base.upload(std::vector<double>(in->x0_base, in->x0_base + n), s);
+    c->targetBase.upload(std::vector<double>(in->target_base, in->target_base + n), s);
+    c->modeBase.upload(std::vector<int>(in->base_row_mode, in->base_row_mode + n), s);
+    if (in->x0_per_scenario) {
+        c->x0Per.alloc((size_t)K * n * 8);
+        HIP_OK(hipMemcpyAsync(c->x0Per.p, in->x0_per_scenario, (size_t)K * n * 8, hipMemcpyHostToDevice, s));
+    }
+    if (in->n_row_mode_entries > 0)
+        c->modeEntries.upload(std::vector<int>(in->row_mode_entries, in->row_mode_entries + 3 * in->n_row_mode_entries), s);
+    if (in->n_target_overrides > 0) {
+        c->targetIdx.upload(std::vector<long long>(in->target_override_index, in->target_override_index + in->n_target_overrides), s);
+        c->targetVal.upload(std::vector<double>(in->target_override_value, in->target_override_value + in->n_target_overrides), s);
+    }
+    if (in->reactive_limits_seed_state && c->nRL > 0) {
+        c->seed.alloc((size_t)K * c->nRL);
+        HIP_OK(hipMemcpyAsync(c->seed.p, in->reactive_limits_seed_state, (size_t)K * c->nRL, hipMemcpyHostToDevice, s));
+    }
+    NrParams prm{};
+    prm.maxIter = in->max_iter;
+    prm.tol = in->tol;
+    prm.reactiveLimits = !in->skip_reactive_limits;
+    prm.rlTol = in->reactive_limit_tol;
+    prm.maxOuter = in->max_outer_iterations > 0 ? in->max_outer_iterations : 20;
+    prm.maxSwitch = in->max_pq_pv_switch >= 0 ? in->max_pq_pv_switch : 3;
+    prm.scaling = in->state_vector_scaling_mode;
+    prm.maxDv = in->max_dv;
+    prm.maxDphi = in->max_dphi;
+    prm.refine = c->mode == OLFHIP_LU_FP32_REFINED;
+    prm.batchOfOne = K == 1;
+    const DevPlan dp = c->st->dp;
+    const DevState S = c->devState();
+    const DevRL R = c->devRL();
+    const bool useGraph = !std::getenv("OLFHIP_NO_GRAPH");
+    const int maxSteps = (prm.maxIter + 1) * (prm.maxOuter + 1) + 1;
+    int steps = 0;
+    withT(c->T, [&](auto TC) {
+        constexpr int T = decltype(TC)::value;
+        const size_t work = (size_t)c->nTiles * P.nb * T;
+        installKernel<T><<<gridFor(work, 256), 256, 0, s>>>(dp, S, c->x0Base.as<double>(),
+                                                            in->x0_per_scenario ? c->x0Per.as<double>() : nullptr,
+                                                            c->targetBase.as<double>(), c->modeBase.as<int>());
+        if (in->n_row_mode_entries + in->n_target_overrides > 0)
+            overrideKernel<T><<<gridFor(in->n_row_mode_entries + in->n_target_overrides, 256), 256, 0, s>>>(
+                dp, S, in->n_row_mode_entries, c->modeEntries.as<int>(), in->n_target_overrides,
+                c->targetIdx.as<long long>(), c->targetVal.as<double>());
+        laneInitKernel<T><<<gridFor(lanes * (1 + (size_t)c->nRL), 256), 256, 0, s>>>(
+            dp, S, R, (in->reactive_limits_seed_state && c->nRL > 0) ? c->seed.as<signed char>() : nullptr,
+            c->modeBase.as<int>());
+        HIP_OK(hipGetLastError());
+        auto enqueueStep = [&]() {
+            if (c->mode == OLFHIP_LU_FP32_REFINED)
+                nrStepKernel<T, float><<<c->nTiles, c->NT, 0, s>>>(dp, S, R, prm);
+            else
+                nrStepKernel<T, double><<<c->nTiles, c->NT, 0, s>>>(dp, S, R, prm);
+            countRunningKernel<<<1, 256, 0, s>>>(S.tileRunning, c->nTiles, c->count.as<int>());
+            HIP_OK(hipMemcpyAsync(c->pinnedCount, c->count.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        };
+        hipGraphExec_t exec = nullptr;
+        hipGraph_t graph = nullptr;
+        if (useGraph) {
+            HIP_OK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            enqueueStep();
+            HIP_OK(hipStreamEndCapture(s, &graph));
+            HIP_OK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        }
+        for (steps = 0; steps < maxSteps; steps++) {
+            if (useGraph) HIP_OK(hipGraphLaunch(exec, s));
+            else enqueueStep();
+            HIP_OK(hipStreamSynchronize(s));
+            if (traceOn()) std::fprintf(stderr, "OLFHIP step %d running tiles %d\n", steps, *c->pinnedCount);
+            if (*c->pinnedCount == 0) {
+                steps++;
+                break;
+            }
+        }
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    });
+    HIP_OK(hipGetLastError());
+    // results
+    std::vector<int> st(K), itv(K), tot(K);
+    std::vector<double> nm(K);
+    HIP_OK(hipMemcpyAsync(st.data(), c->status.p, K * 4, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(itv.data(), c->it.p, K * 4, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(tot.data(), c->total.p, K * 4, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(nm.data(), c->norm.p, K * 8, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    for (int k = 0; k < K; k++)
+        if (st[k] == LANE_RUNNING) st[k] = ST_REJECTED;   // batched-iteration cap (newton_iteration.cu:410-429)
+    c->lastSteps = steps;
+    const bool wantStates = out && out->results && in->states_to_host;
+    std::vector<double> xs;
+    if (wantStates) {
+        c->states.alloc((size_t)K * n * 8);
+        withT(c->T, [&](auto TC) {
+            constexpr int T = decltype(TC)::value;
+            gatherStatesKernel<T><<<gridFor((size_t)K * P.nb, 256), 256, 0, s>>>(dp, S, K, c->states.as<double>());
+        });
+        xs.resize((size_t)K * n);
+        HIP_OK(hipMemcpyAsync(xs.data(), c->states.p, (size_t)K * n * 8, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+    }
+    if (out) {
+        for (int k = 0; k < K; k++) {
+            if (out->status) out->status[k] = st[k];
+            if (out->iterations) out->iterations[k] = itv[k];
+            if (out->total_iterations) out->total_iterations[k] = tot[k];
+            if (out->norm) out->norm[k] = nm[k];
+            if (out->results) {
+                double *row = out->results + (size_t)k * (n + 3);
+                if (wantStates) std::memcpy(row, xs.data() + (size_t)k * n, (size_t)n * 8);
+                row[n] = itv[k];
+                row[n + 1] = nm[k];
+                row[n + 2] = st[k];
+            }
+        }
+    }
+    c->lastSolveMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (traceOn()) std::fprintf(stderr, "OLFHIP solve K=%d T=%d NT=%d steps=%d %.3f ms\n", K, c->T, c->NT, steps, c->lastSolveMs);
+}
+
+static void readStates(ContextImpl *c, int K, double *x) {
+    if (K <= 0 || K > c->lastInstalled) throw ArgumentError("n_scenarios exceeds the last solve's installed count");
+    if (!x) throw ArgumentError("x is null");
+    const Plan &P = c->st->plan;
+    c->states.alloc((size_t)K * P.n * 8);
+    const DevPlan dp = c->st->dp;
+    const DevState S = c->devState();
+    withT(c->T, [&](auto TC) {
+        constexpr int T = decltype(TC)::value;
+        gatherStatesKernel<T><<<gridFor((size_t)K * P.nb, 256), 256, 0, c->stream>>>(dp, S, K, c->states.as<double>());
+    });
+    HIP_OK(hipMemcpyAsync(x, c->states.p, (size_t)K * P.n * 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+}
+
+static void luFactorizeAndSolve(StructureImpl *st, int mode, int capacity, const double *values, const double *rhs,
+                                int nLanes, const int *laneMap, double *solutions, int *clampsOut) {
+    if (!st || !values || !rhs || !laneMap || !solutions) throw ArgumentError("null argument");
+    if (nLanes <= 0 || capacity <= 0) throw ArgumentError("lane counts must be positive");
+    if (mode != OLFHIP_LU_FP64 && mode != OLFHIP_LU_FP32_REFINED) throw ArgumentError("unknown mode");
+    st->ensureDevice();
+    const Plan &P = st->plan;
+    const int nnz = (int)P.colIdx.size();
+    const int T = nLanes >= 8 ? 8 : 1;
+    const int nTiles = (nLanes + T - 1) / T;
+    const size_t vb = luBytes(mode);
+    std::vector<char> lu((size_t)nTiles * P.nSlots * 4 * T * vb, 0);
+    std::vector<double> b((size_t)nTiles * P.nb * 2 * T, 0.0);
+    for (int i = 0; i < nLanes; i++) {
+        const int lane = laneMap[i];
+        if (lane < 0 || lane >= capacity) throw ArgumentError("lane_map entry out of range");
+        const int tile = i / T, t = i % T;
+        for (int q = 0; q < nnz; q++) {
+            const int sc = P.csrSlot[q];
+            const size_t idx = (((size_t)tile * P.nSlots + sc / 4) * T + t) * 4 + sc % 4;
+            const double v = values[(size_t)lane * nnz + q];
+            if (vb == 8) reinterpret_cast<double *>(lu.data())[idx] = v;
+            else reinterpret_cast<float *>(lu.data())[idx] = (float)v;
+        }
+        for (int bus = 0; bus < P.nb; bus++)
+            for (int c = 0; c < 2; c++)
+                b[(((size_t)tile * P.nb + P.pos[bus]) * T + t) * 2 + c] = rhs[(size_t)lane * P.n + 2 * bus + c];
+    }
+    DBuf dlu, dy, dxo, drhs, dcl;
+    dlu.upload(lu);
+    drhs.upload(b);
+    dy.alloc((size_t)nTiles * P.nb * 2 * T * vb);
+    dxo.alloc((size_t)nTiles * P.nb * 2 * T * vb);
+    dcl.alloc((size_t)nTiles * T * 4);
+    const DevPlan dp = st->dp;
+    withT(T, [&](auto TC) {
+        constexpr int TT = decltype(TC)::value;
+        if (vb == 8)
+            luSolveKernel<TT, double><<<nTiles, 512>>>(dp, dlu.as<double>(), dy.as<double>(), dxo.as<double>(),
+                                                       drhs.as<double>(), dcl.as<int>());
+        else
+            luSolveKernel<TT, float><<<nTiles, 512>>>(dp, dlu.as<float>(), dy.as<float>(), dxo.as<float>(),
+                                                      drhs.as<double>(), dcl.as<int>());
+    });
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipDeviceSynchronize());
+    std::vector<char> xo((size_t)nTiles * P.nb * 2 * T * vb);
+    std::vector<int> cl((size_t)nTiles * T);
+    HIP_OK(hipMemcpy(xo.data(), dxo.p, xo.size(), hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(cl.data(), dcl.p, cl.size() * 4, hipMemcpyDeviceToHost));
+    for (int i = 0; i < nLanes; i++) {
+        const int lane = laneMap[i];
+        const int tile = i / T, t = i % T;
+        for (int bus = 0; bus < P.nb; bus++)
+            for (int c = 0; c < 2; c++) {
+                const size_t idx = (((size_t)tile * P.nb + P.pos[bus]) * T + t) * 2 + c;
+                solutions[(size_t)lane * P.n + 2 * bus + c] =
+                    vb == 8 ? reinterpret_cast<double *>(xo.data())[idx] : reinterpret_cast<float *>(xo.data())[idx];
+            }
+        if (clampsOut) clampsOut[i] = cl[(size_t)tile * T + t];
+    }
+}
+
+}  // namespace olfhip
+
+using namespace olfhip;
+
+template <typename Fn>
+static int guarded(Fn &&fn) {
+    try {
+        gLastError.clear();
+        fn();
+        return OLFHIP_OK;
+    } catch (const ArgumentError &e) {
+        gLastError = e.what();
+        return OLFHIP_E_ARGUMENT;
+    } catch (const UnsupportedError &e) {
+        gLastError = e.what();
+        return OLFHIP_E_UNSUPPORTED;
+    } catch (const MemoryError &e) {
+        gLastError = e.what();
+        return OLFHIP_E_MEMORY;
+    } catch (const std::bad_alloc &) {
+        gLastError = "host allocation failed";
+        return OLFHIP_E_MEMORY;
+    } catch (const std::exception &e) {
+        gLastError = std::string("GPU batched Newton-Raphson failed: ") + e.what();
+        return OLFHIP_E_DEVICE;
+    } catch (...) {
+        gLastError = "unknown failure";
+        return OLFHIP_E_DEVICE;
+    }
+}
+
+extern "C" {
+
+const char *olfhip_last_error(void) { return gLastError.c_str(); }
+
+int olfhip_device_count(void) {
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) return 0;
+    return c;
+}
+
+int olfhip_create_structure(int n, int n_closed_branch, const double *closed_branch_parameters,
+                            const int *closed_branch_indices, int n_shunt, const double *shunt_parameters,
+                            const int *shunt_indices, olfhip_structure **out) {
+    return guarded([&] {
+        if (!out) throw ArgumentError("out is null");
+        *out = nullptr;
+        std::unique_ptr<StructureImpl> s(new StructureImpl());
+        s->plan = buildPlan(n, n_closed_branch, closed_branch_parameters, closed_branch_indices, n_shunt,
+                            shunt_parameters, shunt_indices);
+        s->bus1.resize(n_closed_branch);
+        s->bus2.resize(n_closed_branch);
+        for (int e = 0; e < n_closed_branch; e++) {
+            s->bus1[e] = closed_branch_indices[12 * e] / 2;
+            s->bus2[e] = closed_branch_indices[12 * e + 2] / 2;
+        }
+        const int nb = n / 2;
+        s->incPtr.assign(nb + 1, 0);
+        for (int e = 0; e < n_closed_branch; e++) {
+            s->incPtr[s->bus1[e] + 1]++;
+            s->incPtr[s->bus2[e] + 1]++;
+        }
+        for (int b = 0; b < nb; b++) s->incPtr[b + 1] += s->incPtr[b];
+        s->inc.assign(s->incPtr[nb], 0);
+        std::vector<int> fill(s->incPtr.begin(), s->incPtr.end() - 1);
+        for (int e = 0; e < n_closed_branch; e++) {
+            s->inc[fill[s->bus1[e]]++] = e;
+            s->inc[fill[s->bus2[e]]++] = e;
+        }
+        *out = reinterpret_cast<olfhip_structure *>(s.release());
+    });
+}
+
+void olfhip_destroy_structure(olfhip_structure *structure) { release(reinterpret_cast<StructureImpl *>(structure)); }
+
+int olfhip_structure_statistics(const olfhip_structure *structure, long long *stats6) {
+    return guarded([&] {
+        if (!structure || !stats6) throw ArgumentError("null argument");
+        const Plan &P = reinterpret_cast<const StructureImpl *>(structure)->plan;
+        stats6[0] = P.nb;
+        stats6[1] = P.nSlots;
+        stats6[2] = P.nLevels;
+        stats6[3] = P.factorMads;
+        stats6[4] = (long long)P.colIdx.size();
+        stats6[5] = 4LL * P.nSlots;
+    });
+}
+
+int olfhip_structure_order(const olfhip_structure *structure, int *bus_order) {
+    return guarded([&] {
+        if (!structure || !bus_order) throw ArgumentError("null argument");
+        const Plan &P = reinterpret_cast<const StructureImpl *>(structure)->plan;
+        std::memcpy(bus_order, P.perm.data(), P.perm.size() * sizeof(int));
+    });
+}
+
+int olfhip_structure_pattern(const olfhip_structure *structure, int *nnz, int *rowptr, int *colidx) {
+    return guarded([&] {
+        if (!structure || !nnz) throw ArgumentError("null argument");
+        const Plan &P = reinterpret_cast<const StructureImpl *>(structure)->plan;
+        *nnz = (int)P.colIdx.size();
+        if (rowptr) std::memcpy(rowptr, P.rowPtr.data(), P.rowPtr.size() * sizeof(int));
+        if (colidx) std::memcpy(colidx, P.colIdx.data(), P.colIdx.size() * sizeof(int));
+    });
+}
+
+int olfhip_recommend_batch_size(const olfhip_structure *structure, int max_cap, int mode, int *out) {
+    return guarded([&] {
+        if (!structure || !out) throw ArgumentError("null argument");
+        if (max_cap <= 0) throw ArgumentError("max_cap must be positive");
+        const Plan &P = reinterpret_cast<const StructureImpl *>(structure)->plan;
+        size_t freeB = 0, totalB = 0;
+        HIP_OK(hipMemGetInfo(&freeB, &totalB));
+        const size_t per = P.bytesPerScenario((int)luBytes(mode));
+        const size_t fit = (size_t)(0.7 * (double)freeB) / per;
+        *out = (int)std::max<size_t>(1, std::min<size_t>(fit, (size_t)max_cap));
+    });
+}
+
+int olfhip_create_batch_context(olfhip_structure *structure, int n_scenarios, int mode,
+                                const olfhip_reactive_limits *reactive_limits, const olfhip_tuning *tuning,
+                                olfhip_context **out) {
+    return guarded([&] {
+        if (!out) throw ArgumentError("out is null");
+        *out = nullptr;
+        *out = reinterpret_cast<olfhip_context *>(
+            createContext(reinterpret_cast<StructureImpl *>(structure), n_scenarios, mode, reactive_limits, tuning));
+    });
+}
+
+void olfhip_destroy_batch_context(olfhip_context *context) { delete reinterpret_cast<ContextImpl *>(context); }
+
+int olfhip_solve_batch_context(olfhip_context *context, const olfhip_solve_inputs *inputs,
+                               olfhip_solve_outputs *outputs) {
+    return guarded([&] {
+        if (!context) throw ArgumentError("context is null");
+        solve(reinterpret_cast<ContextImpl *>(context), inputs, outputs);
+    });
+}
+
+int olfhip_read_states(olfhip_context *context, int n_scenarios, double *x) {
+    return guarded([&] {
+        if (!context) throw ArgumentError("context is null");
+        readStates(reinterpret_cast<ContextImpl *>(context), n_scenarios, x);
+    });
+}
+
+int olfhip_islanded_buses(const olfhip_structure *structure, int slack_bus, int n_outaged, const int *outaged,
+                          unsigned char *islanded, int *n_islanded) {
+    return guarded([&] {
+        if (!structure || !islanded) throw ArgumentError("null argument");
+        const StructureImpl *s = reinterpret_cast<const StructureImpl *>(structure);
+        const int nb = s->plan.nb;
+        if (slack_bus < 0 || slack_bus >= nb) throw ArgumentError("slack bus out of range");
+        if (n_outaged > 0 && !outaged) throw ArgumentError("outaged is null");
+        std::vector<char> out(s->bus1.size(), 0);
+        for (int q = 0; q < n_outaged; q++)
+            if (outaged[q] >= 0 && outaged[q] < (int)out.size()) out[outaged[q]] = 1;
+        std::vector<int> queue{slack_bus};
+        std::vector<char> seen(nb, 0);
+        seen[slack_bus] = 1;
+        for (size_t h = 0; h < queue.size(); h++) {
+            const int u = queue[h];
+            for (int q = s->incPtr[u]; q < s->incPtr[u + 1]; q++) {
+                const int e = s->inc[q];
+                if (out[e]) continue;
+                const int w = s->bus1[e] == u ? s->bus2[e] : s->bus1[e];
+                if (!seen[w]) {
+                    seen[w] = 1;
+                    queue.push_back(w);
+                }
+            }
+        }
+        for (int b = 0; b < nb; b++) islanded[b] = seen[b] ? 0 : 1;
+        if (n_islanded) *n_islanded = nb - (int)queue.size();
+    });
+}
+
+int olfhip_lu_factorize_and_solve(olfhip_structure *structure, int mode, int capacity, const double *values,
+                                  const double *rhs, int n_lanes, const int *lane_map, double *solutions,
+                                  int *pivot_clamps) {
+    return guarded([&] {
+        luFactorizeAndSolve(reinterpret_cast<StructureImpl *>(structure), mode, capacity, values, rhs, n_lanes,
+                            lane_map, solutions, pivot_clamps);
+    });
+}
+
+int olfhip_autotune(olfhip_structure *structure, int mode, const olfhip_reactive_limits *reactive_limits,
+                    const olfhip_solve_inputs *inputs, olfhip_tuning *best, double *best_ms) {
+    return guarded([&] {
+        if (!structure || !inputs || !best) throw ArgumentError("null argument");
+        StructureImpl *st = reinterpret_cast<StructureImpl *>(structure);
+        const int K = inputs->n_scenarios;
+        double bestMs = 1e300;
+        olfhip_tuning bestT{0, 0};
+        for (int T : {1, 2, 4, 8, 16, 32}) {
+            if (T > 1 && T > K) break;
+            for (int NT : {256, 512, 1024}) {
+                olfhip_tuning tu{T, NT};
+                std::unique_ptr<ContextImpl> c(createContext(st, K, mode, reactive_limits, &tu));
+                olfhip_solve_inputs in = *inputs;
+                in.states_to_host = 0;
+                solve(c.get(), &in, nullptr);   // warm-up (code objects, caches)
+                auto t0 = std::chrono::steady_clock::now();
+                solve(c.get(), &in, nullptr);
+                const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+                if (traceOn()) std::fprintf(stderr, "OLFHIP autotune T=%d NT=%d %.3f ms\n", T, NT, ms);
+                if (ms < bestMs) {
+                    bestMs = ms;
+                    bestT = tu;
+                }
+            }
+        }
+        *best = bestT;
+        if (best_ms) *best_ms = bestMs;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,884 @@
+// Device code of the batched Newton-Raphson: one workgroup owns one tile of T scenarios ("lanes") and runs a whole
+// Newton step for them — fill, convergence test, reactive-limit check, block LU with the forward solve fused in,
+// backward solve, optional FP64-residual refinement, linear-solve guard, step scaling and update — with workgroup
+// barriers between elimination levels and no global synchronisation. Tiles are independent, so a launch of
+// nTiles workgroups is one batched Newton iteration of the whole chunk.
+//
+// Layouts (lane fastest, so that the T lanes of one item read one contiguous run):
+//   per-node vectors  v[((tile * nb + k) * T + t) * 2 + c]        c = 0: phi / P row, 1: V / Q row
+//   LU block slots    lu[((tile * nSlots + s) * T + t) * 4 + c]   c = (P,phi) (P,V) (Q,phi) (Q,V)
+//   per-entry state   e[(tile * nEntries + i) * T + t]
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace olfhip {
+
+constexpr int LANE_RUNNING = 100;   // still solving
+constexpr int LANE_PADDING = 99;    // a slot of the last tile beyond the installed scenarios
+constexpr int MAX_OUTAGES = 4;
+constexpr int MAX_THREADS = 1024;
+
+// status codes (reference GpuAcSolverStatus, docs/gpu-vs-olf.md:196)
+constexpr int ST_CONVERGED = 0, ST_MAX_ITER = 1, ST_DIVERGED = -1, ST_REJECTED = -2, ST_LIN_FAILED = -3;
+constexpr signed char MODE_FIXED = 0, MODE_EQUATION = 1;
+constexpr signed char QLIM_NONE = 0, QLIM_MIN = 1, QLIM_MAX = 2, QLIM_FROZEN = 5;
+constexpr double LIN_SOLVE_FAIL_RATIO = 1e-2;   // reference native/cuda/solver/iterative_refinement.cuh:23
+
+struct DevPlan {
+    int nb, nSlots, nLevels;
+    const int *levelNodePtr, *levelNodes;
+    const int *diagContribPtr, *diagContribA, *diagContribB;
+    const int *levelPairPtr, *pairSlot, *pairNode, *pairContribPtr, *pairContribA, *pairContribB;
+    const int *nodeBase, *sPtr, *sIdx;
+    const int *hbPtr, *hbBranch;
+    const double4 *hbCoef;
+    const int *grpPtr, *grpEnd, *grpFar, *grpSlot;
+    const int *fillPtr, *fillSlot;
+    const double *shuntG, *shuntB;
+    const int *perm, *pos;
+};
+
+struct DevState {
+    int nTiles, nInstalled;
+    double *x, *F, *dx, *r, *target, *qinj;
+    signed char *mode;
+    void *lu, *y, *dx1;
+    int *it, *status, *solves, *outer, *total, *clamps;
+    double *norm, *ratio;
+    const int *outages;
+    int maxOut;
+    signed char *rlType, *rlCount, *rlPass;
+    int *tileRunning;
+};
+
+struct DevRL {
+    int n;
+    const int *node;
+    const double *minQ, *maxQ, *loadQ, *targetV;
+};
+
+struct NrParams {
+    int maxIter;
+    double tol;
+    int reactiveLimits;
+    double rlTol;
+    int maxOuter;
+    int maxSwitch;
+    int scaling;
+    double maxDv, maxDphi;
+    int refine;
+    int batchOfOne;
+};
+
+template <typename VT>
+struct Blk {
+    VT a, b, c, d;  // [[a b] [c d]]
+};
+template <typename VT>
+struct V2 {
+    VT p, q;
+};
+
+template <typename VT>
+__device__ __forceinline__ Blk<VT> ldBlk(const VT *p);
+template <>
+__device__ __forceinline__ Blk<float> ldBlk<float>(const float *p) {
+    float4 v = *reinterpret_cast<const float4 *>(p);
+    return {v.x, v.y, v.z, v.w};
+}
+template <>
+__device__ __forceinline__ Blk<double> ldBlk<double>(const double *p) {
+    double2 u = reinterpret_cast<const double2 *>(p)[0];
+    double2 v = reinterpret_cast<const double2 *>(p)[1];
+    return {u.x, u.y, v.x, v.y};
+}
+template <typename VT>
+__device__ __forceinline__ void stBlk(VT *p, const Blk<VT> &b);
+template <>
+__device__ __forceinline__ void stBlk<float>(float *p, const Blk<float> &b) {
+    *reinterpret_cast<float4 *>(p) = make_float4(b.a, b.b, b.c, b.d);
+}
+template <>
+__device__ __forceinline__ void stBlk<double>(double *p, const Blk<double> &b) {
+    reinterpret_cast<double2 *>(p)[0] = make_double2(b.a, b.b);
+    reinterpret_cast<double2 *>(p)[1] = make_double2(b.c, b.d);
+}
+template <typename VT>
+__device__ __forceinline__ V2<VT> ldV(const VT *p);
+template <>
+__device__ __forceinline__ V2<float> ldV<float>(const float *p) {
+    float2 v = *reinterpret_cast<const float2 *>(p);
+    return {v.x, v.y};
+}
+template <>
+__device__ __forceinline__ V2<double> ldV<double>(const double *p) {
+    double2 v = *reinterpret_cast<const double2 *>(p);
+    return {v.x, v.y};
+}
+template <typename VT>
+__device__ __forceinline__ void stV(VT *p, V2<VT> v);
+template <>
+__device__ __forceinline__ void stV<float>(float *p, V2<float> v) {
+    *reinterpret_cast<float2 *>(p) = make_float2(v.p, v.q);
+}
+template <>
+__device__ __forceinline__ void stV<double>(double *p, V2<double> v) {
+    *reinterpret_cast<double2 *>(p) = make_double2(v.p, v.q);
+}
+
+// acc -= X * Y (2x2 blocks)
+template <typename VT>
+__device__ __forceinline__ void mulSub(Blk<VT> &acc, const Blk<VT> &X, const Blk<VT> &Y) {
+    acc.a -= X.a * Y.a + X.b * Y.c;
+    acc.b -= X.a * Y.b + X.b * Y.d;
+    acc.c -= X.c * Y.a + X.d * Y.c;
+    acc.d -= X.c * Y.b + X.d * Y.d;
+}
+template <typename VT>
+__device__ __forceinline__ Blk<VT> mul(const Blk<VT> &X, const Blk<VT> &Y) {
+    return {X.a * Y.a + X.b * Y.c, X.a * Y.b + X.b * Y.d, X.c * Y.a + X.d * Y.c, X.c * Y.b + X.d * Y.d};
+}
+// v -= X * w
+template <typename VT>
+__device__ __forceinline__ void mulSubV(V2<VT> &v, const Blk<VT> &X, const V2<VT> &w) {
+    v.p -= X.a * w.p + X.b * w.q;
+    v.q -= X.c * w.p + X.d * w.q;
+}
+
+// Inverse of a diagonal block with a static-pivot clamp (the reference clamps near-zero scalar pivots to +-tiny
+// rather than failing, native/cuda/lu/lu_pivot.cuh:12-21; here on the block determinant).
+template <typename VT>
+__device__ __forceinline__ Blk<VT> inv2(const Blk<VT> &D, bool &clamped) {
+    VT det = D.a * D.d - D.b * D.c;
+    const VT tiny = (VT)1e-26;
+    clamped = !(fabs(det) > tiny);
+    if (clamped) det = det < (VT)0 ? -tiny : tiny;
+    VT id = (VT)1 / det;
+    return {D.d * id, -D.b * id, -D.c * id, D.a * id};
+}
+
+template <int T>
+__device__ __forceinline__ size_t vIdx(int tile, int nb, int k, int t) {
+    return ((size_t)tile * nb + k) * (2 * T) + 2 * t;
+}
+template <int T>
+__device__ __forceinline__ size_t qIdx(int tile, int nb, int k, int t) {
+    return ((size_t)tile * nb + k) * T + t;
+}
+template <int T>
+__device__ __forceinline__ size_t bIdx(int tile, int nSlots, int s, int t) {
+    return ((size_t)tile * nSlots + s) * (4 * T) + 4 * t;
+}
+
+__device__ __forceinline__ bool isOut(int br, const int *o) {
+    return br == o[0] || br == o[1] || br == o[2] || br == o[3];
+}
+
+// The half-branch terms of node k, lane t: the mismatch sums, the diagonal block and per neighbour group the
+// off-diagonal block (WRITE_J), or J * v for node k (APPLY_VEC). Row modes mask the rows of FIXED_VARIABLE (identity)
+// rows as the reference's applyRowModeBatch does (native/cuda/equations/fill_launch.cu:82-115). The closed-branch
+// formulas are the reference's (native/cuda/equations/generated/closed_branch.cuh:31-229, OLF
+// ClosedBranchSide1/2*FlowEquationTerm) in own-bus / far-bus form:
+//   p = A vo^2 - B vo vf sin(th),  q = C vo^2 - B vo vf cos(th),  th = psi - phi_o + phi_f.
+template <int T, typename VT, bool WRITE_J, bool APPLY_VEC>
+__device__ __forceinline__ void nodeTerms(const DevPlan &P, int tile, int k, int t, const double *__restrict__ x,
+                                          const int *out, VT *__restrict__ lu, const double *__restrict__ v,
+                                          signed char modeP, signed char modeQ, double &sumP, double &sumQ,
+                                          double &jvP, double &jvQ) {
+    const int nb = P.nb;
+    const double2 xo = *reinterpret_cast<const double2 *>(x + vIdx<T>(tile, nb, k, t));
+    const double pho = xo.x, vo = xo.y;
+    double dPdth = 0, dPdv = 0, dQdth = 0, dQdv = 0;
+    double sP = 0, sQ = 0, jP = 0, jQ = 0;
+    int h = P.hbPtr[k];
+    const int g0 = P.grpPtr[k], g1 = P.grpPtr[k + 1];
+    for (int g = g0; g < g1; g++) {
+        const int f = P.grpFar[g];
+        const int hEnd = P.grpEnd[g];
+        const double2 xf = *reinterpret_cast<const double2 *>(x + vIdx<T>(tile, nb, f, t));
+        const double phf = xf.x, vf = xf.y;
+        double oPth = 0, oPv = 0, oQth = 0, oQv = 0;
+        for (; h < hEnd; h++) {
+            if (isOut(P.hbBranch[h], out)) continue;
+            const double4 cf = P.hbCoef[h];
+            const double th = cf.w - pho + phf;
+            double s, c;
+            sincos(th, &s, &c);
+            const double Bvv = cf.y * vo * vf;
+            sP += cf.x * vo * vo - Bvv * s;
+            sQ += cf.z * vo * vo - Bvv * c;
+            dPdv += 2.0 * cf.x * vo - cf.y * vf * s;
+            dPdth += Bvv * c;
+            dQdv += 2.0 * cf.z * vo - cf.y * vf * c;
+            dQdth -= Bvv * s;
+            oPv -= cf.y * vo * s;
+            oPth -= Bvv * c;
+            oQv -= cf.y * vo * c;
+            oQth += Bvv * s;
+        }
+        if (modeP == MODE_FIXED) {
+            oPth = 0;
+            oPv = 0;
+        }
+        if (modeQ == MODE_FIXED) {
+            oQth = 0;
+            oQv = 0;
+        }
+        if (WRITE_J) {
+            Blk<VT> b = {(VT)oPth, (VT)oPv, (VT)oQth, (VT)oQv};
+            stBlk<VT>(lu + bIdx<T>(tile, P.nSlots, P.grpSlot[g], t), b);
+        }
+        if (APPLY_VEC) {
+            const double2 vf2 = *reinterpret_cast<const double2 *>(v + vIdx<T>(tile, nb, f, t));
+            jP += oPth * vf2.x + oPv * vf2.y;
+            jQ += oQth * vf2.x + oQv * vf2.y;
+        }
+    }
+    // bus shunt: p = g v^2, q = -b v^2
+    const double gs = P.shuntG[k], bs = P.shuntB[k];
+    sP += gs * vo * vo;
+    sQ -= bs * vo * vo;
+    dPdv += 2.0 * gs * vo;
+    dQdv -= 2.0 * bs * vo;
+    if (modeP == MODE_FIXED) {
+        dPdth = 1;
+        dPdv = 0;
+    }
+    if (modeQ == MODE_FIXED) {
+        dQdth = 0;
+        dQdv = 1;
+    }
+    if (WRITE_J) {
+        Blk<VT> d = {(VT)dPdth, (VT)dPdv, (VT)dQdth, (VT)dQdv};
+        stBlk<VT>(lu + bIdx<T>(tile, P.nSlots, P.nodeBase[k], t), d);
+        const Blk<VT> z = {(VT)0, (VT)0, (VT)0, (VT)0};
+        for (int q = P.fillPtr[k]; q < P.fillPtr[k + 1]; q++) {
+            const int s0 = P.fillSlot[q];
+            stBlk<VT>(lu + bIdx<T>(tile, P.nSlots, s0, t), z);
+            stBlk<VT>(lu + bIdx<T>(tile, P.nSlots, s0 + 1, t), z);
+        }
+    }
+    if (APPLY_VEC) {
+        const double2 vk = *reinterpret_cast<const double2 *>(v + vIdx<T>(tile, nb, k, t));
+        jvP = jP + dPdth * vk.x + dPdv * vk.y;
+        jvQ = jQ + dQdth * vk.x + dQdv * vk.y;
+    }
+    sumP = sP;
+    sumQ = sQ;
+}
+
+// Tree reduction over the item dimension, per lane: threads tid and tid + stride share a lane because strides are
+// multiples of T. Lane t's result ends in red[t]. Every thread of the block must call it.
+template <int T, bool IS_MAX>
+__device__ __forceinline__ void laneReduce(double *red, int tid, int nt) {
+    for (int stride = nt >> 1; stride >= T; stride >>= 1) {
+        __syncthreads();
+        if (tid < stride) red[tid] = IS_MAX ? fmax(red[tid], red[tid + stride]) : red[tid] + red[tid + stride];
+    }
+    __syncthreads();
+}
+template <int T>
+__device__ __forceinline__ void laneReduceAnd(int *red, int tid, int nt) {
+    for (int stride = nt >> 1; stride >= T; stride >>= 1) {
+        __syncthreads();
+        if (tid < stride) red[tid] = red[tid] & red[tid + stride];
+    }
+    __syncthreads();
+}
+
+// The factorization, left-looking by elimination levels, with the forward solve of b = sign * rhs fused into the
+// diagonal items. Ends with a barrier.
+template <int T, typename VT>
+__device__ void factorForward(const DevPlan &P, int tile, int tid, int nt, VT *__restrict__ lu, VT *__restrict__ y,
+                              const double *__restrict__ rhs, double sign) {
+    const int nb = P.nb;
+    for (int l = 0; l < P.nLevels; l++) {
+        const int n0 = P.levelNodePtr[l], n1 = P.levelNodePtr[l + 1];
+        const int p0 = P.levelPairPtr[l], p1 = P.levelPairPtr[l + 1];
+        const int nNodes = n1 - n0;
+        const int units = nNodes + (p1 - p0);
+        for (int w = tid; w < units * T; w += nt) {
+            const int u = w / T, t = w % T;
+            if (u < nNodes) {
+                const int k = P.levelNodes[n0 + u];
+                VT *dslot = lu + bIdx<T>(tile, P.nSlots, P.nodeBase[k], t);
+                Blk<VT> D = ldBlk<VT>(dslot);
+                const double2 b = *reinterpret_cast<const double2 *>(rhs + vIdx<T>(tile, nb, k, t));
+                V2<VT> yk = {(VT)(sign * b.x), (VT)(sign * b.y)};
+                const int c1 = P.diagContribPtr[k + 1];
+                for (int c = P.diagContribPtr[k]; c < c1; c++) {
+                    const int a = P.diagContribA[c];
+                    const int j = P.diagContribB[c];
+                    const Blk<VT> Lkj = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, a, t));
+                    const Blk<VT> Ujk = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, a + 1, t));
+                    const V2<VT> yj = ldV<VT>(y + vIdx<T>(tile, nb, j, t));
+                    mulSub(D, Lkj, Ujk);
+                    mulSubV(yk, Lkj, yj);
+                }
+                stBlk<VT>(dslot, D);
+                stV<VT>(y + vIdx<T>(tile, nb, k, t), yk);
+            } else {
+                const int q = p0 + (u - nNodes);
+                const int slot = P.pairSlot[q];
+                VT *lslot = lu + bIdx<T>(tile, P.nSlots, slot, t);
+                Blk<VT> L = ldBlk<VT>(lslot);
+                Blk<VT> U = ldBlk<VT>(lslot + 4 * T);
+                const int c1 = P.pairContribPtr[q + 1];
+                for (int c = P.pairContribPtr[q]; c < c1; c++) {
+                    const int a = P.pairContribA[c];
+                    const int b = P.pairContribB[c];
+                    const Blk<VT> Lkj = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, a, t));
+                    const Blk<VT> Ujk = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, a + 1, t));
+                    const Blk<VT> Lij = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, b, t));
+                    const Blk<VT> Uji = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, b + 1, t));
+                    mulSub(L, Lij, Ujk);
+                    mulSub(U, Lkj, Uji);
+                }
+                stBlk<VT>(lslot, L);
+                stBlk<VT>(lslot + 4 * T, U);
+            }
+        }
+        __syncthreads();
+        for (int w = tid; w < (p1 - p0) * T; w += nt) {
+            const int q = p0 + w / T, t = w % T;
+            const int k = P.pairNode[q];
+            bool cl;
+            const Blk<VT> Dinv = inv2(ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, P.nodeBase[k], t)), cl);
+            VT *lslot = lu + bIdx<T>(tile, P.nSlots, P.pairSlot[q], t);
+            stBlk<VT>(lslot, mul(ldBlk<VT>(lslot), Dinv));
+        }
+        __syncthreads();
+    }
+}
+
+// Forward solve with existing factors: y = L^-1 (sign * rhs). Ends with a barrier.
+template <int T, typename VT>
+__device__ void forwardOnly(const DevPlan &P, int tile, int tid, int nt, const VT *__restrict__ lu,
+                            VT *__restrict__ y, const double *__restrict__ rhs, double sign) {
+    const int nb = P.nb;
+    for (int l = 0; l < P.nLevels; l++) {
+        const int n0 = P.levelNodePtr[l], n1 = P.levelNodePtr[l + 1];
+        for (int w = tid; w < (n1 - n0) * T; w += nt) {
+            const int k = P.levelNodes[n0 + w / T], t = w % T;
+            const double2 b = *reinterpret_cast<const double2 *>(rhs + vIdx<T>(tile, nb, k, t));
+            V2<VT> yk = {(VT)(sign * b.x), (VT)(sign * b.y)};
+            const int c1 = P.diagContribPtr[k + 1];
+            for (int c = P.diagContribPtr[k]; c < c1; c++) {
+                const Blk<VT> Lkj = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, P.diagContribA[c], t));
+                mulSubV(yk, Lkj, ldV<VT>(y + vIdx<T>(tile, nb, P.diagContribB[c], t)));
+            }
+            stV<VT>(y + vIdx<T>(tile, nb, k, t), yk);
+        }
+        __syncthreads();
+    }
+}
+
+// Backward solve: x(k) = D(k)^-1 (y(k) - sum U(k,i) x(i)), levels from the root down. Counts clamped pivots per lane
+// into clampLane (shared). Ends with a barrier.
+template <int T, typename VT>
+__device__ void backward(const DevPlan &P, int tile, int tid, int nt, const VT *__restrict__ lu,
+                         const VT *__restrict__ y, VT *__restrict__ xo, int *clampLane) {
+    const int nb = P.nb;
+    for (int l = P.nLevels - 1; l >= 0; l--) {
+        const int n0 = P.levelNodePtr[l], n1 = P.levelNodePtr[l + 1];
+        for (int w = tid; w < (n1 - n0) * T; w += nt) {
+            const int k = P.levelNodes[n0 + w / T], t = w % T;
+            V2<VT> yk = ldV<VT>(y + vIdx<T>(tile, nb, k, t));
+            const int base = P.nodeBase[k];
+            const int e0 = P.sPtr[k], e1 = P.sPtr[k + 1];
+            for (int e = e0; e < e1; e++) {
+                const int i = P.sIdx[e];
+                const Blk<VT> U = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, base + 2 + 2 * (e - e0), t));
+                mulSubV(yk, U, ldV<VT>(xo + vIdx<T>(tile, nb, i, t)));
+            }
+            bool cl;
+            const Blk<VT> Dinv = inv2(ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, base, t)), cl);
+            if (cl && clampLane) atomicAdd(&clampLane[t], 1);
+            V2<VT> r = {Dinv.a * yk.p + Dinv.b * yk.q, Dinv.c * yk.p + Dinv.d * yk.q};
+            stV<VT>(xo + vIdx<T>(tile, nb, k, t), r);
+        }
+        __syncthreads();
+    }
+}
+
+// Mismatch and Jacobian fill of the lanes flagged in fillLane; per-thread partial max|F| and finiteness of the
+// thread's lane (tid % T) are returned.
+template <int T, typename VT>
+__device__ void fillPass(const DevPlan &P, const DevState &S, int tile, int tid, int nt, const int *fillLane,
+                         double &pmax, int &pfin) {
+    const int nb = P.nb;
+    VT *lu = reinterpret_cast<VT *>(S.lu);
+    for (int w = tid; w < nb * T; w += nt) {
+        const int k = w / T, t = w % T;
+        if (!fillLane[t]) continue;
+        const int lane = tile * T + t;
+        int o[MAX_OUTAGES] = {-1, -1, -1, -1};
+        for (int q = 0; q < S.maxOut && q < MAX_OUTAGES; q++) o[q] = S.outages[(size_t)lane * S.maxOut + q];
+        const size_t vi = vIdx<T>(tile, nb, k, t);
+        const signed char mP = S.mode[vi], mQ = S.mode[vi + 1];
+        double sP, sQ, jP, jQ;
+        nodeTerms<T, VT, true, false>(P, tile, k, t, S.x, o, lu, nullptr, mP, mQ, sP, sQ, jP, jQ);
+        const double2 tg = *reinterpret_cast<const double2 *>(S.target + vi);
+        const double2 xk = *reinterpret_cast<const double2 *>(S.x + vi);
+        const double fP = (mP == MODE_FIXED) ? xk.x - tg.x : sP - tg.x;
+        const double fQ = (mQ == MODE_FIXED) ? xk.y - tg.y : sQ - tg.y;
+        *reinterpret_cast<double2 *>(S.F + vi) = make_double2(fP, fQ);
+        S.qinj[qIdx<T>(tile, nb, k, t)] = sQ;
+        if (!isfinite(fP) || !isfinite(fQ)) pfin = 0;
+        pmax = fmax(pmax, fmax(fabs(fP), fabs(fQ)));
+    }
+}
+
+// out = J * v + add (lanes flagged), per-thread max |out| of the thread's lane.
+template <int T, typename VT>
+__device__ void applyJPass(const DevPlan &P, const DevState &S, int tile, int tid, int nt, const int *laneOn,
+                           const double *v, const double *add, double addSign, double *out, double &pmax) {
+    const int nb = P.nb;
+    for (int w = tid; w < nb * T; w += nt) {
+        const int k = w / T, t = w % T;
+        if (!laneOn[t]) continue;
+        const int lane = tile * T + t;
+        int o[MAX_OUTAGES] = {-1, -1, -1, -1};
+        for (int q = 0; q < S.maxOut && q < MAX_OUTAGES; q++) o[q] = S.outages[(size_t)lane * S.maxOut + q];
+        const size_t vi = vIdx<T>(tile, nb, k, t);
+        double sP, sQ, jP, jQ;
+        nodeTerms<T, VT, false, true>(P, tile, k, t, S.x, o, nullptr, v, S.mode[vi], S.mode[vi + 1], sP, sQ, jP, jQ);
+        const double2 a = *reinterpret_cast<const double2 *>(add + vi);
+        const double rP = jP + addSign * a.x, rQ = jQ + addSign * a.y;
+        if (out) *reinterpret_cast<double2 *>(out + vi) = make_double2(rP, rQ);
+        pmax = fmax(pmax, fmax(fabs(rP), fabs(rQ)));
+    }
+}
+
+// One batched Newton step for the tile of blockIdx.x. Reference loop body: native/cuda/solver/newton_iteration.cu:116-203
+// (test half :447-495, testScenario :276-397, solveUniformBatch :580-660), reactive limits
+// native/cuda/outerloop/reactive_limits.cu:321-371 plus the forced-PV election, step scaling
+// native/cuda/solver/state_vector_scaling.cu:57-97.
+template <int T, typename VT>
+__global__ __launch_bounds__(MAX_THREADS) void nrStepKernel(DevPlan P, DevState S, DevRL R, NrParams prm) {
+    const int tile = blockIdx.x;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int myLane = tid % T;
+    if (S.tileRunning[tile] == 0) return;   // uniform per block
+
+    __shared__ int sStatus[T], sIt[T], sSolves[T], sOuter[T], sTotal[T], sClamp[T];
+    __shared__ int sFill[T], sSolve[T], sConv[T], sRl[T], sRemain[T], sSwitched[T], sFin[T];
+    __shared__ double sNorm[T], sRatio[T], sFactor[T];
+    __shared__ int sFlag[4];
+    __shared__ double red[MAX_THREADS];
+    __shared__ int redI[MAX_THREADS];
+
+    const int nb = P.nb;
+    VT *lu = reinterpret_cast<VT *>(S.lu);
+    VT *y = reinterpret_cast<VT *>(S.y);
+    VT *dx1 = reinterpret_cast<VT *>(S.dx1);
+
+    if (tid < T) {
+        const int lane = tile * T + tid;
+        sStatus[tid] = S.status[lane];
+        sIt[tid] = S.it[lane];
+        sSolves[tid] = S.solves[lane];
+        sOuter[tid] = S.outer[lane];
+        sTotal[tid] = S.total[lane];
+        sRatio[tid] = S.ratio[lane];
+        sClamp[tid] = 0;
+        sFill[tid] = sStatus[tid] == LANE_RUNNING;
+        sSolve[tid] = 0;
+        sConv[tid] = 0;
+        sRl[tid] = 0;
+        sRemain[tid] = 0;
+        sSwitched[tid] = 0;
+        sFactor[tid] = 1.0;
+    }
+    __syncthreads();
+
+    // ---- 1. mismatch + Jacobian of the running lanes
+    {
+        double pmax = 0.0;
+        int pfin = 1;
+        fillPass<T, VT>(P, S, tile, tid, nt, sFill, pmax, pfin);
+        red[tid] = pmax;
+        redI[tid] = pfin;
+        laneReduce<T, true>(red, tid, nt);
+        laneReduceAnd<T>(redI, tid, nt);
+    }
+    // ---- 2. the test of each running lane at its own iteration
+    if (tid < T && sFill[tid]) {
+        const double norm = red[tid];
+        const bool fin = redI[tid] != 0;
+        const int it = sIt[tid];
+        sNorm[tid] = norm;
+        if (!fin) {
+            sStatus[tid] = ST_DIVERGED;
+        } else if (it > 0 && norm < prm.tol && it < prm.maxIter) {
+            sConv[tid] = 1;
+        } else if (it > 0 && it >= prm.maxIter) {
+            const bool bad = !(sRatio[tid] <= LIN_SOLVE_FAIL_RATIO);
+            sStatus[tid] = bad ? (prm.batchOfOne ? ST_LIN_FAILED : ST_REJECTED) : ST_MAX_ITER;
+        } else {
+            sSolve[tid] = 1;
+        }
+    }
+    if (tid == 0) {
+        int any = 0;
+        for (int t = 0; t < T; t++) any |= sConv[t];
+        sFlag[0] = any;
+    }
+    __syncthreads();
+
+    // ---- 3. outer loop (reactive limits) of the lanes whose solve converged
+    if (sFlag[0]) {
+        if (tid < T) {
+            sRl[tid] = sConv[tid] && prm.reactiveLimits && R.n > 0 && sOuter[tid] < prm.maxOuter;
+            sFill[tid] = 0;
+        }
+        __syncthreads();
+        for (int w = tid; w < R.n * T; w += nt) {
+            const int e = w / T, t = w % T;
+            if (!sRl[t]) continue;
+            const size_t ei = ((size_t)tile * R.n + e) * T + t;
+            signed char lt = S.rlType[ei];
+            signed char pass = 0;
+            if (lt != QLIM_FROZEN) {
+                const int k = R.node[e];
+                const size_t vi = vIdx<T>(tile, nb, k, t) + 1;
+                if (S.mode[vi] == MODE_FIXED && lt == QLIM_NONE) {
+                    const double q = S.qinj[qIdx<T>(tile, nb, k, t)] + R.loadQ[e];
+                    if (q < R.minQ[e] - prm.rlTol) {
+                        S.mode[vi] = MODE_EQUATION;
+                        S.target[vi] = R.minQ[e] - R.loadQ[e];
+                        S.rlType[ei] = QLIM_MIN;
+                        S.rlCount[ei]++;
+                        pass = 1;
+                    } else if (q > R.maxQ[e] + prm.rlTol) {
+                        S.mode[vi] = MODE_EQUATION;
+                        S.target[vi] = R.maxQ[e] - R.loadQ[e];
+                        S.rlType[ei] = QLIM_MAX;
+                        S.rlCount[ei]++;
+                        pass = 1;
+                    } else {
+                        atomicAdd(&sRemain[t], 1);
+                    }
+                } else if (lt != QLIM_NONE) {
+                    const double v = S.x[vi], tv = R.targetV[e];
+                    const bool wants = (lt == QLIM_MAX && v > tv) || (lt == QLIM_MIN && v < tv);
+                    if (wants && S.rlCount[ei] < prm.maxSwitch) {
+                        S.mode[vi] = MODE_FIXED;
+                        S.target[vi] = tv;
+                        S.rlType[ei] = QLIM_NONE;
+                        pass = 2;
+                    }
+                }
+            }
+            S.rlPass[ei] = pass;
+            if (pass) atomicOr(&sSwitched[t], 1);
+        }
+        __syncthreads();
+        if (tid < T && sRl[tid] && sRemain[tid] == 0) {
+            // forced PV: no bus that was PV at this check stays PV -> the strongest bus switched PV -> PQ in this
+            // check stays PV (equal nominal voltages: highest P target, then lowest entry)
+            int best = -1;
+            double bp = -1e300;
+            for (int e = 0; e < R.n; e++) {
+                const size_t ei = ((size_t)tile * R.n + e) * T + tid;
+                if (S.rlPass[ei] != 1) continue;
+                const double p = S.target[vIdx<T>(tile, nb, R.node[e], tid)];
+                if (p > bp) {
+                    bp = p;
+                    best = e;
+                }
+            }
+            if (best >= 0) {
+                const size_t ei = ((size_t)tile * R.n + best) * T + tid;
+                const size_t vi = vIdx<T>(tile, nb, R.node[best], tid) + 1;
+                S.mode[vi] = MODE_FIXED;
+                S.target[vi] = R.targetV[best];
+                S.rlType[ei] = QLIM_NONE;
+                S.rlCount[ei]--;
+                S.rlPass[ei] = 0;
+                int any = 0;
+                for (int e = 0; e < R.n; e++) any |= S.rlPass[((size_t)tile * R.n + e) * T + tid] != 0;
+                sSwitched[tid] = any;
+            }
+        }
+        __syncthreads();
+        if (tid < T && sConv[tid]) {
+            sSolves[tid]++;
+            if (sRl[tid] && sSwitched[tid]) {
+                sOuter[tid]++;
+                sIt[tid] = 0;
+                sFill[tid] = 1;     // restarted at its new modes and targets: refill, no test at iteration 0
+            } else {
+                sStatus[tid] = ST_CONVERGED;
+            }
+        }
+        if (tid == 0) {
+            int any = 0;
+            for (int t = 0; t < T; t++) any |= sFill[t];
+            sFlag[1] = any;
+        }
+        __syncthreads();
+        if (sFlag[1]) {
+            double pmax = 0.0;
+            int pfin = 1;
+            fillPass<T, VT>(P, S, tile, tid, nt, sFill, pmax, pfin);
+            red[tid] = pmax;
+            redI[tid] = pfin;
+            laneReduce<T, true>(red, tid, nt);
+            laneReduceAnd<T>(redI, tid, nt);
+            if (tid < T && sFill[tid]) {
+                sNorm[tid] = red[tid];
+                if (redI[tid] == 0) sStatus[tid] = ST_DIVERGED;
+                else sSolve[tid] = 1;
+            }
+        }
+    }
+    if (tid == 0) {
+        int any = 0;
+        for (int t = 0; t < T; t++) any |= sSolve[t];
+        sFlag[2] = any;
+    }
+    __syncthreads();
+
+    if (sFlag[2]) {
+        // ---- 4. factorization with the forward solve of -F, then the backward solve
+        factorForward<T, VT>(P, tile, tid, nt, lu, y, S.F, -1.0);
+        backward<T, VT>(P, tile, tid, nt, lu, y, dx1, sClamp);
+        // ---- 5. step in FP64 (+ one FP64-residual refinement step on the FP32 factors)
+        for (int w = tid; w < nb * T; w += nt) {
+            const int k = w / T, t = w % T;
+            const V2<VT> d = ldV<VT>(dx1 + vIdx<T>(tile, nb, k, t));
+            *reinterpret_cast<double2 *>(S.dx + vIdx<T>(tile, nb, k, t)) = make_double2((double)d.p, (double)d.q);
+        }
+        __syncthreads();
+        if (prm.refine) {
+            double pm = 0.0;
+            // r = -F - J dx
+            applyJPass<T, VT>(P, S, tile, tid, nt, sSolve, S.dx, S.F, 1.0, S.r, pm);
+            __syncthreads();
+            for (int w = tid; w < nb * T; w += nt) {   // negate: r = -(J dx + F)
+                const int k = w / T, t = w % T;
+                double2 *rp = reinterpret_cast<double2 *>(S.r + vIdx<T>(tile, nb, k, t));
+                const double2 rv = *rp;
+                *rp = make_double2(-rv.x, -rv.y);
+            }
+            __syncthreads();
+            forwardOnly<T, VT>(P, tile, tid, nt, lu, y, S.r, 1.0);
+            backward<T, VT>(P, tile, tid, nt, lu, y, dx1, nullptr);
+            for (int w = tid; w < nb * T; w += nt) {
+                const int k = w / T, t = w % T;
+                const V2<VT> d = ldV<VT>(dx1 + vIdx<T>(tile, nb, k, t));
+                double2 *dp = reinterpret_cast<double2 *>(S.dx + vIdx<T>(tile, nb, k, t));
+                const double2 dv = *dp;
+                *dp = make_double2(dv.x + (double)d.p, dv.y + (double)d.q);
+            }
+            __syncthreads();
+        }
+        // ---- 6. linear-solve guard at the 2nd iteration of the first solve, and the residual of the solve before
+        //         the budget (reference newton_iteration.cu:596-652, :369-389)
+        if (tid == 0) {
+            int due = 0;
+            for (int t = 0; t < T; t++)
+                due |= sSolve[t] && ((sSolves[t] == 0 && sIt[t] == 1) || sIt[t] == prm.maxIter - 1);
+            sFlag[3] = due;
+        }
+        __syncthreads();
+        if (sFlag[3]) {
+            double pm = 0.0;
+            applyJPass<T, VT>(P, S, tile, tid, nt, sSolve, S.dx, S.F, 1.0, nullptr, pm);
+            red[tid] = pm;
+            laneReduce<T, true>(red, tid, nt);
+            if (tid < T && sSolve[tid]) {
+                const double ratio = sNorm[tid] > 0.0 ? red[tid] / sNorm[tid] : 0.0;
+                if (sSolves[tid] == 0 && sIt[tid] == 1 && !(ratio <= LIN_SOLVE_FAIL_RATIO)) {
+                    sStatus[tid] = ST_LIN_FAILED;
+                    sSolve[tid] = 0;
+                }
+                sRatio[tid] = ratio;
+            }
+            __syncthreads();
+        }
+        // ---- 7. step scaling (MAX_VOLTAGE_CHANGE) and update
+        if (prm.scaling == 2) {
+            double mdv = 0.0, mdp = 0.0;
+            for (int w = tid; w < nb * T; w += nt) {
+                const int k = w / T, t = w % T;
+                const double2 d = *reinterpret_cast<const double2 *>(S.dx + vIdx<T>(tile, nb, k, t));
+                mdp = fmax(mdp, fabs(d.x));
+                mdv = fmax(mdv, fabs(d.y));
+            }
+            red[tid] = mdv;
+            laneReduce<T, true>(red, tid, nt);
+            if (tid < T) sFactor[tid] = red[tid] > prm.maxDv ? prm.maxDv / red[tid] : 1.0;
+            __syncthreads();
+            red[tid] = mdp;
+            laneReduce<T, true>(red, tid, nt);
+            if (tid < T && red[tid] > prm.maxDphi) sFactor[tid] = fmin(sFactor[tid], prm.maxDphi / red[tid]);
+            __syncthreads();
+        }
+        for (int w = tid; w < nb * T; w += nt) {
+            const int k = w / T, t = w % T;
+            if (!sSolve[t]) continue;
+            const double f = sFactor[t];
+            const size_t vi = vIdx<T>(tile, nb, k, t);
+            const double2 d = *reinterpret_cast<const double2 *>(S.dx + vi);
+            double2 *xp = reinterpret_cast<double2 *>(S.x + vi);
+            const double2 xv = *xp;
+            *xp = make_double2(xv.x + f * d.x, xv.y + f * d.y);
+        }
+        if (tid < T && sSolve[tid]) {
+            sIt[tid]++;
+            sTotal[tid]++;
+        }
+    }
+    __syncthreads();
+    (void)myLane;
+    if (tid < T) {
+        const int lane = tile * T + tid;
+        S.status[lane] = sStatus[tid];
+        S.it[lane] = sIt[tid];
+        S.solves[lane] = sSolves[tid];
+        S.outer[lane] = sOuter[tid];
+        S.total[lane] = sTotal[tid];
+        S.ratio[lane] = sRatio[tid];
+        if (sFill[tid] || sConv[tid]) S.norm[lane] = sNorm[tid];
+        S.clamps[lane] += sClamp[tid];
+    }
+    if (tid == 0) {
+        int any = 0;
+        for (int t = 0; t < T; t++) any |= sStatus[t] == LANE_RUNNING;
+        S.tileRunning[tile] = any;
+    }
+}
+
+// ---- install: every slot starts from the base (or its own x0), base targets and modes.
+template <int T>
+__global__ void installKernel(DevPlan P, DevState S, const double *x0Base, const double *x0Per, const double *targetBase,
+                              const int *modeBase) {
+    const int nb = P.nb;
+    const size_t total = (size_t)S.nTiles * nb * T;
+    for (size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x; w < total; w += (size_t)gridDim.x * blockDim.x) {
+        const int t = (int)(w % T);
+        const size_t kt = w / T;
+        const int k = (int)(kt % nb);
+        const int tile = (int)(kt / nb);
+        const int lane = tile * T + t;
+        const int b = P.perm[k];
+        const size_t vi = vIdx<T>(tile, nb, k, t);
+        const size_t n = 2 * (size_t)nb;
+        const bool installed = lane < S.nInstalled;
+        double2 xv;
+        if (installed && x0Per) xv = make_double2(x0Per[lane * n + 2 * b], x0Per[lane * n + 2 * b + 1]);
+        else xv = make_double2(x0Base[2 * b], x0Base[2 * b + 1]);
+        *reinterpret_cast<double2 *>(S.x + vi) = xv;
+        *reinterpret_cast<double2 *>(S.target + vi) = make_double2(targetBase[2 * b], targetBase[2 * b + 1]);
+        S.mode[vi] = (signed char)modeBase[2 * b];
+        S.mode[vi + 1] = (signed char)modeBase[2 * b + 1];
+    }
+}
+
+// per-scenario (scenario, row, mode) entries and target overrides (scenario * n + row -> value)
+template <int T>
+__global__ void overrideKernel(DevPlan P, DevState S, int nModes, const int *modeEntries, int nTargets,
+                               const long long *targetIndex, const double *targetValue) {
+    const int nb = P.nb;
+    const size_t n = 2 * (size_t)nb;
+    for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < nModes + nTargets; w += gridDim.x * blockDim.x) {
+        if (w < nModes) {
+            const int lane = modeEntries[3 * w], row = modeEntries[3 * w + 1];
+            const int tile = lane / T, t = lane % T;
+            S.mode[vIdx<T>(tile, nb, P.pos[row >> 1], t) + (row & 1)] = (signed char)modeEntries[3 * w + 2];
+        } else {
+            const long long g = targetIndex[w - nModes];
+            const int lane = (int)(g / (long long)n), row = (int)(g % (long long)n);
+            const int tile = lane / T, t = lane % T;
+            S.target[vIdx<T>(tile, nb, P.pos[row >> 1], t) + (row & 1)] = targetValue[w - nModes];
+        }
+    }
+}
+
+// lane counters and reactive-limit entry state; an entry whose bus's P row a scenario froze (an islanded bus) leaves
+// that scenario's outer loop (reference reactive_limits.cu:62-69)
+template <int T>
+__global__ void laneInitKernel(DevPlan P, DevState S, DevRL R, const signed char *seed, const int *modeBase) {
+    const int nb = P.nb;
+    const int lanes = S.nTiles * T;
+    const size_t totalE = (size_t)lanes * R.n;
+    for (size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x; w < (size_t)lanes + totalE;
+         w += (size_t)gridDim.x * blockDim.x) {
+        if (w < (size_t)lanes) {
+            const int lane = (int)w;
+            S.status[lane] = lane < S.nInstalled ? LANE_RUNNING : LANE_PADDING;
+            S.it[lane] = 0;
+            S.solves[lane] = 0;
+            S.outer[lane] = 0;
+            S.total[lane] = 0;
+            S.clamps[lane] = 0;
+            S.norm[lane] = 0.0;
+            S.ratio[lane] = 0.0;
+            if (lane % T == 0) S.tileRunning[lane / T] = lane < S.nInstalled;
+        } else {
+            const size_t q = w - lanes;
+            const int t = (int)(q % T);
+            const size_t et = q / T;
+            const int e = (int)(et % R.n);
+            const int tile = (int)(et / R.n);
+            const int lane = tile * T + t;
+            const size_t ei = ((size_t)tile * R.n + e) * T + t;
+            const int k = R.node[e];
+            const int pRow = 2 * P.perm[k];
+            signed char lt = (seed && lane < S.nInstalled) ? seed[(size_t)lane * R.n + e] : QLIM_NONE;
+            const signed char pm = S.mode[vIdx<T>(tile, nb, k, t)];
+            if (pm == MODE_FIXED && modeBase[pRow] != MODE_FIXED) lt = QLIM_FROZEN;
+            S.rlType[ei] = lt;
+            S.rlCount[ei] = 0;
+            S.rlPass[ei] = 0;
+        }
+    }
+}
+
+// states back in stable-row order, [lane][n]
+template <int T>
+__global__ void gatherStatesKernel(DevPlan P, DevState S, int nLanes, double *out) {
+    const int nb = P.nb;
+    const size_t n = 2 * (size_t)nb;
+    for (size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x; w < (size_t)nLanes * nb;
+         w += (size_t)gridDim.x * blockDim.x) {
+        const int lane = (int)(w / nb), b = (int)(w % nb);
+        const int tile = lane / T, t = lane % T;
+        const double2 v = *reinterpret_cast<const double2 *>(S.x + vIdx<T>(tile, nb, P.pos[b], t));
+        out[lane * n + 2 * b] = v.x;
+        out[lane * n + 2 * b + 1] = v.y;
+    }
+}
+
+// running lanes after a step: sum over tiles into *count (one block)
+__global__ void countRunningKernel(const int *tileRunning, int nTiles, int *count) {
+    __shared__ int acc[256];
+    int s = 0;
+    for (int i = threadIdx.x; i < nTiles; i += blockDim.x) s += tileRunning[i] != 0;
+    acc[threadIdx.x] = s;
+    __syncthreads();
+    for (int stride = blockDim.x / 2; stride > 0; stride >>= 1) {
+        if ((int)threadIdx.x < stride) acc[threadIdx.x] += acc[threadIdx.x + stride];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *count = acc[0];
+}
+
+// standalone batched LU: factor + forward + backward per tile, rhs/solution in the node-ordered vector layout
+template <int T, typename VT>
+__global__ __launch_bounds__(MAX_THREADS) void luSolveKernel(DevPlan P, VT *lu, VT *y, VT *xo, const double *rhs,
+                                                             int *clamps) {
+    __shared__ int sClamp[T];
+    const int tile = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+    if (tid < T) sClamp[tid] = 0;
+    __syncthreads();
+    factorForward<T, VT>(P, tile, tid, nt, lu, y, rhs, 1.0);
+    backward<T, VT>(P, tile, tid, nt, lu, y, xo, sClamp);
+    if (tid < T) clamps[tile * T + tid] = sClamp[tid];
+}
+
+}  // namespace olfhip

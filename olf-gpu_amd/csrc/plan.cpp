@@ -1,0 +1,309 @@
+// Host-side structure and symbolic plan of the batched block LU. See plan.hpp.
+#include "plan.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <queue>
+#include <stdexcept>
+#include <string>
+#include <utility>
+
+#include "errors.hpp"
+
+namespace olfhip {
+
+void minimumDegree(int nb, const std::vector<int> &adjPtr, const std::vector<int> &adj, std::vector<int> &perm,
+                   std::vector<std::vector<int>> &structureAtElimination) {
+    std::vector<std::vector<int>> g(nb);
+    for (int i = 0; i < nb; i++) {
+        g[i].assign(adj.begin() + adjPtr[i], adj.begin() + adjPtr[i + 1]);
+    }
+    using Entry = std::pair<int, int>;  // (degree, bus): lowest degree, then lowest bus
+    std::priority_queue<Entry, std::vector<Entry>, std::greater<Entry>> heap;
+    for (int i = 0; i < nb; i++) heap.push({(int)g[i].size(), i});
+    std::vector<char> dead(nb, 0);
+    perm.assign(nb, -1);
+    structureAtElimination.assign(nb, {});
+    std::vector<int> merged;
+    for (int step = 0; step < nb; step++) {
+        int v = -1;
+        while (!heap.empty()) {
+            Entry top = heap.top();
+            heap.pop();
+            if (!dead[top.second] && top.first == (int)g[top.second].size()) {
+                v = top.second;
+                break;
+            }
+        }
+        if (v < 0) throw std::logic_error("minimum degree: heap exhausted");
+        perm[step] = v;
+        dead[v] = 1;
+        const std::vector<int> nbr = g[v];
+        structureAtElimination[v] = nbr;
+        for (int u : nbr) {
+            merged.clear();
+            std::set_union(g[u].begin(), g[u].end(), nbr.begin(), nbr.end(), std::back_inserter(merged));
+            g[u].clear();
+            for (int c : merged)
+                if (c != u && c != v) g[u].push_back(c);
+            heap.push({(int)g[u].size(), u});
+        }
+        std::vector<int>().swap(g[v]);
+    }
+}
+
+static int indexIn(const std::vector<int> &idx, int lo, int hi, int value) {
+    auto it = std::lower_bound(idx.begin() + lo, idx.begin() + hi, value);
+    if (it == idx.begin() + hi || *it != value) return -1;
+    return (int)(it - (idx.begin() + lo));
+}
+
+Plan buildPlan(int n, int nBranch, const double *p8, const int *idx12, int nShunt, const double *shuntP,
+               const int *shuntIdx) {
+    if (n <= 0 || (n & 1)) throw ArgumentError("n must be a positive even row count (2 rows per bus)");
+    if (nBranch < 0 || nShunt < 0) throw ArgumentError("negative element count");
+    if (nBranch > 0 && (!p8 || !idx12)) throw ArgumentError("closed-branch arrays are null");
+    if (nShunt > 0 && (!shuntP || !shuntIdx)) throw ArgumentError("shunt arrays are null");
+    Plan P;
+    P.n = n;
+    P.nb = n / 2;
+    P.nBranch = nBranch;
+    P.nShunt = nShunt;
+    const int nb = P.nb;
+
+    // --- validate the stable-row packing and read the endpoints
+    std::vector<int> bus1(nBranch), bus2(nBranch);
+    for (int e = 0; e < nBranch; e++) {
+        const int *ix = idx12 + 12 * e;
+        int a = ix[0] / 2, c = ix[2] / 2;
+        bool ok = ix[0] >= 0 && ix[2] >= 0 && ix[0] % 2 == 0 && ix[2] % 2 == 0 && a < nb && c < nb && a != c &&
+                  ix[1] == ix[0] + 1 && ix[3] == ix[2] + 1 && ix[4] == ix[0] && ix[5] == ix[1] && ix[6] == ix[2] &&
+                  ix[7] == ix[3];
+        if (!ok)
+            throw UnsupportedError("closed branch " + std::to_string(e) +
+                                   ": rows/columns are not stable bus rows (P on phi row 2i, Q/V on row 2i+1)");
+        if (ix[8] != -1 || ix[9] != -1 || ix[10] != -1 || ix[11] != -1)
+            throw UnsupportedError("closed branch " + std::to_string(e) +
+                                   ": phase/ratio control columns and relocated reactive rows are not modelled");
+        for (int k = 0; k < 8; k++)
+            if (!std::isfinite(p8[8 * e + k])) throw ArgumentError("closed branch " + std::to_string(e) + ": non-finite parameter");
+        bus1[e] = a;
+        bus2[e] = c;
+    }
+    P.shuntG.assign(nb, 0.0);
+    P.shuntB.assign(nb, 0.0);
+    std::vector<double> gBus(nb, 0.0), bBus(nb, 0.0);
+    for (int s = 0; s < nShunt; s++) {
+        const int *ix = shuntIdx + 3 * s;
+        int b = ix[0] / 2;
+        if (ix[0] < 0 || ix[0] % 2 || b >= nb || ix[1] != ix[0] + 1 || ix[2] != ix[1])
+            throw UnsupportedError("shunt " + std::to_string(s) + ": not on its bus's stable rows");
+        gBus[b] += shuntP[2 * s];
+        bBus[b] += shuntP[2 * s + 1];
+    }
+
+    // --- bus adjacency (unique, sorted)
+    std::vector<int> deg(nb + 1, 0);
+    for (int e = 0; e < nBranch; e++) {
+        deg[bus1[e]]++;
+        deg[bus2[e]]++;
+    }
+    std::vector<std::vector<int>> nbrs(nb);
+    for (int i = 0; i < nb; i++) nbrs[i].reserve(deg[i]);
+    for (int e = 0; e < nBranch; e++) {
+        nbrs[bus1[e]].push_back(bus2[e]);
+        nbrs[bus2[e]].push_back(bus1[e]);
+    }
+    std::vector<int> adjPtr(nb + 1, 0), adj;
+    for (int i = 0; i < nb; i++) {
+        std::sort(nbrs[i].begin(), nbrs[i].end());
+        nbrs[i].erase(std::unique(nbrs[i].begin(), nbrs[i].end()), nbrs[i].end());
+        adj.insert(adj.end(), nbrs[i].begin(), nbrs[i].end());
+        adjPtr[i + 1] = (int)adj.size();
+    }
+
+    // --- fill-reducing order and the symbolic structure
+    std::vector<std::vector<int>> sAt;
+    minimumDegree(nb, adjPtr, adj, P.perm, sAt);
+    P.pos.assign(nb, 0);
+    for (int k = 0; k < nb; k++) P.pos[P.perm[k]] = k;
+    P.sPtr.assign(nb + 1, 0);
+    for (int k = 0; k < nb; k++) {
+        std::vector<int> s;
+        for (int b : sAt[P.perm[k]]) s.push_back(P.pos[b]);
+        std::sort(s.begin(), s.end());
+        for (int i : s)
+            if (i <= k) throw std::logic_error("symbolic structure: non-ascending member");
+        P.sIdx.insert(P.sIdx.end(), s.begin(), s.end());
+        P.sPtr[k + 1] = (int)P.sIdx.size();
+    }
+    P.nodeBase.assign(nb + 1, 0);
+    for (int k = 0; k < nb; k++) P.nodeBase[k + 1] = P.nodeBase[k] + 1 + 2 * (P.sPtr[k + 1] - P.sPtr[k]);
+    P.nSlots = P.nodeBase[nb];
+    auto lslot = [&](int j, int e) { return P.nodeBase[j] + 1 + 2 * e; };
+
+    // --- elimination tree levels
+    std::vector<int> level(nb, 0);
+    for (int k = 0; k < nb; k++) {
+        if (P.sPtr[k + 1] > P.sPtr[k]) {
+            int parent = P.sIdx[P.sPtr[k]];
+            level[parent] = std::max(level[parent], level[k] + 1);
+        }
+    }
+    P.nLevels = 0;
+    for (int k = 0; k < nb; k++) P.nLevels = std::max(P.nLevels, level[k] + 1);
+
+    // --- contributions (ascending contributing node j)
+    std::vector<int> pairItemBase(nb + 1, 0);
+    for (int k = 0; k < nb; k++) pairItemBase[k + 1] = pairItemBase[k] + (P.sPtr[k + 1] - P.sPtr[k]);
+    std::vector<std::vector<std::pair<int, int>>> diagC(nb), pairC(pairItemBase[nb]);
+    for (int j = 0; j < nb; j++) {
+        const int lo = P.sPtr[j], hi = P.sPtr[j + 1];
+        for (int ea = 0; ea < hi - lo; ea++) {
+            const int a = P.sIdx[lo + ea];
+            diagC[a].push_back({lslot(j, ea), j});
+            for (int eb = ea + 1; eb < hi - lo; eb++) {
+                const int b = P.sIdx[lo + eb];
+                int e = indexIn(P.sIdx, P.sPtr[a], P.sPtr[a + 1], b);
+                if (e < 0) throw std::logic_error("symbolic structure is not closed under elimination");
+                pairC[pairItemBase[a] + e].push_back({lslot(j, ea), lslot(j, eb)});
+            }
+        }
+    }
+    P.diagContribPtr.assign(nb + 1, 0);
+    for (int k = 0; k < nb; k++) {
+        for (auto &c : diagC[k]) {
+            P.diagContribA.push_back(c.first);
+            P.diagContribB.push_back(c.second);
+        }
+        P.diagContribPtr[k + 1] = (int)P.diagContribA.size();
+        P.contributions += (long long)diagC[k].size();
+        P.factorMads += (long long)diagC[k].size();
+    }
+    std::vector<std::vector<int>> byLevel(P.nLevels);
+    for (int k = 0; k < nb; k++) byLevel[level[k]].push_back(k);
+    P.levelNodePtr.assign(P.nLevels + 1, 0);
+    P.levelPairPtr.assign(P.nLevels + 1, 0);
+    P.pairContribPtr.push_back(0);
+    for (int l = 0; l < P.nLevels; l++) {
+        for (int k : byLevel[l]) {
+            P.levelNodes.push_back(k);
+            for (int e = 0; e < P.sPtr[k + 1] - P.sPtr[k]; e++) {
+                const auto &lst = pairC[pairItemBase[k] + e];
+                for (auto &c : lst) {
+                    P.pairContribA.push_back(c.first);
+                    P.pairContribB.push_back(c.second);
+                }
+                P.pairContribPtr.push_back((int)P.pairContribA.size());
+                P.contributions += (long long)lst.size();
+                P.factorMads += 2LL * (long long)lst.size() + 1;  // + the D^-1 scaling
+                P.pairSlot.push_back(lslot(k, e));
+                P.pairNode.push_back(k);
+            }
+        }
+        P.levelNodePtr[l + 1] = (int)P.levelNodes.size();
+        P.levelPairPtr[l + 1] = (int)P.pairSlot.size();
+    }
+
+    // --- fill gather tables
+    std::vector<std::vector<int>> inc(nb);
+    for (int e = 0; e < nBranch; e++) {
+        inc[bus1[e]].push_back(e);
+        inc[bus2[e]].push_back(e);
+    }
+    P.hbPtr.assign(nb + 1, 0);
+    P.grpPtr.assign(nb + 1, 0);
+    P.fillPtr.assign(nb + 1, 0);
+    struct Half { int far, branch; double A, B, C, psi; };
+    std::vector<Half> hs;
+    for (int k = 0; k < nb; k++) {
+        const int b = P.perm[k];
+        hs.clear();
+        for (int e : inc[b]) {
+            const double *p = p8 + 8 * e;
+            const double y = p[0], g1 = p[1], g2 = p[2], b1 = p[3], b2 = p[4], ksi = p[5], r1 = p[6], a1 = p[7];
+            const double sk = std::sin(ksi), ck = std::cos(ksi);
+            if (bus1[e] == b) {
+                // side 1: p1 = r1^2 (g1 + y sin ksi) v1^2 - y r1 v1 v2 sin(ksi - a1 - ph1 + ph2)
+                hs.push_back({P.pos[bus2[e]], e, r1 * r1 * (g1 + y * sk), y * r1, r1 * r1 * (-b1 + y * ck), ksi - a1});
+            } else {
+                // side 2: p2 = (g2 + y sin ksi) v2^2 - y r1 v1 v2 sin(ksi + a1 - ph2 + ph1)
+                hs.push_back({P.pos[bus1[e]], e, g2 + y * sk, y * r1, -b2 + y * ck, ksi + a1});
+            }
+        }
+        std::sort(hs.begin(), hs.end(), [](const Half &u, const Half &v) {
+            return u.far != v.far ? u.far < v.far : u.branch < v.branch;
+        });
+        for (size_t h = 0; h < hs.size(); h++) {
+            P.hbBranch.push_back(hs[h].branch);
+            P.hbCoef.push_back(hs[h].A);
+            P.hbCoef.push_back(hs[h].B);
+            P.hbCoef.push_back(hs[h].C);
+            P.hbCoef.push_back(hs[h].psi);
+            bool last = h + 1 == hs.size() || hs[h + 1].far != hs[h].far;
+            if (last) {
+                const int f = hs[h].far;
+                int slot;
+                if (f > k) {
+                    int e = indexIn(P.sIdx, P.sPtr[k], P.sPtr[k + 1], f);
+                    if (e < 0) throw std::logic_error("pattern entry missing from the LU structure");
+                    slot = lslot(k, e) + 1;      // U(k, f)
+                } else {
+                    int e = indexIn(P.sIdx, P.sPtr[f], P.sPtr[f + 1], k);
+                    if (e < 0) throw std::logic_error("pattern entry missing from the LU structure");
+                    slot = lslot(f, e);          // L(k, f)
+                }
+                P.grpEnd.push_back((int)P.hbBranch.size());
+                P.grpFar.push_back(f);
+                P.grpSlot.push_back(slot);
+            }
+        }
+        P.hbPtr[k + 1] = (int)P.hbBranch.size();
+        P.grpPtr[k + 1] = (int)P.grpFar.size();
+        for (int e = 0; e < P.sPtr[k + 1] - P.sPtr[k]; e++) {
+            const int i = P.sIdx[P.sPtr[k] + e];
+            if (!std::binary_search(nbrs[b].begin(), nbrs[b].end(), P.perm[i])) P.fillSlot.push_back(lslot(k, e));
+        }
+        P.fillPtr[k + 1] = (int)P.fillSlot.size();
+        P.shuntG[k] = gBus[b];
+        P.shuntB[k] = bBus[b];
+    }
+
+    // --- scalar CSR pattern (stable rows) and its slot map
+    P.rowPtr.assign(n + 1, 0);
+    for (int b = 0; b < nb; b++) {
+        std::vector<int> cols = nbrs[b];
+        cols.push_back(b);
+        std::sort(cols.begin(), cols.end());
+        for (int r = 0; r < 2; r++) {
+            const int k = P.pos[b];
+            for (int c : cols) {
+                int slot;
+                if (c == b) {
+                    slot = P.nodeBase[k];
+                } else {
+                    const int f = P.pos[c];
+                    if (f > k) slot = lslot(k, indexIn(P.sIdx, P.sPtr[k], P.sPtr[k + 1], f)) + 1;
+                    else slot = lslot(f, indexIn(P.sIdx, P.sPtr[f], P.sPtr[f + 1], k));
+                }
+                for (int cc = 0; cc < 2; cc++) {
+                    P.colIdx.push_back(2 * c + cc);
+                    P.csrSlot.push_back(slot * 4 + 2 * r + cc);
+                }
+            }
+            P.rowPtr[2 * b + r + 1] = (int)P.colIdx.size();
+        }
+    }
+    return P;
+}
+
+size_t Plan::bytesPerScenario(int luValueBytes) const {
+    size_t s = (size_t)nSlots * 4 * luValueBytes;     // LU values
+    s += (size_t)n * 2 * luValueBytes;                // forward-solve vector and first step
+    s += (size_t)n * 8 * 6;                           // x, F, step, residual, target, next state
+    s += (size_t)nb * 8;                              // reactive injections
+    s += (size_t)n;                                   // row modes
+    return s + 256;
+}
+
+}  // namespace olfhip

@@ -1,0 +1,81 @@
+// Host-side structure and symbolic plan of the batched block LU (built once per network, shared by contexts).
+//
+// The Jacobian of the stable-row AC system is a 2x2-block matrix over the bus graph: block (i, j) holds
+// [dP_i/dphi_j, dP_i/dV_j; dQ_i/dphi_j, dQ_i/dV_j]. Its pattern is structurally symmetric (a branch couples both
+// ends both ways), so one fill-reducing order of the bus graph, one elimination tree and one set of block columns
+// serve L and U alike. Every scenario shares the plan; only values differ per scenario (outaged branches leave
+// zero blocks in the superset pattern, reference docs/gpu-vs-olf.md:14-40).
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace olfhip {
+
+struct Plan {
+    int nb = 0;          // buses (= LU nodes)
+    int n = 0;           // 2 * nb stable rows
+    int nBranch = 0;
+    int nShunt = 0;
+
+    // --- order: node k is bus perm[k]; bus b is node pos[b]
+    std::vector<int> perm, pos;
+
+    // --- block LU structure in node numbering
+    // S(k): the nodes i > k with a nonzero block L(i,k) (= U(k,i)), ascending.
+    std::vector<int> sPtr, sIdx;
+    // slots: node k owns [nodeBase[k], nodeBase[k] + 1 + 2|S(k)|): its diagonal block, then for the e-th member
+    // i of S(k) the pair L(i,k) at nodeBase[k] + 1 + 2e and U(k,i) right after it.
+    std::vector<int> nodeBase;
+    int nSlots = 0;
+
+    // --- level schedule (elimination-tree height; leaves are level 0)
+    int nLevels = 0;
+    std::vector<int> levelNodePtr, levelNodes;     // nodes of each level
+    // Phase 1 of a level has two kinds of items, each with a contribution list ascending in the contributing
+    // node j: the diagonal item of node k (D(k) = A(k,k) - sum L(k,j) U(j,k), and the fused forward-solve entry
+    // y(k) = b(k) - sum L(k,j) y(j)), and the pair items (k, i) (L~(i,k) = A(i,k) - sum L(i,j) U(j,k) and
+    // U(k,i) = A(k,i) - sum L(k,j) U(j,i)). Phase 2 scales the level's pairs: L(i,k) = L~(i,k) D(k)^-1.
+    std::vector<int> diagContribPtr;               // per node k (node numbering)
+    std::vector<int> diagContribA;                 // L slot of pair (j, k): L(k,j) there, U(j,k) right after
+    std::vector<int> diagContribB;                 // node j
+    std::vector<int> levelPairPtr, pairSlot, pairNode;
+    std::vector<int> pairContribPtr;
+    std::vector<int> pairContribA;                 // L slot of pair (j, k)
+    std::vector<int> pairContribB;                 // L slot of pair (j, i)
+
+    // --- fill gather tables, per node, half-branches grouped by the far node (ascending)
+    std::vector<int> hbPtr;                        // half-branches of node k
+    std::vector<int> hbBranch;                     // branch id (for the per-scenario outage lists)
+    std::vector<double> hbCoef;                    // 4 per half-branch: A, B, C, psi (see fill)
+    std::vector<int> grpPtr;                       // neighbour groups of node k
+    std::vector<int> grpEnd;                       // end of the group's half-branches in hbBranch
+    std::vector<int> grpFar;                       // far node
+    std::vector<int> grpSlot;                      // block slot of (k, far): U(k,far) or L(k,far)
+    std::vector<int> fillPtr, fillSlot;            // pairs of node k absent from the pattern (zeroed per fill)
+    std::vector<double> shuntG, shuntB;            // per node, pu
+
+    // --- scalar CSR pattern of the stable-row Jacobian (the reference's superset pattern) and its map to slots
+    std::vector<int> rowPtr, colIdx;
+    std::vector<int> csrSlot;                      // slot * 4 + component of each CSR entry
+
+    // --- statistics
+    long long factorMads = 0;                      // 2x2 block multiply-adds of one factorization
+    long long contributions = 0;
+
+    // device-independent per-scenario cost, bytes (without the LU value precision factor)
+    size_t bytesPerScenario(int luValueBytes) const;
+};
+
+// Builds the plan from the reference's closed-branch pack (8 doubles, 12 ints per branch) and optional shunts.
+// Throws std::invalid_argument on inputs this path does not model.
+Plan buildPlan(int n, int nBranch, const double *p8, const int *idx12, int nShunt, const double *shuntP,
+               const int *shuntIdx);
+
+// Minimum degree order of a symmetric graph (CSR adjacency), lowest bus index on ties. Also returns, for each
+// eliminated node, its neighbour set at elimination (the column structure of the Cholesky factor).
+void minimumDegree(int nb, const std::vector<int> &adjPtr, const std::vector<int> &adj, std::vector<int> &perm,
+                   std::vector<std::vector<int>> &structureAtElimination);
+
+}  // namespace olfhip

@@ -1,0 +1,286 @@
+"""ctypes binding of libolfhip.so (the C ABI declared in include/olfhip.h).
+
+The library is required: every entry point raises if it is missing — there is no CPU fallback on this path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(HERE), "libolfhip.so")
+
+OK, E_ARGUMENT, E_DEVICE, E_MEMORY, E_UNSUPPORTED = 0, -1, -2, -3, -4
+CONVERGED, MAX_ITERATION_REACHED, DIVERGED, REJECTED, LINEAR_SOLVE_FAILED = 0, 1, -1, -2, -3
+STATUS_NAMES = {0: "CONVERGED", 1: "MAX_ITERATION_REACHED", -1: "DIVERGED", -2: "REJECTED", -3: "LINEAR_SOLVE_FAILED"}
+LU_FP64, LU_FP32_REFINED = 0, 1
+SCALING_NONE, SCALING_MAX_VOLTAGE_CHANGE = 0, 2
+
+# every entry point include/olfhip.h declares
+EXPORTS = [
+    "olfhip_last_error", "olfhip_device_count", "olfhip_create_structure", "olfhip_destroy_structure",
+    "olfhip_structure_statistics", "olfhip_structure_order", "olfhip_recommend_batch_size",
+    "olfhip_create_batch_context", "olfhip_destroy_batch_context", "olfhip_solve_batch_context",
+    "olfhip_read_states", "olfhip_islanded_buses", "olfhip_lu_factorize_and_solve", "olfhip_structure_pattern",
+    "olfhip_autotune",
+]
+
+
+class OlfHipError(RuntimeError):
+    """A failed C-ABI call: `code` is the OLFHIP_E_* code, the message is olfhip_last_error()."""
+
+    def __init__(self, code, message):
+        super().__init__(message)
+        self.code = code
+
+
+class ReactiveLimits(C.Structure):
+    _fields_ = [("n_entries", C.c_int), ("voltage_row", C.c_void_p), ("min_q", C.c_void_p), ("max_q", C.c_void_p),
+                ("load_q", C.c_void_p), ("target_v", C.c_void_p)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("lanes_per_tile", C.c_int), ("threads_per_tile", C.c_int)]
+
+
+class SolveInputs(C.Structure):
+    _fields_ = [("n_scenarios", C.c_int), ("x0_base", C.c_void_p), ("target_base", C.c_void_p),
+                ("base_row_mode", C.c_void_p), ("n_row_mode_entries", C.c_int), ("row_mode_entries", C.c_void_p),
+                ("n_target_overrides", C.c_int), ("target_override_index", C.c_void_p),
+                ("target_override_value", C.c_void_p), ("max_outages", C.c_int), ("outaged_branches", C.c_void_p),
+                ("x0_per_scenario", C.c_void_p), ("reactive_limits_seed_state", C.c_void_p),
+                ("max_iter", C.c_int), ("tol", C.c_double), ("reactive_limit_tol", C.c_double),
+                ("state_vector_scaling_mode", C.c_int), ("max_dv", C.c_double), ("max_dphi", C.c_double),
+                ("max_outer_iterations", C.c_int), ("max_pq_pv_switch", C.c_int), ("skip_reactive_limits", C.c_int),
+                ("states_to_host", C.c_int)]
+
+
+class SolveOutputs(C.Structure):
+    _fields_ = [("results", C.c_void_p), ("status", C.c_void_p), ("iterations", C.c_void_p),
+                ("total_iterations", C.c_void_p), ("norm", C.c_void_p)]
+
+
+_LIB = None
+
+
+def lib():
+    """The loaded library; raises if it was not built (no fallback)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() (no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        vp, i, d = C.c_void_p, C.c_int, C.c_double
+        L.olfhip_last_error.restype = C.c_char_p
+        L.olfhip_device_count.restype = i
+        L.olfhip_create_structure.argtypes = [i, i, vp, vp, i, vp, vp, C.POINTER(vp)]
+        L.olfhip_destroy_structure.argtypes = [vp]
+        L.olfhip_destroy_structure.restype = None
+        L.olfhip_structure_statistics.argtypes = [vp, vp]
+        L.olfhip_structure_order.argtypes = [vp, vp]
+        L.olfhip_structure_pattern.argtypes = [vp, C.POINTER(i), vp, vp]
+        L.olfhip_recommend_batch_size.argtypes = [vp, i, i, C.POINTER(i)]
+        L.olfhip_create_batch_context.argtypes = [vp, i, i, C.POINTER(ReactiveLimits), C.POINTER(Tuning),
+                                                  C.POINTER(vp)]
+        L.olfhip_destroy_batch_context.argtypes = [vp]
+        L.olfhip_destroy_batch_context.restype = None
+        L.olfhip_solve_batch_context.argtypes = [vp, C.POINTER(SolveInputs), C.POINTER(SolveOutputs)]
+        L.olfhip_read_states.argtypes = [vp, i, vp]
+        L.olfhip_islanded_buses.argtypes = [vp, i, i, vp, vp, C.POINTER(i)]
+        L.olfhip_lu_factorize_and_solve.argtypes = [vp, i, i, vp, vp, i, vp, vp, vp]
+        L.olfhip_autotune.argtypes = [vp, i, C.POINTER(ReactiveLimits), C.POINTER(SolveInputs), C.POINTER(Tuning),
+                                      C.POINTER(d)]
+        _LIB = L
+    return _LIB
+
+
+def check(rc):
+    if rc != OK:
+        raise OlfHipError(rc, lib().olfhip_last_error().decode())
+
+
+def ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def device_count() -> int:
+    return lib().olfhip_device_count()
+
+
+class Structure:
+    """Owner of an olfhip_structure (StructureNative.createStructure / destroyStructure)."""
+
+    def __init__(self, n, closed_branch_parameters, closed_branch_indices, shunt_parameters=None, shunt_indices=None):
+        self.n = int(n)
+        self._p = np.ascontiguousarray(closed_branch_parameters, dtype=np.float64).reshape(-1)
+        self._i = np.ascontiguousarray(closed_branch_indices, dtype=np.int32).reshape(-1)
+        ns = 0 if shunt_parameters is None else len(shunt_parameters)
+        self._sp = None if ns == 0 else np.ascontiguousarray(shunt_parameters, dtype=np.float64).reshape(-1)
+        self._si = None if ns == 0 else np.ascontiguousarray(shunt_indices, dtype=np.int32).reshape(-1)
+        h = C.c_void_p()
+        check(lib().olfhip_create_structure(self.n, self._i.size // 12, ptr(self._p), ptr(self._i), ns, ptr(self._sp),
+                                            ptr(self._si), C.byref(h)))
+        self.h = h
+
+    @classmethod
+    def from_network(cls, net):
+        return cls(net.n, net.pi_model(), net.branch_indices())
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().olfhip_destroy_structure(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def statistics(self) -> dict:
+        s = np.zeros(6, np.int64)
+        check(lib().olfhip_structure_statistics(self.h, ptr(s)))
+        return dict(buses=int(s[0]), lu_slots=int(s[1]), levels=int(s[2]), factor_block_mads=int(s[3]),
+                    pattern_nnz=int(s[4]), lu_nnz=int(s[5]))
+
+    def order(self) -> np.ndarray:
+        o = np.zeros(self.n // 2, np.int32)
+        check(lib().olfhip_structure_order(self.h, ptr(o)))
+        return o
+
+    def pattern(self):
+        nnz = C.c_int()
+        check(lib().olfhip_structure_pattern(self.h, C.byref(nnz), None, None))
+        rp = np.zeros(self.n + 1, np.int32)
+        ci = np.zeros(nnz.value, np.int32)
+        check(lib().olfhip_structure_pattern(self.h, C.byref(nnz), ptr(rp), ptr(ci)))
+        return rp, ci
+
+    def islanded_buses(self, outaged, slack_bus=0) -> np.ndarray:
+        o = np.ascontiguousarray(np.asarray(outaged, dtype=np.int32).reshape(-1))
+        isl = np.zeros(self.n // 2, np.uint8)
+        cnt = C.c_int()
+        check(lib().olfhip_islanded_buses(self.h, slack_bus, o.size, ptr(o), ptr(isl), C.byref(cnt)))
+        return isl.astype(bool)
+
+    def recommend_batch_size(self, max_cap, mode=LU_FP64) -> int:
+        out = C.c_int()
+        check(lib().olfhip_recommend_batch_size(self.h, max_cap, mode, C.byref(out)))
+        return out.value
+
+    def lu_factorize_and_solve(self, values, rhs, lane_map=None, mode=LU_FP64):
+        """BatchedLuNative.factorizeAndSolve: values [capacity, nnz] on pattern(), rhs [capacity, n]."""
+        values = np.ascontiguousarray(values, dtype=np.float64)
+        rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+        cap = values.shape[0]
+        lm = np.arange(cap, dtype=np.int32) if lane_map is None else np.ascontiguousarray(lane_map, np.int32)
+        sol = np.zeros((cap, self.n))
+        clamps = np.zeros(lm.size, np.int32)
+        check(lib().olfhip_lu_factorize_and_solve(self.h, mode, cap, ptr(values), ptr(rhs), lm.size, ptr(lm),
+                                                  ptr(sol), ptr(clamps)))
+        return sol, clamps
+
+
+def reactive_limits_struct(rl: dict | None):
+    """(ctypes struct, keep-alive arrays) from a dict with voltage_row, min_q, max_q, load_q, target_v."""
+    if rl is None or len(rl["voltage_row"]) == 0:
+        return ReactiveLimits(0, None, None, None, None, None), []
+    keep = [np.ascontiguousarray(rl["voltage_row"], np.int32)] + [
+        np.ascontiguousarray(rl[k], np.float64) for k in ("min_q", "max_q", "load_q", "target_v")]
+    return ReactiveLimits(len(keep[0]), *[ptr(a) for a in keep]), keep
+
+
+class BatchContext:
+    """Owner of an olfhip_context (BatchSolveNative.createBatchContext / solveBatchContext / destroyBatchContext)."""
+
+    def __init__(self, structure: Structure, n_scenarios: int, mode=LU_FP64, reactive_limits=None,
+                 lanes_per_tile=0, threads_per_tile=0):
+        self.structure = structure
+        self.capacity = n_scenarios
+        self.mode = mode
+        self._rl, self._rl_keep = reactive_limits_struct(reactive_limits)
+        self._tuning = Tuning(lanes_per_tile, threads_per_tile)
+        h = C.c_void_p()
+        check(lib().olfhip_create_batch_context(structure.h, n_scenarios, mode, C.byref(self._rl),
+                                                C.byref(self._tuning), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().olfhip_destroy_batch_context(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def make_inputs(n_scenarios, x0_base, target_base, base_row_mode, row_mode_entries=None, target_overrides=None,
+                    outaged_branches=None, x0_per_scenario=None, reactive_limits_seed_state=None, max_iter=15,
+                    tol=1e-4, reactive_limit_tol=1e-4, state_vector_scaling_mode=SCALING_NONE, max_dv=0.1,
+                    max_dphi=0.174533, max_outer_iterations=20, max_pq_pv_switch=3, skip_reactive_limits=False,
+                    states_to_host=True):
+        keep = []
+
+        def arr(a, dt):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=dt)
+            keep.append(a)
+            return a
+
+        x0 = arr(x0_base, np.float64)
+        tg = arr(target_base, np.float64)
+        md = arr(base_row_mode, np.int32)
+        rme = arr(None if row_mode_entries is None or len(row_mode_entries) == 0 else
+                  np.asarray(row_mode_entries).reshape(-1, 3), np.int32)
+        if target_overrides is not None and len(target_overrides[0]) > 0:
+            ti = arr(target_overrides[0], np.int64)
+            tv = arr(target_overrides[1], np.float64)
+        else:
+            ti = tv = None
+        ob = None
+        mo = 0
+        if outaged_branches is not None:
+            ob = np.asarray(outaged_branches, dtype=np.int32)
+            if ob.ndim == 1:
+                ob = ob[:, None]
+            mo = ob.shape[1]
+            ob = arr(ob, np.int32)
+        xp = arr(x0_per_scenario, np.float64)
+        sd = arr(reactive_limits_seed_state, np.int8)
+        si = SolveInputs(n_scenarios, ptr(x0), ptr(tg), ptr(md), 0 if rme is None else rme.shape[0], ptr(rme),
+                         0 if ti is None else ti.size, ptr(ti), ptr(tv), mo, ptr(ob), ptr(xp), ptr(sd), max_iter, tol,
+                         reactive_limit_tol, state_vector_scaling_mode, max_dv, max_dphi, max_outer_iterations,
+                         max_pq_pv_switch, int(skip_reactive_limits), int(states_to_host))
+        return si, keep
+
+    def solve(self, inputs: SolveInputs, want_states=True):
+        k = inputs.n_scenarios
+        n = self.structure.n
+        res = np.zeros((k, n + 3)) if want_states else None
+        st = np.zeros(k, np.int32)
+        it = np.zeros(k, np.int32)
+        tot = np.zeros(k, np.int32)
+        nm = np.zeros(k)
+        out = SolveOutputs(ptr(res), ptr(st), ptr(it), ptr(tot), ptr(nm))
+        check(lib().olfhip_solve_batch_context(self.h, C.byref(inputs), C.byref(out)))
+        return dict(x=None if res is None else res[:, :n], results=res, status=st, iterations=it,
+                    total_iterations=tot, norm=nm)
+
+    def read_states(self, k):
+        x = np.zeros((k, self.structure.n))
+        check(lib().olfhip_read_states(self.h, k, ptr(x)))
+        return x
+
+
+def autotune(structure: Structure, inputs: SolveInputs, mode=LU_FP64, reactive_limits=None):
+    rl, keep = reactive_limits_struct(reactive_limits)
+    best = Tuning()
+    ms = C.c_double()
+    check(lib().olfhip_autotune(structure.h, mode, C.byref(rl), C.byref(inputs), C.byref(best), C.byref(ms)))
+    return dict(lanes_per_tile=best.lanes_per_tile, threads_per_tile=best.threads_per_tile, ms=ms.value)

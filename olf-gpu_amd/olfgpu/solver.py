@@ -1,0 +1,256 @@
+"""Host-side mirror of the reference's Java surface for this path, over the C ABI.
+
+* ``GpuLoadFlowParameters``  — the parameter extension (names and defaults of
+  src/main/java/com/powsybl/openloadflow/ac/solver/gpu/GpuLoadFlowParameters.java:37-63);
+* ``GpuNewtonRaphson``       — the GPU_NEWTON_RAPHSON AcSolver's batch-of-one load flow
+  (GpuNewtonRaphson.java:510-602, factory GpuNewtonRaphsonFactory.java:35-86);
+* ``GpuBatchedSecurityAnalysisSolver`` — the batched N-1 (GpuBatchedSecurityAnalysisSolver.java:107-373 with
+  chunking as GpuBatchChunkSolver.java:101-160 and islands as batch/plan/IslandedBuses.java:50-60).
+
+The numerics all run in libolfhip.so; this layer plans scenarios and marshals arrays, as the Java layer does.
+"""
+from __future__ import annotations
+
+import json
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _native as N
+
+NAME = "GPU_NEWTON_RAPHSON"          # AcSolverFactory name (GpuNewtonRaphsonFactory.java:35-38)
+PROVIDER_NAME = "OpenLoadFlowGpu"    # LoadFlowProvider / SecurityAnalysisProvider name
+
+
+@dataclass
+class GpuLoadFlowParameters:
+    """The ``gpu-load-flow-parameters`` extension (GpuLoadFlowParameters.java:39-63): same names and defaults.
+
+    linearSolver selects the device linear solve: BATCHED_LU (FP64 block LU) or BATCHED_LU_FP32_REFINED (FP32 factors
+    and solves plus one FP64-residual refinement step). CUDSS, the reference's closed-library alternative, is refused.
+    The cuDSS-specific knobs (reorderingAlgorithm, batchedLuOrdering, batchedLuSplitLongRows, matchingAlgorithm,
+    pivotType, pivotThreshold) are accepted and carried but have no effect: the block LU always uses its own
+    minimum-degree order and static 2x2-block pivots. iterativeRefinementBaseSteps is the number of FP64-residual
+    refinement steps of the FP32 mode (1 supported)."""
+
+    NAME = "gpu-load-flow-parameters"
+    MODULE = "open-loadflow-gpu-default-parameters"
+
+    linearSolver: str = "BATCHED_LU"
+    reorderingAlgorithm: str = "AUTO"
+    batchedLuOrdering: str = "CUDSS"
+    batchedLuSplitLongRows: bool = True
+    matchingAlgorithm: str = "AUTO"
+    pivotType: str = "BY_ORDERING"
+    pivotThreshold: float = float("nan")
+    iterativeRefinementBaseSteps: int = 1
+    iterativeRefinementMaxSteps: int = 4
+    iterativeRefinementTolerance: float = 1e-12
+
+    LINEAR_SOLVERS = ("BATCHED_LU", "BATCHED_LU_FP32_REFINED")
+
+    def __post_init__(self):
+        self.validate()
+
+    def validate(self):
+        if self.linearSolver == "CUDSS":
+            raise ValueError("linearSolver CUDSS is not available: the MI355X build has its own batched LU")
+        if self.linearSolver not in self.LINEAR_SOLVERS:
+            raise ValueError(f"unknown linearSolver {self.linearSolver!r}")
+        if self.iterativeRefinementBaseSteps not in (0, 1):
+            raise ValueError("iterativeRefinementBaseSteps: the FP32 mode runs exactly one refinement step")
+
+    @property
+    def lu_mode(self) -> int:
+        return N.LU_FP32_REFINED if self.linearSolver == "BATCHED_LU_FP32_REFINED" else N.LU_FP64
+
+    @classmethod
+    def load(cls, values: dict) -> "GpuLoadFlowParameters":
+        """From a name -> value map (the reference's PlatformConfig / parameter-map route)."""
+        p = cls()
+        for k, v in values.items():
+            if not hasattr(p, k) or k.isupper():
+                raise ValueError(f"unknown parameter {k!r}")
+            cur = getattr(p, k)
+            if isinstance(cur, bool):
+                v = v if isinstance(v, bool) else str(v).lower() == "true"
+            elif isinstance(cur, int):
+                v = int(v)
+            elif isinstance(cur, float):
+                v = float(v)
+            setattr(p, k, v)
+        p.validate()
+        return p
+
+    def to_json(self) -> str:
+        d = {k: getattr(self, k) for k in self.__dataclass_fields__}
+        d["pivotThreshold"] = None if math.isnan(self.pivotThreshold) else self.pivotThreshold
+        return json.dumps(d)
+
+
+@dataclass
+class NewtonRaphsonParameters:
+    """The OLF Newton-Raphson parameters the AcSolver copies (GpuNewtonRaphsonFactory.java:62-79)."""
+
+    maxIterations: int = 15
+    maxMismatch: float = 1e-4                 # ||F||inf stopping threshold, pu
+    stateVectorScalingMode: str = "NONE"      # NONE or MAX_VOLTAGE_CHANGE
+    maxVoltageChangeV: float = 0.1
+    maxVoltageChangePhi: float = 0.174533
+    reactiveLimits: bool = True
+    maxReactivePowerMismatch: float = 1e-4    # pu
+    maxOuterLoopIterations: int = 20
+    maxPqPvSwitch: int = 3
+
+    def scaling_code(self) -> int:
+        if self.stateVectorScalingMode == "NONE":
+            return N.SCALING_NONE
+        if self.stateVectorScalingMode == "MAX_VOLTAGE_CHANGE":
+            return N.SCALING_MAX_VOLTAGE_CHANGE
+        raise ValueError(f"state vector scaling {self.stateVectorScalingMode!r} is not modelled")
+
+
+@dataclass
+class AcSolverResult:
+    status: str
+    iterations: int
+    total_iterations: int
+    norm: float
+    x: np.ndarray
+    row_mode: np.ndarray = field(default=None, repr=False)
+
+
+class GpuNewtonRaphson:
+    """Batch-of-one device load flow on a cached structure and context (GpuNewtonRaphson.run/runImpl)."""
+
+    def __init__(self, structure: N.Structure, reactive_limits: dict | None = None,
+                 gpu_parameters: GpuLoadFlowParameters | None = None,
+                 nr_parameters: NewtonRaphsonParameters | None = None):
+        self.structure = structure
+        self.gpu_parameters = gpu_parameters or GpuLoadFlowParameters()
+        self.nr = nr_parameters or NewtonRaphsonParameters()
+        self.reactive_limits = reactive_limits
+        self.context = N.BatchContext(structure, 1, self.gpu_parameters.lu_mode, reactive_limits)
+
+    def run(self, x0, target, row_mode, seed_limit_types=None) -> AcSolverResult:
+        nr = self.nr
+        inp, keep = N.BatchContext.make_inputs(
+            1, x0, target, row_mode, reactive_limits_seed_state=None if seed_limit_types is None else
+            np.asarray(seed_limit_types, np.int8)[None, :], max_iter=nr.maxIterations, tol=nr.maxMismatch,
+            reactive_limit_tol=nr.maxReactivePowerMismatch, state_vector_scaling_mode=nr.scaling_code(),
+            max_dv=nr.maxVoltageChangeV, max_dphi=nr.maxVoltageChangePhi,
+            max_outer_iterations=nr.maxOuterLoopIterations, max_pq_pv_switch=nr.maxPqPvSwitch,
+            skip_reactive_limits=not nr.reactiveLimits)
+        r = self.context.solve(inp)
+        return AcSolverResult(N.STATUS_NAMES[int(r["status"][0])], int(r["iterations"][0]),
+                              int(r["total_iterations"][0]), float(r["norm"][0]), r["x"][0].copy())
+
+
+@dataclass
+class BatchScenarioResult:
+    """Per-contingency result (the reference's BatchScenarioResult)."""
+    status: np.ndarray
+    iterations: np.ndarray
+    total_iterations: np.ndarray
+    norm: np.ndarray
+    x: np.ndarray | None
+    islanded_count: np.ndarray
+    chunks: int
+    device_seconds: float
+
+
+class GpuBatchedSecurityAnalysisSolver:
+    """Batched N-1 from a converged base case: islands frozen at 1<0 in their scenario, chunks sized from device
+    memory, every chunk one batched solve."""
+
+    def __init__(self, structure: N.Structure, reactive_limits: dict | None = None,
+                 gpu_parameters: GpuLoadFlowParameters | None = None,
+                 nr_parameters: NewtonRaphsonParameters | None = None, slack_bus: int = 0, max_chunk: int = 4096,
+                 lanes_per_tile: int = 0, threads_per_tile: int = 0):
+        self.structure = structure
+        self.reactive_limits = reactive_limits
+        self.gpu_parameters = gpu_parameters or GpuLoadFlowParameters()
+        self.nr = nr_parameters or NewtonRaphsonParameters()
+        self.slack_bus = slack_bus
+        self.max_chunk = max_chunk
+        self.lanes_per_tile = lanes_per_tile
+        self.threads_per_tile = threads_per_tile
+        self._ctx = None
+        self._bridges = None
+
+    def plan_islands(self, contingencies: np.ndarray):
+        """(row_mode_entries, target_override_index, target_override_value, islanded_count) for the chunk."""
+        n = self.structure.n
+        rme, tix, tvl = [], [], []
+        counts = np.zeros(len(contingencies), np.int32)
+        for s, outs in enumerate(contingencies):
+            outs = [int(o) for o in np.atleast_1d(outs) if o >= 0]
+            isl = self.structure.islanded_buses(outs, self.slack_bus)
+            buses = np.nonzero(isl)[0]
+            counts[s] = buses.size
+            for b in buses:
+                for r, val in ((2 * b, 0.0), (2 * b + 1, 1.0)):
+                    rme.append((s, r, 0))   # FIXED_VARIABLE
+                    tix.append(s * n + r)
+                    tvl.append(val)
+        return (np.asarray(rme, np.int32).reshape(-1, 3), np.asarray(tix, np.int64), np.asarray(tvl, np.float64),
+                counts)
+
+    def context(self, chunk: int) -> N.BatchContext:
+        if self._ctx is None or self._ctx.capacity < chunk:
+            if self._ctx is not None:
+                self._ctx.close()
+            self._ctx = N.BatchContext(self.structure, chunk, self.gpu_parameters.lu_mode, self.reactive_limits,
+                                       self.lanes_per_tile, self.threads_per_tile)
+        return self._ctx
+
+    def solve_batch_n1(self, x_base, target_base, row_mode_base, limit_types_base, contingencies,
+                       x0_per_scenario=None, states_to_host=True) -> BatchScenarioResult:
+        """solveBatchN1: contingencies is [K] or [K, m] closed-branch ids (-1 padded)."""
+        import time
+
+        ctg = np.asarray(contingencies, dtype=np.int32)
+        if ctg.ndim == 1:
+            ctg = ctg[:, None]
+        k = ctg.shape[0]
+        chunk = min(k, self.max_chunk, self.structure.recommend_batch_size(self.max_chunk, self.gpu_parameters.lu_mode))
+        nr = self.nr
+        n = self.structure.n
+        st = np.zeros(k, np.int32)
+        it = np.zeros(k, np.int32)
+        tot = np.zeros(k, np.int32)
+        nm = np.zeros(k)
+        xs = np.zeros((k, n)) if states_to_host else None
+        islanded = np.zeros(k, np.int32)
+        seed = None
+        lt = None if limit_types_base is None else np.asarray(limit_types_base, np.int8)
+        device_s = 0.0
+        chunks = 0
+        for c0 in range(0, k, chunk):
+            c1 = min(k, c0 + chunk)
+            part = ctg[c0:c1]
+            rme, tix, tvl, cnt = self.plan_islands(part)
+            islanded[c0:c1] = cnt
+            if lt is not None and lt.size:
+                seed = np.tile(lt, (c1 - c0, 1))
+            inp, keep = N.BatchContext.make_inputs(
+                c1 - c0, x_base, target_base, row_mode_base, row_mode_entries=rme, target_overrides=(tix, tvl),
+                outaged_branches=part, x0_per_scenario=None if x0_per_scenario is None else x0_per_scenario[c0:c1],
+                reactive_limits_seed_state=seed, max_iter=nr.maxIterations, tol=nr.maxMismatch,
+                reactive_limit_tol=nr.maxReactivePowerMismatch, state_vector_scaling_mode=nr.scaling_code(),
+                max_dv=nr.maxVoltageChangeV, max_dphi=nr.maxVoltageChangePhi,
+                max_outer_iterations=nr.maxOuterLoopIterations, max_pq_pv_switch=nr.maxPqPvSwitch,
+                skip_reactive_limits=not nr.reactiveLimits, states_to_host=states_to_host)
+            ctx = self.context(c1 - c0)
+            t0 = time.perf_counter()
+            r = ctx.solve(inp, want_states=states_to_host)
+            device_s += time.perf_counter() - t0
+            chunks += 1
+            st[c0:c1] = r["status"]
+            it[c0:c1] = r["iterations"]
+            tot[c0:c1] = r["total_iterations"]
+            nm[c0:c1] = r["norm"]
+            if states_to_host:
+                xs[c0:c1] = r["x"]
+        return BatchScenarioResult(st, it, tot, nm, xs, islanded, chunks, device_s)

@@ -1,0 +1,187 @@
+"""GPU parity: the HIP path (through the C ABI) against the FP64 CPU oracle on the same seeded inputs.
+
+Bar (from the workload definition): per scenario identical status; bus |V| within 1e-6 pu and angle within 1e-6 rad
+of the oracle (both stop at ||F||inf < 1e-4 pu). The standalone LU matches the oracle's pivoted sparse solve to
+1e-9 relative in FP64 and to 1e-3 relative in FP32.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import pyoracle as po
+from olfgpu import _native as N
+from olfgpu import gridgen
+from olfgpu.solver import (GpuBatchedSecurityAnalysisSolver, GpuLoadFlowParameters, GpuNewtonRaphson,
+                           NewtonRaphsonParameters)
+
+pytestmark = pytest.mark.gpu
+
+V_TOL = 1e-6
+ANGLE_TOL = 1e-6
+
+
+@pytest.fixture(scope="module", autouse=True)
+def need_gpu():
+    if N.device_count() == 0:
+        pytest.fail("no GPU visible: the gpu-marked tests need an MI355X")
+
+
+_BASE = {}
+
+
+def base_case(net, name):
+    """Converged base case from the oracle: (x, mode, target, limit types)."""
+    if name not in _BASE:
+        osys = po.OracleSystem(net)
+        x0, tgt, mode, _ = net.system()
+        xb, mb, tb, ltb, res = osys.solve(x0, mode, tgt)
+        assert po.STATUS[res.status] == "CONVERGED"
+        _BASE[name] = (osys, xb, mb, tb, ltb)
+    return _BASE[name]
+
+
+def compare_states(xg, xo, what=""):
+    dv = np.abs(xg[..., 1::2] - xo[..., 1::2]).max()
+    da = np.abs(xg[..., 0::2] - xo[..., 0::2]).max()
+    assert dv <= V_TOL and da <= ANGLE_TOL, f"{what}: |dV|={dv:.3e} |dtheta|={da:.3e}"
+    return dv, da
+
+
+@pytest.mark.parametrize("name", ["grid118", "grid2000"])
+@pytest.mark.parametrize("mode", [N.LU_FP64, N.LU_FP32_REFINED])
+def test_standalone_lu_matches_oracle_solve(grid, name, mode):
+    net = grid(name)
+    print(f"[lu] {name} mode={mode}", flush=True)
+    osys = po.OracleSystem(net)
+    st = N.Structure.from_network(net)
+    x0, tgt, md, _ = net.system()
+    rng = np.random.default_rng(1)
+    lanes = 12
+    vals = np.zeros((lanes, osys.nnz))
+    rhs = rng.normal(size=(lanes, net.n))
+    ref = np.zeros((lanes, net.n))
+    for l in range(lanes):
+        x = x0.copy()
+        x[0::2] = rng.normal(0, 0.1, net.n_bus)
+        x[1::2] = 1 + rng.normal(0, 0.03, net.n_bus)
+        _, _, J = osys.fill(x, md, tgt)
+        vals[l] = J
+        ref[l] = po.lu_solve(osys.rowptr, osys.colidx, J, rhs[l])
+    sol, clamps = st.lu_factorize_and_solve(vals, rhs, mode=mode)
+    assert (clamps == 0).all()
+    rel = np.abs(sol - ref).max(axis=1) / np.abs(ref).max(axis=1)
+    assert rel.max() < (1e-9 if mode == N.LU_FP64 else 1e-3), rel.max()
+    # and A x = b directly on one lane
+    A = sp.csr_matrix((vals[0], osys.colidx, osys.rowptr), shape=(net.n, net.n))
+    assert np.abs(A @ sol[0] - rhs[0]).max() < (1e-8 if mode == N.LU_FP64 else 1e-2)
+
+
+def test_flat_start_118_matches_oracle(grid):
+    net = grid("grid118")
+    print("[base] grid118 flat start", flush=True)
+    osys = po.OracleSystem(net)
+    x0, tgt, md, _ = net.system()
+    xo, mo, to, lto, ro = osys.solve(x0, md, tgt)
+    solver = GpuNewtonRaphson(N.Structure.from_network(net), net.reactive_limits())
+    r = solver.run(x0, tgt, md)
+    assert r.status == po.STATUS[ro.status] == "CONVERGED"
+    assert r.iterations == ro.iterations and r.total_iterations == ro.total_iterations
+    compare_states(r.x, xo, "grid118")
+
+
+@pytest.mark.parametrize("lu", ["BATCHED_LU", "BATCHED_LU_FP32_REFINED"])
+def test_base_2000_with_reactive_limits(grid, lu):
+    net = grid("grid2000")
+    print(f"[base] grid2000 {lu}", flush=True)
+    osys, xb, mb, tb, ltb = base_case(net, "grid2000")
+    x0, tgt, md, _ = net.system()
+    solver = GpuNewtonRaphson(N.Structure.from_network(net), net.reactive_limits(),
+                              GpuLoadFlowParameters(linearSolver=lu))
+    r = solver.run(x0, tgt, md)
+    assert r.status == "CONVERGED"
+    compare_states(r.x, xb, f"grid2000 {lu}")
+
+
+def run_n1(net, name, count, bridge_fraction, seed, lu, lanes_per_tile=0, threads=0):
+    osys, xb, mb, tb, ltb = base_case(net, name)
+    ctg = gridgen.sample_contingencies(net, count, bridge_fraction, seed)
+    ref = osys.n1(xb, mb, tb, ltb, ctg)
+    sa = GpuBatchedSecurityAnalysisSolver(N.Structure.from_network(net), net.reactive_limits(),
+                                          GpuLoadFlowParameters(linearSolver=lu), lanes_per_tile=lanes_per_tile,
+                                          threads_per_tile=threads)
+    res = sa.solve_batch_n1(xb, tb, mb, ltb, ctg)
+    return ctg, ref, res
+
+
+@pytest.mark.parametrize("lu", ["BATCHED_LU", "BATCHED_LU_FP32_REFINED"])
+def test_n1_2000_matches_oracle(grid, lu):
+    net = grid("grid2000")
+    print(f"[n1] grid2000 {lu}", flush=True)
+    ctg, ref, res = run_n1(net, "grid2000", 48, 0.25, 5, lu)
+    assert np.array_equal(res.status, ref["status"]), (res.status, ref["status"])
+    assert (res.islanded_count > 0).sum() >= 10
+    conv = res.status == 0
+    compare_states(res.x[conv], ref["x"][conv], f"n1 grid2000 {lu}")
+    if lu == "BATCHED_LU":
+        assert np.array_equal(res.iterations, ref["iterations"])
+        assert np.array_equal(res.total_iterations, ref["total_iterations"])
+
+
+@pytest.mark.parametrize("lu", ["BATCHED_LU", "BATCHED_LU_FP32_REFINED"])
+def test_n1_9241_matches_oracle(grid, lu):
+    net = grid("grid9241")
+    print(f"[n1] grid9241 {lu}", flush=True)
+    ctg, ref, res = run_n1(net, "grid9241", 40, 0.1, 2, lu)
+    assert np.array_equal(res.status, ref["status"])
+    conv = res.status == 0
+    compare_states(res.x[conv], ref["x"][conv], f"n1 grid9241 {lu}")
+
+
+def test_tile_shapes_give_bitwise_identical_results(grid):
+    net = grid("grid2000")
+    print("[tiles] grid2000", flush=True)
+    outs = []
+    for T, NT in [(1, 256), (4, 512), (8, 1024), (16, 256)]:
+        ctg, ref, res = run_n1(net, "grid2000", 24, 0.25, 6, "BATCHED_LU", T, NT)
+        outs.append(res)
+    for r in outs[1:]:
+        assert np.array_equal(r.status, outs[0].status)
+        assert np.array_equal(r.x, outs[0].x)
+
+
+def test_rerun_is_bitwise_identical(grid):
+    net = grid("grid2000")
+    a = run_n1(net, "grid2000", 16, 0.25, 8, "BATCHED_LU_FP32_REFINED")[2]
+    b = run_n1(net, "grid2000", 16, 0.25, 8, "BATCHED_LU_FP32_REFINED")[2]
+    assert np.array_equal(a.x, b.x) and np.array_equal(a.status, b.status)
+
+
+def test_statuses_forced_from_the_host(grid):
+    net = grid("grid118")
+    print("[status] grid118", flush=True)
+    st = N.Structure.from_network(net)
+    x0, tgt, md, _ = net.system()
+    ctx = N.BatchContext(st, 3, N.LU_FP64, net.reactive_limits())
+    xp = np.tile(x0, (3, 1))
+    xp[1, 7] = np.nan
+    inp, keep = N.BatchContext.make_inputs(3, x0, tgt, md, x0_per_scenario=xp)
+    r = ctx.solve(inp)
+    assert [N.STATUS_NAMES[s] for s in r["status"]] == ["CONVERGED", "DIVERGED", "CONVERGED"]
+    assert np.array_equal(r["x"][0], r["x"][2])
+    inp, keep = N.BatchContext.make_inputs(3, x0, tgt, md, max_iter=1, skip_reactive_limits=True)
+    r = ctx.solve(inp)
+    assert all(N.STATUS_NAMES[s] == "MAX_ITERATION_REACHED" for s in r["status"])
+    assert (r["iterations"] == 1).all()
+
+
+def test_states_left_on_device_read_back(grid):
+    net = grid("grid118")
+    st = N.Structure.from_network(net)
+    x0, tgt, md, _ = net.system()
+    ctx = N.BatchContext(st, 2, N.LU_FP64, net.reactive_limits())
+    inp, keep = N.BatchContext.make_inputs(2, x0, tgt, md, states_to_host=True)
+    a = ctx.solve(inp)
+    inp, keep = N.BatchContext.make_inputs(2, x0, tgt, md, states_to_host=False)
+    b = ctx.solve(inp, want_states=False)
+    assert np.array_equal(ctx.read_states(2), a["x"])
+    assert np.array_equal(a["status"], b["status"])

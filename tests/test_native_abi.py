@@ -1,0 +1,98 @@
+"""CPU-side checks of the C ABI: the library loads and exports every entry point of include/olfhip.h, the host plan
+(pattern, order, islands) agrees with the oracle, contract violations come back as error codes, and the parameter
+surface keeps the reference's names and defaults. No compute call needs a GPU here."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+from olfgpu import _native as N
+from olfgpu import gridgen
+from olfgpu.solver import GpuLoadFlowParameters, NewtonRaphsonParameters
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol():
+    header = open(os.path.join(REPO, "include", "olfhip.h")).read()
+    declared = sorted(set(re.findall(r"\b(olfhip_[a-z_0-9]+)\s*\(", header)))
+    assert declared, "no declarations parsed"
+    L = N.lib()
+    for name in declared:
+        assert hasattr(L, name), name
+    assert sorted(N.EXPORTS) == declared
+
+
+def test_device_count_never_fails():
+    assert N.device_count() >= 0
+
+
+def test_pattern_matches_the_oracle_pattern(grid):
+    net = grid("grid2000")
+    st = N.Structure.from_network(net)
+    rp, ci = st.pattern()
+    osys = po.OracleSystem(net)
+    assert np.array_equal(rp, osys.rowptr)
+    assert np.array_equal(ci, osys.colidx)
+
+
+def test_plan_statistics_and_order(grid):
+    net = grid("grid9241")
+    st = N.Structure.from_network(net)
+    s = st.statistics()
+    assert s["buses"] == 9241
+    assert s["pattern_nnz"] == 4 * 9241 + 8 * 16000 - 8 * 0  # no parallel branches in the generator
+    assert s["lu_nnz"] >= s["pattern_nnz"]
+    assert 10 < s["levels"] < 1000
+    order = st.order()
+    assert sorted(order.tolist()) == list(range(9241))
+
+
+def test_islands_match_the_oracle(grid):
+    net = grid("grid2000")
+    st = N.Structure.from_network(net)
+    osys = po.OracleSystem(net)
+    ctg = gridgen.sample_contingencies(net, 40, 0.5, 4)
+    for e in ctg:
+        assert np.array_equal(st.islanded_buses([e]), osys.islands([e]))
+    two = [int(ctg[0]), int(ctg[1])]
+    assert np.array_equal(st.islanded_buses(two), osys.islands(two))
+
+
+def test_unsupported_and_invalid_inputs_return_codes(grid):
+    net = grid("grid118")
+    idx = net.branch_indices().copy()
+    idx[0, 9] = 7  # a ratio-control column: not modelled
+    with pytest.raises(N.OlfHipError) as e:
+        N.Structure(net.n, net.pi_model(), idx)
+    assert e.value.code == N.E_UNSUPPORTED
+    idx = net.branch_indices().copy()
+    idx[3, 1] = idx[3, 0]  # Q row not the bus's V row
+    with pytest.raises(N.OlfHipError) as e:
+        N.Structure(net.n, net.pi_model(), idx)
+    assert e.value.code == N.E_UNSUPPORTED
+    with pytest.raises(N.OlfHipError) as e:
+        N.Structure(net.n + 1, net.pi_model(), net.branch_indices())
+    assert e.value.code == N.E_ARGUMENT
+    assert "even" in N.lib().olfhip_last_error().decode()
+
+
+def test_parameters_keep_the_reference_surface():
+    p = GpuLoadFlowParameters()
+    assert p.NAME == "gpu-load-flow-parameters"
+    assert p.MODULE == "open-loadflow-gpu-default-parameters"
+    assert (p.linearSolver, p.reorderingAlgorithm, p.batchedLuOrdering, p.batchedLuSplitLongRows,
+            p.matchingAlgorithm, p.pivotType) == ("BATCHED_LU", "AUTO", "CUDSS", True, "AUTO", "BY_ORDERING")
+    assert np.isnan(p.pivotThreshold)
+    assert (p.iterativeRefinementBaseSteps, p.iterativeRefinementMaxSteps, p.iterativeRefinementTolerance) == (1, 4, 1e-12)
+    q = GpuLoadFlowParameters.load({"linearSolver": "BATCHED_LU_FP32_REFINED", "batchedLuSplitLongRows": "false"})
+    assert q.lu_mode == N.LU_FP32_REFINED and q.batchedLuSplitLongRows is False
+    with pytest.raises(ValueError):
+        GpuLoadFlowParameters(linearSolver="CUDSS")
+    with pytest.raises(ValueError):
+        GpuLoadFlowParameters.load({"noSuchParameter": 1})
+    assert NewtonRaphsonParameters().scaling_code() == N.SCALING_NONE
+    with pytest.raises(ValueError):
+        NewtonRaphsonParameters(stateVectorScalingMode="LINE_SEARCH").scaling_code()
