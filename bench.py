@@ -1,0 +1,208 @@
+"""Benchmark: N-1 contingencies/s at 1e-4 pu on the 9,241-bus synthetic grid (load x1.1).
+
+One step = one batched N-1 solve of the whole contingency set (2,000 single-branch outages, 5 % of them bridges
+that island buses) from the converged base case, FP32 block LU + one FP64-residual refinement step, reactive-limit
+outer loop on, states left on the device (inputs resident; the per-call upload is a few hundred KB). The plain FP64
+one-case-at-a-time GPU Newton-Raphson (the same library at batch 1) is timed on a sample beside it, and the CPU
+oracle (FP64, OpenMP over contingencies) on a bounded sample.
+
+    python bench.py [--gpus 1] [--steps 5] [--warmup 2]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "olf-gpu_amd"))
+
+from olfgpu import _native as N  # noqa: E402
+from olfgpu import gridgen  # noqa: E402
+from olfgpu.solver import (GpuBatchedSecurityAnalysisSolver, GpuLoadFlowParameters,  # noqa: E402
+                           GpuNewtonRaphson, NewtonRaphsonParameters)
+
+METRIC = "N-1 contingencies/sec @1e-4pu, 9241-bus: batched FP32-LU+warm-start vs plain FP64"
+HBM_PEAK_GBS = 8000.0
+
+
+def device_synchronize():
+    """hipDeviceSynchronize (what torch.cuda.synchronize() does), without importing torch."""
+    hip = ctypes.CDLL("libamdhip64.so")
+    rc = hip.hipDeviceSynchronize()
+    if rc != 0:
+        raise RuntimeError(f"hipDeviceSynchronize failed: {rc}")
+
+
+def algorithmic_bytes(stats, n, mode, lane_iterations, solves):
+    """Algorithmic HBM bytes of the Newton steps: per lane-iteration (fill, norm, factorization, solves, update and,
+    in the FP32 mode, the FP64-residual refinement), per the per-unit model of the hot path, with L = LU nonzeros of
+    the symbolic plan and m = pattern nonzeros; plus one mismatch fill per ended solve (its convergence test)."""
+    m = stats["pattern_nnz"]
+    L = stats["lu_nnz"]
+    w = 4 if mode == N.LU_FP32_REFINED else 8
+    fill = 8 * n + w * m + 8 * n
+    norm = 8 * n
+    factor = w * m + w * L
+    solve = w * L + 32 * n
+    update = 24 * n
+    refine = (8 * m + 24 * n + w * L + 32 * n + 16 * n) if mode == N.LU_FP32_REFINED else 0
+    per_iter = fill + norm + factor + solve + update + refine
+    return lane_iterations * per_iter + solves * (16 * n), per_iter
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--contingencies", type=int, default=2000)
+    ap.add_argument("--plain-sample", type=int, default=20)
+    ap.add_argument("--cpu-sample", type=int, default=300)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--lanes-per-tile", type=int, default=0)
+    ap.add_argument("--threads-per-tile", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU: the N-1 batch shards by contingency across GPUs "
+                                   "(see the sharding design); only --gpus 1 is measured"}))
+        sys.exit(2)
+    if N.device_count() == 0:
+        print(json.dumps({"error": "no GPU visible"}))
+        sys.exit(1)
+
+    t_setup = time.perf_counter()
+    net = gridgen.workload("grid9241")
+    structure = N.Structure.from_network(net)
+    stats = structure.statistics()
+    rl = net.reactive_limits()
+    x0, tgt, mode, _ = net.system()
+    # base case on the device (batch of one, FP64)
+    base = GpuNewtonRaphson(structure, rl).run(x0, tgt, mode)
+    if base.status != "CONVERGED":
+        print(json.dumps({"error": f"base case {base.status}"}))
+        sys.exit(1)
+    # contingencies start from the base case's converged state, row modes, targets and reactive-limit types
+    xb, mb, tb, lt = base.x, base.row_mode, base.target, base.reactive_limit_types
+
+    ctg = gridgen.sample_contingencies(net, args.contingencies, 0.05, 2)
+    nrp = NewtonRaphsonParameters()
+    sa = GpuBatchedSecurityAnalysisSolver(structure, rl, GpuLoadFlowParameters(linearSolver="BATCHED_LU_FP32_REFINED"),
+                                          nrp, lanes_per_tile=args.lanes_per_tile,
+                                          threads_per_tile=args.threads_per_tile)
+    K = len(ctg)
+    rme, tix, tvl, isl = sa.plan_islands(ctg[:, None])
+    inputs, keep = N.BatchContext.make_inputs(
+        K, xb, tb, mb, row_mode_entries=rme, target_overrides=(tix, tvl), outaged_branches=ctg[:, None],
+        reactive_limits_seed_state=np.tile(lt, (K, 1)), max_iter=nrp.maxIterations, tol=nrp.maxMismatch,
+        reactive_limit_tol=nrp.maxReactivePowerMismatch, states_to_host=False)
+    ctx = sa.context(K)
+    setup_s = time.perf_counter() - t_setup
+
+    for _ in range(args.warmup):
+        ctx.solve(inputs, want_states=False)
+    device_synchronize()
+    t0 = time.perf_counter()
+    step_ms = 0.0
+    steps_launched = 0
+    last = None
+    for _ in range(args.steps):
+        last = ctx.solve(inputs, want_states=False)
+        ms, st = ctx.timing()
+        step_ms += ms
+        steps_launched += st
+    device_synchronize()
+    wall = time.perf_counter() - t0
+    value = K * args.steps / wall
+    status = last["status"]
+    lane_iters = int(last["total_iterations"].sum())
+
+    # plain FP64 one-case-at-a-time GPU NR on a sample (the same library at batch 1, FP64, base-case start)
+    plain = GpuBatchedSecurityAnalysisSolver(structure, rl, GpuLoadFlowParameters(linearSolver="BATCHED_LU"), nrp)
+    sample = ctg[: args.plain_sample]
+    prme, ptix, ptvl, _ = plain.plan_islands(sample[:, None])
+    pctx = plain.context(1)
+    plain_inputs = []
+    for s in range(len(sample)):
+        sel = prme[:, 0] == s
+        r1 = prme[sel].copy()
+        r1[:, 0] = 0
+        tsel = (ptix // net.n) == s
+        pin, pk = N.BatchContext.make_inputs(
+            1, xb, tb, mb, row_mode_entries=r1, target_overrides=(ptix[tsel] - s * net.n, ptvl[tsel]),
+            outaged_branches=sample[s:s + 1, None], reactive_limits_seed_state=lt[None, :],
+            max_iter=nrp.maxIterations, tol=nrp.maxMismatch, reactive_limit_tol=nrp.maxReactivePowerMismatch,
+            states_to_host=False)
+        plain_inputs.append((pin, pk))
+    pctx.solve(plain_inputs[0][0], want_states=False)   # warm-up
+    device_synchronize()
+    tp = time.perf_counter()
+    plain_status = []
+    for pin, _ in plain_inputs:
+        plain_status.append(int(pctx.solve(pin, want_states=False)["status"][0]))
+    device_synchronize()
+    plain_s = time.perf_counter() - tp
+    plain_value = len(sample) / plain_s
+    plain_match = bool(np.array_equal(np.array(plain_status), status[: len(sample)]))
+
+    # roofline of the Newton-step kernel
+    ab, per_iter = algorithmic_bytes(stats, net.n, N.LU_FP32_REFINED, lane_iters, K)
+    step_ms_per_solve = step_ms / args.steps
+    launches_per_solve = steps_launched / args.steps
+    achieved = ab / (step_ms_per_solve * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "nrStepKernel<T,float>", "launches_per_step": launches_per_solve,
+                "avg_launch_ms": round(step_ms_per_solve / max(launches_per_solve, 1), 4),
+                "algorithmic_bytes_per_launch": int(ab / max(launches_per_solve, 1)),
+                "bytes_per_lane_iteration": int(per_iter), "lane_iterations_per_step": lane_iters}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(net, xb, mb, tb, lt, ctg[: args.cpu_sample])
+
+    out = {
+        "metric": METRIC, "value": round(value, 2), "unit": "contingencies/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32-lu+fp64-residual", "data": "synthetic",
+        "config": {"workload": "grid9241 x1.1 load, 2000 single-branch outages (5% bridges, seed 2), FP32 block LU + "
+                               "1 FP64 refinement, reactive limits on, base-case start (no learned warm start yet)",
+                   "buses": net.n_bus, "branches": net.n_branch, "contingencies": K, "tol_pu": nrp.maxMismatch,
+                   "lanes_per_tile": ctx_lanes(ctx), "lu_levels": stats["levels"], "lu_nnz": stats["lu_nnz"]},
+        "statuses": {N.STATUS_NAMES[int(s)]: int((status == s).sum()) for s in np.unique(status)},
+        "newton_lane_iterations_per_step": lane_iters,
+        "plain_fp64_gpu": {"value": round(plain_value, 2), "unit": "contingencies/s", "sample": len(sample),
+                           "statuses_match_batched": plain_match},
+        "speedup_vs_plain_fp64_gpu": round(value / plain_value, 2),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "setup_s": round(setup_s, 2),
+    }
+    print(json.dumps(out), flush=True)
+
+
+def ctx_lanes(ctx):
+    return ctx._tuning.lanes_per_tile or "auto"
+
+
+def cpu_baseline(net, xb, mb, tb, lt, sample):
+    """The FP64 CPU oracle (own sparse LU, OpenMP over contingencies) on a bounded sample, all host cores."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import pyoracle as po
+
+    osys = po.OracleSystem(net)
+    t = time.perf_counter()
+    r = osys.n1(xb, mb, tb, lt, sample)
+    dt = time.perf_counter() - t
+    return {"value": round(len(sample) / dt, 2), "unit": "contingencies/s", "cores": int(r["threads"]),
+            "kind": "port", "sample": f"first {len(sample)} of the 2000 contingencies, same base state and rules, "
+                                       f"{dt:.1f} s wall on {int(r['threads'])} threads"}
+
+
+if __name__ == "__main__":
+    main()

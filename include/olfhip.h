@@ -170,6 +170,19 @@ typedef struct {
 int olfhip_solve_batch_context(olfhip_context *context, const olfhip_solve_inputs *inputs,
                                olfhip_solve_outputs *outputs);
 
+/* Device time of the last solve's Newton-step launches, measured with HIP events on the context's stream around each
+ * step (the step kernel, plus the few-microsecond running-lane count and its 4-byte copy), and the step count.
+ * No reference counterpart: the reference's phase timers are print-only (OLF_GPU_PHASE_TIME,
+ * native/cuda/jni/batch_solve_jni.cu:12-26). */
+int olfhip_context_timing(const olfhip_context *context, double *step_ms, int *steps);
+
+/* Row modes and targets ([n_scenarios][n], stable rows) and reactive-limit types ([n_scenarios][n_entries], may be
+ * NULL) the outer loop left in each scenario after the last solve: what the reference writes back into OLF's buses
+ * after a device solve (acceptDeviceSolution, GpuNewtonRaphson.java:789-824), and the seed state a later N-1 chunk
+ * starts from. */
+int olfhip_read_outer_loop_state(olfhip_context *context, int n_scenarios, int *row_mode, double *target,
+                                 signed char *reactive_limit_types);
+
 /* States of the last solve (states_to_host = 0): [n_scenarios][n] into x. Reference: the StateHandback readback of
  * states left on the device (BatchSolveNative.setBatchStatesToHost). */
 int olfhip_read_states(olfhip_context *context, int n_scenarios, double *x);

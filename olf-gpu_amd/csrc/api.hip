@@ -166,6 +166,8 @@ struct ContextImpl {
     int lastInstalled = 0;
     double lastSolveMs = 0.0;
     int lastSteps = 0;
+    double lastStepMs = 0.0;                 // event-timed Newton-step launches of the last solve
+    hipEvent_t evStart = nullptr, evStop = nullptr;
 
     DevState devState() const {
         DevState S{};
@@ -203,6 +205,8 @@ struct ContextImpl {
     }
     ~ContextImpl() {
         if (pinnedCount) (void)hipHostFree(pinnedCount);
+        if (evStart) (void)hipEventDestroy(evStart);
+        if (evStop) (void)hipEventDestroy(evStop);
         if (stream) (void)hipStreamDestroy(stream);
         release(st);
     }
@@ -228,6 +232,8 @@ static ContextImpl *createContext(StructureImpl *st, int nScenarios, int mode, c
     c->nTiles = (nScenarios + c->T - 1) / c->T;
     HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     HIP_OK(hipHostMalloc((void **)&c->pinnedCount, sizeof(int), hipHostMallocDefault));
+    HIP_OK(hipEventCreate(&c->evStart));
+    HIP_OK(hipEventCreate(&c->evStop));
     const Plan &P = st->plan;
     const size_t lanes = (size_t)c->nTiles * c->T;
     const size_t nv = lanes * P.nb * 2;
@@ -383,10 +389,16 @@ static void solve(ContextImpl *c, const olfhip_solve_inputs *in, olfhip_solve_ou
             HIP_OK(hipStreamEndCapture(s, &graph));
             HIP_OK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
         }
+        c->lastStepMs = 0.0;
         for (steps = 0; steps < maxSteps; steps++) {
+            HIP_OK(hipEventRecord(c->evStart, s));
             if (useGraph) HIP_OK(hipGraphLaunch(exec, s));
             else enqueueStep();
+            HIP_OK(hipEventRecord(c->evStop, s));
             HIP_OK(hipStreamSynchronize(s));
+            float ms = 0.0f;
+            HIP_OK(hipEventElapsedTime(&ms, c->evStart, c->evStop));
+            c->lastStepMs += ms;
             if (traceOn()) std::fprintf(stderr, "OLFHIP step %d running tiles %d\n", steps, *c->pinnedCount);
             if (*c->pinnedCount == 0) {
                 steps++;
@@ -654,6 +666,45 @@ int olfhip_solve_batch_context(olfhip_context *context, const olfhip_solve_input
     return guarded([&] {
         if (!context) throw ArgumentError("context is null");
         solve(reinterpret_cast<ContextImpl *>(context), inputs, outputs);
+    });
+}
+
+int olfhip_context_timing(const olfhip_context *context, double *step_ms, int *steps) {
+    return guarded([&] {
+        if (!context) throw ArgumentError("context is null");
+        const ContextImpl *c = reinterpret_cast<const ContextImpl *>(context);
+        if (step_ms) *step_ms = c->lastStepMs;
+        if (steps) *steps = c->lastSteps;
+    });
+}
+
+int olfhip_read_outer_loop_state(olfhip_context *context, int n_scenarios, int *row_mode, double *target,
+                                 signed char *reactive_limit_types) {
+    return guarded([&] {
+        ContextImpl *c = reinterpret_cast<ContextImpl *>(context);
+        if (!c || !row_mode || !target) throw ArgumentError("null argument");
+        if (n_scenarios <= 0 || n_scenarios > c->lastInstalled)
+            throw ArgumentError("n_scenarios exceeds the last solve's installed count");
+        const Plan &P = c->st->plan;
+        const size_t K = (size_t)n_scenarios;
+        DBuf dm, dt, dl;
+        dm.alloc(K * P.n * 4);
+        dt.alloc(K * P.n * 8);
+        dl.alloc(K * std::max(c->nRL, 1));
+        const DevPlan dp = c->st->dp;
+        const DevState S = c->devState();
+        const DevRL R = c->devRL();
+        withT(c->T, [&](auto TC) {
+            constexpr int T = decltype(TC)::value;
+            gatherOuterStateKernel<T><<<gridFor(K * (P.nb + c->nRL), 256), 256, 0, c->stream>>>(
+                dp, S, R, n_scenarios, dm.as<int>(), dt.as<double>(), dl.as<signed char>());
+        });
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipMemcpyAsync(row_mode, dm.p, K * P.n * 4, hipMemcpyDeviceToHost, c->stream));
+        HIP_OK(hipMemcpyAsync(target, dt.p, K * P.n * 8, hipMemcpyDeviceToHost, c->stream));
+        if (reactive_limit_types && c->nRL > 0)
+            HIP_OK(hipMemcpyAsync(reactive_limit_types, dl.p, K * c->nRL, hipMemcpyDeviceToHost, c->stream));
+        HIP_OK(hipStreamSynchronize(c->stream));
     });
 }
 

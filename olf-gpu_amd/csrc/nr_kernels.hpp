@@ -854,6 +854,33 @@ __global__ void gatherStatesKernel(DevPlan P, DevState S, int nLanes, double *ou
     }
 }
 
+// row modes, targets (stable-row order, [lane][n]) and reactive-limit types ([lane][entries]) of the installed lanes
+template <int T>
+__global__ void gatherOuterStateKernel(DevPlan P, DevState S, DevRL R, int nLanes, int *modeOut, double *targetOut,
+                                       signed char *typeOut) {
+    const int nb = P.nb;
+    const size_t n = 2 * (size_t)nb;
+    const size_t rows = (size_t)nLanes * nb;
+    for (size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x; w < rows + (size_t)nLanes * R.n;
+         w += (size_t)gridDim.x * blockDim.x) {
+        if (w < rows) {
+            const int lane = (int)(w / nb), b = (int)(w % nb);
+            const int tile = lane / T, t = lane % T;
+            const size_t vi = vIdx<T>(tile, nb, P.pos[b], t);
+            modeOut[lane * n + 2 * b] = S.mode[vi];
+            modeOut[lane * n + 2 * b + 1] = S.mode[vi + 1];
+            targetOut[lane * n + 2 * b] = S.target[vi];
+            targetOut[lane * n + 2 * b + 1] = S.target[vi + 1];
+        } else {
+            const size_t q = w - rows;
+            const int lane = (int)(q / R.n), e = (int)(q % R.n);
+            const int tile = lane / T, t = lane % T;
+            const signed char lt = S.rlType[((size_t)tile * R.n + e) * T + t];
+            typeOut[q] = lt == QLIM_FROZEN ? QLIM_NONE : lt;
+        }
+    }
+}
+
 // running lanes after a step: sum over tiles into *count (one block)
 __global__ void countRunningKernel(const int *tileRunning, int nTiles, int *count) {
     __shared__ int acc[256];

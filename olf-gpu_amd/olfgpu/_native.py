@@ -24,7 +24,7 @@ EXPORTS = [
     "olfhip_structure_statistics", "olfhip_structure_order", "olfhip_recommend_batch_size",
     "olfhip_create_batch_context", "olfhip_destroy_batch_context", "olfhip_solve_batch_context",
     "olfhip_read_states", "olfhip_islanded_buses", "olfhip_lu_factorize_and_solve", "olfhip_structure_pattern",
-    "olfhip_autotune",
+    "olfhip_autotune", "olfhip_context_timing", "olfhip_read_outer_loop_state",
 ]
 
 
@@ -88,6 +88,8 @@ def lib():
         L.olfhip_destroy_batch_context.restype = None
         L.olfhip_solve_batch_context.argtypes = [vp, C.POINTER(SolveInputs), C.POINTER(SolveOutputs)]
         L.olfhip_read_states.argtypes = [vp, i, vp]
+        L.olfhip_context_timing.argtypes = [vp, C.POINTER(d), C.POINTER(i)]
+        L.olfhip_read_outer_loop_state.argtypes = [vp, i, vp, vp, vp]
         L.olfhip_islanded_buses.argtypes = [vp, i, i, vp, vp, C.POINTER(i)]
         L.olfhip_lu_factorize_and_solve.argtypes = [vp, i, i, vp, vp, i, vp, vp, vp]
         L.olfhip_autotune.argtypes = [vp, i, C.POINTER(ReactiveLimits), C.POINTER(SolveInputs), C.POINTER(Tuning),
@@ -271,6 +273,22 @@ class BatchContext:
         check(lib().olfhip_solve_batch_context(self.h, C.byref(inputs), C.byref(out)))
         return dict(x=None if res is None else res[:, :n], results=res, status=st, iterations=it,
                     total_iterations=tot, norm=nm)
+
+    def timing(self):
+        """(event-timed Newton-step milliseconds, step launches) of the last solve."""
+        ms = C.c_double()
+        steps = C.c_int()
+        check(lib().olfhip_context_timing(self.h, C.byref(ms), C.byref(steps)))
+        return ms.value, steps.value
+
+    def read_outer_loop_state(self, k, n_entries):
+        """(row modes [k, n], targets [k, n], reactive-limit types [k, n_entries]) left by the last solve."""
+        n = self.structure.n
+        m = np.zeros((k, n), np.int32)
+        t = np.zeros((k, n))
+        lt = np.zeros((k, max(n_entries, 1)), np.int8)
+        check(lib().olfhip_read_outer_loop_state(self.h, k, ptr(m), ptr(t), ptr(lt)))
+        return m, t, lt[:, :n_entries]
 
     def read_states(self, k):
         x = np.zeros((k, self.structure.n))

@@ -119,6 +119,8 @@ class AcSolverResult:
     norm: float
     x: np.ndarray
     row_mode: np.ndarray = field(default=None, repr=False)
+    target: np.ndarray = field(default=None, repr=False)
+    reactive_limit_types: np.ndarray = field(default=None, repr=False)
 
 
 class GpuNewtonRaphson:
@@ -143,8 +145,11 @@ class GpuNewtonRaphson:
             max_outer_iterations=nr.maxOuterLoopIterations, max_pq_pv_switch=nr.maxPqPvSwitch,
             skip_reactive_limits=not nr.reactiveLimits)
         r = self.context.solve(inp)
+        n_rl = 0 if self.reactive_limits is None else len(self.reactive_limits["voltage_row"])
+        m, t, lt = self.context.read_outer_loop_state(1, n_rl)
         return AcSolverResult(N.STATUS_NAMES[int(r["status"][0])], int(r["iterations"][0]),
-                              int(r["total_iterations"][0]), float(r["norm"][0]), r["x"][0].copy())
+                              int(r["total_iterations"][0]), float(r["norm"][0]), r["x"][0].copy(), m[0], t[0],
+                              lt[0])
 
 
 @dataclass

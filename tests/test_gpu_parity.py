@@ -69,11 +69,16 @@ def test_standalone_lu_matches_oracle_solve(grid, name, mode):
         ref[l] = po.lu_solve(osys.rowptr, osys.colidx, J, rhs[l])
     sol, clamps = st.lu_factorize_and_solve(vals, rhs, mode=mode)
     assert (clamps == 0).all()
-    rel = np.abs(sol - ref).max(axis=1) / np.abs(ref).max(axis=1)
-    assert rel.max() < (1e-9 if mode == N.LU_FP64 else 1e-3), rel.max()
-    # and A x = b directly on one lane
-    A = sp.csr_matrix((vals[0], osys.colidx, osys.rowptr), shape=(net.n, net.n))
-    assert np.abs(A @ sol[0] - rhs[0]).max() < (1e-8 if mode == N.LU_FP64 else 1e-2)
+    if mode == N.LU_FP64:
+        rel = np.abs(sol - ref).max(axis=1) / np.abs(ref).max(axis=1)
+        assert rel.max() < 1e-9, rel.max()
+    # normwise backward error ||A x - b|| / (||A|| ||x|| + ||b||): the property a stable factorization guarantees at
+    # its working precision (forward error also scales with the condition number)
+    for l in range(lanes):
+        A = sp.csr_matrix((vals[l], osys.colidx, osys.rowptr), shape=(net.n, net.n))
+        anorm = np.abs(A).sum(axis=1).max()
+        berr = np.abs(A @ sol[l] - rhs[l]).max() / (anorm * np.abs(sol[l]).max() + np.abs(rhs[l]).max())
+        assert berr < (1e-13 if mode == N.LU_FP64 else 1e-5), (l, berr)
 
 
 def test_flat_start_118_matches_oracle(grid):
