@@ -67,6 +67,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--lanes-per-tile", type=int, default=0)
     ap.add_argument("--threads-per-tile", type=int, default=0)
+    ap.add_argument("--autotune", action="store_true", help="time every tile shape first and use the fastest")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: the N-1 batch shards by contingency across GPUs "
@@ -101,6 +102,12 @@ def main():
         K, xb, tb, mb, row_mode_entries=rme, target_overrides=(tix, tvl), outaged_branches=ctg[:, None],
         reactive_limits_seed_state=np.tile(lt, (K, 1)), max_iter=nrp.maxIterations, tol=nrp.maxMismatch,
         reactive_limit_tol=nrp.maxReactivePowerMismatch, states_to_host=False)
+    tuned = None
+    if args.autotune:
+        tuned = N.autotune(structure, inputs, N.LU_FP32_REFINED, rl)
+        print(json.dumps({"autotune": tuned}), file=sys.stderr, flush=True)
+        sa.lanes_per_tile = tuned["lanes_per_tile"]
+        sa.threads_per_tile = tuned["threads_per_tile"]
     ctx = sa.context(K)
     setup_s = time.perf_counter() - t_setup
 
@@ -173,7 +180,7 @@ def main():
         "config": {"workload": "grid9241 x1.1 load, 2000 single-branch outages (5% bridges, seed 2), FP32 block LU + "
                                "1 FP64 refinement, reactive limits on, base-case start (no learned warm start yet)",
                    "buses": net.n_bus, "branches": net.n_branch, "contingencies": K, "tol_pu": nrp.maxMismatch,
-                   "lanes_per_tile": ctx_lanes(ctx), "lu_levels": stats["levels"], "lu_nnz": stats["lu_nnz"]},
+                   "tile": ctx_lanes(ctx), "lu_levels": stats["levels"], "lu_nnz": stats["lu_nnz"]},
         "statuses": {N.STATUS_NAMES[int(s)]: int((status == s).sum()) for s in np.unique(status)},
         "newton_lane_iterations_per_step": lane_iters,
         "plain_fp64_gpu": {"value": round(plain_value, 2), "unit": "contingencies/s", "sample": len(sample),
@@ -187,7 +194,8 @@ def main():
 
 
 def ctx_lanes(ctx):
-    return ctx._tuning.lanes_per_tile or "auto"
+    return {"lanes_per_tile": ctx._tuning.lanes_per_tile or "default", "threads_per_tile":
+            ctx._tuning.threads_per_tile or "default"}
 
 
 def cpu_baseline(net, xb, mb, tb, lt, sample):

@@ -93,16 +93,10 @@ struct StructureImpl {
         dp.nSlots = P.nSlots;
         dp.nLevels = P.nLevels;
         dp.levelNodePtr = put(P.levelNodePtr);
-        dp.levelNodes = put(P.levelNodes);
-        dp.diagContribPtr = put(P.diagContribPtr);
-        dp.diagContribA = put(P.diagContribA);
-        dp.diagContribB = put(P.diagContribB);
-        dp.levelPairPtr = put(P.levelPairPtr);
-        dp.pairSlot = put(P.pairSlot);
-        dp.pairNode = put(P.pairNode);
-        dp.pairContribPtr = put(P.pairContribPtr);
-        dp.pairContribA = put(P.pairContribA);
-        dp.pairContribB = put(P.pairContribB);
+        dp.levelItemPtr = put(P.levelItemPtr);
+        dp.itemRec = reinterpret_cast<const int4 *>(put(P.itemRec));
+        dp.contribRec = reinterpret_cast<const int4 *>(put(P.contribRec));
+        dp.bwdRec = reinterpret_cast<const int4 *>(put(P.bwdRec));
         dp.nodeBase = put(P.nodeBase);
         dp.sPtr = put(P.sPtr);
         dp.sIdx = put(P.sIdx);
@@ -168,6 +162,8 @@ struct ContextImpl {
     int lastSteps = 0;
     double lastStepMs = 0.0;                 // event-timed Newton-step launches of the last solve
     hipEvent_t evStart = nullptr, evStop = nullptr;
+    DBuf prof;                               // opt-in section stamps of tile 0 (OLFHIP_PROFILE=1)
+    int profCap = 0;
 
     DevState devState() const {
         DevState S{};
@@ -193,6 +189,8 @@ struct ContextImpl {
         S.ratio = ratio.as<double>();
         S.outages = outages.as<int>();
         S.maxOut = MAX_OUTAGES;
+        S.prof = prof.as<unsigned long long>();
+        S.profCap = profCap;
         S.rlType = rlType.as<signed char>();
         S.rlCount = rlCount.as<signed char>();
         S.rlPass = rlPass.as<signed char>();
@@ -353,11 +351,21 @@ static void solve(ContextImpl *c, const olfhip_solve_inputs *in, olfhip_solve_ou
     prm.maxDphi = in->max_dphi;
     prm.refine = c->mode == OLFHIP_LU_FP32_REFINED;
     prm.batchOfOne = K == 1;
+    const int maxSteps = (prm.maxIter + 1) * (prm.maxOuter + 1) + 1;
+    const bool profile = std::getenv("OLFHIP_PROFILE") && std::strcmp(std::getenv("OLFHIP_PROFILE"), "0") != 0;
+    if (profile) {
+        c->profCap = maxSteps;
+        const size_t bytes = (size_t)PROF_SECTIONS * (maxSteps + 1) * 8;
+        c->prof.alloc(bytes);
+        HIP_OK(hipMemsetAsync(c->prof.p, 0, bytes, s));   // before any capture: no memset node in the graph
+    } else {
+        c->profCap = 0;
+        c->prof.release();
+    }
     const DevPlan dp = c->st->dp;
     const DevState S = c->devState();
     const DevRL R = c->devRL();
     const bool useGraph = !std::getenv("OLFHIP_NO_GRAPH");
-    const int maxSteps = (prm.maxIter + 1) * (prm.maxOuter + 1) + 1;
     int steps = 0;
     withT(c->T, [&](auto TC) {
         constexpr int T = decltype(TC)::value;
@@ -409,6 +417,27 @@ static void solve(ContextImpl *c, const olfhip_solve_inputs *in, olfhip_solve_ou
         if (graph) (void)hipGraphDestroy(graph);
     });
     HIP_OK(hipGetLastError());
+    if (profile) {
+        std::vector<unsigned long long> h((size_t)PROF_SECTIONS * (maxSteps + 1));
+        HIP_OK(hipMemcpy(h.data(), c->prof.p, h.size() * 8, hipMemcpyDeviceToHost));
+        int rateKHz = 0;
+        HIP_OK(hipDeviceGetAttribute(&rateKHz, hipDeviceAttributeWallClockRate, c->st->device));
+        const double usPerTick = rateKHz > 0 ? 1e3 / rateKHz : 0.0;
+        const char *names[9] = {"start", "fill", "test+outer", "factor+fwd", "backward", "refine-J", "refine-solve",
+                                "guard", "update"};
+        const unsigned long long launches = std::min<unsigned long long>(h[0], (unsigned long long)maxSteps);
+        for (unsigned long long k = 0; k < launches; k++) {
+            const unsigned long long *r = h.data() + PROF_SECTIONS * (k + 1);
+            std::fprintf(stderr, "OLFHIP_PROFILE T=%d NT=%d launch %llu:", c->T, c->NT, k);
+            unsigned long long prev = r[0];
+            for (int i = 1; i <= 8; i++) {
+                if (r[i] == 0) continue;
+                std::fprintf(stderr, " %s=%.1fus", i < 9 ? names[i] : "end", (r[i] - prev) * usPerTick);
+                prev = r[i];
+            }
+            std::fprintf(stderr, " total=%.1fus\n", (prev - r[0]) * usPerTick);
+        }
+    }
     // results
     std::vector<int> st(K), itv(K), tot(K);
     std::vector<double> nm(K);

@@ -28,9 +28,8 @@ constexpr double LIN_SOLVE_FAIL_RATIO = 1e-2;   // reference native/cuda/solver/
 
 struct DevPlan {
     int nb, nSlots, nLevels;
-    const int *levelNodePtr, *levelNodes;
-    const int *diagContribPtr, *diagContribA, *diagContribB;
-    const int *levelPairPtr, *pairSlot, *pairNode, *pairContribPtr, *pairContribA, *pairContribB;
+    const int *levelNodePtr, *levelItemPtr;
+    const int4 *itemRec, *contribRec, *bwdRec;
     const int *nodeBase, *sPtr, *sIdx;
     const int *hbPtr, *hbBranch;
     const double4 *hbCoef;
@@ -51,7 +50,12 @@ struct DevState {
     int maxOut;
     signed char *rlType, *rlCount, *rlPass;
     int *tileRunning;
+    // opt-in section timing of one tile: prof[0] counts the launches that recorded, then PROF_SECTIONS stamps per
+    // launch (wall clock) from prof[PROF_SECTIONS] on, at most profCap launches
+    unsigned long long *prof;
+    int profCap;
 };
+constexpr int PROF_SECTIONS = 16;
 
 struct DevRL {
     int n;
@@ -288,116 +292,127 @@ __device__ __forceinline__ void laneReduceAnd(int *red, int tid, int nt) {
     __syncthreads();
 }
 
-// The factorization, left-looking by elimination levels, with the forward solve of b = sign * rhs fused into the
-// diagonal items. Ends with a barrier.
+// The factorization, left-looking by elimination levels, one phase per level, with the forward solve of
+// b = sign * rhs fused into the diagonal items. Stores D(k)^-1 in the diagonal slot, the unscaled L~(i,k) and U(k,i)
+// in the pair slots, and z(k) = D(k)^-1 y(k) (y = L^-1 b) in z. Counts clamped pivots per lane into clampLane
+// (shared, may be null). Ends with a barrier.
 template <int T, typename VT>
-__device__ void factorForward(const DevPlan &P, int tile, int tid, int nt, VT *__restrict__ lu, VT *__restrict__ y,
-                              const double *__restrict__ rhs, double sign) {
+__device__ void factorForward(const DevPlan &P, int tile, int tid, int nt, VT *__restrict__ lu, VT *__restrict__ z,
+                              const double *__restrict__ rhs, double sign, int *clampLane) {
     const int nb = P.nb;
     for (int l = 0; l < P.nLevels; l++) {
-        const int n0 = P.levelNodePtr[l], n1 = P.levelNodePtr[l + 1];
-        const int p0 = P.levelPairPtr[l], p1 = P.levelPairPtr[l + 1];
-        const int nNodes = n1 - n0;
-        const int units = nNodes + (p1 - p0);
-        for (int w = tid; w < units * T; w += nt) {
-            const int u = w / T, t = w % T;
-            if (u < nNodes) {
-                const int k = P.levelNodes[n0 + u];
-                VT *dslot = lu + bIdx<T>(tile, P.nSlots, P.nodeBase[k], t);
-                Blk<VT> D = ldBlk<VT>(dslot);
-                const double2 b = *reinterpret_cast<const double2 *>(rhs + vIdx<T>(tile, nb, k, t));
+        const int i0 = P.levelItemPtr[l], i1 = P.levelItemPtr[l + 1];
+        for (int w = tid; w < (i1 - i0) * T; w += nt) {
+            const int t = w % T;
+            const int4 it = P.itemRec[i0 + w / T];
+            VT *own = lu + bIdx<T>(tile, P.nSlots, it.x, t);
+            if (it.w >= 0) {
+                // diagonal item of node it.w: pivot block, its inverse, and the forward-solve entry
+                Blk<VT> D = ldBlk<VT>(own);
+                const double2 b = *reinterpret_cast<const double2 *>(rhs + vIdx<T>(tile, nb, it.w, t));
                 V2<VT> yk = {(VT)(sign * b.x), (VT)(sign * b.y)};
-                const int c1 = P.diagContribPtr[k + 1];
-                for (int c = P.diagContribPtr[k]; c < c1; c++) {
-                    const int a = P.diagContribA[c];
-                    const int j = P.diagContribB[c];
-                    const Blk<VT> Lkj = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, a, t));
-                    const Blk<VT> Ujk = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, a + 1, t));
-                    const V2<VT> yj = ldV<VT>(y + vIdx<T>(tile, nb, j, t));
-                    mulSub(D, Lkj, Ujk);
-                    mulSubV(yk, Lkj, yj);
+                for (int c = it.y; c < it.z; c++) {
+                    const int4 cr = P.contribRec[c];
+                    const Blk<VT> Lkj = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, cr.x, t));
+                    const Blk<VT> Ujk = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, cr.x + 1, t));
+                    const Blk<VT> Dj = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, cr.z, t));
+                    const V2<VT> zj = ldV<VT>(z + vIdx<T>(tile, nb, cr.y, t));
+                    mulSub(D, Lkj, mul(Dj, Ujk));
+                    mulSubV(yk, Lkj, zj);
                 }
-                stBlk<VT>(dslot, D);
-                stV<VT>(y + vIdx<T>(tile, nb, k, t), yk);
+                bool cl;
+                const Blk<VT> Dinv = inv2(D, cl);
+                if (cl && clampLane) atomicAdd(&clampLane[t], 1);
+                stBlk<VT>(own, Dinv);
+                const V2<VT> zk = {Dinv.a * yk.p + Dinv.b * yk.q, Dinv.c * yk.p + Dinv.d * yk.q};
+                stV<VT>(z + vIdx<T>(tile, nb, it.w, t), zk);
             } else {
-                const int q = p0 + (u - nNodes);
-                const int slot = P.pairSlot[q];
-                VT *lslot = lu + bIdx<T>(tile, P.nSlots, slot, t);
-                Blk<VT> L = ldBlk<VT>(lslot);
-                Blk<VT> U = ldBlk<VT>(lslot + 4 * T);
-                const int c1 = P.pairContribPtr[q + 1];
-                for (int c = P.pairContribPtr[q]; c < c1; c++) {
-                    const int a = P.pairContribA[c];
-                    const int b = P.pairContribB[c];
-                    const Blk<VT> Lkj = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, a, t));
-                    const Blk<VT> Ujk = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, a + 1, t));
-                    const Blk<VT> Lij = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, b, t));
-                    const Blk<VT> Uji = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, b + 1, t));
-                    mulSub(L, Lij, Ujk);
-                    mulSub(U, Lkj, Uji);
+                // pair item: L~(i,k) at it.x, U(k,i) right after
+                Blk<VT> L = ldBlk<VT>(own);
+                Blk<VT> U = ldBlk<VT>(own + 4 * T);
+                int c = it.y;
+                for (; c + 1 < it.z; c += 2) {
+                    const int4 c0 = P.contribRec[c];
+                    const int4 c1 = P.contribRec[c + 1];
+                    const Blk<VT> Lkj0 = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, c0.x, t));
+                    const Blk<VT> Ujk0 = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, c0.x + 1, t));
+                    const Blk<VT> Lij0 = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, c0.y, t));
+                    const Blk<VT> Uji0 = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, c0.y + 1, t));
+                    const Blk<VT> Dj0 = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, c0.z, t));
+                    const Blk<VT> Lkj1 = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, c1.x, t));
+                    const Blk<VT> Ujk1 = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, c1.x + 1, t));
+                    const Blk<VT> Lij1 = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, c1.y, t));
+                    const Blk<VT> Uji1 = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, c1.y + 1, t));
+                    const Blk<VT> Dj1 = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, c1.z, t));
+                    mulSub(L, Lij0, mul(Dj0, Ujk0));
+                    mulSub(U, Lkj0, mul(Dj0, Uji0));
+                    mulSub(L, Lij1, mul(Dj1, Ujk1));
+                    mulSub(U, Lkj1, mul(Dj1, Uji1));
                 }
-                stBlk<VT>(lslot, L);
-                stBlk<VT>(lslot + 4 * T, U);
+                if (c < it.z) {
+                    const int4 c0 = P.contribRec[c];
+                    const Blk<VT> Lkj = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, c0.x, t));
+                    const Blk<VT> Ujk = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, c0.x + 1, t));
+                    const Blk<VT> Lij = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, c0.y, t));
+                    const Blk<VT> Uji = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, c0.y + 1, t));
+                    const Blk<VT> Dj = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, c0.z, t));
+                    mulSub(L, Lij, mul(Dj, Ujk));
+                    mulSub(U, Lkj, mul(Dj, Uji));
+                }
+                stBlk<VT>(own, L);
+                stBlk<VT>(own + 4 * T, U);
             }
-        }
-        __syncthreads();
-        for (int w = tid; w < (p1 - p0) * T; w += nt) {
-            const int q = p0 + w / T, t = w % T;
-            const int k = P.pairNode[q];
-            bool cl;
-            const Blk<VT> Dinv = inv2(ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, P.nodeBase[k], t)), cl);
-            VT *lslot = lu + bIdx<T>(tile, P.nSlots, P.pairSlot[q], t);
-            stBlk<VT>(lslot, mul(ldBlk<VT>(lslot), Dinv));
         }
         __syncthreads();
     }
 }
 
-// Forward solve with existing factors: y = L^-1 (sign * rhs). Ends with a barrier.
+// Forward solve with existing factors: z = D^-1-scaled L^-1 (sign * rhs), z(k) = D(k)^-1 (b(k) - sum L~(k,j) z(j)).
+// Ends with a barrier.
 template <int T, typename VT>
 __device__ void forwardOnly(const DevPlan &P, int tile, int tid, int nt, const VT *__restrict__ lu,
-                            VT *__restrict__ y, const double *__restrict__ rhs, double sign) {
+                            VT *__restrict__ z, const double *__restrict__ rhs, double sign) {
     const int nb = P.nb;
     for (int l = 0; l < P.nLevels; l++) {
-        const int n0 = P.levelNodePtr[l], n1 = P.levelNodePtr[l + 1];
-        for (int w = tid; w < (n1 - n0) * T; w += nt) {
-            const int k = P.levelNodes[n0 + w / T], t = w % T;
-            const double2 b = *reinterpret_cast<const double2 *>(rhs + vIdx<T>(tile, nb, k, t));
+        const int i0 = P.levelItemPtr[l];
+        const int nNodes = P.levelNodePtr[l + 1] - P.levelNodePtr[l];   // diagonal items come first
+        for (int w = tid; w < nNodes * T; w += nt) {
+            const int t = w % T;
+            const int4 it = P.itemRec[i0 + w / T];
+            const double2 b = *reinterpret_cast<const double2 *>(rhs + vIdx<T>(tile, nb, it.w, t));
             V2<VT> yk = {(VT)(sign * b.x), (VT)(sign * b.y)};
-            const int c1 = P.diagContribPtr[k + 1];
-            for (int c = P.diagContribPtr[k]; c < c1; c++) {
-                const Blk<VT> Lkj = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, P.diagContribA[c], t));
-                mulSubV(yk, Lkj, ldV<VT>(y + vIdx<T>(tile, nb, P.diagContribB[c], t)));
+            for (int c = it.y; c < it.z; c++) {
+                const int4 cr = P.contribRec[c];
+                const Blk<VT> Lkj = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, cr.x, t));
+                mulSubV(yk, Lkj, ldV<VT>(z + vIdx<T>(tile, nb, cr.y, t)));
             }
-            stV<VT>(y + vIdx<T>(tile, nb, k, t), yk);
+            const Blk<VT> Dinv = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, it.x, t));
+            const V2<VT> zk = {Dinv.a * yk.p + Dinv.b * yk.q, Dinv.c * yk.p + Dinv.d * yk.q};
+            stV<VT>(z + vIdx<T>(tile, nb, it.w, t), zk);
         }
         __syncthreads();
     }
 }
 
-// Backward solve: x(k) = D(k)^-1 (y(k) - sum U(k,i) x(i)), levels from the root down. Counts clamped pivots per lane
-// into clampLane (shared). Ends with a barrier.
+// Backward solve: x(k) = z(k) - D(k)^-1 sum U(k,i) x(i), levels from the root down. Ends with a barrier.
 template <int T, typename VT>
 __device__ void backward(const DevPlan &P, int tile, int tid, int nt, const VT *__restrict__ lu,
-                         const VT *__restrict__ y, VT *__restrict__ xo, int *clampLane) {
+                         const VT *__restrict__ z, VT *__restrict__ xo) {
     const int nb = P.nb;
     for (int l = P.nLevels - 1; l >= 0; l--) {
         const int n0 = P.levelNodePtr[l], n1 = P.levelNodePtr[l + 1];
         for (int w = tid; w < (n1 - n0) * T; w += nt) {
-            const int k = P.levelNodes[n0 + w / T], t = w % T;
-            V2<VT> yk = ldV<VT>(y + vIdx<T>(tile, nb, k, t));
-            const int base = P.nodeBase[k];
-            const int e0 = P.sPtr[k], e1 = P.sPtr[k + 1];
-            for (int e = e0; e < e1; e++) {
-                const int i = P.sIdx[e];
-                const Blk<VT> U = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, base + 2 + 2 * (e - e0), t));
-                mulSubV(yk, U, ldV<VT>(xo + vIdx<T>(tile, nb, i, t)));
+            const int t = w % T;
+            const int4 br = P.bwdRec[n0 + w / T];
+            V2<VT> s = {(VT)0, (VT)0};
+            for (int e = br.y; e < br.z; e++) {
+                const Blk<VT> U = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, br.x + 2 + 2 * (e - br.y), t));
+                mulSubV(s, U, ldV<VT>(xo + vIdx<T>(tile, nb, P.sIdx[e], t)));
             }
-            bool cl;
-            const Blk<VT> Dinv = inv2(ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, base, t)), cl);
-            if (cl && clampLane) atomicAdd(&clampLane[t], 1);
-            V2<VT> r = {Dinv.a * yk.p + Dinv.b * yk.q, Dinv.c * yk.p + Dinv.d * yk.q};
-            stV<VT>(xo + vIdx<T>(tile, nb, k, t), r);
+            const Blk<VT> Dinv = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, br.x, t));
+            const V2<VT> zk = ldV<VT>(z + vIdx<T>(tile, nb, br.w, t));
+            const V2<VT> r = {zk.p + Dinv.a * s.p + Dinv.b * s.q, zk.q + Dinv.c * s.p + Dinv.d * s.q};
+            stV<VT>(xo + vIdx<T>(tile, nb, br.w, t), r);
         }
         __syncthreads();
     }
@@ -410,12 +425,12 @@ __device__ void fillPass(const DevPlan &P, const DevState &S, int tile, int tid,
                          double &pmax, int &pfin) {
     const int nb = P.nb;
     VT *lu = reinterpret_cast<VT *>(S.lu);
+    const int t = tid % T;   // fixed per thread: the stride nt is a multiple of T
+    if (!fillLane[t]) return;
+    int o[MAX_OUTAGES] = {-1, -1, -1, -1};
+    for (int q = 0; q < S.maxOut && q < MAX_OUTAGES; q++) o[q] = S.outages[(size_t)(tile * T + t) * S.maxOut + q];
     for (int w = tid; w < nb * T; w += nt) {
-        const int k = w / T, t = w % T;
-        if (!fillLane[t]) continue;
-        const int lane = tile * T + t;
-        int o[MAX_OUTAGES] = {-1, -1, -1, -1};
-        for (int q = 0; q < S.maxOut && q < MAX_OUTAGES; q++) o[q] = S.outages[(size_t)lane * S.maxOut + q];
+        const int k = w / T;
         const size_t vi = vIdx<T>(tile, nb, k, t);
         const signed char mP = S.mode[vi], mQ = S.mode[vi + 1];
         double sP, sQ, jP, jQ;
@@ -436,12 +451,12 @@ template <int T, typename VT>
 __device__ void applyJPass(const DevPlan &P, const DevState &S, int tile, int tid, int nt, const int *laneOn,
                            const double *v, const double *add, double addSign, double *out, double &pmax) {
     const int nb = P.nb;
+    const int t = tid % T;   // fixed per thread: the stride nt is a multiple of T
+    if (!laneOn[t]) return;
+    int o[MAX_OUTAGES] = {-1, -1, -1, -1};
+    for (int q = 0; q < S.maxOut && q < MAX_OUTAGES; q++) o[q] = S.outages[(size_t)(tile * T + t) * S.maxOut + q];
     for (int w = tid; w < nb * T; w += nt) {
-        const int k = w / T, t = w % T;
-        if (!laneOn[t]) continue;
-        const int lane = tile * T + t;
-        int o[MAX_OUTAGES] = {-1, -1, -1, -1};
-        for (int q = 0; q < S.maxOut && q < MAX_OUTAGES; q++) o[q] = S.outages[(size_t)lane * S.maxOut + q];
+        const int k = w / T;
         const size_t vi = vIdx<T>(tile, nb, k, t);
         double sP, sQ, jP, jQ;
         nodeTerms<T, VT, false, true>(P, tile, k, t, S.x, o, nullptr, v, S.mode[vi], S.mode[vi + 1], sP, sQ, jP, jQ);
@@ -469,12 +484,25 @@ __global__ __launch_bounds__(MAX_THREADS) void nrStepKernel(DevPlan P, DevState 
     __shared__ int sFlag[4];
     __shared__ double red[MAX_THREADS];
     __shared__ int redI[MAX_THREADS];
+    __shared__ unsigned long long *sProf;
 
     const int nb = P.nb;
     VT *lu = reinterpret_cast<VT *>(S.lu);
     VT *y = reinterpret_cast<VT *>(S.y);
     VT *dx1 = reinterpret_cast<VT *>(S.dx1);
 
+    if (tid == 0) {
+        sProf = nullptr;
+        if (S.prof && tile == 0) {
+            const unsigned long long k = atomicAdd(S.prof, 1ull);
+            if (k < (unsigned long long)S.profCap) sProf = S.prof + PROF_SECTIONS * (k + 1);
+        }
+        if (sProf) sProf[0] = wall_clock64();
+    }
+#define OLFHIP_MARK(i)                                 \
+    do {                                               \
+        if (tid == 0 && sProf) sProf[i] = wall_clock64(); \
+    } while (0)
     if (tid < T) {
         const int lane = tile * T + tid;
         sStatus[tid] = S.status[lane];
@@ -504,6 +532,7 @@ __global__ __launch_bounds__(MAX_THREADS) void nrStepKernel(DevPlan P, DevState 
         laneReduce<T, true>(red, tid, nt);
         laneReduceAnd<T>(redI, tid, nt);
     }
+    OLFHIP_MARK(1);
     // ---- 2. the test of each running lane at its own iteration
     if (tid < T && sFill[tid]) {
         const double norm = red[tid];
@@ -644,8 +673,11 @@ __global__ __launch_bounds__(MAX_THREADS) void nrStepKernel(DevPlan P, DevState 
 
     if (sFlag[2]) {
         // ---- 4. factorization with the forward solve of -F, then the backward solve
-        factorForward<T, VT>(P, tile, tid, nt, lu, y, S.F, -1.0);
-        backward<T, VT>(P, tile, tid, nt, lu, y, dx1, sClamp);
+        OLFHIP_MARK(2);
+        factorForward<T, VT>(P, tile, tid, nt, lu, y, S.F, -1.0, sClamp);
+        OLFHIP_MARK(3);
+        backward<T, VT>(P, tile, tid, nt, lu, y, dx1);
+        OLFHIP_MARK(4);
         // ---- 5. step in FP64 (+ one FP64-residual refinement step on the FP32 factors)
         for (int w = tid; w < nb * T; w += nt) {
             const int k = w / T, t = w % T;
@@ -655,18 +687,13 @@ __global__ __launch_bounds__(MAX_THREADS) void nrStepKernel(DevPlan P, DevState 
         __syncthreads();
         if (prm.refine) {
             double pm = 0.0;
-            // r = -F - J dx
+            // FP64 residual r = J dx + F, solved for the correction with sign -1: J dx2 = -r
             applyJPass<T, VT>(P, S, tile, tid, nt, sSolve, S.dx, S.F, 1.0, S.r, pm);
             __syncthreads();
-            for (int w = tid; w < nb * T; w += nt) {   // negate: r = -(J dx + F)
-                const int k = w / T, t = w % T;
-                double2 *rp = reinterpret_cast<double2 *>(S.r + vIdx<T>(tile, nb, k, t));
-                const double2 rv = *rp;
-                *rp = make_double2(-rv.x, -rv.y);
-            }
-            __syncthreads();
-            forwardOnly<T, VT>(P, tile, tid, nt, lu, y, S.r, 1.0);
-            backward<T, VT>(P, tile, tid, nt, lu, y, dx1, nullptr);
+            OLFHIP_MARK(5);
+            forwardOnly<T, VT>(P, tile, tid, nt, lu, y, S.r, -1.0);
+            backward<T, VT>(P, tile, tid, nt, lu, y, dx1);
+            OLFHIP_MARK(6);
             for (int w = tid; w < nb * T; w += nt) {
                 const int k = w / T, t = w % T;
                 const V2<VT> d = ldV<VT>(dx1 + vIdx<T>(tile, nb, k, t));
@@ -700,6 +727,7 @@ __global__ __launch_bounds__(MAX_THREADS) void nrStepKernel(DevPlan P, DevState 
             }
             __syncthreads();
         }
+        OLFHIP_MARK(7);
         // ---- 7. step scaling (MAX_VOLTAGE_CHANGE) and update
         if (prm.scaling == 2) {
             double mdv = 0.0, mdp = 0.0;
@@ -734,6 +762,8 @@ __global__ __launch_bounds__(MAX_THREADS) void nrStepKernel(DevPlan P, DevState 
         }
     }
     __syncthreads();
+    OLFHIP_MARK(8);
+#undef OLFHIP_MARK
     (void)myLane;
     if (tid < T) {
         const int lane = tile * T + tid;
@@ -903,8 +933,8 @@ __global__ __launch_bounds__(MAX_THREADS) void luSolveKernel(DevPlan P, VT *lu, 
     const int tile = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
     if (tid < T) sClamp[tid] = 0;
     __syncthreads();
-    factorForward<T, VT>(P, tile, tid, nt, lu, y, rhs, 1.0);
-    backward<T, VT>(P, tile, tid, nt, lu, y, xo, sClamp);
+    factorForward<T, VT>(P, tile, tid, nt, lu, y, rhs, 1.0, sClamp);
+    backward<T, VT>(P, tile, tid, nt, lu, y, xo);
     if (tid < T) clamps[tile * T + tid] = sClamp[tid];
 }
 

@@ -170,39 +170,51 @@ Plan buildPlan(int n, int nBranch, const double *p8, const int *idx12, int nShun
             }
         }
     }
-    P.diagContribPtr.assign(nb + 1, 0);
-    for (int k = 0; k < nb; k++) {
-        for (auto &c : diagC[k]) {
-            P.diagContribA.push_back(c.first);
-            P.diagContribB.push_back(c.second);
-        }
-        P.diagContribPtr[k + 1] = (int)P.diagContribA.size();
-        P.contributions += (long long)diagC[k].size();
-        P.factorMads += (long long)diagC[k].size();
-    }
+    // level-ordered packed records: a level's diagonal items first (one per node), then its pair items
     std::vector<std::vector<int>> byLevel(P.nLevels);
     for (int k = 0; k < nb; k++) byLevel[level[k]].push_back(k);
     P.levelNodePtr.assign(P.nLevels + 1, 0);
-    P.levelPairPtr.assign(P.nLevels + 1, 0);
-    P.pairContribPtr.push_back(0);
+    P.levelItemPtr.assign(P.nLevels + 1, 0);
+    auto pushContrib = [&](int a, int b, int d) {
+        P.contribRec.push_back(a);
+        P.contribRec.push_back(b);
+        P.contribRec.push_back(d);
+        P.contribRec.push_back(0);
+    };
+    int nItems = 0, nContrib = 0;
     for (int l = 0; l < P.nLevels; l++) {
         for (int k : byLevel[l]) {
             P.levelNodes.push_back(k);
+            const int c0 = nContrib;
+            for (auto &c : diagC[k]) {
+                pushContrib(c.first, c.second, P.nodeBase[c.second]);
+                nContrib++;
+            }
+            P.itemRec.insert(P.itemRec.end(), {P.nodeBase[k], c0, nContrib, k});
+            nItems++;
+            P.bwdRec.insert(P.bwdRec.end(), {P.nodeBase[k], P.sPtr[k], P.sPtr[k + 1], k});
+            P.contributions += (long long)diagC[k].size();
+            P.factorMads += 2LL * (long long)diagC[k].size() + 1;   // W = D^-1 U, D -= L W; the inverse
+        }
+        for (int k : byLevel[l]) {
             for (int e = 0; e < P.sPtr[k + 1] - P.sPtr[k]; e++) {
                 const auto &lst = pairC[pairItemBase[k] + e];
+                const int c0 = nContrib;
                 for (auto &c : lst) {
-                    P.pairContribA.push_back(c.first);
-                    P.pairContribB.push_back(c.second);
+                    // contributing node j owns both slots: recover it from the slot range
+                    const int j = (int)(std::upper_bound(P.nodeBase.begin(), P.nodeBase.end(), c.first) -
+                                        P.nodeBase.begin()) - 1;
+                    pushContrib(c.first, c.second, P.nodeBase[j]);
+                    nContrib++;
                 }
-                P.pairContribPtr.push_back((int)P.pairContribA.size());
+                P.itemRec.insert(P.itemRec.end(), {lslot(k, e), c0, nContrib, -1});
+                nItems++;
                 P.contributions += (long long)lst.size();
-                P.factorMads += 2LL * (long long)lst.size() + 1;  // + the D^-1 scaling
-                P.pairSlot.push_back(lslot(k, e));
-                P.pairNode.push_back(k);
+                P.factorMads += 4LL * (long long)lst.size();   // two W = D^-1 U and two updates
             }
         }
         P.levelNodePtr[l + 1] = (int)P.levelNodes.size();
-        P.levelPairPtr[l + 1] = (int)P.pairSlot.size();
+        P.levelItemPtr[l + 1] = nItems;
     }
 
     // --- fill gather tables

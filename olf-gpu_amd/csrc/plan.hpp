@@ -33,17 +33,20 @@ struct Plan {
     // --- level schedule (elimination-tree height; leaves are level 0)
     int nLevels = 0;
     std::vector<int> levelNodePtr, levelNodes;     // nodes of each level
-    // Phase 1 of a level has two kinds of items, each with a contribution list ascending in the contributing
-    // node j: the diagonal item of node k (D(k) = A(k,k) - sum L(k,j) U(j,k), and the fused forward-solve entry
-    // y(k) = b(k) - sum L(k,j) y(j)), and the pair items (k, i) (L~(i,k) = A(i,k) - sum L(i,j) U(j,k) and
-    // U(k,i) = A(k,i) - sum L(k,j) U(j,i)). Phase 2 scales the level's pairs: L(i,k) = L~(i,k) D(k)^-1.
-    std::vector<int> diagContribPtr;               // per node k (node numbering)
-    std::vector<int> diagContribA;                 // L slot of pair (j, k): L(k,j) there, U(j,k) right after
-    std::vector<int> diagContribB;                 // node j
-    std::vector<int> levelPairPtr, pairSlot, pairNode;
-    std::vector<int> pairContribPtr;
-    std::vector<int> pairContribA;                 // L slot of pair (j, k)
-    std::vector<int> pairContribB;                 // L slot of pair (j, i)
+    // The factors A = L U (L unit block-lower, U with the pivot blocks D(k) on its diagonal) are kept in the form the
+    // kernels use: per node k the inverse D(k)^-1 (in the diagonal slot), per pair the unscaled L~(i,k) = L(i,k) D(k)
+    // and U(k,i). Every
+    // product L(i,j) U(j,k) is then evaluated as L~(i,j) (D(j)^-1 U(j,k)), so a level needs one phase only.
+    // A level's items: the diagonal item of node k (D(k) = A(k,k) - sum L~(k,j) D(j)^-1 U(j,k), inverted, and the
+    // fused forward solve z(k) = D(k)^-1 (b(k) - sum L~(k,j) z(j))), then the pair items (k, i)
+    // (L~(i,k) = A(i,k) - sum L~(i,j) D(j)^-1 U(j,k), U(k,i) = A(k,i) - sum L~(k,j) D(j)^-1 U(j,i)); every
+    // contribution list ascends in the contributing node j.
+    std::vector<int> levelItemPtr;                 // items of level l in [levelItemPtr[l], levelItemPtr[l+1])
+    std::vector<int> itemRec;                      // 4 per item: slot (diag or L), contrib begin, end, node | -1
+    std::vector<int> contribRec;                   // 4 per contribution: diag item: L slot of pair (j,k), j,
+                                                   // diag slot of j, 0; pair item: L slot of (j,k), L slot of
+                                                   // (j,i), diag slot of j, 0 (U(j,x) sits right after L(x,j))
+    std::vector<int> bwdRec;                       // 4 per node in level order: diag slot, S begin, S end, node
 
     // --- fill gather tables, per node, half-branches grouped by the far node (ascending)
     std::vector<int> hbPtr;                        // half-branches of node k
