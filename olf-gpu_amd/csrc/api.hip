@@ -97,6 +97,8 @@ struct StructureImpl {
         dp.itemRec = reinterpret_cast<const int4 *>(put(P.itemRec));
         dp.contribRec = reinterpret_cast<const int4 *>(put(P.contribRec));
         dp.bwdRec = reinterpret_cast<const int4 *>(put(P.bwdRec));
+        dp.levelSplit = put(P.levelSplit);
+        dp.levelSplitBwd = put(P.levelSplitBwd);
         dp.nodeBase = put(P.nodeBase);
         dp.sPtr = put(P.sPtr);
         dp.sIdx = put(P.sIdx);

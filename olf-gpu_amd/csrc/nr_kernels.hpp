@@ -30,6 +30,7 @@ struct DevPlan {
     int nb, nSlots, nLevels;
     const int *levelNodePtr, *levelItemPtr;
     const int4 *itemRec, *contribRec, *bwdRec;
+    const int *levelSplit, *levelSplitBwd;
     const int *nodeBase, *sPtr, *sIdx;
     const int *hbPtr, *hbBranch;
     const double4 *hbCoef;
@@ -292,26 +293,49 @@ __device__ __forceinline__ void laneReduceAnd(int *red, int tid, int nt) {
     __syncthreads();
 }
 
+// Cross-lane sum over the S threads that share an item (lanes t, t + T, ..., t + (S-1) T of one wave).
+template <int T, typename VT>
+__device__ __forceinline__ VT groupSum(VT v, int S) {
+    for (int off = T; off < T * S; off <<= 1) v += __shfl_xor(v, off);
+    return v;
+}
+template <int T, typename VT>
+__device__ __forceinline__ void groupSumBlk(Blk<VT> &b, int S) {
+    b.a = groupSum<T>(b.a, S);
+    b.b = groupSum<T>(b.b, S);
+    b.c = groupSum<T>(b.c, S);
+    b.d = groupSum<T>(b.d, S);
+}
+
+// Threads per item at level l: the plan's split, capped so that an item's threads stay in one wave.
+template <int T>
+__device__ __forceinline__ int splitOf(const int *levelSplit, int l) {
+    const int s = levelSplit[l];
+    return s < 64 / T ? s : 64 / T;
+}
+
 // The factorization, left-looking by elimination levels, one phase per level, with the forward solve of
 // b = sign * rhs fused into the diagonal items. Stores D(k)^-1 in the diagonal slot, the unscaled L~(i,k) and U(k,i)
-// in the pair slots, and z(k) = D(k)^-1 y(k) (y = L^-1 b) in z. Counts clamped pivots per lane into clampLane
-// (shared, may be null). Ends with a barrier.
+// in the pair slots, and z(k) = D(k)^-1 y(k) (y = L^-1 b) in z. An item's contribution list is summed by S threads
+// (stride S) and reduced across lanes. Counts clamped pivots per lane into clampLane (shared, may be null). Ends with
+// a barrier.
 template <int T, typename VT>
 __device__ void factorForward(const DevPlan &P, int tile, int tid, int nt, VT *__restrict__ lu, VT *__restrict__ z,
                               const double *__restrict__ rhs, double sign, int *clampLane) {
     const int nb = P.nb;
     for (int l = 0; l < P.nLevels; l++) {
         const int i0 = P.levelItemPtr[l], i1 = P.levelItemPtr[l + 1];
-        for (int w = tid; w < (i1 - i0) * T; w += nt) {
+        const int S = splitOf<T>(P.levelSplit, l);
+        for (int w = tid; w < (i1 - i0) * T * S; w += nt) {
             const int t = w % T;
-            const int4 it = P.itemRec[i0 + w / T];
+            const int s = (w / T) % S;
+            const int4 it = P.itemRec[i0 + w / (T * S)];
             VT *own = lu + bIdx<T>(tile, P.nSlots, it.x, t);
             if (it.w >= 0) {
                 // diagonal item of node it.w: pivot block, its inverse, and the forward-solve entry
-                Blk<VT> D = ldBlk<VT>(own);
-                const double2 b = *reinterpret_cast<const double2 *>(rhs + vIdx<T>(tile, nb, it.w, t));
-                V2<VT> yk = {(VT)(sign * b.x), (VT)(sign * b.y)};
-                for (int c = it.y; c < it.z; c++) {
+                Blk<VT> D = {(VT)0, (VT)0, (VT)0, (VT)0};
+                V2<VT> yk = {(VT)0, (VT)0};
+                for (int c = it.y + s; c < it.z; c += S) {
                     const int4 cr = P.contribRec[c];
                     const Blk<VT> Lkj = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, cr.x, t));
                     const Blk<VT> Ujk = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, cr.x + 1, t));
@@ -320,36 +344,28 @@ __device__ void factorForward(const DevPlan &P, int tile, int tid, int nt, VT *_
                     mulSub(D, Lkj, mul(Dj, Ujk));
                     mulSubV(yk, Lkj, zj);
                 }
-                bool cl;
-                const Blk<VT> Dinv = inv2(D, cl);
-                if (cl && clampLane) atomicAdd(&clampLane[t], 1);
-                stBlk<VT>(own, Dinv);
-                const V2<VT> zk = {Dinv.a * yk.p + Dinv.b * yk.q, Dinv.c * yk.p + Dinv.d * yk.q};
-                stV<VT>(z + vIdx<T>(tile, nb, it.w, t), zk);
+                if (S > 1) {
+                    groupSumBlk<T>(D, S);
+                    yk.p = groupSum<T>(yk.p, S);
+                    yk.q = groupSum<T>(yk.q, S);
+                }
+                if (s == 0) {
+                    const Blk<VT> A = ldBlk<VT>(own);
+                    const Blk<VT> Dk = {A.a + D.a, A.b + D.b, A.c + D.c, A.d + D.d};
+                    const double2 b = *reinterpret_cast<const double2 *>(rhs + vIdx<T>(tile, nb, it.w, t));
+                    const V2<VT> y = {(VT)(sign * b.x) + yk.p, (VT)(sign * b.y) + yk.q};
+                    bool cl;
+                    const Blk<VT> Dinv = inv2(Dk, cl);
+                    if (cl && clampLane) atomicAdd(&clampLane[t], 1);
+                    stBlk<VT>(own, Dinv);
+                    const V2<VT> zk = {Dinv.a * y.p + Dinv.b * y.q, Dinv.c * y.p + Dinv.d * y.q};
+                    stV<VT>(z + vIdx<T>(tile, nb, it.w, t), zk);
+                }
             } else {
                 // pair item: L~(i,k) at it.x, U(k,i) right after
-                Blk<VT> L = ldBlk<VT>(own);
-                Blk<VT> U = ldBlk<VT>(own + 4 * T);
-                int c = it.y;
-                for (; c + 1 < it.z; c += 2) {
-                    const int4 c0 = P.contribRec[c];
-                    const int4 c1 = P.contribRec[c + 1];
-                    const Blk<VT> Lkj0 = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, c0.x, t));
-                    const Blk<VT> Ujk0 = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, c0.x + 1, t));
-                    const Blk<VT> Lij0 = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, c0.y, t));
-                    const Blk<VT> Uji0 = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, c0.y + 1, t));
-                    const Blk<VT> Dj0 = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, c0.z, t));
-                    const Blk<VT> Lkj1 = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, c1.x, t));
-                    const Blk<VT> Ujk1 = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, c1.x + 1, t));
-                    const Blk<VT> Lij1 = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, c1.y, t));
-                    const Blk<VT> Uji1 = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, c1.y + 1, t));
-                    const Blk<VT> Dj1 = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, c1.z, t));
-                    mulSub(L, Lij0, mul(Dj0, Ujk0));
-                    mulSub(U, Lkj0, mul(Dj0, Uji0));
-                    mulSub(L, Lij1, mul(Dj1, Ujk1));
-                    mulSub(U, Lkj1, mul(Dj1, Uji1));
-                }
-                if (c < it.z) {
+                Blk<VT> L = {(VT)0, (VT)0, (VT)0, (VT)0};
+                Blk<VT> U = {(VT)0, (VT)0, (VT)0, (VT)0};
+                for (int c = it.y + s; c < it.z; c += S) {
                     const int4 c0 = P.contribRec[c];
                     const Blk<VT> Lkj = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, c0.x, t));
                     const Blk<VT> Ujk = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, c0.x + 1, t));
@@ -359,16 +375,24 @@ __device__ void factorForward(const DevPlan &P, int tile, int tid, int nt, VT *_
                     mulSub(L, Lij, mul(Dj, Ujk));
                     mulSub(U, Lkj, mul(Dj, Uji));
                 }
-                stBlk<VT>(own, L);
-                stBlk<VT>(own + 4 * T, U);
+                if (S > 1) {
+                    groupSumBlk<T>(L, S);
+                    groupSumBlk<T>(U, S);
+                }
+                if (s == 0) {
+                    const Blk<VT> A = ldBlk<VT>(own);
+                    const Blk<VT> B = ldBlk<VT>(own + 4 * T);
+                    stBlk<VT>(own, Blk<VT>{A.a + L.a, A.b + L.b, A.c + L.c, A.d + L.d});
+                    stBlk<VT>(own + 4 * T, Blk<VT>{B.a + U.a, B.b + U.b, B.c + U.c, B.d + U.d});
+                }
             }
         }
         __syncthreads();
     }
 }
 
-// Forward solve with existing factors: z = D^-1-scaled L^-1 (sign * rhs), z(k) = D(k)^-1 (b(k) - sum L~(k,j) z(j)).
-// Ends with a barrier.
+// Forward solve with existing factors: z(k) = D(k)^-1 (b(k) - sum L~(k,j) z(j)), b = sign * rhs. Ends with a
+// barrier.
 template <int T, typename VT>
 __device__ void forwardOnly(const DevPlan &P, int tile, int tid, int nt, const VT *__restrict__ lu,
                             VT *__restrict__ z, const double *__restrict__ rhs, double sign) {
@@ -376,43 +400,61 @@ __device__ void forwardOnly(const DevPlan &P, int tile, int tid, int nt, const V
     for (int l = 0; l < P.nLevels; l++) {
         const int i0 = P.levelItemPtr[l];
         const int nNodes = P.levelNodePtr[l + 1] - P.levelNodePtr[l];   // diagonal items come first
-        for (int w = tid; w < nNodes * T; w += nt) {
+        const int S = splitOf<T>(P.levelSplit, l);
+        for (int w = tid; w < nNodes * T * S; w += nt) {
             const int t = w % T;
-            const int4 it = P.itemRec[i0 + w / T];
-            const double2 b = *reinterpret_cast<const double2 *>(rhs + vIdx<T>(tile, nb, it.w, t));
-            V2<VT> yk = {(VT)(sign * b.x), (VT)(sign * b.y)};
-            for (int c = it.y; c < it.z; c++) {
+            const int s = (w / T) % S;
+            const int4 it = P.itemRec[i0 + w / (T * S)];
+            V2<VT> yk = {(VT)0, (VT)0};
+            for (int c = it.y + s; c < it.z; c += S) {
                 const int4 cr = P.contribRec[c];
                 const Blk<VT> Lkj = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, cr.x, t));
                 mulSubV(yk, Lkj, ldV<VT>(z + vIdx<T>(tile, nb, cr.y, t)));
             }
-            const Blk<VT> Dinv = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, it.x, t));
-            const V2<VT> zk = {Dinv.a * yk.p + Dinv.b * yk.q, Dinv.c * yk.p + Dinv.d * yk.q};
-            stV<VT>(z + vIdx<T>(tile, nb, it.w, t), zk);
+            if (S > 1) {
+                yk.p = groupSum<T>(yk.p, S);
+                yk.q = groupSum<T>(yk.q, S);
+            }
+            if (s == 0) {
+                const double2 b = *reinterpret_cast<const double2 *>(rhs + vIdx<T>(tile, nb, it.w, t));
+                const V2<VT> y = {(VT)(sign * b.x) + yk.p, (VT)(sign * b.y) + yk.q};
+                const Blk<VT> Dinv = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, it.x, t));
+                const V2<VT> zk = {Dinv.a * y.p + Dinv.b * y.q, Dinv.c * y.p + Dinv.d * y.q};
+                stV<VT>(z + vIdx<T>(tile, nb, it.w, t), zk);
+            }
         }
         __syncthreads();
     }
 }
 
-// Backward solve: x(k) = z(k) - D(k)^-1 sum U(k,i) x(i), levels from the root down. Ends with a barrier.
+// Backward solve: x(k) = z(k) - D(k)^-1 sum U(k,i) x(i), levels from the root down; a node's sum is split across
+// S threads as in the factorization. Ends with a barrier.
 template <int T, typename VT>
 __device__ void backward(const DevPlan &P, int tile, int tid, int nt, const VT *__restrict__ lu,
                          const VT *__restrict__ z, VT *__restrict__ xo) {
     const int nb = P.nb;
     for (int l = P.nLevels - 1; l >= 0; l--) {
         const int n0 = P.levelNodePtr[l], n1 = P.levelNodePtr[l + 1];
-        for (int w = tid; w < (n1 - n0) * T; w += nt) {
+        const int S = splitOf<T>(P.levelSplitBwd, l);
+        for (int w = tid; w < (n1 - n0) * T * S; w += nt) {
             const int t = w % T;
-            const int4 br = P.bwdRec[n0 + w / T];
-            V2<VT> s = {(VT)0, (VT)0};
-            for (int e = br.y; e < br.z; e++) {
+            const int s = (w / T) % S;
+            const int4 br = P.bwdRec[n0 + w / (T * S)];
+            V2<VT> acc = {(VT)0, (VT)0};
+            for (int e = br.y + s; e < br.z; e += S) {
                 const Blk<VT> U = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, br.x + 2 + 2 * (e - br.y), t));
-                mulSubV(s, U, ldV<VT>(xo + vIdx<T>(tile, nb, P.sIdx[e], t)));
+                mulSubV(acc, U, ldV<VT>(xo + vIdx<T>(tile, nb, P.sIdx[e], t)));
             }
-            const Blk<VT> Dinv = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, br.x, t));
-            const V2<VT> zk = ldV<VT>(z + vIdx<T>(tile, nb, br.w, t));
-            const V2<VT> r = {zk.p + Dinv.a * s.p + Dinv.b * s.q, zk.q + Dinv.c * s.p + Dinv.d * s.q};
-            stV<VT>(xo + vIdx<T>(tile, nb, br.w, t), r);
+            if (S > 1) {
+                acc.p = groupSum<T>(acc.p, S);
+                acc.q = groupSum<T>(acc.q, S);
+            }
+            if (s == 0) {
+                const Blk<VT> Dinv = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, br.x, t));
+                const V2<VT> zk = ldV<VT>(z + vIdx<T>(tile, nb, br.w, t));
+                const V2<VT> r = {zk.p + Dinv.a * acc.p + Dinv.b * acc.q, zk.q + Dinv.c * acc.p + Dinv.d * acc.q};
+                stV<VT>(xo + vIdx<T>(tile, nb, br.w, t), r);
+            }
         }
         __syncthreads();
     }

@@ -216,6 +216,24 @@ Plan buildPlan(int n, int nBranch, const double *p8, const int *idx12, int nShun
         P.levelNodePtr[l + 1] = (int)P.levelNodes.size();
         P.levelItemPtr[l + 1] = nItems;
     }
+    // split factors: about four contributions per thread on the level's longest list, at most 16 threads
+    auto splitFor = [](int longest) {
+        int s = 1;
+        while (s < 16 && 4 * s < longest) s *= 2;
+        return s;
+    };
+    P.levelSplit.assign(P.nLevels, 1);
+    P.levelSplitBwd.assign(P.nLevels, 1);
+    for (int l = 0; l < P.nLevels; l++) {
+        int longest = 0;
+        for (int i = P.levelItemPtr[l]; i < P.levelItemPtr[l + 1]; i++)
+            longest = std::max(longest, P.itemRec[4 * i + 2] - P.itemRec[4 * i + 1]);
+        P.levelSplit[l] = splitFor(longest);
+        int widest = 0;
+        for (int q = P.levelNodePtr[l]; q < P.levelNodePtr[l + 1]; q++)
+            widest = std::max(widest, P.bwdRec[4 * q + 2] - P.bwdRec[4 * q + 1]);
+        P.levelSplitBwd[l] = splitFor(widest);
+    }
 
     // --- fill gather tables
     std::vector<std::vector<int>> inc(nb);

@@ -47,6 +47,10 @@ struct Plan {
                                                    // diag slot of j, 0; pair item: L slot of (j,k), L slot of
                                                    // (j,i), diag slot of j, 0 (U(j,x) sits right after L(x,j))
     std::vector<int> bwdRec;                       // 4 per node in level order: diag slot, S begin, S end, node
+    // Threads that share one item's contribution list (a power of two, from the level's longest list): long lists
+    // at the top of the tree are summed by several threads of a wave and reduced across lanes. The kernels cap it
+    // at 64 / T so that an item's threads stay in one wave.
+    std::vector<int> levelSplit, levelSplitBwd;
 
     // --- fill gather tables, per node, half-branches grouped by the far node (ascending)
     std::vector<int> hbPtr;                        // half-branches of node k

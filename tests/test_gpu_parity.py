@@ -142,16 +142,21 @@ def test_n1_9241_matches_oracle(grid, lu):
     compare_states(res.x[conv], ref["x"][conv], f"n1 grid9241 {lu}")
 
 
-def test_tile_shapes_give_bitwise_identical_results(grid):
+def test_tile_shapes_agree(grid):
+    """The workgroup size never changes a result (bitwise); the lane-tile width only changes how many threads share
+    a long contribution sum (capped at 64 / T), hence the summation order: results agree to rounding."""
     net = grid("grid2000")
     print("[tiles] grid2000", flush=True)
-    outs = []
-    for T, NT in [(1, 256), (4, 512), (8, 1024), (16, 256)]:
+    outs = {}
+    for T, NT in [(1, 256), (1, 1024), (4, 512), (4, 1024), (8, 1024), (16, 256), (32, 512)]:
         ctg, ref, res = run_n1(net, "grid2000", 24, 0.25, 6, "BATCHED_LU", T, NT)
-        outs.append(res)
-    for r in outs[1:]:
-        assert np.array_equal(r.status, outs[0].status)
-        assert np.array_equal(r.x, outs[0].x)
+        outs[(T, NT)] = res
+    for (a, b) in [((1, 256), (1, 1024)), ((4, 512), (4, 1024))]:
+        assert np.array_equal(outs[a].x, outs[b].x), (a, b)
+    base = outs[(1, 256)]
+    for key, r in outs.items():
+        assert np.array_equal(r.status, base.status), key
+        assert np.abs(r.x - base.x).max() < 1e-9, key
 
 
 def test_rerun_is_bitwise_identical(grid):
