@@ -335,6 +335,7 @@ __device__ void factorForward(const DevPlan &P, int tile, int tid, int nt, VT *_
                 // diagonal item of node it.w: pivot block, its inverse, and the forward-solve entry
                 Blk<VT> D = {(VT)0, (VT)0, (VT)0, (VT)0};
                 V2<VT> yk = {(VT)0, (VT)0};
+#pragma unroll 2
                 for (int c = it.y + s; c < it.z; c += S) {
                     const int4 cr = P.contribRec[c];
                     const Blk<VT> Lkj = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, cr.x, t));
@@ -365,6 +366,7 @@ __device__ void factorForward(const DevPlan &P, int tile, int tid, int nt, VT *_
                 // pair item: L~(i,k) at it.x, U(k,i) right after
                 Blk<VT> L = {(VT)0, (VT)0, (VT)0, (VT)0};
                 Blk<VT> U = {(VT)0, (VT)0, (VT)0, (VT)0};
+#pragma unroll 2
                 for (int c = it.y + s; c < it.z; c += S) {
                     const int4 c0 = P.contribRec[c];
                     const Blk<VT> Lkj = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, c0.x, t));
@@ -406,6 +408,7 @@ __device__ void forwardOnly(const DevPlan &P, int tile, int tid, int nt, const V
             const int s = (w / T) % S;
             const int4 it = P.itemRec[i0 + w / (T * S)];
             V2<VT> yk = {(VT)0, (VT)0};
+#pragma unroll 4
             for (int c = it.y + s; c < it.z; c += S) {
                 const int4 cr = P.contribRec[c];
                 const Blk<VT> Lkj = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, cr.x, t));
@@ -441,6 +444,7 @@ __device__ void backward(const DevPlan &P, int tile, int tid, int nt, const VT *
             const int s = (w / T) % S;
             const int4 br = P.bwdRec[n0 + w / (T * S)];
             V2<VT> acc = {(VT)0, (VT)0};
+#pragma unroll 4
             for (int e = br.y + s; e < br.z; e += S) {
                 const Blk<VT> U = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, br.x + 2 + 2 * (e - br.y), t));
                 mulSubV(acc, U, ldV<VT>(xo + vIdx<T>(tile, nb, P.sIdx[e], t)));

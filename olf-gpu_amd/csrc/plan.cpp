@@ -199,6 +199,7 @@ Plan buildPlan(int n, int nBranch, const double *p8, const int *idx12, int nShun
         for (int k : byLevel[l]) {
             for (int e = 0; e < P.sPtr[k + 1] - P.sPtr[k]; e++) {
                 const auto &lst = pairC[pairItemBase[k] + e];
+                if (lst.empty()) continue;   // nothing to subtract: the fill's values are final
                 const int c0 = nContrib;
                 for (auto &c : lst) {
                     // contributing node j owns both slots: recover it from the slot range
@@ -216,10 +217,10 @@ Plan buildPlan(int n, int nBranch, const double *p8, const int *idx12, int nShun
         P.levelNodePtr[l + 1] = (int)P.levelNodes.size();
         P.levelItemPtr[l + 1] = nItems;
     }
-    // split factors: about four contributions per thread on the level's longest list, at most 16 threads
+    // split factors: about two contributions per thread on the level's longest list, at most 64 threads
     auto splitFor = [](int longest) {
         int s = 1;
-        while (s < 16 && 4 * s < longest) s *= 2;
+        while (s < 64 && 2 * s < longest) s *= 2;
         return s;
     };
     P.levelSplit.assign(P.nLevels, 1);
