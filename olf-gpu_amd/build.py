@@ -8,7 +8,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libolfhip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["api.hip", "plan.cpp"]
+SOURCES = ["api.hip", "plan.cpp", "mlp.hip"]
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
          # no packed FP32 instructions in device code (some of their forms gave wrong results on these boxes)
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

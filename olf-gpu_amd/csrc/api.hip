@@ -20,6 +20,7 @@
 namespace olfhip {
 
 static thread_local std::string gLastError;
+std::string &lastErrorRef() { return gLastError; }
 
 #define HIP_OK(expr)                                                                                     \
     do {                                                                                                 \
@@ -327,9 +328,13 @@ static void solve(ContextImpl *c, const olfhip_solve_inputs *in, olfhip_solve_ou
     c->x0Base.upload(std::vector<double>(in->x0_base, in->x0_base + n), s);
     c->targetBase.upload(std::vector<double>(in->target_base, in->target_base + n), s);
     c->modeBase.upload(std::vector<int>(in->base_row_mode, in->base_row_mode + n), s);
+    if (in->x0_per_scenario && in->x0_per_scenario_device)
+        throw ArgumentError("x0_per_scenario and x0_per_scenario_device are exclusive");
+    const double *x0PerDev = in->x0_per_scenario_device;
     if (in->x0_per_scenario) {
         c->x0Per.alloc((size_t)K * n * 8);
         HIP_OK(hipMemcpyAsync(c->x0Per.p, in->x0_per_scenario, (size_t)K * n * 8, hipMemcpyHostToDevice, s));
+        x0PerDev = c->x0Per.as<double>();
     }
     if (in->n_row_mode_entries > 0)
         c->modeEntries.upload(std::vector<int>(in->row_mode_entries, in->row_mode_entries + 3 * in->n_row_mode_entries), s);
@@ -373,7 +378,7 @@ static void solve(ContextImpl *c, const olfhip_solve_inputs *in, olfhip_solve_ou
         constexpr int T = decltype(TC)::value;
         const size_t work = (size_t)c->nTiles * P.nb * T;
         installKernel<T><<<gridFor(work, 256), 256, 0, s>>>(dp, S, c->x0Base.as<double>(),
-                                                            in->x0_per_scenario ? c->x0Per.as<double>() : nullptr,
+                                                            x0PerDev,
                                                             c->targetBase.as<double>(), c->modeBase.as<int>());
         if (in->n_row_mode_entries + in->n_target_overrides > 0)
             overrideKernel<T><<<gridFor(in->n_row_mode_entries + in->n_target_overrides, 256), 256, 0, s>>>(

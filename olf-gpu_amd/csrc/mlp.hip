@@ -1,0 +1,575 @@
+// Learned warm start: a 3-layer MLP trained and served by hand-written gfx950 kernels (BF16 operands on the matrix
+// cores, FP32 accumulation, FP32 master weights with Adam). It maps a scenario's base-case state, its outaged branch
+// (an embedding row: the one-hot input of the first layer) and a few endpoint features to the normalised
+// post-contingency state change; the prediction seeds the Newton solve through the per-scenario x0 hook
+// (BatchSolveNative x0PerScenario, BatchSolveNative.java:119). The Newton mismatch stays the sole judge of the answer.
+//
+//   z1 = Win [x_norm ; f] + Emb[branch] + b1,  h1 = gelu(z1)      Win: H x (N + F), Emb: E x H
+//   z2 = W2 h1 + b2,                           h2 = gelu(z2)      W2:  H x H
+//   y  = W3 h2 + b3                                               W3:  N x H
+// with N state rows, F endpoint features, E branches, H hidden units; loss = mean over batch and rows of
+// (y - target)^2 on normalised targets.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/olfhip.h"
+#include "errors.hpp"
+
+namespace olfhip {
+
+std::string &lastErrorRef();
+
+#define MLP_OK(expr)                                                                                      \
+    do {                                                                                                  \
+        hipError_t e_ = (expr);                                                                           \
+        if (e_ != hipSuccess) {                                                                           \
+            if (e_ == hipErrorOutOfMemory) throw MemoryError(std::string(#expr) + ": out of device memory"); \
+            throw DeviceError(std::string(#expr) + ": " + hipGetErrorString(e_));                         \
+        }                                                                                                 \
+    } while (0)
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ unsigned short toBf16(float f) {
+    // round to nearest even
+    unsigned int u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (unsigned short)((u >> 16) | 0x40);   // NaN stays NaN
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float fromBf16(unsigned short h) { return __uint_as_float(((unsigned int)h) << 16); }
+
+// ---- GEMM: C[M x N] (+)= A[M x K] B[K x N], BF16 operands with arbitrary strides, FP32 accumulation on
+// v_mfma_f32_16x16x16bf16_1k. A(m,k) = A[m*sam + k*sak], B(k,n) = B[k*sbk + n*sbn], C row-major [M][ldc].
+// A 256-thread workgroup computes a 64x64 tile (each wave 32x32 = 2x2 MFMA tiles) over K range
+// [kz*kChunk, (kz+1)*kChunk); with several K chunks the partial tiles are added atomically (C zeroed first).
+constexpr int GT = 64, GK = 16;
+
+__global__ __launch_bounds__(256) void gemmBf16(int M, int N, int K, const unsigned short *__restrict__ A, long sam,
+                                                long sak, const unsigned short *__restrict__ B, long sbk, long sbn,
+                                                float *__restrict__ C, long ldc, int kChunk, int atomic) {
+    __shared__ unsigned short As[GT][GK + 4];
+    __shared__ unsigned short Bs[GK][GT + 4];
+    const int tid = threadIdx.x;
+    const int wave = tid / 64, lane = tid % 64;
+    const int m0 = blockIdx.y * GT, n0 = blockIdx.x * GT;
+    const int kBeg = blockIdx.z * kChunk;
+    const int kEnd = min(K, kBeg + kChunk);
+    floatx4 acc[2][2];
+    for (int i = 0; i < 2; i++)
+        for (int j = 0; j < 2; j++) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+    const int wm = (wave / 2) * 32, wn = (wave % 2) * 32;
+    for (int k0 = kBeg; k0 < kEnd; k0 += GK) {
+        // stage: 64x16 of A and 16x64 of B, 4 elements per thread each, zero outside the matrix
+        for (int q = 0; q < 4; q++) {
+            const int e = tid * 4 + q;          // 0..1023
+            const int am = e / GK, ak = e % GK;
+            const int gm = m0 + am, gk = k0 + ak;
+            As[am][ak] = (gm < M && gk < kEnd) ? A[(long)gm * sam + (long)gk * sak] : (unsigned short)0;
+            const int bk = e / GT, bn = e % GT;
+            const int hk = k0 + bk, hn = n0 + bn;
+            Bs[bk][bn] = (hk < kEnd && hn < N) ? B[(long)hk * sbk + (long)hn * sbn] : (unsigned short)0;
+        }
+        __syncthreads();
+        const int r = lane % 16, g = lane / 16;
+        for (int i = 0; i < 2; i++) {
+            bf16x4 a;
+            a[0] = (short)As[wm + i * 16 + r][g * 4 + 0];
+            a[1] = (short)As[wm + i * 16 + r][g * 4 + 1];
+            a[2] = (short)As[wm + i * 16 + r][g * 4 + 2];
+            a[3] = (short)As[wm + i * 16 + r][g * 4 + 3];
+            for (int j = 0; j < 2; j++) {
+                bf16x4 b;
+                b[0] = (short)Bs[g * 4 + 0][wn + j * 16 + r];
+                b[1] = (short)Bs[g * 4 + 1][wn + j * 16 + r];
+                b[2] = (short)Bs[g * 4 + 2][wn + j * 16 + r];
+                b[3] = (short)Bs[g * 4 + 3][wn + j * 16 + r];
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[i][j], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    // lane holds D[4g + q][r] of each 16x16 tile
+    const int r = lane % 16, g = lane / 16;
+    for (int i = 0; i < 2; i++)
+        for (int j = 0; j < 2; j++)
+            for (int q = 0; q < 4; q++) {
+                const int gm = m0 + wm + i * 16 + g * 4 + q, gn = n0 + wn + j * 16 + r;
+                if (gm < M && gn < N) {
+                    if (atomic) atomicAdd(&C[(long)gm * ldc + gn], acc[i][j][q]);
+                    else C[(long)gm * ldc + gn] = acc[i][j][q];
+                }
+            }
+}
+
+__global__ void zeroF(float *p, long n) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = 0.f;
+}
+__global__ void f32ToBf16(const float *in, unsigned short *out, long n) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        out[i] = toBf16(in[i]);
+}
+
+__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.f + erff(x * 0.70710678118654752f)); }
+__device__ __forceinline__ float geluGrad(float x) {
+    const float cdf = 0.5f * (1.f + erff(x * 0.70710678118654752f));
+    const float pdf = 0.39894228040143268f * expf(-0.5f * x * x);
+    return cdf + x * pdf;
+}
+
+// z = c + bias (+ emb[branch] row); h = gelu(z) as BF16 (the next GEMM's operand); z kept for the backward pass
+__global__ void biasGeluFwd(int B, int H, const float *c, const float *bias, const float *emb, const int *branch,
+                            float *z, unsigned short *h) {
+    const long n = (long)B * H;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const int b = (int)(i / H), u = (int)(i % H);
+        float v = c[i] + bias[u];
+        if (emb) v += emb[(long)branch[b] * H + u];
+        z[i] = v;
+        h[i] = toBf16(gelu(v));
+    }
+}
+
+// y = c + bias
+__global__ void biasAdd(int B, int N, float *c, const float *bias) {
+    const long n = (long)B * N;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        c[i] += bias[i % N];
+}
+
+// dy = 2 (y - t) / (B N) as FP32 and BF16; loss sum into *loss
+__global__ void mseGrad(int B, int N, const float *y, const float *t, float *dy, unsigned short *dyh, double *loss) {
+    const long n = (long)B * N;
+    double part = 0.0;
+    const float scale = 2.f / (float)n;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const float d = y[i] - t[i];
+        part += (double)d * d;
+        dy[i] = scale * d;
+        dyh[i] = toBf16(scale * d);
+    }
+    __shared__ double red[256];
+    red[threadIdx.x] = part;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(loss, red[0]);
+}
+
+// dz = dh * gelu'(z), as FP32 and BF16
+__global__ void geluBwd(long n, const float *dh, const float *z, float *dz, unsigned short *dzh) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const float v = dh[i] * geluGrad(z[i]);
+        dz[i] = v;
+        dzh[i] = toBf16(v);
+    }
+}
+
+// bias gradient: column sums of d [B x H] (one thread per column)
+__global__ void colSum(int B, int H, const float *d, float *g) {
+    for (int u = blockIdx.x * blockDim.x + threadIdx.x; u < H; u += gridDim.x * blockDim.x) {
+        float s = 0.f;
+        for (int b = 0; b < B; b++) s += d[(long)b * H + u];
+        g[u] = s;
+    }
+}
+
+// embedding gradient: gEmb[branch[b]] += dz1[b]
+__global__ void embScatter(int B, int H, const int *branch, const float *dz, float *gEmb) {
+    const long n = (long)B * H;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const int b = (int)(i / H), u = (int)(i % H);
+        atomicAdd(&gEmb[(long)branch[b] * H + u], dz[i]);
+    }
+}
+
+// Adam on FP32 masters, refreshed BF16 copy (bf may be null for parameters only used in FP32)
+__global__ void adam(long n, float *w, const float *g, float *m, float *v, unsigned short *bf, float lr, float b1,
+                     float b2, float c1, float c2, float eps) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const float gi = g[i];
+        const float mi = b1 * m[i] + (1.f - b1) * gi;
+        const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+        w[i] = wi;
+        if (bf) bf[i] = toBf16(wi);
+    }
+}
+
+// layer-1 input rows [x_norm ; f] as BF16: x_norm = (x_base - mean) / std
+__global__ void buildInput(int B, int N, int F, const double *xBase, long xStride, const float *xMean,
+                           const float *xStd, const float *feat, unsigned short *out) {
+    const long D = (long)N + F;
+    const long n = (long)B * D;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const int b = (int)(i / D);
+        const long c = i % D;
+        float v;
+        if (c < N) v = (float)((xBase[(long)b * xStride + c] - (double)xMean[c]) / (double)xStd[c]);
+        else v = feat[(long)b * F + (c - N)];
+        out[i] = toBf16(v);
+    }
+}
+
+// x0[b][r] = x_base[r] + y[b][r] * std_r + mean_r (the warm start of scenario b)
+__global__ void warmStartOut(int B, int N, const double *xBase, long xStride, const float *y, const float *yMean,
+                             const float *yStd, double *x0) {
+    const long n = (long)B * N;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const int b = (int)(i / N), r = (int)(i % N);
+        x0[i] = xBase[(long)b * xStride + r] + (double)y[i] * (double)yStd[r] + (double)yMean[r];
+    }
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+struct Buf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    ~Buf() {
+        if (p) (void)hipFree(p);
+    }
+    void alloc(size_t b) {
+        if (p && b <= bytes) return;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        MLP_OK(hipMalloc(&p, b ? b : 16));
+        bytes = b;
+    }
+    template <typename X>
+    X *as() const { return reinterpret_cast<X *>(p); }
+};
+
+static int gridOf(long n) { return (int)std::max<long>(1, std::min<long>((n + 255) / 256, 8192)); }
+
+struct Mlp {
+    int N = 0, E = 0, F = 0, H = 0;
+    long D() const { return (long)N + F; }
+    // parameters: FP32 masters, BF16 copies of the GEMM weights, Adam moments, gradients
+    struct Param {
+        long n = 0;
+        Buf w, g, m, v, bf;
+    };
+    Param Win, Emb, b1, W2, b2, W3, b3;
+    Buf xMean, xStd, yMean, yStd;
+    // activations for a batch of up to cap
+    int cap = 0;
+    Buf X, c1, z1, h1, c2, z2, h2, y, dy, dyh, dh2, dz2, dz2h, dh1, dz1, dz1h, tgt, branch, feat, loss, x0;
+    long step = 0;
+    hipStream_t s = nullptr;
+
+    void initParam(Param &p, long n, float scale, std::mt19937_64 &rng, bool bf) {
+        p.n = n;
+        std::vector<float> h(n);
+        std::normal_distribution<float> nd(0.f, 1.f);
+        for (long i = 0; i < n; i++) h[i] = scale * nd(rng);
+        p.w.alloc(n * 4);
+        MLP_OK(hipMemcpy(p.w.p, h.data(), n * 4, hipMemcpyHostToDevice));
+        p.g.alloc(n * 4);
+        p.m.alloc(n * 4);
+        p.v.alloc(n * 4);
+        zeroF<<<gridOf(n), 256, 0, s>>>(p.m.as<float>(), n);
+        zeroF<<<gridOf(n), 256, 0, s>>>(p.v.as<float>(), n);
+        if (bf) {
+            p.bf.alloc(n * 2);
+            f32ToBf16<<<gridOf(n), 256, 0, s>>>(p.w.as<float>(), p.bf.as<unsigned short>(), n);
+        }
+    }
+    void ensureBatch(int B) {
+        if (B <= cap) return;
+        cap = B;
+        X.alloc((size_t)B * D() * 2);
+        for (Buf *b : {&c1, &z1, &c2, &z2, &dh2, &dz2, &dh1, &dz1}) b->alloc((size_t)B * H * 4);
+        for (Buf *b : {&h1, &h2, &dz2h, &dz1h}) b->alloc((size_t)B * H * 2);
+        for (Buf *b : {&y, &dy, &tgt}) b->alloc((size_t)B * N * 4);
+        dyh.alloc((size_t)B * N * 2);
+        x0.alloc((size_t)B * N * 8);
+        branch.alloc((size_t)B * 4);
+        feat.alloc((size_t)B * std::max(F, 1) * 4);
+        loss.alloc(8);
+    }
+    // C[M x N] = A B with K chunked so that the grid has at least ~512 workgroups
+    void gemm(int M, int Nn, int K, const unsigned short *A, long sam, long sak, const unsigned short *B, long sbk,
+              long sbn, float *C, long ldc) {
+        const int gx = (Nn + GT - 1) / GT, gy = (M + GT - 1) / GT;
+        int splits = 1;
+        while (gx * gy * splits < 512 && K / (splits * 2) >= 256) splits *= 2;
+        int kChunk = (K + splits - 1) / splits;
+        kChunk = (kChunk + GK - 1) / GK * GK;
+        splits = (K + kChunk - 1) / kChunk;
+        if (splits > 1) zeroF<<<gridOf((long)M * ldc), 256, 0, s>>>(C, (long)M * ldc);
+        gemmBf16<<<dim3(gx, gy, splits), 256, 0, s>>>(M, Nn, K, A, sam, sak, B, sbk, sbn, C, ldc, kChunk, splits > 1);
+    }
+    void forward(int B) {
+        const int Hh = H;
+        // c1 = X Win^T : [B x D] [D x H]
+        gemm(B, Hh, (int)D(), X.as<unsigned short>(), D(), 1, Win.bf.as<unsigned short>(), 1, D(), c1.as<float>(), Hh);
+        biasGeluFwd<<<gridOf((long)B * Hh), 256, 0, s>>>(B, Hh, c1.as<float>(), b1.w.as<float>(), Emb.w.as<float>(),
+                                                         branch.as<int>(), z1.as<float>(), h1.as<unsigned short>());
+        gemm(B, Hh, Hh, h1.as<unsigned short>(), Hh, 1, W2.bf.as<unsigned short>(), 1, Hh, c2.as<float>(), Hh);
+        biasGeluFwd<<<gridOf((long)B * Hh), 256, 0, s>>>(B, Hh, c2.as<float>(), b2.w.as<float>(), nullptr, nullptr,
+                                                         z2.as<float>(), h2.as<unsigned short>());
+        gemm(B, N, Hh, h2.as<unsigned short>(), Hh, 1, W3.bf.as<unsigned short>(), 1, Hh, y.as<float>(), N);
+        biasAdd<<<gridOf((long)B * N), 256, 0, s>>>(B, N, y.as<float>(), b3.w.as<float>());
+    }
+    // gradients of the mean squared error into the .g buffers; returns the loss
+    double backward(int B) {
+        const int Hh = H;
+        MLP_OK(hipMemsetAsync(loss.p, 0, 8, s));
+        mseGrad<<<gridOf((long)B * N), 256, 0, s>>>(B, N, y.as<float>(), tgt.as<float>(), dy.as<float>(),
+                                                   dyh.as<unsigned short>(), loss.as<double>());
+        // W3: g = dy^T h2 : [N x B][B x H]
+        gemm(N, Hh, B, dyh.as<unsigned short>(), 1, N, h2.as<unsigned short>(), Hh, 1, W3.g.as<float>(), Hh);
+        colSum<<<gridOf(N), 256, 0, s>>>(B, N, dy.as<float>(), b3.g.as<float>());
+        // dh2 = dy W3 : [B x N][N x H]
+        gemm(B, Hh, N, dyh.as<unsigned short>(), N, 1, W3.bf.as<unsigned short>(), Hh, 1, dh2.as<float>(), Hh);
+        geluBwd<<<gridOf((long)B * Hh), 256, 0, s>>>((long)B * Hh, dh2.as<float>(), z2.as<float>(), dz2.as<float>(),
+                                                     dz2h.as<unsigned short>());
+        gemm(Hh, Hh, B, dz2h.as<unsigned short>(), 1, Hh, h1.as<unsigned short>(), Hh, 1, W2.g.as<float>(), Hh);
+        colSum<<<gridOf(Hh), 256, 0, s>>>(B, Hh, dz2.as<float>(), b2.g.as<float>());
+        gemm(B, Hh, Hh, dz2h.as<unsigned short>(), Hh, 1, W2.bf.as<unsigned short>(), Hh, 1, dh1.as<float>(), Hh);
+        geluBwd<<<gridOf((long)B * Hh), 256, 0, s>>>((long)B * Hh, dh1.as<float>(), z1.as<float>(), dz1.as<float>(),
+                                                     dz1h.as<unsigned short>());
+        gemm(Hh, (int)D(), B, dz1h.as<unsigned short>(), 1, Hh, X.as<unsigned short>(), D(), 1, Win.g.as<float>(), D());
+        colSum<<<gridOf(Hh), 256, 0, s>>>(B, Hh, dz1.as<float>(), b1.g.as<float>());
+        zeroF<<<gridOf(Emb.n), 256, 0, s>>>(Emb.g.as<float>(), Emb.n);
+        embScatter<<<gridOf((long)B * Hh), 256, 0, s>>>(B, Hh, branch.as<int>(), dz1.as<float>(), Emb.g.as<float>());
+        double l = 0.0;
+        MLP_OK(hipMemcpyAsync(&l, loss.p, 8, hipMemcpyDeviceToHost, s));
+        MLP_OK(hipStreamSynchronize(s));
+        return l / ((double)B * N);
+    }
+    void adamStep(float lr) {
+        step++;
+        const float b1c = 0.9f, b2c = 0.999f;
+        const float c1 = 1.f - std::pow(b1c, (float)step), c2 = 1.f - std::pow(b2c, (float)step);
+        for (Param *p : {&Win, &Emb, &b1, &W2, &b2, &W3, &b3})
+            adam<<<gridOf(p->n), 256, 0, s>>>(p->n, p->w.as<float>(), p->g.as<float>(), p->m.as<float>(),
+                                              p->v.as<float>(), p->bf.p ? p->bf.as<unsigned short>() : nullptr, lr, b1c,
+                                              b2c, c1, c2, 1e-8f);
+    }
+    void upload(int B, const double *xBaseDev, long xStride, const int *br, const float *ft, const float *tg) {
+        ensureBatch(B);
+        MLP_OK(hipMemcpyAsync(branch.p, br, (size_t)B * 4, hipMemcpyHostToDevice, s));
+        if (F > 0) MLP_OK(hipMemcpyAsync(feat.p, ft, (size_t)B * F * 4, hipMemcpyHostToDevice, s));
+        if (tg) MLP_OK(hipMemcpyAsync(tgt.p, tg, (size_t)B * N * 4, hipMemcpyHostToDevice, s));
+        buildInput<<<gridOf((long)B * D()), 256, 0, s>>>(B, N, F, xBaseDev, xStride, xMean.as<float>(),
+                                                        xStd.as<float>(), feat.as<float>(), X.as<unsigned short>());
+    }
+    ~Mlp() {
+        if (s) (void)hipStreamDestroy(s);
+    }
+};
+
+}  // namespace olfhip
+
+using namespace olfhip;
+
+template <typename Fn>
+static int mlpGuarded(Fn &&fn) {
+    try {
+        lastErrorRef().clear();
+        fn();
+        return OLFHIP_OK;
+    } catch (const ArgumentError &e) {
+        lastErrorRef() = e.what();
+        return OLFHIP_E_ARGUMENT;
+    } catch (const MemoryError &e) {
+        lastErrorRef() = e.what();
+        return OLFHIP_E_MEMORY;
+    } catch (const std::exception &e) {
+        lastErrorRef() = std::string("warm-start network failed: ") + e.what();
+        return OLFHIP_E_DEVICE;
+    }
+}
+
+// per-call device copy of a host base-state array ([rows][stride] doubles)
+struct HostX {
+    Buf d;
+    void put(const double *h, size_t n, hipStream_t s) {
+        d.alloc(n * 8);
+        MLP_OK(hipMemcpyAsync(d.p, h, n * 8, hipMemcpyHostToDevice, s));
+    }
+};
+
+extern "C" {
+
+int olfhip_mlp_create(int n_state, int n_branch, int n_features, int hidden, unsigned long long seed,
+                      olfhip_mlp **out) {
+    return mlpGuarded([&] {
+        if (!out) throw ArgumentError("out is null");
+        *out = nullptr;
+        if (n_state <= 0 || n_branch <= 0 || n_features < 0 || hidden <= 0 || hidden % 16)
+            throw ArgumentError("sizes must be positive and hidden a multiple of 16");
+        std::unique_ptr<Mlp> m(new Mlp());
+        m->N = n_state;
+        m->E = n_branch;
+        m->F = n_features;
+        m->H = hidden;
+        MLP_OK(hipStreamCreateWithFlags(&m->s, hipStreamNonBlocking));
+        std::mt19937_64 rng(seed);
+        const float sIn = 1.f / std::sqrt((float)m->D()), sH = 1.f / std::sqrt((float)hidden);
+        m->initParam(m->Win, (long)hidden * m->D(), sIn, rng, true);
+        m->initParam(m->Emb, (long)n_branch * hidden, 0.1f, rng, false);
+        m->initParam(m->b1, hidden, 0.f, rng, false);
+        m->initParam(m->W2, (long)hidden * hidden, sH, rng, true);
+        m->initParam(m->b2, hidden, 0.f, rng, false);
+        m->initParam(m->W3, (long)n_state * hidden, sH, rng, true);
+        m->initParam(m->b3, n_state, 0.f, rng, false);
+        std::vector<float> zeros(n_state, 0.f), ones(n_state, 1.f);
+        for (Buf *b : {&m->xMean, &m->yMean}) {
+            b->alloc(n_state * 4);
+            MLP_OK(hipMemcpy(b->p, zeros.data(), n_state * 4, hipMemcpyHostToDevice));
+        }
+        for (Buf *b : {&m->xStd, &m->yStd}) {
+            b->alloc(n_state * 4);
+            MLP_OK(hipMemcpy(b->p, ones.data(), n_state * 4, hipMemcpyHostToDevice));
+        }
+        MLP_OK(hipStreamSynchronize(m->s));
+        *out = reinterpret_cast<olfhip_mlp *>(m.release());
+    });
+}
+
+void olfhip_mlp_destroy(olfhip_mlp *mlp) { delete reinterpret_cast<Mlp *>(mlp); }
+
+int olfhip_mlp_set_normalization(olfhip_mlp *mlp, const float *x_mean, const float *x_std, const float *y_mean,
+                                 const float *y_std) {
+    return mlpGuarded([&] {
+        Mlp *m = reinterpret_cast<Mlp *>(mlp);
+        if (!m || !x_mean || !x_std || !y_mean || !y_std) throw ArgumentError("null argument");
+        for (int r = 0; r < m->N; r++)
+            if (!(x_std[r] > 0.f) || !(y_std[r] > 0.f)) throw ArgumentError("standard deviations must be positive");
+        MLP_OK(hipMemcpy(m->xMean.p, x_mean, m->N * 4, hipMemcpyHostToDevice));
+        MLP_OK(hipMemcpy(m->xStd.p, x_std, m->N * 4, hipMemcpyHostToDevice));
+        MLP_OK(hipMemcpy(m->yMean.p, y_mean, m->N * 4, hipMemcpyHostToDevice));
+        MLP_OK(hipMemcpy(m->yStd.p, y_std, m->N * 4, hipMemcpyHostToDevice));
+    });
+}
+
+int olfhip_mlp_train_step(olfhip_mlp *mlp, int batch, const double *x_base, const int *branch, const float *features,
+                          const float *target, float learning_rate, double *loss) {
+    return mlpGuarded([&] {
+        Mlp *m = reinterpret_cast<Mlp *>(mlp);
+        if (!m || !x_base || !branch || !target || (m->F > 0 && !features)) throw ArgumentError("null argument");
+        if (batch <= 0) throw ArgumentError("batch must be positive");
+        for (int b = 0; b < batch; b++)
+            if (branch[b] < 0 || branch[b] >= m->E) throw ArgumentError("branch id out of range");
+        HostX hx;
+        hx.put(x_base, (size_t)batch * m->N, m->s);
+        m->upload(batch, hx.d.as<double>(), m->N, branch, features, target);
+        m->forward(batch);
+        const double l = m->backward(batch);
+        m->adamStep(learning_rate);
+        MLP_OK(hipStreamSynchronize(m->s));
+        if (loss) *loss = l;
+    });
+}
+
+int olfhip_mlp_forward(olfhip_mlp *mlp, int batch, const double *x_base, const int *branch, const float *features,
+                       float *y_out) {
+    return mlpGuarded([&] {
+        Mlp *m = reinterpret_cast<Mlp *>(mlp);
+        if (!m || !x_base || !branch || !y_out || (m->F > 0 && !features)) throw ArgumentError("null argument");
+        if (batch <= 0) throw ArgumentError("batch must be positive");
+        for (int b = 0; b < batch; b++)
+            if (branch[b] < 0 || branch[b] >= m->E) throw ArgumentError("branch id out of range");
+        HostX hx;
+        hx.put(x_base, (size_t)batch * m->N, m->s);
+        m->upload(batch, hx.d.as<double>(), m->N, branch, features, nullptr);
+        m->forward(batch);
+        MLP_OK(hipMemcpyAsync(y_out, m->y.p, (size_t)batch * m->N * 4, hipMemcpyDeviceToHost, m->s));
+        MLP_OK(hipStreamSynchronize(m->s));
+    });
+}
+
+int olfhip_mlp_gradients(olfhip_mlp *mlp, int batch, const double *x_base, const int *branch, const float *features,
+                         const float *target, int which, float *grad_out, double *loss) {
+    return mlpGuarded([&] {
+        Mlp *m = reinterpret_cast<Mlp *>(mlp);
+        if (!m || !x_base || !branch || !target || !grad_out) throw ArgumentError("null argument");
+        if (batch <= 0) throw ArgumentError("batch must be positive");
+        Mlp::Param *ps[7] = {&m->Win, &m->Emb, &m->b1, &m->W2, &m->b2, &m->W3, &m->b3};
+        if (which < 0 || which > 6) throw ArgumentError("which selects one of the 7 parameter arrays");
+        HostX hx;
+        hx.put(x_base, (size_t)batch * m->N, m->s);
+        m->upload(batch, hx.d.as<double>(), m->N, branch, features, target);
+        m->forward(batch);
+        const double l = m->backward(batch);
+        MLP_OK(hipMemcpy(grad_out, ps[which]->g.p, ps[which]->n * 4, hipMemcpyDeviceToHost));
+        if (loss) *loss = l;
+    });
+}
+
+int olfhip_mlp_parameters(olfhip_mlp *mlp, int which, float *values_out, const float *values_in) {
+    return mlpGuarded([&] {
+        Mlp *m = reinterpret_cast<Mlp *>(mlp);
+        if (!m) throw ArgumentError("null argument");
+        Mlp::Param *ps[7] = {&m->Win, &m->Emb, &m->b1, &m->W2, &m->b2, &m->W3, &m->b3};
+        if (which < 0 || which > 6) throw ArgumentError("which selects one of the 7 parameter arrays");
+        Mlp::Param &p = *ps[which];
+        if (values_in) {
+            MLP_OK(hipMemcpy(p.w.p, values_in, p.n * 4, hipMemcpyHostToDevice));
+            if (p.bf.p) f32ToBf16<<<gridOf(p.n), 256, 0, m->s>>>(p.w.as<float>(), p.bf.as<unsigned short>(), p.n);
+            MLP_OK(hipStreamSynchronize(m->s));
+        }
+        if (values_out) MLP_OK(hipMemcpy(values_out, p.w.p, p.n * 4, hipMemcpyDeviceToHost));
+    });
+}
+
+int olfhip_mlp_predict_warm_start(olfhip_mlp *mlp, int batch, const double *x_base, int x_base_per_scenario,
+                                  const int *branch, const float *features, double **x0_device) {
+    return mlpGuarded([&] {
+        Mlp *m = reinterpret_cast<Mlp *>(mlp);
+        if (!m || !x_base || !branch || !x0_device || (m->F > 0 && !features)) throw ArgumentError("null argument");
+        if (batch <= 0) throw ArgumentError("batch must be positive");
+        for (int b = 0; b < batch; b++)
+            if (branch[b] < 0 || branch[b] >= m->E) throw ArgumentError("branch id out of range");
+        HostX hx;
+        const long stride = x_base_per_scenario ? m->N : 0;
+        hx.put(x_base, x_base_per_scenario ? (size_t)batch * m->N : (size_t)m->N, m->s);
+        m->upload(batch, hx.d.as<double>(), stride, branch, features, nullptr);
+        m->forward(batch);
+        warmStartOut<<<gridOf((long)batch * m->N), 256, 0, m->s>>>(batch, m->N, hx.d.as<double>(), stride,
+                                                                   m->y.as<float>(), m->yMean.as<float>(),
+                                                                   m->yStd.as<float>(), m->x0.as<double>());
+        MLP_OK(hipStreamSynchronize(m->s));
+        *x0_device = m->x0.as<double>();
+    });
+}
+
+// GEMM on the matrix cores, for the unit test of the lane layout: C = A B, A [M x K] and B [K x N] row-major FP32
+// (rounded to BF16 on upload), C FP32.
+int olfhip_mlp_gemm_test(int M, int N, int K, const float *A, const float *B, float *C) {
+    return mlpGuarded([&] {
+        if (M <= 0 || N <= 0 || K <= 0 || !A || !B || !C) throw ArgumentError("bad arguments");
+        Buf a32, b32, a16, b16, c;
+        a32.alloc((size_t)M * K * 4);
+        b32.alloc((size_t)K * N * 4);
+        a16.alloc((size_t)M * K * 2);
+        b16.alloc((size_t)K * N * 2);
+        c.alloc((size_t)M * N * 4);
+        MLP_OK(hipMemcpy(a32.p, A, (size_t)M * K * 4, hipMemcpyHostToDevice));
+        MLP_OK(hipMemcpy(b32.p, B, (size_t)K * N * 4, hipMemcpyHostToDevice));
+        f32ToBf16<<<gridOf((long)M * K), 256>>>(a32.as<float>(), a16.as<unsigned short>(), (long)M * K);
+        f32ToBf16<<<gridOf((long)K * N), 256>>>(b32.as<float>(), b16.as<unsigned short>(), (long)K * N);
+        Mlp tmp;
+        tmp.s = nullptr;
+        tmp.gemm(M, N, K, a16.as<unsigned short>(), K, 1, b16.as<unsigned short>(), N, 1, c.as<float>(), N);
+        MLP_OK(hipDeviceSynchronize());
+        MLP_OK(hipMemcpy(C, c.p, (size_t)M * N * 4, hipMemcpyDeviceToHost));
+    });
+}
+
+}  // extern "C"

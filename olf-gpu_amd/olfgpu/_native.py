@@ -25,6 +25,9 @@ EXPORTS = [
     "olfhip_create_batch_context", "olfhip_destroy_batch_context", "olfhip_solve_batch_context",
     "olfhip_read_states", "olfhip_islanded_buses", "olfhip_lu_factorize_and_solve", "olfhip_structure_pattern",
     "olfhip_autotune", "olfhip_context_timing", "olfhip_read_outer_loop_state",
+    "olfhip_mlp_create", "olfhip_mlp_destroy", "olfhip_mlp_set_normalization", "olfhip_mlp_train_step",
+    "olfhip_mlp_forward", "olfhip_mlp_gradients", "olfhip_mlp_parameters", "olfhip_mlp_predict_warm_start",
+    "olfhip_mlp_gemm_test",
 ]
 
 
@@ -54,7 +57,7 @@ class SolveInputs(C.Structure):
                 ("max_iter", C.c_int), ("tol", C.c_double), ("reactive_limit_tol", C.c_double),
                 ("state_vector_scaling_mode", C.c_int), ("max_dv", C.c_double), ("max_dphi", C.c_double),
                 ("max_outer_iterations", C.c_int), ("max_pq_pv_switch", C.c_int), ("skip_reactive_limits", C.c_int),
-                ("states_to_host", C.c_int)]
+                ("states_to_host", C.c_int), ("x0_per_scenario_device", C.c_void_p)]
 
 
 class SolveOutputs(C.Structure):
@@ -225,7 +228,7 @@ class BatchContext:
                     outaged_branches=None, x0_per_scenario=None, reactive_limits_seed_state=None, max_iter=15,
                     tol=1e-4, reactive_limit_tol=1e-4, state_vector_scaling_mode=SCALING_NONE, max_dv=0.1,
                     max_dphi=0.174533, max_outer_iterations=20, max_pq_pv_switch=3, skip_reactive_limits=False,
-                    states_to_host=True):
+                    states_to_host=True, x0_per_scenario_device=None):
         keep = []
 
         def arr(a, dt):
@@ -258,7 +261,7 @@ class BatchContext:
         si = SolveInputs(n_scenarios, ptr(x0), ptr(tg), ptr(md), 0 if rme is None else rme.shape[0], ptr(rme),
                          0 if ti is None else ti.size, ptr(ti), ptr(tv), mo, ptr(ob), ptr(xp), ptr(sd), max_iter, tol,
                          reactive_limit_tol, state_vector_scaling_mode, max_dv, max_dphi, max_outer_iterations,
-                         max_pq_pv_switch, int(skip_reactive_limits), int(states_to_host))
+                         max_pq_pv_switch, int(skip_reactive_limits), int(states_to_host), x0_per_scenario_device)
         return si, keep
 
     def solve(self, inputs: SolveInputs, want_states=True):

@@ -71,6 +71,11 @@ class Network:
     def n(self) -> int:
         return 2 * self.n_bus
 
+    def scaled(self, factor: float) -> "Network":
+        """The same grid with every load and the generation scaled by `factor` (generation stays 1.02 x load)."""
+        import dataclasses
+        return dataclasses.replace(self, pload=self.pload * factor, qload=self.qload * factor, pgen=self.pgen * factor)
+
     # ---- packing (the solver's inputs) ----
     def pi_model(self) -> np.ndarray:
         """Closed-branch PiModel pack, 8 doubles per branch: y, g1, g2, b1, b2, ksi, r1, a1."""

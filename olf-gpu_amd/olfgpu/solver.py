@@ -163,6 +163,7 @@ class BatchScenarioResult:
     islanded_count: np.ndarray
     chunks: int
     device_seconds: float
+    warm_start_fallbacks: int = 0
 
 
 class GpuBatchedSecurityAnalysisSolver:
@@ -211,8 +212,11 @@ class GpuBatchedSecurityAnalysisSolver:
         return self._ctx
 
     def solve_batch_n1(self, x_base, target_base, row_mode_base, limit_types_base, contingencies,
-                       x0_per_scenario=None, states_to_host=True) -> BatchScenarioResult:
-        """solveBatchN1: contingencies is [K] or [K, m] closed-branch ids (-1 padded)."""
+                       x0_per_scenario=None, states_to_host=True, warm_start=None,
+                       warm_start_features=None) -> BatchScenarioResult:
+        """solveBatchN1: contingencies is [K] or [K, m] closed-branch ids (-1 padded). With a warm_start network
+        (olfgpu.warmstart.WarmStartMLP, single outages) every scenario starts from its prediction, and any that does
+        not converge is solved again from the base-case start."""
         import time
 
         ctg = np.asarray(contingencies, dtype=np.int32)
@@ -232,30 +236,52 @@ class GpuBatchedSecurityAnalysisSolver:
         lt = None if limit_types_base is None else np.asarray(limit_types_base, np.int8)
         device_s = 0.0
         chunks = 0
-        for c0 in range(0, k, chunk):
-            c1 = min(k, c0 + chunk)
-            part = ctg[c0:c1]
+        fallback = 0
+
+        def solve_part(part, x0_host=None, x0_dev=None):
+            nonlocal device_s, chunks, seed
             rme, tix, tvl, cnt = self.plan_islands(part)
-            islanded[c0:c1] = cnt
             if lt is not None and lt.size:
-                seed = np.tile(lt, (c1 - c0, 1))
+                seed = np.tile(lt, (len(part), 1))
             inp, keep = N.BatchContext.make_inputs(
-                c1 - c0, x_base, target_base, row_mode_base, row_mode_entries=rme, target_overrides=(tix, tvl),
-                outaged_branches=part, x0_per_scenario=None if x0_per_scenario is None else x0_per_scenario[c0:c1],
-                reactive_limits_seed_state=seed, max_iter=nr.maxIterations, tol=nr.maxMismatch,
-                reactive_limit_tol=nr.maxReactivePowerMismatch, state_vector_scaling_mode=nr.scaling_code(),
-                max_dv=nr.maxVoltageChangeV, max_dphi=nr.maxVoltageChangePhi,
-                max_outer_iterations=nr.maxOuterLoopIterations, max_pq_pv_switch=nr.maxPqPvSwitch,
-                skip_reactive_limits=not nr.reactiveLimits, states_to_host=states_to_host)
-            ctx = self.context(c1 - c0)
+                len(part), x_base, target_base, row_mode_base, row_mode_entries=rme, target_overrides=(tix, tvl),
+                outaged_branches=part, x0_per_scenario=x0_host, reactive_limits_seed_state=seed,
+                max_iter=nr.maxIterations, tol=nr.maxMismatch, reactive_limit_tol=nr.maxReactivePowerMismatch,
+                state_vector_scaling_mode=nr.scaling_code(), max_dv=nr.maxVoltageChangeV,
+                max_dphi=nr.maxVoltageChangePhi, max_outer_iterations=nr.maxOuterLoopIterations,
+                max_pq_pv_switch=nr.maxPqPvSwitch, skip_reactive_limits=not nr.reactiveLimits,
+                states_to_host=states_to_host, x0_per_scenario_device=x0_dev)
+            ctx = self.context(len(part))
             t0 = time.perf_counter()
             r = ctx.solve(inp, want_states=states_to_host)
             device_s += time.perf_counter() - t0
             chunks += 1
+            return r, cnt
+
+        for c0 in range(0, k, chunk):
+            c1 = min(k, c0 + chunk)
+            part = ctg[c0:c1]
+            x0_dev = None
+            if warm_start is not None:
+                x0_dev = warm_start.predict_warm_start(x_base, part[:, 0], warm_start_features[c0:c1])
+            r, cnt = solve_part(part, None if x0_per_scenario is None else x0_per_scenario[c0:c1], x0_dev)
+            islanded[c0:c1] = cnt
             st[c0:c1] = r["status"]
             it[c0:c1] = r["iterations"]
             tot[c0:c1] = r["total_iterations"]
             nm[c0:c1] = r["norm"]
             if states_to_host:
                 xs[c0:c1] = r["x"]
-        return BatchScenarioResult(st, it, tot, nm, xs, islanded, chunks, device_s)
+            if warm_start is not None:
+                # a warm start that did not converge: solve again from the base-case start, whose answer stands
+                redo = np.nonzero(r["status"] != 0)[0]
+                if redo.size:
+                    fallback += redo.size
+                    r2, _ = solve_part(part[redo])
+                    st[c0 + redo] = r2["status"]
+                    it[c0 + redo] = r2["iterations"]
+                    tot[c0 + redo] += r2["total_iterations"]
+                    nm[c0 + redo] = r2["norm"]
+                    if states_to_host:
+                        xs[c0 + redo] = r2["x"]
+        return BatchScenarioResult(st, it, tot, nm, xs, islanded, chunks, device_s, fallback)
