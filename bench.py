@@ -68,6 +68,9 @@ def main():
     ap.add_argument("--lanes-per-tile", type=int, default=0)
     ap.add_argument("--threads-per-tile", type=int, default=0)
     ap.add_argument("--autotune", action="store_true", help="time every tile shape first and use the fastest")
+    ap.add_argument("--no-warm-start", action="store_true", help="base-case start only (no learned warm start)")
+    ap.add_argument("--train-samples", type=int, default=20000)
+    ap.add_argument("--train-epochs", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: the N-1 batch shards by contingency across GPUs "
@@ -109,25 +112,63 @@ def main():
         sa.lanes_per_tile = tuned["lanes_per_tile"]
         sa.threads_per_tile = tuned["threads_per_tile"]
     ctx = sa.context(K)
+
+    # learned warm start: trained on post-contingency states the solver itself computes (load scalings U(0.9, 1.1)),
+    # before the timed region
+    mlp = None
+    warm_info = None
+    if not args.no_warm_start:
+        from olfgpu import warmstart as W
+        tw = time.perf_counter()
+        data = W.make_training_set(net, structure, n_samples=args.train_samples, levels=40, seed=3)
+        tgen = time.perf_counter() - tw
+        mlp, hist = W.train(data, net.n_branch, hidden=512, epochs=args.train_epochs, batch=256, lr=1e-3, seed=0)
+        feats = W.endpoint_features(net, xb, ctg, 1.0, gridgen.bridges(net))
+        warm_info = {"training_samples": int(len(data["branch"])), "generation_s": round(tgen, 1),
+                     "training_s": round(time.perf_counter() - tw - tgen, 1), "epochs": args.train_epochs,
+                     "loss_first_last": [round(hist[0], 4), round(hist[-1], 4)], "hidden": 512}
     setup_s = time.perf_counter() - t_setup
 
+    def one_step():
+        """One batched N-1 of the whole set: warm starts (network inference on the device), the solve, and a
+        base-start solve of any warm-started scenario that did not converge."""
+        ms_total, launches = 0.0, 0
+        if mlp is not None:
+            inputs.x0_per_scenario_device = mlp.predict_warm_start(xb, ctg, feats)
+        r = ctx.solve(inputs, want_states=False)
+        ms, st = ctx.timing()
+        ms_total += ms
+        launches += st
+        fallbacks = 0
+        if mlp is not None:
+            redo = np.nonzero(r["status"] != 0)[0]
+            if redo.size:
+                fallbacks = int(redo.size)
+                rr = sa.solve_batch_n1(xb, tb, mb, lt, ctg[redo], states_to_host=False)
+                r["status"][redo] = rr.status
+                r["total_iterations"][redo] += rr.total_iterations
+        return r, ms_total, launches, fallbacks
+
     for _ in range(args.warmup):
-        ctx.solve(inputs, want_states=False)
+        one_step()
     device_synchronize()
     t0 = time.perf_counter()
     step_ms = 0.0
     steps_launched = 0
     last = None
+    fallbacks = 0
     for _ in range(args.steps):
-        last = ctx.solve(inputs, want_states=False)
-        ms, st = ctx.timing()
+        last, ms, st, fb = one_step()
         step_ms += ms
         steps_launched += st
+        fallbacks += fb
     device_synchronize()
     wall = time.perf_counter() - t0
     value = K * args.steps / wall
     status = last["status"]
     lane_iters = int(last["total_iterations"].sum())
+    if warm_info is not None:
+        warm_info["fallbacks_per_step"] = fallbacks / args.steps
 
     # plain FP64 one-case-at-a-time GPU NR on a sample (the same library at batch 1, FP64, base-case start)
     plain = GpuBatchedSecurityAnalysisSolver(structure, rl, GpuLoadFlowParameters(linearSolver="BATCHED_LU"), nrp)
@@ -178,11 +219,13 @@ def main():
         "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "fp32-lu+fp64-residual", "data": "synthetic",
         "config": {"workload": "grid9241 x1.1 load, 2000 single-branch outages (5% bridges, seed 2), FP32 block LU + "
-                               "1 FP64 refinement, reactive limits on, base-case start (no learned warm start yet)",
+                               "1 FP64 refinement, reactive limits on, " +
+                               ("base-case start" if mlp is None else "learned warm start (base-start fallback)"),
                    "buses": net.n_bus, "branches": net.n_branch, "contingencies": K, "tol_pu": nrp.maxMismatch,
                    "tile": ctx_lanes(ctx), "lu_levels": stats["levels"], "lu_nnz": stats["lu_nnz"]},
         "statuses": {N.STATUS_NAMES[int(s)]: int((status == s).sum()) for s in np.unique(status)},
         "newton_lane_iterations_per_step": lane_iters,
+        "warm_start": warm_info,
         "plain_fp64_gpu": {"value": round(plain_value, 2), "unit": "contingencies/s", "sample": len(sample),
                            "statuses_match_batched": plain_match},
         "speedup_vs_plain_fp64_gpu": round(value / plain_value, 2),
