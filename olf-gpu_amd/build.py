@@ -21,6 +21,11 @@ def build(verbose=True):
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(LIB + ".tmp", LIB)
+    # the HBM-counter calibration program (a plain copy of known size)
+    probe = os.path.join(HERE, "hbm_copy")
+    subprocess.run([HIPCC, "--offload-arch=gfx950", "-O3", "-Xclang", "-target-feature", "-Xclang",
+                    "-packed-fp32-ops", os.path.join(CSRC, "hbm_copy.hip"), "-o", probe + ".tmp"], check=True)
+    os.replace(probe + ".tmp", probe)
     return LIB
 
 

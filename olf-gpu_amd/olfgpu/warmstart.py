@@ -209,8 +209,13 @@ def train(data, n_branch, hidden=512, epochs=20, batch=256, lr=1e-3, seed=0, ver
     n = tg.shape[1]
     x_mean = xb.mean(0)
     x_std = np.maximum(xb.std(0), 1e-3)
-    y_mean = tg.mean(0)
-    y_std = np.maximum(tg.std(0), 1e-5)
+    # one scale per variable type (angle rows, voltage rows), no offset: a row's change is large only near the
+    # outage, and those large changes are what decide whether a start converges in one Newton step — a per-row scale
+    # would blow up rows that hardly move
+    y_mean = np.zeros(n)
+    y_std = np.empty(n)
+    y_std[0::2] = max(float(np.sqrt((tg[:, 0::2].astype(np.float64) ** 2).mean())), 1e-6)
+    y_std[1::2] = max(float(np.sqrt((tg[:, 1::2].astype(np.float64) ** 2).mean())), 1e-6)
     net = WarmStartMLP(n, n_branch, ft.shape[1], hidden, seed)
     net.set_normalization(x_mean, x_std, y_mean, y_std)
     tgn = ((tg - y_mean) / y_std).astype(np.float32)
