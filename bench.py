@@ -70,7 +70,7 @@ def main():
     ap.add_argument("--autotune", action="store_true", help="time every tile shape first and use the fastest")
     ap.add_argument("--no-warm-start", action="store_true", help="base-case start only (no learned warm start)")
     ap.add_argument("--train-samples", type=int, default=20000)
-    ap.add_argument("--train-epochs", type=int, default=20)
+    ap.add_argument("--train-epochs", type=int, default=60)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: the N-1 batch shards by contingency across GPUs "
@@ -120,13 +120,15 @@ def main():
     if not args.no_warm_start:
         from olfgpu import warmstart as W
         tw = time.perf_counter()
-        data = W.make_training_set(net, structure, n_samples=args.train_samples, levels=40, seed=3)
+        mlp = W.WarmStartMLP(net, hidden=512, seed=0)
+        data = W.make_training_set(net, structure, mlp.hoods, n_samples=args.train_samples, levels=20, seed=3)
         tgen = time.perf_counter() - tw
-        mlp, hist = W.train(data, net.n_branch, hidden=512, epochs=args.train_epochs, batch=256, lr=1e-3, seed=0)
+        hist = W.train(mlp, data, epochs=args.train_epochs, batch=128, lr=2e-3, seed=0)
         feats = W.endpoint_features(net, xb, ctg, 1.0, gridgen.bridges(net))
         warm_info = {"training_samples": int(len(data["branch"])), "generation_s": round(tgen, 1),
                      "training_s": round(time.perf_counter() - tw - tgen, 1), "epochs": args.train_epochs,
-                     "loss_first_last": [round(hist[0], 4), round(hist[-1], 4)], "hidden": 512}
+                     "loss_first_last": [round(hist[0], 4), round(hist[-1], 4)], "hidden": 512,
+                     "neighbourhood_buses": mlp.n_local}
     setup_s = time.perf_counter() - t_setup
 
     def one_step():

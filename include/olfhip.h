@@ -216,35 +216,44 @@ int olfhip_autotune(olfhip_structure *structure, int mode, const olfhip_reactive
 
 /* ---- Learned warm start (no reference counterpart: the reference seeds scenarios from the base case or from
  * caller-given states only, BatchSolveNative.java:119). A 3-layer MLP, hidden width `hidden`, GELU, BF16 weights on
- * the matrix cores with FP32 accumulation and FP32 master weights (Adam). Input: a scenario's base-case state
- * (n_state values, normalised), its outaged branch (an embedding row, the one-hot input of the first layer) and
- * n_features endpoint features; output: the normalised change of the state. The Newton solve it seeds stays the
- * sole judge of the answer. */
+ * the matrix cores with FP32 accumulation and FP32 master weights (Adam). It works on each outage's neighbourhood:
+ * the n_local buses nearest the outaged branch by the sum of |weight| (e.g. branch reactance) along paths, endpoints
+ * first, -1 padded. Input: the neighbourhood's base-case angles relative to endpoint 1 and voltages minus 1
+ * (2 n_local rows, normalised), the outaged branch (an embedding row, the one-hot input of the first layer) and
+ * n_features endpoint features; output: 2 n_local rows [angle changes ; voltage changes], normalised. The Newton solve
+ * it seeds stays the sole judge of the answer. */
 typedef struct olfhip_mlp olfhip_mlp;
 
-int olfhip_mlp_create(int n_state, int n_branch, int n_features, int hidden, unsigned long long seed,
-                      olfhip_mlp **out);
+int olfhip_mlp_create(int n_bus, int n_branch, const int *bus1, const int *bus2, const double *weight, int n_local,
+                      int n_features, int hidden, unsigned long long seed, olfhip_mlp **out);
 void olfhip_mlp_destroy(olfhip_mlp *mlp);
-/* x_norm = (x - x_mean) / x_std on input; prediction = y * y_std + y_mean (all [n_state], FP32) */
+/* the neighbourhoods olfhip_mlp_create builds, into buses [n_branch][n_local] (host computation, no device use):
+ * multi-source Dijkstra over |weight| from both endpoints; endpoint 1, endpoint 2, then buses in order of settlement
+ * (ties by bus index); -1 padded when the component is smaller than n_local */
+int olfhip_neighbourhoods(int n_bus, int n_branch, const int *bus1, const int *bus2, const double *weight,
+                          int n_local, int *buses);
+/* x_norm = (x_loc - x_mean) / x_std on input; prediction = y * y_std + y_mean (all [2 n_local], FP32) */
 int olfhip_mlp_set_normalization(olfhip_mlp *mlp, const float *x_mean, const float *x_std, const float *y_mean,
                                  const float *y_std);
-/* one Adam step on a batch: x_base [batch][n_state], branch [batch], features [batch][n_features], target
- * [batch][n_state] (normalised); returns the batch's mean squared error */
-int olfhip_mlp_train_step(olfhip_mlp *mlp, int batch, const double *x_base, const int *branch, const float *features,
+/* the base states the batches refer to: x_base [n_rows][2 n_bus] (stable rows), copied to the device */
+int olfhip_mlp_set_base_states(olfhip_mlp *mlp, int n_rows, const double *x_base);
+/* one Adam step on a batch: base_row [batch] (NULL: row 0), branch [batch], features [batch][n_features], target
+ * [batch][2 n_local] (normalised); returns the batch's mean squared error */
+int olfhip_mlp_train_step(olfhip_mlp *mlp, int batch, const int *base_row, const int *branch, const float *features,
                           const float *target, float learning_rate, double *loss);
-/* normalised outputs y_out [batch][n_state] */
-int olfhip_mlp_forward(olfhip_mlp *mlp, int batch, const double *x_base, const int *branch, const float *features,
+/* normalised outputs y_out [batch][2 n_local] */
+int olfhip_mlp_forward(olfhip_mlp *mlp, int batch, const int *base_row, const int *branch, const float *features,
                        float *y_out);
 /* loss gradient of parameter array `which` (0 Win, 1 Emb, 2 b1, 3 W2, 4 b2, 5 W3, 6 b3) for a batch, no update */
-int olfhip_mlp_gradients(olfhip_mlp *mlp, int batch, const double *x_base, const int *branch, const float *features,
+int olfhip_mlp_gradients(olfhip_mlp *mlp, int batch, const int *base_row, const int *branch, const float *features,
                          const float *target, int which, float *grad_out, double *loss);
 /* read (values_out) and/or write (values_in) parameter array `which` (FP32 masters; BF16 copies refreshed) */
 int olfhip_mlp_parameters(olfhip_mlp *mlp, int which, float *values_out, const float *values_in);
-/* warm starts x0 = x_base + y * y_std + y_mean for `batch` scenarios into a device buffer owned by the network (valid
- * until its next call), to pass as olfhip_solve_inputs.x0_per_scenario_device; x_base is [n_state] shared by all
- * scenarios (x_base_per_scenario = 0) or [batch][n_state] */
-int olfhip_mlp_predict_warm_start(olfhip_mlp *mlp, int batch, const double *x_base, int x_base_per_scenario,
-                                  const int *branch, const float *features, double **x0_device);
+/* warm starts for `batch` scenarios into a device buffer owned by the network (valid until its next call), to pass as
+ * olfhip_solve_inputs.x0_per_scenario_device: x0 = the base state of base_row[b] (NULL: row 0), plus
+ * (y * y_std + y_mean) * scale[b] (scale NULL: 1) on the rows of the outage's neighbourhood */
+int olfhip_mlp_predict_warm_start(olfhip_mlp *mlp, int batch, const int *base_row, const int *branch,
+                                  const float *features, const float *scale, double **x0_device);
 /* C = A B on the matrix-core GEMM of the network (A [M][K], B [K][N] row-major FP32, rounded to BF16), for tests */
 int olfhip_mlp_gemm_test(int M, int N, int K, const float *A, const float *B, float *C);
 

@@ -1,14 +1,20 @@
 // Learned warm start: a 3-layer MLP trained and served by hand-written gfx950 kernels (BF16 operands on the matrix
-// cores, FP32 accumulation, FP32 master weights with Adam). It maps a scenario's base-case state, its outaged branch
-// (an embedding row: the one-hot input of the first layer) and a few endpoint features to the normalised
-// post-contingency state change; the prediction seeds the Newton solve through the per-scenario x0 hook
-// (BatchSolveNative x0PerScenario, BatchSolveNative.java:119). The Newton mismatch stays the sole judge of the answer.
+// cores, FP32 accumulation, FP32 master weights with Adam). The prediction seeds the Newton solve through the
+// per-scenario x0 hook (BatchSolveNative x0PerScenario, BatchSolveNative.java:119); the Newton mismatch stays the
+// sole judge of the answer.
 //
-//   z1 = Win [x_norm ; f] + Emb[branch] + b1,  h1 = gelu(z1)      Win: H x (N + F), Emb: E x H
-//   z2 = W2 h1 + b2,                           h2 = gelu(z2)      W2:  H x H
-//   y  = W3 h2 + b3                                               W3:  N x H
-// with N state rows, F endpoint features, E branches, H hidden units; loss = mean over batch and rows of
-// (y - target)^2 on normalised targets.
+// The network works on the outage's neighbourhood: the K buses electrically nearest the outaged branch (multi-source
+// Dijkstra over |x| from both endpoints, endpoints first, -1 padded), computed once per branch at creation. Its N = 2K
+// rows are [angle of slot 0..K-1 ; voltage of slot 0..K-1]:
+//   input  x_loc = [phi(hood) - phi(endpoint 1) ; V(hood) - 1] of the base state, normalised per row
+//   z1 = Win [x_loc ; f] + Emb[branch] + b1,  h1 = gelu(z1)      Win: H x (N + F), Emb: E x H
+//   z2 = W2 h1 + b2,                          h2 = gelu(z2)      W2:  H x H
+//   y  = W3 h2 + b3                                              W3:  N x H
+// with F endpoint features, E branches, H hidden units; loss = mean over batch and rows of (y - target)^2 on
+// normalised targets. The warm start keeps the base state outside the neighbourhood and adds
+// (y * y_std + y_mean) * scale_b to the neighbourhood's rows, scale_b a per-scenario factor given by the caller.
+// A Newton step's remaining error depends on the start's branch-wise differences, so a local prediction with no
+// step at its border (angles taken relative to the neighbourhood's outer ring) is what pays.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -209,28 +215,62 @@ __global__ void adam(long n, float *w, const float *g, float *m, float *v, unsig
     }
 }
 
-// layer-1 input rows [x_norm ; f] as BF16: x_norm = (x_base - mean) / std
-__global__ void buildInput(int B, int N, int F, const double *xBase, long xStride, const float *xMean,
-                           const float *xStd, const float *feat, unsigned short *out) {
+// The neighbourhood's local value of row c (c < K: angle of slot c relative to slot 0; c >= K: voltage of slot c - K
+// minus 1) in base state x (stable rows: 2 bus = angle, 2 bus + 1 = voltage); 0 for a padding slot.
+__device__ __forceinline__ double localValue(const double *x, const int *hood, int K, int c) {
+    if (c < K) {
+        const int bus = hood[c];
+        return bus < 0 ? 0.0 : x[2 * (long)bus] - x[2 * (long)hood[0]];
+    }
+    const int bus = hood[c - K];
+    return bus < 0 ? 0.0 : x[2 * (long)bus + 1] - 1.0;
+}
+
+// layer-1 input rows [x_loc_norm ; f] as BF16: x_loc_norm = (x_loc - mean) / std, x_loc from base state row
+// baseRow[b] (row 0 when baseRow is null) of xBase [rows][nFull]
+__global__ void buildInput(int B, int K, int F, const double *xBase, long nFull, const int *baseRow, const int *branch,
+                           const int *hoods, const float *xMean, const float *xStd, const float *feat,
+                           unsigned short *out) {
+    const int N = 2 * K;
     const long D = (long)N + F;
     const long n = (long)B * D;
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
         const int b = (int)(i / D);
-        const long c = i % D;
+        const int c = (int)(i % D);
         float v;
-        if (c < N) v = (float)((xBase[(long)b * xStride + c] - (double)xMean[c]) / (double)xStd[c]);
-        else v = feat[(long)b * F + (c - N)];
+        if (c < N) {
+            const double *x = xBase + (baseRow ? (long)baseRow[b] : 0L) * nFull;
+            const double loc = localValue(x, hoods + (long)branch[b] * K, K, c);
+            v = (float)((loc - (double)xMean[c]) / (double)xStd[c]);
+        } else {
+            v = feat[(long)b * F + (c - N)];
+        }
         out[i] = toBf16(v);
     }
 }
 
-// x0[b][r] = x_base[r] + y[b][r] * std_r + mean_r (the warm start of scenario b)
-__global__ void warmStartOut(int B, int N, const double *xBase, long xStride, const float *y, const float *yMean,
-                             const float *yStd, double *x0) {
+// x0[b] = base state of scenario b (the rows outside its neighbourhood keep their base value)
+__global__ void warmStartBase(int B, long nFull, const double *xBase, const int *baseRow, double *x0) {
+    const long n = (long)B * nFull;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const int b = (int)(i / nFull);
+        const long r = i % nFull;
+        x0[i] = xBase[(baseRow ? (long)baseRow[b] : 0L) * nFull + r];
+    }
+}
+
+// x0[b][row of slot c] += (y[b][c] * std_c + mean_c) * scale_b over the neighbourhood (slots of one neighbourhood are
+// distinct buses, so no two threads add to one element)
+__global__ void warmStartScatter(int B, int K, long nFull, const int *branch, const int *hoods, const float *y,
+                                 const float *yMean, const float *yStd, const float *scale, double *x0) {
+    const int N = 2 * K;
     const long n = (long)B * N;
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-        const int b = (int)(i / N), r = (int)(i % N);
-        x0[i] = xBase[(long)b * xStride + r] + (double)y[i] * (double)yStd[r] + (double)yMean[r];
+        const int b = (int)(i / N), c = (int)(i % N);
+        const int bus = hoods[(long)branch[b] * K + (c < K ? c : c - K)];
+        if (bus < 0) continue;
+        const double d = ((double)y[i] * (double)yStd[c] + (double)yMean[c]) * (scale ? (double)scale[b] : 1.0);
+        x0[(long)b * nFull + 2 * (long)bus + (c < K ? 0 : 1)] += d;
     }
 }
 
@@ -254,8 +294,80 @@ struct Buf {
 
 static int gridOf(long n) { return (int)std::max<long>(1, std::min<long>((n + 255) / 256, 8192)); }
 
+// Neighbourhoods of every branch: the K buses nearest the branch by the sum of |weight| along paths, from both
+// endpoints at distance 0 (endpoint 1, endpoint 2, then in order of settlement; ties by bus index), -1 padded when the
+// component is smaller than K.
+static std::vector<int> neighbourhoods(int nBus, int E, const int *bus1, const int *bus2, const double *weight, int K) {
+    std::vector<int> deg(nBus + 1, 0);
+    for (int e = 0; e < E; e++) {
+        deg[bus1[e] + 1]++;
+        deg[bus2[e] + 1]++;
+    }
+    for (int i = 0; i < nBus; i++) deg[i + 1] += deg[i];
+    std::vector<int> adj(2 * (size_t)E), fill(deg.begin(), deg.end() - 1);
+    std::vector<double> w(2 * (size_t)E);
+    for (int e = 0; e < E; e++) {
+        adj[fill[bus1[e]]] = bus2[e];
+        w[fill[bus1[e]]++] = std::fabs(weight[e]);
+        adj[fill[bus2[e]]] = bus1[e];
+        w[fill[bus2[e]]++] = std::fabs(weight[e]);
+    }
+    std::vector<int> out((size_t)E * K, -1);
+    std::vector<double> dist(nBus, INFINITY);
+    std::vector<char> done(nBus, 0);
+    std::vector<int> touched;
+    typedef std::pair<double, int> QE;
+    std::vector<QE> heap;
+    auto cmp = [](const QE &a, const QE &b) { return a.first > b.first || (a.first == b.first && a.second > b.second); };
+    for (int e = 0; e < E; e++) {
+        int *h = &out[(size_t)e * K];
+        int k = 0;
+        h[k++] = bus1[e];
+        if (bus2[e] != bus1[e] && k < K) h[k++] = bus2[e];
+        heap.clear();
+        for (int s = 0; s < k; s++) {
+            dist[h[s]] = 0.0;
+            touched.push_back(h[s]);
+            heap.push_back(QE(0.0, h[s]));
+        }
+        std::make_heap(heap.begin(), heap.end(), cmp);
+        int settled = 0;
+        while (!heap.empty() && settled < K) {
+            std::pop_heap(heap.begin(), heap.end(), cmp);
+            const QE q = heap.back();
+            heap.pop_back();
+            const int u = q.second;
+            if (done[u] || q.first > dist[u]) continue;
+            done[u] = 1;
+            settled++;
+            if (u != bus1[e] && u != bus2[e]) h[k++] = u;
+            for (int p = deg[u]; p < deg[u + 1]; p++) {
+                const int v = adj[p];
+                const double nd = q.first + w[p];
+                if (nd < dist[v]) {
+                    if (dist[v] == INFINITY) touched.push_back(v);
+                    dist[v] = nd;
+                    heap.push_back(QE(nd, v));
+                    std::push_heap(heap.begin(), heap.end(), cmp);
+                }
+            }
+        }
+        for (int v : touched) {
+            dist[v] = INFINITY;
+            done[v] = 0;
+        }
+        touched.clear();
+    }
+    return out;
+}
+
 struct Mlp {
-    int N = 0, E = 0, F = 0, H = 0;
+    int K = 0, N = 0, E = 0, F = 0, H = 0;   // N = 2K network rows
+    long nFull = 0;                           // state rows of the grid (2 per bus)
+    std::vector<int> hoodsHost;
+    Buf hoods;                                // [E][K]
+    Buf bases;                                // [baseRows][nFull] base states
+    int baseRows = 0;
     long D() const { return (long)N + F; }
     // parameters: FP32 masters, BF16 copies of the GEMM weights, Adam moments, gradients
     struct Param {
@@ -266,7 +378,8 @@ struct Mlp {
     Buf xMean, xStd, yMean, yStd;
     // activations for a batch of up to cap
     int cap = 0;
-    Buf X, c1, z1, h1, c2, z2, h2, y, dy, dyh, dh2, dz2, dz2h, dh1, dz1, dz1h, tgt, branch, feat, loss, x0;
+    Buf X, c1, z1, h1, c2, z2, h2, y, dy, dyh, dh2, dz2, dz2h, dh1, dz1, dz1h, tgt, branch, baseRow, feat, scale, loss,
+        x0;
     long step = 0;
     hipStream_t s = nullptr;
 
@@ -295,8 +408,9 @@ struct Mlp {
         for (Buf *b : {&h1, &h2, &dz2h, &dz1h}) b->alloc((size_t)B * H * 2);
         for (Buf *b : {&y, &dy, &tgt}) b->alloc((size_t)B * N * 4);
         dyh.alloc((size_t)B * N * 2);
-        x0.alloc((size_t)B * N * 8);
         branch.alloc((size_t)B * 4);
+        baseRow.alloc((size_t)B * 4);
+        scale.alloc((size_t)B * 4);
         feat.alloc((size_t)B * std::max(F, 1) * 4);
         loss.alloc(8);
     }
@@ -360,13 +474,24 @@ struct Mlp {
                                               p->v.as<float>(), p->bf.p ? p->bf.as<unsigned short>() : nullptr, lr, b1c,
                                               b2c, c1, c2, 1e-8f);
     }
-    void upload(int B, const double *xBaseDev, long xStride, const int *br, const float *ft, const float *tg) {
+    // checks a batch's arguments, uploads them and builds the layer-1 input; rows may be null (base row 0)
+    void upload(int B, const int *rows, const int *br, const float *ft, const float *tg) {
+        if (B <= 0) throw ArgumentError("batch must be positive");
+        if (!br || (F > 0 && !ft)) throw ArgumentError("null argument");
+        if (baseRows == 0) throw ArgumentError("no base states set (olfhip_mlp_set_base_states)");
+        for (int b = 0; b < B; b++) {
+            if (br[b] < 0 || br[b] >= E) throw ArgumentError("branch id out of range");
+            if (rows && (rows[b] < 0 || rows[b] >= baseRows)) throw ArgumentError("base row out of range");
+        }
         ensureBatch(B);
         MLP_OK(hipMemcpyAsync(branch.p, br, (size_t)B * 4, hipMemcpyHostToDevice, s));
+        if (rows) MLP_OK(hipMemcpyAsync(baseRow.p, rows, (size_t)B * 4, hipMemcpyHostToDevice, s));
         if (F > 0) MLP_OK(hipMemcpyAsync(feat.p, ft, (size_t)B * F * 4, hipMemcpyHostToDevice, s));
         if (tg) MLP_OK(hipMemcpyAsync(tgt.p, tg, (size_t)B * N * 4, hipMemcpyHostToDevice, s));
-        buildInput<<<gridOf((long)B * D()), 256, 0, s>>>(B, N, F, xBaseDev, xStride, xMean.as<float>(),
-                                                        xStd.as<float>(), feat.as<float>(), X.as<unsigned short>());
+        buildInput<<<gridOf((long)B * D()), 256, 0, s>>>(B, K, F, bases.as<double>(), nFull,
+                                                        rows ? baseRow.as<int>() : nullptr, branch.as<int>(),
+                                                        hoods.as<int>(), xMean.as<float>(), xStd.as<float>(),
+                                                        feat.as<float>(), X.as<unsigned short>());
     }
     ~Mlp() {
         if (s) (void)hipStreamDestroy(s);
@@ -395,47 +520,49 @@ static int mlpGuarded(Fn &&fn) {
     }
 }
 
-// per-call device copy of a host base-state array ([rows][stride] doubles)
-struct HostX {
-    Buf d;
-    void put(const double *h, size_t n, hipStream_t s) {
-        d.alloc(n * 8);
-        MLP_OK(hipMemcpyAsync(d.p, h, n * 8, hipMemcpyHostToDevice, s));
-    }
-};
-
 extern "C" {
 
-int olfhip_mlp_create(int n_state, int n_branch, int n_features, int hidden, unsigned long long seed,
-                      olfhip_mlp **out) {
+int olfhip_mlp_create(int n_bus, int n_branch, const int *bus1, const int *bus2, const double *weight, int n_local,
+                      int n_features, int hidden, unsigned long long seed, olfhip_mlp **out) {
     return mlpGuarded([&] {
         if (!out) throw ArgumentError("out is null");
         *out = nullptr;
-        if (n_state <= 0 || n_branch <= 0 || n_features < 0 || hidden <= 0 || hidden % 16)
-            throw ArgumentError("sizes must be positive and hidden a multiple of 16");
+        if (!bus1 || !bus2 || !weight) throw ArgumentError("null argument");
+        if (n_bus <= 0 || n_branch <= 0 || n_local < 2 || n_local > n_bus || n_features < 0 || hidden <= 0 ||
+            hidden % 16)
+            throw ArgumentError("sizes must be positive, 2 <= n_local <= n_bus and hidden a multiple of 16");
+        for (int e = 0; e < n_branch; e++)
+            if (bus1[e] < 0 || bus1[e] >= n_bus || bus2[e] < 0 || bus2[e] >= n_bus)
+                throw ArgumentError("branch end out of range");
         std::unique_ptr<Mlp> m(new Mlp());
-        m->N = n_state;
+        m->K = n_local;
+        m->N = 2 * m->K;
+        m->nFull = 2L * n_bus;
         m->E = n_branch;
         m->F = n_features;
         m->H = hidden;
+        m->hoodsHost = neighbourhoods(n_bus, n_branch, bus1, bus2, weight, m->K);
         MLP_OK(hipStreamCreateWithFlags(&m->s, hipStreamNonBlocking));
+        m->hoods.alloc(m->hoodsHost.size() * 4);
+        MLP_OK(hipMemcpy(m->hoods.p, m->hoodsHost.data(), m->hoodsHost.size() * 4, hipMemcpyHostToDevice));
         std::mt19937_64 rng(seed);
+        const int N = m->N;
         const float sIn = 1.f / std::sqrt((float)m->D()), sH = 1.f / std::sqrt((float)hidden);
         m->initParam(m->Win, (long)hidden * m->D(), sIn, rng, true);
         m->initParam(m->Emb, (long)n_branch * hidden, 0.1f, rng, false);
         m->initParam(m->b1, hidden, 0.f, rng, false);
         m->initParam(m->W2, (long)hidden * hidden, sH, rng, true);
         m->initParam(m->b2, hidden, 0.f, rng, false);
-        m->initParam(m->W3, (long)n_state * hidden, sH, rng, true);
-        m->initParam(m->b3, n_state, 0.f, rng, false);
-        std::vector<float> zeros(n_state, 0.f), ones(n_state, 1.f);
+        m->initParam(m->W3, (long)N * hidden, sH, rng, true);
+        m->initParam(m->b3, N, 0.f, rng, false);
+        std::vector<float> zeros(N, 0.f), ones(N, 1.f);
         for (Buf *b : {&m->xMean, &m->yMean}) {
-            b->alloc(n_state * 4);
-            MLP_OK(hipMemcpy(b->p, zeros.data(), n_state * 4, hipMemcpyHostToDevice));
+            b->alloc(N * 4);
+            MLP_OK(hipMemcpy(b->p, zeros.data(), N * 4, hipMemcpyHostToDevice));
         }
         for (Buf *b : {&m->xStd, &m->yStd}) {
-            b->alloc(n_state * 4);
-            MLP_OK(hipMemcpy(b->p, ones.data(), n_state * 4, hipMemcpyHostToDevice));
+            b->alloc(N * 4);
+            MLP_OK(hipMemcpy(b->p, ones.data(), N * 4, hipMemcpyHostToDevice));
         }
         MLP_OK(hipStreamSynchronize(m->s));
         *out = reinterpret_cast<olfhip_mlp *>(m.release());
@@ -443,6 +570,20 @@ int olfhip_mlp_create(int n_state, int n_branch, int n_features, int hidden, uns
 }
 
 void olfhip_mlp_destroy(olfhip_mlp *mlp) { delete reinterpret_cast<Mlp *>(mlp); }
+
+int olfhip_neighbourhoods(int n_bus, int n_branch, const int *bus1, const int *bus2, const double *weight,
+                          int n_local, int *buses) {
+    return mlpGuarded([&] {
+        if (!bus1 || !bus2 || !weight || !buses) throw ArgumentError("null argument");
+        if (n_bus <= 0 || n_branch <= 0 || n_local < 2 || n_local > n_bus)
+            throw ArgumentError("sizes must be positive and 2 <= n_local <= n_bus");
+        for (int e = 0; e < n_branch; e++)
+            if (bus1[e] < 0 || bus1[e] >= n_bus || bus2[e] < 0 || bus2[e] >= n_bus)
+                throw ArgumentError("branch end out of range");
+        const std::vector<int> h = neighbourhoods(n_bus, n_branch, bus1, bus2, weight, n_local);
+        std::memcpy(buses, h.data(), h.size() * 4);
+    });
+}
 
 int olfhip_mlp_set_normalization(olfhip_mlp *mlp, const float *x_mean, const float *x_std, const float *y_mean,
                                  const float *y_std) {
@@ -458,17 +599,24 @@ int olfhip_mlp_set_normalization(olfhip_mlp *mlp, const float *x_mean, const flo
     });
 }
 
-int olfhip_mlp_train_step(olfhip_mlp *mlp, int batch, const double *x_base, const int *branch, const float *features,
+int olfhip_mlp_set_base_states(olfhip_mlp *mlp, int n_rows, const double *x_base) {
+    return mlpGuarded([&] {
+        Mlp *m = reinterpret_cast<Mlp *>(mlp);
+        if (!m || !x_base) throw ArgumentError("null argument");
+        if (n_rows <= 0) throw ArgumentError("n_rows must be positive");
+        MLP_OK(hipStreamSynchronize(m->s));
+        m->bases.alloc((size_t)n_rows * m->nFull * 8);
+        MLP_OK(hipMemcpy(m->bases.p, x_base, (size_t)n_rows * m->nFull * 8, hipMemcpyHostToDevice));
+        m->baseRows = n_rows;
+    });
+}
+
+int olfhip_mlp_train_step(olfhip_mlp *mlp, int batch, const int *base_row, const int *branch, const float *features,
                           const float *target, float learning_rate, double *loss) {
     return mlpGuarded([&] {
         Mlp *m = reinterpret_cast<Mlp *>(mlp);
-        if (!m || !x_base || !branch || !target || (m->F > 0 && !features)) throw ArgumentError("null argument");
-        if (batch <= 0) throw ArgumentError("batch must be positive");
-        for (int b = 0; b < batch; b++)
-            if (branch[b] < 0 || branch[b] >= m->E) throw ArgumentError("branch id out of range");
-        HostX hx;
-        hx.put(x_base, (size_t)batch * m->N, m->s);
-        m->upload(batch, hx.d.as<double>(), m->N, branch, features, target);
+        if (!m || !target) throw ArgumentError("null argument");
+        m->upload(batch, base_row, branch, features, target);
         m->forward(batch);
         const double l = m->backward(batch);
         m->adamStep(learning_rate);
@@ -477,34 +625,26 @@ int olfhip_mlp_train_step(olfhip_mlp *mlp, int batch, const double *x_base, cons
     });
 }
 
-int olfhip_mlp_forward(olfhip_mlp *mlp, int batch, const double *x_base, const int *branch, const float *features,
+int olfhip_mlp_forward(olfhip_mlp *mlp, int batch, const int *base_row, const int *branch, const float *features,
                        float *y_out) {
     return mlpGuarded([&] {
         Mlp *m = reinterpret_cast<Mlp *>(mlp);
-        if (!m || !x_base || !branch || !y_out || (m->F > 0 && !features)) throw ArgumentError("null argument");
-        if (batch <= 0) throw ArgumentError("batch must be positive");
-        for (int b = 0; b < batch; b++)
-            if (branch[b] < 0 || branch[b] >= m->E) throw ArgumentError("branch id out of range");
-        HostX hx;
-        hx.put(x_base, (size_t)batch * m->N, m->s);
-        m->upload(batch, hx.d.as<double>(), m->N, branch, features, nullptr);
+        if (!m || !y_out) throw ArgumentError("null argument");
+        m->upload(batch, base_row, branch, features, nullptr);
         m->forward(batch);
         MLP_OK(hipMemcpyAsync(y_out, m->y.p, (size_t)batch * m->N * 4, hipMemcpyDeviceToHost, m->s));
         MLP_OK(hipStreamSynchronize(m->s));
     });
 }
 
-int olfhip_mlp_gradients(olfhip_mlp *mlp, int batch, const double *x_base, const int *branch, const float *features,
+int olfhip_mlp_gradients(olfhip_mlp *mlp, int batch, const int *base_row, const int *branch, const float *features,
                          const float *target, int which, float *grad_out, double *loss) {
     return mlpGuarded([&] {
         Mlp *m = reinterpret_cast<Mlp *>(mlp);
-        if (!m || !x_base || !branch || !target || !grad_out) throw ArgumentError("null argument");
-        if (batch <= 0) throw ArgumentError("batch must be positive");
+        if (!m || !target || !grad_out) throw ArgumentError("null argument");
         Mlp::Param *ps[7] = {&m->Win, &m->Emb, &m->b1, &m->W2, &m->b2, &m->W3, &m->b3};
         if (which < 0 || which > 6) throw ArgumentError("which selects one of the 7 parameter arrays");
-        HostX hx;
-        hx.put(x_base, (size_t)batch * m->N, m->s);
-        m->upload(batch, hx.d.as<double>(), m->N, branch, features, target);
+        m->upload(batch, base_row, branch, features, target);
         m->forward(batch);
         const double l = m->backward(batch);
         MLP_OK(hipMemcpy(grad_out, ps[which]->g.p, ps[which]->n * 4, hipMemcpyDeviceToHost));
@@ -528,22 +668,24 @@ int olfhip_mlp_parameters(olfhip_mlp *mlp, int which, float *values_out, const f
     });
 }
 
-int olfhip_mlp_predict_warm_start(olfhip_mlp *mlp, int batch, const double *x_base, int x_base_per_scenario,
-                                  const int *branch, const float *features, double **x0_device) {
+int olfhip_mlp_predict_warm_start(olfhip_mlp *mlp, int batch, const int *base_row, const int *branch,
+                                  const float *features, const float *scale, double **x0_device) {
     return mlpGuarded([&] {
         Mlp *m = reinterpret_cast<Mlp *>(mlp);
-        if (!m || !x_base || !branch || !x0_device || (m->F > 0 && !features)) throw ArgumentError("null argument");
-        if (batch <= 0) throw ArgumentError("batch must be positive");
-        for (int b = 0; b < batch; b++)
-            if (branch[b] < 0 || branch[b] >= m->E) throw ArgumentError("branch id out of range");
-        HostX hx;
-        const long stride = x_base_per_scenario ? m->N : 0;
-        hx.put(x_base, x_base_per_scenario ? (size_t)batch * m->N : (size_t)m->N, m->s);
-        m->upload(batch, hx.d.as<double>(), stride, branch, features, nullptr);
+        if (!m || !x0_device) throw ArgumentError("null argument");
+        if (scale)
+            for (int b = 0; b < batch; b++)
+                if (!std::isfinite(scale[b])) throw ArgumentError("scale must be finite");
+        m->upload(batch, base_row, branch, features, nullptr);
+        if (scale) MLP_OK(hipMemcpyAsync(m->scale.p, scale, (size_t)batch * 4, hipMemcpyHostToDevice, m->s));
         m->forward(batch);
-        warmStartOut<<<gridOf((long)batch * m->N), 256, 0, m->s>>>(batch, m->N, hx.d.as<double>(), stride,
-                                                                   m->y.as<float>(), m->yMean.as<float>(),
-                                                                   m->yStd.as<float>(), m->x0.as<double>());
+        m->x0.alloc((size_t)batch * m->nFull * 8);
+        const int *rows = base_row ? m->baseRow.as<int>() : nullptr;
+        warmStartBase<<<gridOf((long)batch * m->nFull), 256, 0, m->s>>>(batch, m->nFull, m->bases.as<double>(), rows,
+                                                                        m->x0.as<double>());
+        warmStartScatter<<<gridOf((long)batch * m->N), 256, 0, m->s>>>(
+            batch, m->K, m->nFull, m->branch.as<int>(), m->hoods.as<int>(), m->y.as<float>(), m->yMean.as<float>(),
+            m->yStd.as<float>(), scale ? m->scale.as<float>() : nullptr, m->x0.as<double>());
         MLP_OK(hipStreamSynchronize(m->s));
         *x0_device = m->x0.as<double>();
     });

@@ -25,9 +25,9 @@ EXPORTS = [
     "olfhip_create_batch_context", "olfhip_destroy_batch_context", "olfhip_solve_batch_context",
     "olfhip_read_states", "olfhip_islanded_buses", "olfhip_lu_factorize_and_solve", "olfhip_structure_pattern",
     "olfhip_autotune", "olfhip_context_timing", "olfhip_read_outer_loop_state",
-    "olfhip_mlp_create", "olfhip_mlp_destroy", "olfhip_mlp_set_normalization", "olfhip_mlp_train_step",
-    "olfhip_mlp_forward", "olfhip_mlp_gradients", "olfhip_mlp_parameters", "olfhip_mlp_predict_warm_start",
-    "olfhip_mlp_gemm_test",
+    "olfhip_neighbourhoods", "olfhip_mlp_create", "olfhip_mlp_destroy", "olfhip_mlp_set_normalization",
+    "olfhip_mlp_set_base_states", "olfhip_mlp_train_step", "olfhip_mlp_forward", "olfhip_mlp_gradients",
+    "olfhip_mlp_parameters", "olfhip_mlp_predict_warm_start", "olfhip_mlp_gemm_test",
 ]
 
 

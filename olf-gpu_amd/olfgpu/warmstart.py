@@ -1,9 +1,16 @@
 """Learned warm start for the batched N-1: the network (over the C ABI), its features, its training set made by the
 Newton solver itself, and its training loop.
 
-The network maps a scenario's base-case state (all bus V/phi), its outaged branch (embedding) and features of the
-branch's endpoints to the post-contingency state change. Training samples are post-contingency states the batched
-solver computes under random load scalings; a warm-started scenario that does not converge is solved again from the
+The network predicts the post-contingency change on the outage's neighbourhood (the n_local buses electrically
+nearest the outaged branch) from the neighbourhood's base state, the branch (an embedding row) and endpoint features.
+What it learns is the response shape per unit of the branch's base flow: targets are divided by the flow scale
+``|S1| + 0.01`` (S1 the base-case complex flow at side 1, pu) and the prediction is multiplied by it, so one
+training level serves every load level. Angle changes are taken relative to the mean change of the neighbourhood's
+outer quarter, so the start has no step where the neighbourhood ends: a Newton step's remaining error depends on the
+start's branch-wise differences, and a step there would cost more than the prediction gains.
+
+Training samples are post-contingency states the batched solver computes under random load scalings; every branch of
+the grid is covered before any is repeated. A warm-started scenario that does not converge is solved again from the
 base-case start, so the converged Newton mismatch stays the sole judge of every answer.
 """
 from __future__ import annotations
@@ -14,28 +21,34 @@ import time
 import numpy as np
 
 from . import _native as N
-from .gridgen import SB
 
 PARAMS = ("Win", "Emb", "b1", "W2", "b2", "W3", "b3")
 N_FEATURES = 16
+N_LOCAL = 96
 
 
 def _lib():
     L = N.lib()
     if not getattr(L, "_mlp_bound", False):
         vp, i, d, f = C.c_void_p, C.c_int, C.c_double, C.c_float
-        L.olfhip_mlp_create.argtypes = [i, i, i, i, C.c_ulonglong, C.POINTER(vp)]
+        L.olfhip_neighbourhoods.argtypes = [i, i, vp, vp, vp, i, vp]
+        L.olfhip_mlp_create.argtypes = [i, i, vp, vp, vp, i, i, i, C.c_ulonglong, C.POINTER(vp)]
         L.olfhip_mlp_destroy.argtypes = [vp]
         L.olfhip_mlp_destroy.restype = None
         L.olfhip_mlp_set_normalization.argtypes = [vp, vp, vp, vp, vp]
+        L.olfhip_mlp_set_base_states.argtypes = [vp, i, vp]
         L.olfhip_mlp_train_step.argtypes = [vp, i, vp, vp, vp, vp, f, C.POINTER(d)]
         L.olfhip_mlp_forward.argtypes = [vp, i, vp, vp, vp, vp]
         L.olfhip_mlp_gradients.argtypes = [vp, i, vp, vp, vp, vp, i, vp, C.POINTER(d)]
         L.olfhip_mlp_parameters.argtypes = [vp, i, vp, vp]
-        L.olfhip_mlp_predict_warm_start.argtypes = [vp, i, vp, i, vp, vp, C.POINTER(C.c_void_p)]
+        L.olfhip_mlp_predict_warm_start.argtypes = [vp, i, vp, vp, vp, vp, C.POINTER(C.c_void_p)]
         L.olfhip_mlp_gemm_test.argtypes = [i, i, i, vp, vp, vp]
         L._mlp_bound = True
     return L
+
+
+def _ptr_or_none(a):
+    return None if a is None else N.ptr(a)
 
 
 def gemm_test(A, B):
@@ -46,16 +59,69 @@ def gemm_test(A, B):
     return Cm
 
 
-class WarmStartMLP:
-    """Owner of an olfhip_mlp."""
+def neighbourhoods(n_bus, bus1, bus2, weight, n_local):
+    """[E][n_local] bus ids of every branch's neighbourhood (host computation in the library)."""
+    b1 = np.ascontiguousarray(bus1, np.int32)
+    b2 = np.ascontiguousarray(bus2, np.int32)
+    w = np.ascontiguousarray(weight, np.float64)
+    out = np.zeros((len(b1), n_local), np.int32)
+    N.check(_lib().olfhip_neighbourhoods(n_bus, len(b1), N.ptr(b1), N.ptr(b2), N.ptr(w), n_local, N.ptr(out)))
+    return out
 
-    def __init__(self, n_state, n_branch, n_features=N_FEATURES, hidden=512, seed=0):
-        self.n_state, self.n_branch, self.n_features, self.hidden = n_state, n_branch, n_features, hidden
+
+def flow_scale(features):
+    """Per-scenario output scale |S1| + 0.01 from endpoint features 5 and 6 (base-case P1, Q1 in pu)."""
+    f = np.asarray(features, np.float64)
+    return (np.sqrt(f[:, 5] ** 2 + f[:, 6] ** 2) + 0.01).astype(np.float32)
+
+
+def local_state(x_rows, base_row, hoods, branch):
+    """[B][2K] local values of the base states: angles relative to endpoint 1, voltages minus 1 (0 on padding)."""
+    h = hoods[np.asarray(branch)]
+    x = np.asarray(x_rows, np.float64)[np.asarray(base_row)]
+    valid = h >= 0
+    hc = np.where(valid, h, 0)
+    th = np.take_along_axis(x, 2 * hc, 1) - np.take_along_axis(x, 2 * hc[:, :1], 1)
+    v = np.take_along_axis(x, 2 * hc + 1, 1) - 1.0
+    return np.concatenate([np.where(valid, th, 0.0), np.where(valid, v, 0.0)], 1)
+
+
+def local_change(dx, hoods, branch):
+    """[B][2K] neighbourhood change of full state changes dx [B][n]: angle changes minus the mean change of the
+    neighbourhood's outer quarter (slots [3k/4, k) of its k valid slots), voltage changes; 0 on padding."""
+    h = hoods[np.asarray(branch)]
+    valid = h >= 0
+    hc = np.where(valid, h, 0)
+    dth = np.where(valid, np.take_along_axis(dx, 2 * hc, 1), 0.0)
+    dv = np.where(valid, np.take_along_axis(dx, 2 * hc + 1, 1), 0.0)
+    k = valid.sum(1)
+    slot = np.arange(h.shape[1])[None, :]
+    ring = valid & (slot >= (3 * k // 4)[:, None])
+    ref = (dth * ring).sum(1) / np.maximum(ring.sum(1), 1)
+    dth = np.where(valid, dth - ref[:, None], 0.0)
+    return np.concatenate([dth, dv], 1)
+
+
+class WarmStartMLP:
+    """Owner of an olfhip_mlp for one grid (gridgen.Network)."""
+
+    def __init__(self, net, n_local=N_LOCAL, n_features=N_FEATURES, hidden=512, seed=0):
+        self.n_bus, self.n_branch = net.n_bus, net.n_branch
+        self.n_local = min(n_local, net.n_bus)
+        self.n_features, self.hidden = n_features, hidden
+        self.n_rows = 2 * self.n_local
+        b1 = np.ascontiguousarray(net.bus1, np.int32)
+        b2 = np.ascontiguousarray(net.bus2, np.int32)
+        w = np.ascontiguousarray(net.x, np.float64)
         h = C.c_void_p()
-        N.check(_lib().olfhip_mlp_create(n_state, n_branch, n_features, hidden, seed, C.byref(h)))
+        N.check(_lib().olfhip_mlp_create(net.n_bus, net.n_branch, N.ptr(b1), N.ptr(b2), N.ptr(w), self.n_local,
+                                         n_features, hidden, seed, C.byref(h)))
         self.h = h
-        self.shapes = {"Win": (hidden, n_state + n_features), "Emb": (n_branch, hidden), "b1": (hidden,),
-                       "W2": (hidden, hidden), "b2": (hidden,), "W3": (n_state, hidden), "b3": (n_state,)}
+        self.hoods = neighbourhoods(net.n_bus, b1, b2, w, self.n_local)
+        D = self.n_rows + n_features
+        self.shapes = {"Win": (hidden, D), "Emb": (net.n_branch, hidden), "b1": (hidden,), "W2": (hidden, hidden),
+                       "b2": (hidden,), "W3": (self.n_rows, hidden), "b3": (self.n_rows,)}
+        self._base = None
 
     def close(self):
         if getattr(self, "h", None):
@@ -73,32 +139,37 @@ class WarmStartMLP:
         N.check(_lib().olfhip_mlp_set_normalization(self.h, *[N.ptr(a) for a in arrs]))
         self.norm = arrs
 
-    def _batch(self, x_base, branch, feat):
-        x = np.ascontiguousarray(x_base, np.float64)
-        br = np.ascontiguousarray(branch, np.int32)
-        ft = np.ascontiguousarray(feat, np.float32).reshape(len(br), self.n_features)
-        return x, br, ft
+    def set_base_states(self, x_rows):
+        x = np.ascontiguousarray(np.atleast_2d(x_rows), np.float64)
+        N.check(_lib().olfhip_mlp_set_base_states(self.h, x.shape[0], N.ptr(x)))
+        self._base = x.copy()
 
-    def train_step(self, x_base, branch, feat, target, lr):
-        x, br, ft = self._batch(x_base, branch, feat)
+    def _batch(self, base_row, branch, feat):
+        br = np.ascontiguousarray(branch, np.int32)
+        rows = None if base_row is None else np.ascontiguousarray(base_row, np.int32)
+        ft = np.ascontiguousarray(feat, np.float32).reshape(len(br), self.n_features)
+        return rows, br, ft
+
+    def train_step(self, base_row, branch, feat, target, lr):
+        rows, br, ft = self._batch(base_row, branch, feat)
         tg = np.ascontiguousarray(target, np.float32)
         loss = C.c_double()
-        N.check(_lib().olfhip_mlp_train_step(self.h, len(br), N.ptr(x), N.ptr(br), N.ptr(ft), N.ptr(tg), lr,
-                                             C.byref(loss)))
+        N.check(_lib().olfhip_mlp_train_step(self.h, len(br), _ptr_or_none(rows), N.ptr(br), N.ptr(ft), N.ptr(tg),
+                                             lr, C.byref(loss)))
         return loss.value
 
-    def forward(self, x_base, branch, feat):
-        x, br, ft = self._batch(x_base, branch, feat)
-        y = np.zeros((len(br), self.n_state), np.float32)
-        N.check(_lib().olfhip_mlp_forward(self.h, len(br), N.ptr(x), N.ptr(br), N.ptr(ft), N.ptr(y)))
+    def forward(self, base_row, branch, feat):
+        rows, br, ft = self._batch(base_row, branch, feat)
+        y = np.zeros((len(br), self.n_rows), np.float32)
+        N.check(_lib().olfhip_mlp_forward(self.h, len(br), _ptr_or_none(rows), N.ptr(br), N.ptr(ft), N.ptr(y)))
         return y
 
-    def gradients(self, x_base, branch, feat, target, which):
-        x, br, ft = self._batch(x_base, branch, feat)
+    def gradients(self, base_row, branch, feat, target, which):
+        rows, br, ft = self._batch(base_row, branch, feat)
         tg = np.ascontiguousarray(target, np.float32)
         g = np.zeros(self.shapes[which], np.float32)
         loss = C.c_double()
-        N.check(_lib().olfhip_mlp_gradients(self.h, len(br), N.ptr(x), N.ptr(br), N.ptr(ft), N.ptr(tg),
+        N.check(_lib().olfhip_mlp_gradients(self.h, len(br), _ptr_or_none(rows), N.ptr(br), N.ptr(ft), N.ptr(tg),
                                             PARAMS.index(which), N.ptr(g), C.byref(loss)))
         return g, loss.value
 
@@ -111,16 +182,22 @@ class WarmStartMLP:
         v = np.ascontiguousarray(values, np.float32).reshape(self.shapes[which])
         N.check(_lib().olfhip_mlp_parameters(self.h, PARAMS.index(which), None, N.ptr(v)))
 
-    def predict_warm_start(self, x_base, branch, feat):
-        """Device pointer (int) to [K][n_state] start states for olfhip_solve_inputs.x0_per_scenario_device."""
-        x = np.ascontiguousarray(x_base, np.float64)
-        per = 1 if x.ndim == 2 else 0
-        br = np.ascontiguousarray(branch, np.int32)
-        ft = np.ascontiguousarray(feat, np.float32).reshape(len(br), self.n_features)
+    def predict_raw(self, branch, feat, scale=None, base_row=None):
+        """Device pointer (int) to [B][n] start states from the base states already set."""
+        rows, br, ft = self._batch(base_row, branch, feat)
+        sc = None if scale is None else np.ascontiguousarray(scale, np.float32)
         out = C.c_void_p()
-        N.check(_lib().olfhip_mlp_predict_warm_start(self.h, len(br), N.ptr(x), per, N.ptr(br), N.ptr(ft),
-                                                     C.byref(out)))
+        N.check(_lib().olfhip_mlp_predict_warm_start(self.h, len(br), _ptr_or_none(rows), N.ptr(br), N.ptr(ft),
+                                                     _ptr_or_none(sc), C.byref(out)))
         return out.value
+
+    def predict_warm_start(self, x_base, branch, feat):
+        """Device pointer (int) to [B][n] start states for olfhip_solve_inputs.x0_per_scenario_device: x_base plus
+        the predicted neighbourhood change, scaled by each outage's base flow."""
+        x = np.asarray(x_base, np.float64).reshape(1, -1)
+        if self._base is None or self._base.shape != x.shape or not np.array_equal(self._base, x):
+            self.set_base_states(x)
+        return self.predict_raw(branch, feat, flow_scale(feat))
 
 
 def endpoint_features(net, x_base, branches, load_scale, bridge_mask=None):
@@ -157,44 +234,53 @@ def endpoint_features(net, x_base, branches, load_scale, bridge_mask=None):
     return f
 
 
-def make_training_set(net, structure, n_samples=20000, levels=40, seed=3, scale_range=(0.9, 1.1),
-                      bridge_fraction=0.05, verbose=False):
-    """Post-contingency states solved by the batched Newton solver itself, under uniform random load scalings.
+def make_training_set(net, structure, hoods, n_samples=20000, levels=20, seed=3, scale_range=(0.9, 1.1),
+                      verbose=False):
+    """Post-contingency states solved by the batched Newton solver itself, at `levels` load scalings drawn from
+    U(scale_range); the outages are consecutive slices of one random permutation of all branches, so every branch is
+    covered before any repeats.
 
-    Returns a dict with x_base [levels, n], level index, branch, features and the state change target [S, n]
-    (islanded rows set to zero change: they are frozen whatever the start)."""
+    Returns a dict with the base states x_base [levels, n], per sample its level, branch, endpoint features and
+    neighbourhood target (local_change / flow_scale, unnormalised) [S, 2K]. Islanded buses count as unchanged (the
+    solver freezes them whatever the start); outages whose solve does not converge are left out."""
     from . import gridgen
     from .solver import GpuBatchedSecurityAnalysisSolver, GpuLoadFlowParameters, GpuNewtonRaphson
 
     rng = np.random.default_rng(seed)
     scales = rng.uniform(scale_range[0], scale_range[1], levels)
     per = int(np.ceil(n_samples / levels))
+    order = rng.permutation(net.n_branch)
     bridges = gridgen.bridges(net)
     xb_all, lvl, brs, feats, targets = [], [], [], [], []
     t0 = time.perf_counter()
+    pos = 0
     for k, s in enumerate(scales):
+        idx = (pos + np.arange(per)) % net.n_branch
+        pos = (pos + per) % net.n_branch
+        ctg = order[idx].astype(np.int32)
         ns = net.scaled(s)
         rl = ns.reactive_limits()
         x0, tg, md, _ = ns.system()
         base = GpuNewtonRaphson(structure, rl).run(x0, tg, md)
         if base.status != "CONVERGED":
             continue
-        ctg = gridgen.sample_contingencies(net, per, bridge_fraction, int(rng.integers(1 << 31)))
-        sa = GpuBatchedSecurityAnalysisSolver(structure, rl, GpuLoadFlowParameters(linearSolver="BATCHED_LU_FP32_REFINED"))
+        sa = GpuBatchedSecurityAnalysisSolver(structure, rl,
+                                              GpuLoadFlowParameters(linearSolver="BATCHED_LU_FP32_REFINED"))
         res = sa.solve_batch_n1(base.x, base.target, base.row_mode, base.reactive_limit_types, ctg)
         ok = res.status == 0
+        c = ctg[ok]
         d = res.x[ok] - base.x[None, :]
-        for j, e in enumerate(ctg[ok]):
+        for j, e in enumerate(c):
             if bridges[e]:
                 isl = structure.islanded_buses([e])
                 d[j, 0::2][isl] = 0.0
                 d[j, 1::2][isl] = 0.0
+        f = endpoint_features(net, base.x, c, s, bridges)
         xb_all.append(base.x)
-        li = len(xb_all) - 1
-        lvl.append(np.full(ok.sum(), li, np.int32))
-        brs.append(ctg[ok])
-        feats.append(endpoint_features(net, base.x, ctg[ok], s, bridges))
-        targets.append(d.astype(np.float32))
+        lvl.append(np.full(len(c), len(xb_all) - 1, np.int32))
+        brs.append(c)
+        feats.append(f)
+        targets.append((local_change(d, hoods, c) / flow_scale(f)[:, None]).astype(np.float32))
         if verbose:
             print(f"  level {k}: scale {s:.3f}, {ok.sum()}/{len(ctg)} converged, {time.perf_counter() - t0:.1f}s",
                   flush=True)
@@ -203,21 +289,20 @@ def make_training_set(net, structure, n_samples=20000, levels=40, seed=3, scale_
                 seconds=time.perf_counter() - t0)
 
 
-def train(data, n_branch, hidden=512, epochs=20, batch=256, lr=1e-3, seed=0, verbose=False):
-    """Fit a WarmStartMLP to a training set; returns (network, loss history)."""
+def train(mlp, data, epochs=60, batch=128, lr=2e-3, seed=0, verbose=False):
+    """Fit `mlp` to a training set (cosine learning-rate decay); returns the loss history per epoch."""
     xb, lvl, br, ft, tg = data["x_base"], data["level"], data["branch"], data["features"], data["target"]
-    n = tg.shape[1]
-    x_mean = xb.mean(0)
-    x_std = np.maximum(xb.std(0), 1e-3)
-    # one scale per variable type (angle rows, voltage rows), no offset: a row's change is large only near the
-    # outage, and those large changes are what decide whether a start converges in one Newton step — a per-row scale
-    # would blow up rows that hardly move
-    y_mean = np.zeros(n)
-    y_std = np.empty(n)
-    y_std[0::2] = max(float(np.sqrt((tg[:, 0::2].astype(np.float64) ** 2).mean())), 1e-6)
-    y_std[1::2] = max(float(np.sqrt((tg[:, 1::2].astype(np.float64) ** 2).mean())), 1e-6)
-    net = WarmStartMLP(n, n_branch, ft.shape[1], hidden, seed)
-    net.set_normalization(x_mean, x_std, y_mean, y_std)
+    K = mlp.n_local
+    xl = local_state(xb, lvl, mlp.hoods, br)
+    x_mean = xl.mean(0)
+    x_std = np.maximum(xl.std(0), 1e-3)
+    # one output scale per variable type, no offset
+    y_mean = np.zeros(2 * K)
+    y_std = np.empty(2 * K)
+    y_std[:K] = max(float(np.sqrt((tg[:, :K].astype(np.float64) ** 2).mean())), 1e-6)
+    y_std[K:] = max(float(np.sqrt((tg[:, K:].astype(np.float64) ** 2).mean())), 1e-6)
+    mlp.set_normalization(x_mean, x_std, y_mean, y_std)
+    mlp.set_base_states(xb)
     tgn = ((tg - y_mean) / y_std).astype(np.float32)
     rng = np.random.default_rng(seed)
     hist = []
@@ -230,9 +315,10 @@ def train(data, n_branch, hidden=512, epochs=20, batch=256, lr=1e-3, seed=0, ver
         nb = 0
         for q in range(0, S - batch + 1, batch):
             idx = order[q:q + batch]
-            tot += net.train_step(xb[lvl[idx]], br[idx], ft[idx], tgn[idx], lr_ep)
+            tot += mlp.train_step(lvl[idx], br[idx], ft[idx], tgn[idx], lr_ep)
             nb += 1
         hist.append(tot / max(nb, 1))
         if verbose:
             print(f"  epoch {ep}: loss {hist[-1]:.4f} ({time.perf_counter() - t0:.1f}s)", flush=True)
-    return net, hist
+    mlp._base = None
+    return hist

@@ -1,12 +1,17 @@
 """CPU oracle of the warm-start network (numpy, FP64) — TEST INFRASTRUCTURE ONLY.
 
-The same forward and backward passes as the device kernels (olf-gpu_amd/csrc/mlp.hip): a 3-layer MLP with GELU,
-an embedding row added in the first layer, mean squared error on normalised targets. With ``emulate_bf16`` the
+The same computations as the device code (olf-gpu_amd/csrc/mlp.hip): the outage neighbourhoods (multi-source
+Dijkstra, restated with heapq), the layer-1 input built from a base state's neighbourhood, the 3-layer MLP with GELU
+and an embedding row added in the first layer, mean squared error on normalised targets, and the warm start it
+produces. The neighbourhoods are pinned by brute force (all-pairs shortest paths on small graphs) in
+tests/test_mlp_oracle.py, the gradients by finite differences. With ``emulate_bf16`` the
 matrix operands are rounded to BF16 (round to nearest even, from FP32) where the device rounds them, so the device
 gradients can be compared tightly; without it the passes are plain FP64, which the finite-difference test pins.
 The reference has no learned warm start, so nothing here restates reference code.
 """
 from __future__ import annotations
+
+import heapq
 
 import numpy as np
 from scipy.special import erf
@@ -34,10 +39,72 @@ def _r(a, on):
     return bf16(a) if on else np.asarray(a, dtype=np.float64)
 
 
-def inputs(x_base, x_mean, x_std, feat, emulate_bf16=True):
-    xn = ((np.asarray(x_base, np.float64) - x_mean) / x_std).astype(np.float32).astype(np.float64)
+def neighbourhoods(n_bus, bus1, bus2, weight, k):
+    """[E][k]: endpoint 1, endpoint 2, then the buses settled by a Dijkstra over |weight| started from both
+    endpoints at distance 0, in order of (distance, bus id); -1 padded."""
+    adj = [[] for _ in range(n_bus)]
+    for a, b, w in zip(bus1, bus2, weight):
+        adj[int(a)].append((int(b), abs(float(w))))
+        adj[int(b)].append((int(a), abs(float(w))))
+    out = np.full((len(bus1), k), -1, np.int32)
+    for e, (a, b) in enumerate(zip(bus1, bus2)):
+        a, b = int(a), int(b)
+        ends = [a] if a == b else [a, b]
+        row = list(ends)
+        dist = {u: 0.0 for u in ends}
+        heap = [(0.0, u) for u in ends]
+        heapq.heapify(heap)
+        done = set()
+        while heap and len(done) < k:
+            d, u = heapq.heappop(heap)
+            if u in done or d > dist[u]:
+                continue
+            done.add(u)
+            if u not in ends:
+                row.append(u)
+            for v, w in adj[u]:
+                if d + w < dist.get(v, np.inf):
+                    dist[v] = d + w
+                    heapq.heappush(heap, (d + w, v))
+        out[e, :len(row)] = row[:k]
+    return out
+
+
+def local_values(x_rows, base_row, hoods, branch):
+    """[B][2k] neighbourhood values of the base states: angle minus endpoint 1's angle, voltage minus 1; 0 on
+    padding slots."""
+    B = len(branch)
+    k = hoods.shape[1]
+    out = np.zeros((B, 2 * k))
+    for i in range(B):
+        x = x_rows[0 if base_row is None else base_row[i]]
+        h = hoods[branch[i]]
+        for c in range(k):
+            if h[c] >= 0:
+                out[i, c] = x[2 * h[c]] - x[2 * h[0]]
+                out[i, k + c] = x[2 * h[c] + 1] - 1.0
+    return out
+
+
+def inputs(x_rows, base_row, hoods, branch, x_mean, x_std, feat, emulate_bf16=True):
+    loc = local_values(x_rows, base_row, hoods, branch)
+    xn = ((loc - x_mean) / x_std).astype(np.float32).astype(np.float64)
     X = np.concatenate([xn, np.asarray(feat, np.float64).reshape(len(xn), -1)], axis=1)
     return _r(X, emulate_bf16)
+
+
+def warm_start(x_rows, base_row, hoods, branch, y, y_mean, y_std, scale):
+    """[B][n] starts: the base state plus (y * y_std + y_mean) * scale on the neighbourhood's rows."""
+    k = hoods.shape[1]
+    out = np.array([x_rows[0 if base_row is None else base_row[i]] for i in range(len(branch))], np.float64)
+    for i in range(len(branch)):
+        h = hoods[branch[i]]
+        s = 1.0 if scale is None else float(scale[i])
+        for c in range(2 * k):
+            bus = h[c % k]
+            if bus >= 0:
+                out[i, 2 * bus + (c >= k)] += (float(y[i, c]) * float(y_std[c]) + float(y_mean[c])) * s
+    return out
 
 
 def forward(p, X, branch, emulate_bf16=True):

@@ -41,3 +41,52 @@ def test_bf16_rounding():
     assert mo.bf16(1.0 + 2.0 ** -7) == 1.0 + 2.0 ** -7          # representable
     x = np.random.default_rng(2).normal(size=1000)
     assert np.all(np.abs(mo.bf16(x) - x) <= np.abs(x) * 2.0 ** -8 + 1e-30)
+
+
+def _brute_neighbourhoods(n_bus, bus1, bus2, weight, k):
+    """All-pairs shortest paths (Floyd-Warshall), then per branch the endpoints and the k - 2 nearest other buses by
+    (distance to the nearer endpoint, bus id); -1 padded for buses out of reach."""
+    D = np.full((n_bus, n_bus), np.inf)
+    np.fill_diagonal(D, 0.0)
+    for a, b, w in zip(bus1, bus2, weight):
+        D[a, b] = min(D[a, b], abs(w))
+        D[b, a] = min(D[b, a], abs(w))
+    for m in range(n_bus):
+        D = np.minimum(D, D[:, m:m + 1] + D[m:m + 1, :])
+    out = np.full((len(bus1), k), -1, np.int32)
+    for e, (a, b) in enumerate(zip(bus1, bus2)):
+        ends = [a] if a == b else [a, b]
+        d = np.minimum(D[a], D[b])
+        rest = sorted((d[v], v) for v in range(n_bus) if v not in ends and np.isfinite(d[v]))
+        row = ends + [v for _, v in rest]
+        out[e, :min(k, len(row))] = row[:k]
+    return out
+
+
+def test_neighbourhoods_match_brute_force():
+    rng = np.random.default_rng(5)
+    for trial in range(6):
+        n = int(rng.integers(5, 25))
+        m = int(rng.integers(n, 3 * n))
+        bus1 = rng.integers(0, n, m)
+        bus2 = rng.integers(0, n, m)
+        keep = bus1 != bus2
+        bus1, bus2 = bus1[keep], bus2[keep]
+        # integer weights: exact path sums, so distance ties are real ties broken by bus id in both
+        w = rng.integers(1, 6, len(bus1)).astype(float) * np.where(rng.random(len(bus1)) < 0.2, -1, 1)
+        for k in (2, 5, n):
+            got = mo.neighbourhoods(n, bus1, bus2, w, k)
+            assert np.array_equal(got, _brute_neighbourhoods(n, bus1, bus2, w, k)), (trial, k)
+
+
+def test_local_values_and_warm_start_scatter():
+    # 4 buses, neighbourhood of branch 0 = [2, 0, 3, -1]; states are (phi, V) per bus
+    hoods = np.array([[2, 0, 3, -1]], np.int32)
+    x = np.array([[0.1, 1.01, -0.2, 0.99, 0.3, 1.02, 0.05, 0.97]])
+    loc = mo.local_values(x, None, hoods, [0])
+    np.testing.assert_allclose(loc[0], [0.0, 0.1 - 0.3, 0.05 - 0.3, 0.0, 0.02, 0.01, -0.03, 0.0], atol=1e-15)
+    y = np.arange(8, dtype=float).reshape(1, 8)
+    ws = mo.warm_start(x, None, hoods, [0], y, np.zeros(8), np.full(8, 0.5), [2.0])
+    d = ws[0] - x[0]
+    # slot c adds y[c] * 0.5 * 2 = y[c]: angle rows get slots 0..2, voltage rows slots 4..6; bus 1 untouched
+    np.testing.assert_allclose(d, [1.0, 5.0, 0.0, 0.0, 0.0, 4.0, 2.0, 6.0], atol=1e-15)

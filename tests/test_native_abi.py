@@ -96,3 +96,23 @@ def test_parameters_keep_the_reference_surface():
     assert NewtonRaphsonParameters().scaling_code() == N.SCALING_NONE
     with pytest.raises(ValueError):
         NewtonRaphsonParameters(stateVectorScalingMode="LINE_SEARCH").scaling_code()
+
+
+def test_outage_neighbourhoods_match_the_oracle(grid):
+    """The warm-start network's outage neighbourhoods (a host routine of the library) against the oracle's Dijkstra,
+    itself pinned by brute force."""
+    import mlp_oracle as mo
+    from olfgpu import warmstart as W
+
+    net = grid("grid118")
+    for k in (2, 17, 64):
+        got = W.neighbourhoods(net.n_bus, net.bus1, net.bus2, net.x, k)
+        assert np.array_equal(got, mo.neighbourhoods(net.n_bus, net.bus1, net.bus2, net.x, k)), k
+    big = grid("grid2000")
+    sel = np.arange(0, big.n_branch, 37)
+    got = W.neighbourhoods(big.n_bus, big.bus1, big.bus2, big.x, 96)[sel]
+    ref = mo.neighbourhoods(big.n_bus, big.bus1, big.bus2, big.x, 96)[sel]
+    assert np.array_equal(got, ref)
+    with pytest.raises(N.OlfHipError) as e:
+        W.neighbourhoods(net.n_bus, net.bus1, net.bus2, net.x, net.n_bus + 1)
+    assert e.value.code == N.E_ARGUMENT
