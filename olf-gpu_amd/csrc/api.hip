@@ -116,10 +116,66 @@ struct StructureImpl {
         dp.shuntB = put(P.shuntB);
         dp.perm = put(P.perm);
         dp.pos = put(P.pos);
+        dp.levelPos = put(P.levelPos);
+        dp.nodeLevel = put(P.nodeLevel);
+        dp.itemKI = reinterpret_cast<const int2 *>(put(P.itemKI));
         HIP_OK(hipDeviceSynchronize());
         onDevice = true;
     }
 };
+
+// The top block a launch of `kernel` (T lanes, NT threads, vtBytes per LU value) can hold in LDS without costing
+// occupancy: the LDS one workgroup may use when as many workgroups share a CU as the kernel's registers allow, less
+// the kernel's static LDS. OLFHIP_TOP_NODES caps the node count (0: no top block).
+struct TopPlan {
+    DevTop top{};
+    size_t ldsBytes = 0;
+    DBuf slots;
+};
+
+static void planTop(const StructureImpl *st, const void *kernel, int T, int NT, size_t vtBytes, TopPlan &tp) {
+    const Plan &P = st->plan;
+    hipFuncAttributes fa{};
+    HIP_OK(hipFuncGetAttributes(&fa, kernel));
+    int perCU = 0, perBlock = 0;
+    HIP_OK(hipDeviceGetAttribute(&perCU, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, st->device));
+    HIP_OK(hipDeviceGetAttribute(&perBlock, hipDeviceAttributeMaxSharedMemoryPerBlock, st->device));
+    const int vgpr = std::max(8, (fa.numRegs + 7) / 8 * 8);
+    const int wavesPerSimd = std::max(1, std::min(8, 512 / vgpr));
+    const int blocksPerCU = std::max(1, (4 * wavesPerSimd) / std::max(1, NT / 64));
+    long budget = std::min<long>(perBlock, perCU / blocksPerCU) - (long)fa.sharedSizeBytes - 256;
+    int maxNodes = 0;
+    while (budget > 0 && topLdsBytes(T, maxNodes + 1, vtBytes) <= (size_t)budget) maxNodes++;
+    if (const char *e = std::getenv("OLFHIP_TOP_NODES")) maxNodes = std::min(maxNodes, std::max(0, std::atoi(e)));
+    int n = 0;
+    const int l0 = maxNodes > 0 ? P.topLevel(maxNodes, &n) : P.nLevels;
+    if (l0 >= P.nLevels) n = 0;
+    tp.top = DevTop{};
+    tp.top.l0 = n > 0 ? l0 : P.nLevels;
+    tp.top.n = n;
+    tp.top.start = n > 0 ? P.levelNodePtr[l0] : P.nb;
+    tp.top.S = n > 0 ? P.topSplit(l0) : 1;
+    tp.ldsBytes = n > 0 ? topLdsBytes(T, n, vtBytes) : 0;
+    if (n > 0) {
+        std::vector<int> rec;
+        for (int a = 0; a < n; a++) {
+            const int k = P.levelNodes[tp.top.start + a];
+            rec.insert(rec.end(), {P.nodeBase[k], a, a, 0});
+            for (int q = P.sPtr[k]; q < P.sPtr[k + 1]; q++) {
+                const int b = P.levelPos[P.sIdx[q]] - tp.top.start;
+                if (b < 0 || b >= n) throw std::logic_error("top block is not closed under S");
+                const int slot = P.nodeBase[k] + 1 + 2 * (q - P.sPtr[k]);
+                rec.insert(rec.end(), {slot, b, a, 0});       // L~(i,k): row i, column k
+                rec.insert(rec.end(), {slot + 1, a, b, 0});   // U(k,i): row k, column i
+            }
+        }
+        tp.slots.upload(rec);
+        tp.top.slots = tp.slots.as<int4>();
+        tp.top.nSlots = (int)(rec.size() / 4);
+        if (tp.ldsBytes > 65536)
+            HIP_OK(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)tp.ldsBytes));
+    }
+}
 
 static void release(StructureImpl *s) {
     if (s && --s->refs == 0) delete s;
@@ -167,6 +223,7 @@ struct ContextImpl {
     hipEvent_t evStart = nullptr, evStop = nullptr;
     DBuf prof;                               // opt-in section stamps of tile 0 (OLFHIP_PROFILE=1)
     int profCap = 0;
+    TopPlan top;                             // the dense top block of the step kernel
 
     DevState devState() const {
         DevState S{};
@@ -275,6 +332,13 @@ static ContextImpl *createContext(StructureImpl *st, int nScenarios, int mode, c
     c->rlType.alloc(ne);
     c->rlCount.alloc(ne);
     c->rlPass.alloc(ne);
+    withT(c->T, [&](auto TC) {
+        constexpr int T = decltype(TC)::value;
+        if (mode == OLFHIP_LU_FP32_REFINED)
+            planTop(st, reinterpret_cast<const void *>(&nrStepKernel<T, float>), T, c->NT, 4, c->top);
+        else
+            planTop(st, reinterpret_cast<const void *>(&nrStepKernel<T, double>), T, c->NT, 8, c->top);
+    });
     HIP_OK(hipDeviceSynchronize());
     return c.release();
 }
@@ -390,9 +454,9 @@ static void solve(ContextImpl *c, const olfhip_solve_inputs *in, olfhip_solve_ou
         HIP_OK(hipGetLastError());
         auto enqueueStep = [&]() {
             if (c->mode == OLFHIP_LU_FP32_REFINED)
-                nrStepKernel<T, float><<<c->nTiles, c->NT, 0, s>>>(dp, S, R, prm);
+                nrStepKernel<T, float><<<c->nTiles, c->NT, c->top.ldsBytes, s>>>(dp, S, R, prm, c->top.top);
             else
-                nrStepKernel<T, double><<<c->nTiles, c->NT, 0, s>>>(dp, S, R, prm);
+                nrStepKernel<T, double><<<c->nTiles, c->NT, c->top.ldsBytes, s>>>(dp, S, R, prm, c->top.top);
             countRunningKernel<<<1, 256, 0, s>>>(S.tileRunning, c->nTiles, c->count.as<int>());
             HIP_OK(hipMemcpyAsync(c->pinnedCount, c->count.p, sizeof(int), hipMemcpyDeviceToHost, s));
         };
@@ -537,14 +601,20 @@ static void luFactorizeAndSolve(StructureImpl *st, int mode, int capacity, const
     dxo.alloc((size_t)nTiles * P.nb * 2 * T * vb);
     dcl.alloc((size_t)nTiles * T * 4);
     const DevPlan dp = st->dp;
+    TopPlan tp;
     withT(T, [&](auto TC) {
         constexpr int TT = decltype(TC)::value;
-        if (vb == 8)
-            luSolveKernel<TT, double><<<nTiles, 512>>>(dp, dlu.as<double>(), dy.as<double>(), dxo.as<double>(),
-                                                       drhs.as<double>(), dcl.as<int>());
-        else
-            luSolveKernel<TT, float><<<nTiles, 512>>>(dp, dlu.as<float>(), dy.as<float>(), dxo.as<float>(),
-                                                      drhs.as<double>(), dcl.as<int>());
+        if (vb == 8) {
+            planTop(st, reinterpret_cast<const void *>(&luSolveKernel<TT, double>), TT, 512, 8, tp);
+            luSolveKernel<TT, double><<<nTiles, 512, tp.ldsBytes>>>(dp, dlu.as<double>(), dy.as<double>(),
+                                                                    dxo.as<double>(), drhs.as<double>(),
+                                                                    dcl.as<int>(), tp.top);
+        } else {
+            planTop(st, reinterpret_cast<const void *>(&luSolveKernel<TT, float>), TT, 512, 4, tp);
+            luSolveKernel<TT, float><<<nTiles, 512, tp.ldsBytes>>>(dp, dlu.as<float>(), dy.as<float>(),
+                                                                   dxo.as<float>(), drhs.as<double>(), dcl.as<int>(),
+                                                                   tp.top);
+        }
     });
     HIP_OK(hipGetLastError());
     HIP_OK(hipDeviceSynchronize());

@@ -38,6 +38,20 @@ struct DevPlan {
     const int *fillPtr, *fillSlot;
     const double *shuntG, *shuntB;
     const int *perm, *pos;
+    const int *levelPos;        // position of each node in level order
+    const int *nodeLevel;       // elimination-tree level of each node
+    const int2 *itemKI;         // (k, i) of each item
+};
+
+// The top block: the nodes of levels l0.. (an upper set of the elimination tree, `start` = their first position in
+// level order, n of them) factored as one dense block matrix per lane in LDS, so that the levels where the tree is
+// one or two nodes wide cost one LDS barrier per node instead of one global-memory phase per level. slots lists the
+// top nodes' LU slots with their dense coordinates (slot, row, column, 0); S is the thread split of the contribution
+// sums from below the block. n = 0: no top block (l0 = nLevels).
+struct DevTop {
+    int l0, n, start, S;
+    const int4 *slots;
+    int nSlots;
 };
 
 struct DevState {
@@ -321,9 +335,9 @@ __device__ __forceinline__ int splitOf(const int *levelSplit, int l) {
 // a barrier.
 template <int T, typename VT>
 __device__ void factorForward(const DevPlan &P, int tile, int tid, int nt, VT *__restrict__ lu, VT *__restrict__ z,
-                              const double *__restrict__ rhs, double sign, int *clampLane) {
+                              const double *__restrict__ rhs, double sign, int *clampLane, int l0) {
     const int nb = P.nb;
-    for (int l = 0; l < P.nLevels; l++) {
+    for (int l = 0; l < l0; l++) {
         const int i0 = P.levelItemPtr[l], i1 = P.levelItemPtr[l + 1];
         const int S = splitOf<T>(P.levelSplit, l);
         for (int w = tid; w < (i1 - i0) * T * S; w += nt) {
@@ -397,9 +411,9 @@ __device__ void factorForward(const DevPlan &P, int tile, int tid, int nt, VT *_
 // barrier.
 template <int T, typename VT>
 __device__ void forwardOnly(const DevPlan &P, int tile, int tid, int nt, const VT *__restrict__ lu,
-                            VT *__restrict__ z, const double *__restrict__ rhs, double sign) {
+                            VT *__restrict__ z, const double *__restrict__ rhs, double sign, int l0) {
     const int nb = P.nb;
-    for (int l = 0; l < P.nLevels; l++) {
+    for (int l = 0; l < l0; l++) {
         const int i0 = P.levelItemPtr[l];
         const int nNodes = P.levelNodePtr[l + 1] - P.levelNodePtr[l];   // diagonal items come first
         const int S = splitOf<T>(P.levelSplit, l);
@@ -434,9 +448,9 @@ __device__ void forwardOnly(const DevPlan &P, int tile, int tid, int nt, const V
 // S threads as in the factorization. Ends with a barrier.
 template <int T, typename VT>
 __device__ void backward(const DevPlan &P, int tile, int tid, int nt, const VT *__restrict__ lu,
-                         const VT *__restrict__ z, VT *__restrict__ xo) {
+                         const VT *__restrict__ z, VT *__restrict__ xo, int l0) {
     const int nb = P.nb;
-    for (int l = P.nLevels - 1; l >= 0; l--) {
+    for (int l = l0 - 1; l >= 0; l--) {
         const int n0 = P.levelNodePtr[l], n1 = P.levelNodePtr[l + 1];
         const int S = splitOf<T>(P.levelSplitBwd, l);
         for (int w = tid; w < (n1 - n0) * T * S; w += nt) {
@@ -462,6 +476,262 @@ __device__ void backward(const DevPlan &P, int tile, int tid, int nt, const VT *
         }
         __syncthreads();
     }
+}
+
+// ---- top block in LDS. Per lane: M (n x (n+1) blocks; column n holds the right-hand side in its .a/.c entries),
+// the pivot inverses D^-1 (n blocks), z (n pairs) and the backward accumulator (n pairs); lane fastest.
+template <int T, typename VT>
+struct TopLds {
+    VT *base;
+    int n;
+    __device__ __forceinline__ VT *M(int b, int c, int t) const {
+        return base + ((size_t)(b * (n + 1) + c) * T + t) * 4;
+    }
+    __device__ __forceinline__ VT *Dinv(int a, int t) const {
+        return base + ((size_t)n * (n + 1) * T + (size_t)a * T + t) * 4;
+    }
+    __device__ __forceinline__ VT *Z(int a, int t) const {
+        return base + (size_t)n * (n + 2) * T * 4 + ((size_t)a * T + t) * 2;
+    }
+    __device__ __forceinline__ VT *Acc(int a, int t) const {
+        return base + (size_t)n * (n + 2) * T * 4 + (size_t)n * T * 2 + ((size_t)a * T + t) * 2;
+    }
+};
+// LDS bytes of a top block of n nodes for T lanes
+__host__ __device__ constexpr size_t topLdsBytes(int T, int n, size_t vtBytes) {
+    return (size_t)T * ((size_t)4 * n * (n + 1) + 4 * (size_t)n + 4 * (size_t)n) * vtBytes;
+}
+
+// Adds to the top block's right-hand side column (and, with FACTOR, to its matrix blocks) the contributions of the
+// nodes below the block: the items of the top levels summed over their contributors j of levels < l0.
+template <int T, typename VT, bool FACTOR>
+__device__ void topFromBelow(const DevPlan &P, const DevTop &top, const TopLds<T, VT> &L, int tile, int tid, int nt,
+                             const VT *__restrict__ lu, const VT *__restrict__ z) {
+    const int nb = P.nb;
+    const int i0 = P.levelItemPtr[top.l0];
+    const int nItems = FACTOR ? P.levelItemPtr[P.nLevels] - i0 : 0;
+    const int S = top.S < 64 / T ? top.S : 64 / T;
+    // without FACTOR only the diagonal items (one per top node, first in each level) matter: walk the nodes
+    const int units = FACTOR ? nItems : top.n;
+    for (int w = tid; w < units * T * S; w += nt) {
+        const int t = w % T;
+        const int s = (w / T) % S;
+        int item;
+        if (FACTOR) {
+            item = i0 + w / (T * S);
+        } else {
+            const int a = w / (T * S);
+            const int k = P.bwdRec[top.start + a].w;
+            // the diagonal item of node k: levels list their diagonal items first, in node order
+            const int l = P.nodeLevel[k];
+            item = P.levelItemPtr[l] + (top.start + a - P.levelNodePtr[l]);
+        }
+        const int4 it = P.itemRec[item];
+        const int2 ki = P.itemKI[item];
+        const int a = P.levelPos[ki.x] - top.start;
+        if (it.w >= 0) {
+            Blk<VT> D = {(VT)0, (VT)0, (VT)0, (VT)0};
+            V2<VT> yk = {(VT)0, (VT)0};
+            for (int c = it.y + s; c < it.z; c += S) {
+                const int4 cr = P.contribRec[c];
+                if (cr.w >= top.l0) continue;
+                const Blk<VT> Lkj = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, cr.x, t));
+                if (FACTOR) {
+                    const Blk<VT> Ujk = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, cr.x + 1, t));
+                    const Blk<VT> Dj = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, cr.z, t));
+                    mulSub(D, Lkj, mul(Dj, Ujk));
+                }
+                mulSubV(yk, Lkj, ldV<VT>(z + vIdx<T>(tile, nb, cr.y, t)));
+            }
+            if (S > 1) {
+                if (FACTOR) groupSumBlk<T>(D, S);
+                yk.p = groupSum<T>(yk.p, S);
+                yk.q = groupSum<T>(yk.q, S);
+            }
+            if (s == 0) {
+                if (FACTOR) {
+                    VT *m = L.M(a, a, t);
+                    const Blk<VT> A = ldBlk<VT>(m);
+                    stBlk<VT>(m, Blk<VT>{A.a + D.a, A.b + D.b, A.c + D.c, A.d + D.d});
+                }
+                VT *r = L.M(a, top.n, t);
+                const Blk<VT> R = ldBlk<VT>(r);
+                stBlk<VT>(r, Blk<VT>{R.a + yk.p, (VT)0, R.c + yk.q, (VT)0});
+            }
+        } else if (FACTOR) {
+            const int b = P.levelPos[ki.y] - top.start;
+            Blk<VT> Lx = {(VT)0, (VT)0, (VT)0, (VT)0};
+            Blk<VT> Ux = {(VT)0, (VT)0, (VT)0, (VT)0};
+            for (int c = it.y + s; c < it.z; c += S) {
+                const int4 c0 = P.contribRec[c];
+                if (c0.w >= top.l0) continue;
+                const Blk<VT> Lkj = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, c0.x, t));
+                const Blk<VT> Ujk = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, c0.x + 1, t));
+                const Blk<VT> Lij = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, c0.y, t));
+                const Blk<VT> Uji = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, c0.y + 1, t));
+                const Blk<VT> Dj = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, c0.z, t));
+                mulSub(Lx, Lij, mul(Dj, Ujk));
+                mulSub(Ux, Lkj, mul(Dj, Uji));
+            }
+            if (S > 1) {
+                groupSumBlk<T>(Lx, S);
+                groupSumBlk<T>(Ux, S);
+            }
+            if (s == 0) {
+                VT *ml = L.M(b, a, t);   // L~(i,k): row i, column k
+                VT *mu = L.M(a, b, t);   // U(k,i): row k, column i
+                const Blk<VT> A = ldBlk<VT>(ml);
+                const Blk<VT> B = ldBlk<VT>(mu);
+                stBlk<VT>(ml, Blk<VT>{A.a + Lx.a, A.b + Lx.b, A.c + Lx.c, A.d + Lx.d});
+                stBlk<VT>(mu, Blk<VT>{B.a + Ux.a, B.b + Ux.b, B.c + Ux.c, B.d + Ux.d});
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// Factorization of the top block with its forward solve: loads the block's slots and b = sign * rhs, adds the
+// contributions from below, then eliminates right-looking, one barrier per node. Leaves the factors in the stored form
+// of the sparse levels (M below the diagonal L~, above it U, D^-1 apart) and z in LDS. Ends with a barrier.
+template <int T, typename VT>
+__device__ void topFactor(const DevPlan &P, const DevTop &top, const TopLds<T, VT> &L, int tile, int tid, int nt,
+                          const VT *__restrict__ lu, const VT *__restrict__ z, const double *__restrict__ rhs,
+                          double sign, int *clampLane) {
+    const int nb = P.nb, n = top.n;
+    const Blk<VT> zero = {(VT)0, (VT)0, (VT)0, (VT)0};
+    for (int w = tid; w < n * (n + 1) * T; w += nt) stBlk<VT>(L.base + (size_t)w * 4, zero);
+    __syncthreads();
+    for (int w = tid; w < (top.nSlots + n) * T; w += nt) {
+        const int t = w % T, q = w / T;
+        if (q < top.nSlots) {
+            const int4 sr = top.slots[q];
+            stBlk<VT>(L.M(sr.y, sr.z, t), ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, sr.x, t)));
+        } else {
+            const int a = q - top.nSlots;
+            const int k = P.bwdRec[top.start + a].w;
+            const double2 b = *reinterpret_cast<const double2 *>(rhs + vIdx<T>(tile, nb, k, t));
+            stBlk<VT>(L.M(a, n, t), Blk<VT>{(VT)(sign * b.x), (VT)0, (VT)(sign * b.y), (VT)0});
+        }
+    }
+    __syncthreads();
+    topFromBelow<T, VT, true>(P, top, L, tile, tid, nt, lu, z);
+    if (tid < T) {
+        bool cl;
+        stBlk<VT>(L.Dinv(0, tid), inv2(ldBlk<VT>(L.M(0, 0, tid)), cl));
+        if (cl && clampLane) atomicAdd(&clampLane[tid], 1);
+    }
+    __syncthreads();
+    for (int a = 0; a < n; a++) {
+        const int m = n - 1 - a;   // rows below a; columns a+1 .. n (n: right-hand side)
+        if (tid < T) {
+            const Blk<VT> Di = ldBlk<VT>(L.Dinv(a, tid));
+            const Blk<VT> ya = ldBlk<VT>(L.M(a, n, tid));
+            stV<VT>(L.Z(a, tid), V2<VT>{Di.a * ya.a + Di.b * ya.c, Di.c * ya.a + Di.d * ya.c});
+        }
+        for (int w = tid; w < m * (m + 1) * T; w += nt) {
+            const int t = w % T, e = w / T;
+            const int b = a + 1 + e / (m + 1), c = a + 1 + e % (m + 1);
+            const Blk<VT> Di = ldBlk<VT>(L.Dinv(a, t));
+            VT *mb = L.M(b, c, t);
+            Blk<VT> acc = ldBlk<VT>(mb);
+            mulSub(acc, ldBlk<VT>(L.M(b, a, t)), mul(Di, ldBlk<VT>(L.M(a, c, t))));
+            stBlk<VT>(mb, acc);
+            if (b == a + 1 && c == a + 1) {
+                // the next pivot is final: invert it for the next step
+                bool cl;
+                stBlk<VT>(L.Dinv(b, t), inv2(acc, cl));
+                if (cl && clampLane) atomicAdd(&clampLane[t], 1);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Forward solve of the top block with existing factors: b = sign * rhs, contributions from below, then the dense
+// forward substitution into z (LDS). Ends with a barrier.
+template <int T, typename VT>
+__device__ void topForward(const DevPlan &P, const DevTop &top, const TopLds<T, VT> &L, int tile, int tid, int nt,
+                           const VT *__restrict__ lu, const VT *__restrict__ z, const double *__restrict__ rhs,
+                           double sign) {
+    const int nb = P.nb, n = top.n;
+    for (int w = tid; w < n * T; w += nt) {
+        const int t = w % T, a = w / T;
+        const int k = P.bwdRec[top.start + a].w;
+        const double2 b = *reinterpret_cast<const double2 *>(rhs + vIdx<T>(tile, nb, k, t));
+        stBlk<VT>(L.M(a, n, t), Blk<VT>{(VT)(sign * b.x), (VT)0, (VT)(sign * b.y), (VT)0});
+    }
+    __syncthreads();
+    topFromBelow<T, VT, false>(P, top, L, tile, tid, nt, lu, z);
+    for (int a = 0; a < n; a++) {
+        for (int w = tid; w < (n - a) * T; w += nt) {
+            const int t = w % T, b = a + w / T;   // b = a: store z(a); b > a: eliminate
+            const Blk<VT> Di = ldBlk<VT>(L.Dinv(a, t));
+            const Blk<VT> ya = ldBlk<VT>(L.M(a, n, t));
+            const V2<VT> za = {Di.a * ya.a + Di.b * ya.c, Di.c * ya.a + Di.d * ya.c};
+            if (b == a) {
+                stV<VT>(L.Z(a, t), za);
+            } else {
+                VT *mb = L.M(b, n, t);
+                Blk<VT> r = ldBlk<VT>(mb);
+                const Blk<VT> Lba = ldBlk<VT>(L.M(b, a, t));
+                r.a -= Lba.a * za.p + Lba.b * za.q;
+                r.c -= Lba.c * za.p + Lba.d * za.q;
+                stBlk<VT>(mb, r);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Backward solve of the top block: x(a) = z(a) - D(a)^-1 sum_{c > a} U(a,c) x(c), from the last node down, one
+// barrier per node; x of the top nodes goes to xo for the sparse levels below. Ends with a barrier.
+template <int T, typename VT>
+__device__ void topBackward(const DevPlan &P, const DevTop &top, const TopLds<T, VT> &L, int tile, int tid, int nt,
+                            VT *__restrict__ xo) {
+    const int nb = P.nb, n = top.n;
+    for (int w = tid; w < n * T; w += nt) stV<VT>(L.Acc(w / T, w % T), V2<VT>{(VT)0, (VT)0});
+    __syncthreads();
+    for (int a = n - 1; a >= 0; a--) {
+        for (int w = tid; w < (a + 1) * T; w += nt) {
+            const int t = w % T, b = w / T;   // b = a: store x(a); b < a: accumulate U(b,a) x(a)
+            const Blk<VT> Di = ldBlk<VT>(L.Dinv(a, t));
+            const V2<VT> za = ldV<VT>(L.Z(a, t));
+            const V2<VT> ac = ldV<VT>(L.Acc(a, t));
+            const V2<VT> xa = {za.p - (Di.a * ac.p + Di.b * ac.q), za.q - (Di.c * ac.p + Di.d * ac.q)};
+            if (b == a) {
+                const int k = P.bwdRec[top.start + a].w;
+                stV<VT>(xo + vIdx<T>(tile, nb, k, t), xa);
+            } else {
+                const Blk<VT> U = ldBlk<VT>(L.M(b, a, t));
+                V2<VT> ab = ldV<VT>(L.Acc(b, t));
+                ab.p += U.a * xa.p + U.b * xa.q;
+                ab.q += U.c * xa.p + U.d * xa.q;
+                stV<VT>(L.Acc(b, t), ab);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// The whole block LU: sparse levels below the top block, then the top block (n = 0: sparse only).
+template <int T, typename VT>
+__device__ __forceinline__ void luFactorForward(const DevPlan &P, const DevTop &top, const TopLds<T, VT> &L, int tile,
+                                                int tid, int nt, VT *lu, VT *z, const double *rhs, double sign,
+                                                int *clampLane) {
+    factorForward<T, VT>(P, tile, tid, nt, lu, z, rhs, sign, clampLane, top.l0);
+    if (top.n > 0) topFactor<T, VT>(P, top, L, tile, tid, nt, lu, z, rhs, sign, clampLane);
+}
+template <int T, typename VT>
+__device__ __forceinline__ void luForward(const DevPlan &P, const DevTop &top, const TopLds<T, VT> &L, int tile,
+                                          int tid, int nt, const VT *lu, VT *z, const double *rhs, double sign) {
+    forwardOnly<T, VT>(P, tile, tid, nt, lu, z, rhs, sign, top.l0);
+    if (top.n > 0) topForward<T, VT>(P, top, L, tile, tid, nt, lu, z, rhs, sign);
+}
+template <int T, typename VT>
+__device__ __forceinline__ void luBackward(const DevPlan &P, const DevTop &top, const TopLds<T, VT> &L, int tile,
+                                           int tid, int nt, const VT *lu, const VT *z, VT *xo) {
+    if (top.n > 0) topBackward<T, VT>(P, top, L, tile, tid, nt, xo);
+    backward<T, VT>(P, tile, tid, nt, lu, z, xo, top.l0);
 }
 
 // Mismatch and Jacobian fill of the lanes flagged in fillLane; per-thread partial max|F| and finiteness of the
@@ -518,11 +788,13 @@ __device__ void applyJPass(const DevPlan &P, const DevState &S, int tile, int ti
 // native/cuda/outerloop/reactive_limits.cu:321-371 plus the forced-PV election, step scaling
 // native/cuda/solver/state_vector_scaling.cu:57-97.
 template <int T, typename VT>
-__global__ __launch_bounds__(MAX_THREADS) void nrStepKernel(DevPlan P, DevState S, DevRL R, NrParams prm) {
+__global__ __launch_bounds__(MAX_THREADS) void nrStepKernel(DevPlan P, DevState S, DevRL R, NrParams prm, DevTop top) {
     const int tile = blockIdx.x;
     const int tid = threadIdx.x, nt = blockDim.x;
     const int myLane = tid % T;
     if (S.tileRunning[tile] == 0) return;   // uniform per block
+    extern __shared__ float4 olfhipTopLds[];
+    const TopLds<T, VT> TL = {reinterpret_cast<VT *>(olfhipTopLds), top.n};
 
     __shared__ int sStatus[T], sIt[T], sSolves[T], sOuter[T], sTotal[T], sClamp[T];
     __shared__ int sFill[T], sSolve[T], sConv[T], sRl[T], sRemain[T], sSwitched[T], sFin[T];
@@ -720,9 +992,9 @@ __global__ __launch_bounds__(MAX_THREADS) void nrStepKernel(DevPlan P, DevState 
     if (sFlag[2]) {
         // ---- 4. factorization with the forward solve of -F, then the backward solve
         OLFHIP_MARK(2);
-        factorForward<T, VT>(P, tile, tid, nt, lu, y, S.F, -1.0, sClamp);
+        luFactorForward<T, VT>(P, top, TL, tile, tid, nt, lu, y, S.F, -1.0, sClamp);
         OLFHIP_MARK(3);
-        backward<T, VT>(P, tile, tid, nt, lu, y, dx1);
+        luBackward<T, VT>(P, top, TL, tile, tid, nt, lu, y, dx1);
         OLFHIP_MARK(4);
         // ---- 5. step in FP64 (+ one FP64-residual refinement step on the FP32 factors)
         for (int w = tid; w < nb * T; w += nt) {
@@ -737,8 +1009,8 @@ __global__ __launch_bounds__(MAX_THREADS) void nrStepKernel(DevPlan P, DevState 
             applyJPass<T, VT>(P, S, tile, tid, nt, sSolve, S.dx, S.F, 1.0, S.r, pm);
             __syncthreads();
             OLFHIP_MARK(5);
-            forwardOnly<T, VT>(P, tile, tid, nt, lu, y, S.r, -1.0);
-            backward<T, VT>(P, tile, tid, nt, lu, y, dx1);
+            luForward<T, VT>(P, top, TL, tile, tid, nt, lu, y, S.r, -1.0);
+            luBackward<T, VT>(P, top, TL, tile, tid, nt, lu, y, dx1);
             OLFHIP_MARK(6);
             for (int w = tid; w < nb * T; w += nt) {
                 const int k = w / T, t = w % T;
@@ -974,13 +1246,15 @@ __global__ void countRunningKernel(const int *tileRunning, int nTiles, int *coun
 // standalone batched LU: factor + forward + backward per tile, rhs/solution in the node-ordered vector layout
 template <int T, typename VT>
 __global__ __launch_bounds__(MAX_THREADS) void luSolveKernel(DevPlan P, VT *lu, VT *y, VT *xo, const double *rhs,
-                                                             int *clamps) {
+                                                             int *clamps, DevTop top) {
     __shared__ int sClamp[T];
+    extern __shared__ float4 olfhipTopLds[];
+    const TopLds<T, VT> TL = {reinterpret_cast<VT *>(olfhipTopLds), top.n};
     const int tile = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
     if (tid < T) sClamp[tid] = 0;
     __syncthreads();
-    factorForward<T, VT>(P, tile, tid, nt, lu, y, rhs, 1.0, sClamp);
-    backward<T, VT>(P, tile, tid, nt, lu, y, xo);
+    luFactorForward<T, VT>(P, top, TL, tile, tid, nt, lu, y, rhs, 1.0, sClamp);
+    luBackward<T, VT>(P, top, TL, tile, tid, nt, lu, y, xo);
     if (tid < T) clamps[tile * T + tid] = sClamp[tid];
 }
 

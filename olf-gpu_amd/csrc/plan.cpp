@@ -175,22 +175,26 @@ Plan buildPlan(int n, int nBranch, const double *p8, const int *idx12, int nShun
     for (int k = 0; k < nb; k++) byLevel[level[k]].push_back(k);
     P.levelNodePtr.assign(P.nLevels + 1, 0);
     P.levelItemPtr.assign(P.nLevels + 1, 0);
-    auto pushContrib = [&](int a, int b, int d) {
+    auto pushContrib = [&](int a, int b, int d, int j) {
         P.contribRec.push_back(a);
         P.contribRec.push_back(b);
         P.contribRec.push_back(d);
-        P.contribRec.push_back(0);
+        P.contribRec.push_back(level[j]);
     };
+    P.nodeLevel = level;
+    P.levelPos.assign(nb, 0);
     int nItems = 0, nContrib = 0;
     for (int l = 0; l < P.nLevels; l++) {
         for (int k : byLevel[l]) {
+            P.levelPos[k] = (int)P.levelNodes.size();
             P.levelNodes.push_back(k);
             const int c0 = nContrib;
             for (auto &c : diagC[k]) {
-                pushContrib(c.first, c.second, P.nodeBase[c.second]);
+                pushContrib(c.first, c.second, P.nodeBase[c.second], c.second);
                 nContrib++;
             }
             P.itemRec.insert(P.itemRec.end(), {P.nodeBase[k], c0, nContrib, k});
+            P.itemKI.insert(P.itemKI.end(), {k, k});
             nItems++;
             P.bwdRec.insert(P.bwdRec.end(), {P.nodeBase[k], P.sPtr[k], P.sPtr[k + 1], k});
             P.contributions += (long long)diagC[k].size();
@@ -205,10 +209,11 @@ Plan buildPlan(int n, int nBranch, const double *p8, const int *idx12, int nShun
                     // contributing node j owns both slots: recover it from the slot range
                     const int j = (int)(std::upper_bound(P.nodeBase.begin(), P.nodeBase.end(), c.first) -
                                         P.nodeBase.begin()) - 1;
-                    pushContrib(c.first, c.second, P.nodeBase[j]);
+                    pushContrib(c.first, c.second, P.nodeBase[j], j);
                     nContrib++;
                 }
                 P.itemRec.insert(P.itemRec.end(), {lslot(k, e), c0, nContrib, -1});
+                P.itemKI.insert(P.itemKI.end(), {k, P.sIdx[P.sPtr[k] + e]});
                 nItems++;
                 P.contributions += (long long)lst.size();
                 P.factorMads += 4LL * (long long)lst.size();   // two W = D^-1 U and two updates
@@ -326,6 +331,25 @@ Plan buildPlan(int n, int nBranch, const double *p8, const int *idx12, int nShun
         }
     }
     return P;
+}
+
+int Plan::topLevel(int maxNodes, int *nodes) const {
+    int l0 = nLevels;
+    while (l0 > 0 && nb - levelNodePtr[l0 - 1] <= maxNodes) l0--;
+    if (nodes) *nodes = nb - levelNodePtr[l0];
+    return l0;
+}
+
+int Plan::topSplit(int l0) const {
+    int longest = 0;
+    for (int i = levelItemPtr[l0]; i < levelItemPtr[nLevels]; i++) {
+        int c = 0;
+        for (int q = itemRec[4 * i + 1]; q < itemRec[4 * i + 2]; q++) c += contribRec[4 * q + 3] < l0;
+        longest = std::max(longest, c);
+    }
+    int s = 1;
+    while (s < 64 && 2 * s < longest) s *= 2;
+    return s;
 }
 
 size_t Plan::bytesPerScenario(int luValueBytes) const {

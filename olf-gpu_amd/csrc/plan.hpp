@@ -33,6 +33,8 @@ struct Plan {
     // --- level schedule (elimination-tree height; leaves are level 0)
     int nLevels = 0;
     std::vector<int> levelNodePtr, levelNodes;     // nodes of each level
+    std::vector<int> nodeLevel;                    // level of each node
+    std::vector<int> levelPos;                     // position of each node in levelNodes
     // The factors A = L U (L unit block-lower, U with the pivot blocks D(k) on its diagonal) are kept in the form the
     // kernels use: per node k the inverse D(k)^-1 (in the diagonal slot), per pair the unscaled L~(i,k) = L(i,k) D(k)
     // and U(k,i). Every
@@ -43,9 +45,11 @@ struct Plan {
     // contribution list ascends in the contributing node j.
     std::vector<int> levelItemPtr;                 // items of level l in [levelItemPtr[l], levelItemPtr[l+1])
     std::vector<int> itemRec;                      // 4 per item: slot (diag or L), contrib begin, end, node | -1
+    std::vector<int> itemKI;                       // 2 per item: k and i of L~(i,k) / U(k,i) (diag: k, k)
     std::vector<int> contribRec;                   // 4 per contribution: diag item: L slot of pair (j,k), j,
-                                                   // diag slot of j, 0; pair item: L slot of (j,k), L slot of
-                                                   // (j,i), diag slot of j, 0 (U(j,x) sits right after L(x,j))
+                                                   // diag slot of j, level of j; pair item: L slot of (j,k), L slot
+                                                   // of (j,i), diag slot of j, level of j (U(j,x) sits right after
+                                                   // L(x,j))
     std::vector<int> bwdRec;                       // 4 per node in level order: diag slot, S begin, S end, node
     // Threads that share one item's contribution list (a power of two, from the level's longest list): long lists
     // at the top of the tree are summed by several threads of a wave and reduced across lanes. The kernels cap it
@@ -73,6 +77,12 @@ struct Plan {
 
     // device-independent per-scenario cost, bytes (without the LU value precision factor)
     size_t bytesPerScenario(int luValueBytes) const;
+    // The top block for at most maxNodes nodes: the smallest level l0 such that the levels l0.. hold at most
+    // maxNodes nodes (an upper set of the elimination tree, so every member's S set stays inside it). Returns l0
+    // (nLevels when even the root level does not fit) and the node count in *nodes.
+    int topLevel(int maxNodes, int *nodes) const;
+    // Threads per top-block item for the contributions from below the top block (the longest such list).
+    int topSplit(int l0) const;
 };
 
 // Builds the plan from the reference's closed-branch pack (8 doubles, 12 ints per branch) and optional shunts.

@@ -143,16 +143,16 @@ def test_n1_9241_matches_oracle(grid, lu):
 
 
 def test_tile_shapes_agree(grid):
-    """The workgroup size never changes a result (bitwise); the lane-tile width only changes how many threads share
-    a long contribution sum (capped at 64 / T), hence the summation order: results agree to rounding."""
+    """Tile shapes change the summation order only: the lane-tile width sets how many threads share a long
+    contribution sum (capped at 64 / T), and the LDS a workgroup may use (which depends on the workgroup size) sets
+    how many top nodes are factored as one dense block. Results agree to rounding; a rerun of one shape is bitwise
+    identical (test_rerun_is_bitwise_identical)."""
     net = grid("grid2000")
     print("[tiles] grid2000", flush=True)
     outs = {}
     for T, NT in [(1, 256), (1, 1024), (4, 512), (4, 1024), (8, 1024), (16, 256), (32, 512)]:
         ctg, ref, res = run_n1(net, "grid2000", 24, 0.25, 6, "BATCHED_LU", T, NT)
         outs[(T, NT)] = res
-    for (a, b) in [((1, 256), (1, 1024)), ((4, 512), (4, 1024))]:
-        assert np.array_equal(outs[a].x, outs[b].x), (a, b)
     base = outs[(1, 256)]
     for key, r in outs.items():
         assert np.array_equal(r.status, base.status), key
