@@ -214,6 +214,53 @@ int olfhip_structure_pattern(const olfhip_structure *structure, int *nnz, int *r
 int olfhip_autotune(olfhip_structure *structure, int mode, const olfhip_reactive_limits *reactive_limits,
                     const olfhip_solve_inputs *inputs, olfhip_tuning *best, double *best_ms);
 
+/* ---- Limit violations of the scenarios the last solve of a context converged (reference
+ * native/cuda/sa/olf_gpu_violations.cu:1-27,170-300 and its JNI ViolationScreenNative.uploadViolationTable /
+ * screenHandedBack, olf_gpu_violations.cu:430-470,650-690; the decisions of GpuLimitScanner.java:20-262, which are OLF's
+ * LimitViolationManager.detectViolations). A table lists, per monitored closed branch, its limit groups (a side and a
+ * quantity) with their thresholds most severe first, and per bus a high and a low voltage limit (NaN: none). Values
+ * are in per unit of the state: current |S|/V, active power P (|P| compared, the signed value reported), apparent
+ * power |S|, voltage V. Within a group the first exceeded threshold is reported and the group stops there. Outaged
+ * branches, islanded buses and branches with an islanded end are out of service. */
+enum { OLFHIP_QUANTITY_CURRENT = 0, OLFHIP_QUANTITY_ACTIVE_POWER = 1, OLFHIP_QUANTITY_APPARENT_POWER = 2 };
+enum { OLFHIP_VIOLATION_BRANCH = 0, OLFHIP_VIOLATION_HIGH_VOLTAGE = 1, OLFHIP_VIOLATION_LOW_VOLTAGE = 2 };
+typedef struct olfhip_limit_table olfhip_limit_table;
+typedef struct olfhip_limit_table_desc {
+    int n_branches;                 /* monitored branches (table order) */
+    const int *branch;              /* [n_branches] closed-branch index */
+    const int *group_start;         /* [n_branches + 1] groups of table branch b: [group_start[b], group_start[b+1]) */
+    const int *group_side;          /* [n_groups] 1 or 2 */
+    const int *group_quantity;      /* [n_groups] OLFHIP_QUANTITY_* */
+    const int *limit_start;         /* [n_groups + 1] limits of group g: [limit_start[g], limit_start[g+1]) */
+    const double *limit_threshold;  /* [n_limits] */
+    int n_buses;                    /* buses 0 .. n_buses - 1 carry voltage limits */
+    const double *bus_high;         /* [n_buses] pu, NaN: none */
+    const double *bus_low;          /* [n_buses] pu, NaN: none */
+} olfhip_limit_table_desc;
+typedef struct olfhip_violation {
+    int scenario;                   /* index in the last solve's scenarios */
+    int kind;                       /* OLFHIP_VIOLATION_* */
+    int element;                    /* table branch index, or bus index */
+    int side;                       /* 1 or 2 (branch), 0 (bus) */
+    int quantity;                   /* OLFHIP_QUANTITY_* (branch) */
+    int limit;                      /* limit index (branch), -1 (bus) */
+    double value;                   /* the value (signed active power) */
+    double threshold;
+} olfhip_violation;
+int olfhip_create_limit_table(olfhip_structure *structure, const olfhip_limit_table_desc *desc,
+                              olfhip_limit_table **out);
+void olfhip_destroy_limit_table(olfhip_limit_table *table);
+/* The device screen alone (candidates, may over-emit): counts [K] and records [K][capacity] for the K scenarios of the
+ * last solve; record l is branch limit l, n_limits + 2 i / n_limits + 2 i + 1 the high / low voltage of bus i. A
+ * count above capacity means the scenario's candidates overflowed. band: relative screen band (e.g. 1e-7). */
+int olfhip_screen_violations(olfhip_context *context, olfhip_limit_table *table, double band, int capacity,
+                             int *counts, int *records);
+/* Screen plus the exact host decisions: the violations of every converged scenario, ordered by scenario, then
+ * branches in table order (a group's first exceeded limit), then buses (high before low). Fails with
+ * OLFHIP_E_ARGUMENT, *n_violations set to the count, when max_violations is too small. */
+int olfhip_detect_violations(olfhip_context *context, olfhip_limit_table *table, double band, int capacity,
+                             int max_violations, olfhip_violation *violations, int *n_violations);
+
 /* ---- Learned warm start (no reference counterpart: the reference seeds scenarios from the base case or from
  * caller-given states only, BatchSolveNative.java:119). A 3-layer MLP, hidden width `hidden`, GELU, BF16 weights on
  * the matrix cores with FP32 accumulation and FP32 master weights (Adam). It works on each outage's neighbourhood:

@@ -16,6 +16,7 @@
 #include "errors.hpp"
 #include "nr_kernels.hpp"
 #include "plan.hpp"
+#include "violation_kernels.hpp"
 
 namespace olfhip {
 
@@ -224,6 +225,8 @@ struct ContextImpl {
     DBuf prof;                               // opt-in section stamps of tile 0 (OLFHIP_PROFILE=1)
     int profCap = 0;
     TopPlan top;                             // the dense top block of the step kernel
+    std::vector<int> lastStatus;             // statuses of the last solve's scenarios
+    std::vector<int> lastOutages;            // its outage lists, MAX_OUTAGES per lane (-1 padded)
 
     DevState devState() const {
         DevState S{};
@@ -520,6 +523,8 @@ static void solve(ContextImpl *c, const olfhip_solve_inputs *in, olfhip_solve_ou
     for (int k = 0; k < K; k++)
         if (st[k] == LANE_RUNNING) st[k] = ST_REJECTED;   // batched-iteration cap (newton_iteration.cu:410-429)
     c->lastSteps = steps;
+    c->lastStatus = st;
+    c->lastOutages = outs;
     const bool wantStates = out && out->results && in->states_to_host;
     std::vector<double> xs;
     if (wantStates) {
@@ -564,6 +569,245 @@ static void readStates(ContextImpl *c, int K, double *x) {
     });
     HIP_OK(hipMemcpyAsync(x, c->states.p, (size_t)K * P.n * 8, hipMemcpyDeviceToHost, c->stream));
     HIP_OK(hipStreamSynchronize(c->stream));
+}
+
+// ---- limit violations: the table, the device screen and the exact host confirmation
+struct LimitTableImpl {
+    StructureImpl *st = nullptr;
+    int nBranches = 0, nGroups = 0, nLimits = 0, nBuses = 0;
+    std::vector<int> branch, groupStart, groupSide, groupQty, limitStart, groupBranch, limitGroup;
+    std::vector<double> threshold, busHigh, busLow;
+    DBuf dBranch, dGroupStart, dGroupSide, dGroupQty, dLimitStart, dThreshold, dBusHigh, dBusLow, dCoef, dNode;
+    DBuf records, counts, screenLane, req, vals, isl;
+    DevLimits dev() const {
+        return DevLimits{nBranches, nGroups, nLimits, nBuses, dBranch.as<int>(), dGroupStart.as<int>(),
+                         dGroupSide.as<int>(), dGroupQty.as<int>(), dLimitStart.as<int>(), dThreshold.as<double>(),
+                         dBusHigh.as<double>(), dBusLow.as<double>(), dCoef.as<double>(), dNode.as<int>()};
+    }
+    ~LimitTableImpl() { release(st); }
+};
+
+static LimitTableImpl *createLimitTable(StructureImpl *st, const olfhip_limit_table_desc *d) {
+    if (!st || !d) throw ArgumentError("null argument");
+    const Plan &P = st->plan;
+    if (d->n_branches < 0 || d->n_buses < 0 || d->n_buses > P.nb) throw ArgumentError("table sizes out of range");
+    if (d->n_branches > 0 && (!d->branch || !d->group_start || !d->group_side || !d->group_quantity ||
+                              !d->limit_start || !d->limit_threshold))
+        throw ArgumentError("branch limit arrays are null");
+    if (d->n_buses > 0 && (!d->bus_high || !d->bus_low)) throw ArgumentError("bus limit arrays are null");
+    std::unique_ptr<LimitTableImpl> t(new LimitTableImpl());
+    t->nBranches = d->n_branches;
+    t->nBuses = d->n_buses;
+    if (d->n_branches > 0) {
+        if (d->group_start[0] != 0) throw ArgumentError("group_start[0] must be 0");
+        t->nGroups = d->group_start[d->n_branches];
+        for (int b = 0; b < d->n_branches; b++) {
+            if (d->branch[b] < 0 || d->branch[b] >= P.nBranch) throw ArgumentError("table branch out of range");
+            if (d->group_start[b + 1] < d->group_start[b]) throw ArgumentError("group_start must not decrease");
+        }
+        if (t->nGroups > 0 && d->limit_start[0] != 0) throw ArgumentError("limit_start[0] must be 0");
+        t->nLimits = t->nGroups > 0 ? d->limit_start[t->nGroups] : 0;
+        for (int g = 0; g < t->nGroups; g++) {
+            if (d->group_side[g] != 1 && d->group_side[g] != 2) throw ArgumentError("group side must be 1 or 2");
+            if (d->group_quantity[g] < OLFHIP_QUANTITY_CURRENT || d->group_quantity[g] > OLFHIP_QUANTITY_APPARENT_POWER)
+                throw ArgumentError("unknown group quantity");
+            if (d->limit_start[g + 1] < d->limit_start[g]) throw ArgumentError("limit_start must not decrease");
+        }
+        t->branch.assign(d->branch, d->branch + d->n_branches);
+        t->groupStart.assign(d->group_start, d->group_start + d->n_branches + 1);
+        t->groupSide.assign(d->group_side, d->group_side + t->nGroups);
+        t->groupQty.assign(d->group_quantity, d->group_quantity + t->nGroups);
+        t->limitStart.assign(d->limit_start, d->limit_start + t->nGroups + 1);
+        t->threshold.assign(d->limit_threshold, d->limit_threshold + t->nLimits);
+    } else {
+        t->groupStart.assign(1, 0);
+        t->limitStart.assign(1, 0);
+    }
+    t->groupBranch.assign(t->nGroups, 0);
+    for (int b = 0; b < t->nBranches; b++)
+        for (int g = t->groupStart[b]; g < t->groupStart[b + 1]; g++) t->groupBranch[g] = b;
+    t->limitGroup.assign(t->nLimits, 0);
+    for (int g = 0; g < t->nGroups; g++)
+        for (int l = t->limitStart[g]; l < t->limitStart[g + 1]; l++) t->limitGroup[l] = g;
+    if (d->n_buses > 0) {
+        t->busHigh.assign(d->bus_high, d->bus_high + d->n_buses);
+        t->busLow.assign(d->bus_low, d->bus_low + d->n_buses);
+    }
+    st->ensureDevice();
+    t->dBranch.upload(t->branch);
+    t->dGroupStart.upload(t->groupStart);
+    t->dGroupSide.upload(t->groupSide);
+    t->dGroupQty.upload(t->groupQty);
+    t->dLimitStart.upload(t->limitStart);
+    t->dThreshold.upload(t->threshold);
+    t->dBusHigh.upload(t->busHigh);
+    t->dBusLow.upload(t->busLow);
+    t->dCoef.upload(P.branchCoef);
+    t->dNode.upload(P.branchNode);
+    HIP_OK(hipDeviceSynchronize());
+    st->refs++;
+    t->st = st;
+    return t.release();
+}
+
+// Screens the last solve's converged scenarios on the device: counts [K], records [K][capacity] on the host.
+static void screenViolations(ContextImpl *c, LimitTableImpl *t, double band, int capacity, std::vector<int> &counts,
+                             std::vector<int> &records) {
+    if (c->st != t->st) throw ArgumentError("the limit table belongs to another structure");
+    const int K = c->lastInstalled;
+    if (K <= 0 || (int)c->lastStatus.size() != K) throw ArgumentError("no solve to screen");
+    if (capacity <= 0) throw ArgumentError("capacity must be positive");
+    if (!(band >= 0.0)) throw ArgumentError("band must be non-negative");
+    hipStream_t s = c->stream;
+    std::vector<int> lane(K);
+    for (int k = 0; k < K; k++) lane[k] = c->lastStatus[k] == ST_CONVERGED;
+    t->screenLane.upload(lane, s);
+    t->counts.alloc((size_t)K * 4);
+    t->records.alloc((size_t)K * capacity * 4);
+    zeroIntKernel<<<gridFor(K, 256), 256, 0, s>>>(t->counts.as<int>(), K);
+    const DevPlan dp = c->st->dp;
+    const DevState S = c->devState();
+    const DevLimits L = t->dev();
+    withT(c->T, [&](auto TC) {
+        constexpr int T = decltype(TC)::value;
+        if (t->nBranches > 0)
+            screenBranchesKernel<T><<<gridFor((size_t)K * t->nBranches, 256), 256, 0, s>>>(
+                dp, S, L, c->modeBase.as<int>(), t->screenLane.as<int>(), K, band, capacity, t->records.as<int>(),
+                t->counts.as<int>());
+        if (t->nBuses > 0)
+            screenBusesKernel<T><<<gridFor((size_t)K * t->nBuses, 256), 256, 0, s>>>(
+                dp, S, L, c->modeBase.as<int>(), t->screenLane.as<int>(), K, band, capacity, t->records.as<int>(),
+                t->counts.as<int>());
+    });
+    HIP_OK(hipGetLastError());
+    counts.resize(K);
+    records.resize((size_t)K * capacity);
+    HIP_OK(hipMemcpyAsync(counts.data(), t->counts.p, (size_t)K * 4, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(records.data(), t->records.p, records.size() * 4, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+}
+
+// Exact FP64 flows of closed branch e from its end states: side 1 (p, q, v) and side 2 (p, q, v).
+static void branchFlows(const Plan &P, int e, double2 x1, double2 x2, double f[6]) {
+    const double *c = &P.branchCoef[8 * (size_t)e];
+    const double v1 = x1.y, v2 = x2.y;
+    const double t1 = c[3] - x1.x + x2.x, t2 = c[6] - x2.x + x1.x;
+    const double bvv = c[1] * v1 * v2;
+    f[0] = c[0] * v1 * v1 - bvv * std::sin(t1);
+    f[1] = c[2] * v1 * v1 - bvv * std::cos(t1);
+    f[2] = v1;
+    f[3] = c[4] * v2 * v2 - bvv * std::sin(t2);
+    f[4] = c[5] * v2 * v2 - bvv * std::cos(t2);
+    f[5] = v2;
+}
+
+// Screen, then the exact decisions: every candidate re-tested with the exact comparison on the host's FP64 value,
+// the first confirmed limit of a group reported and the rest of the group skipped (GpuLimitScanner.java:178-262,
+// 394-410); a scenario whose candidates overflowed the capacity is scanned in full (all its groups and buses in table
+// order, GpuLimitScanner.java:95-131). Output ordered by scenario, then branches in table order, then buses.
+static void detectViolations(ContextImpl *c, LimitTableImpl *t, double band, int capacity,
+                             std::vector<olfhip_violation> &out) {
+    std::vector<int> counts, records;
+    screenViolations(c, t, band, capacity, counts, records);
+    const Plan &P = c->st->plan;
+    const int K = c->lastInstalled;
+    // per scenario, its sorted candidate records (all records when it overflowed)
+    std::vector<std::vector<int>> cand(K);
+    std::vector<int> allRecords;
+    for (int k = 0; k < K; k++) {
+        if (c->lastStatus[k] != ST_CONVERGED) continue;
+        if (counts[k] > capacity) {
+            if (allRecords.empty()) {
+                for (int l = 0; l < t->nLimits; l++) allRecords.push_back(l);
+                for (int i = 0; i < t->nBuses; i++) {
+                    allRecords.push_back(t->nLimits + 2 * i);
+                    allRecords.push_back(t->nLimits + 2 * i + 1);
+                }
+            }
+            cand[k] = allRecords;
+        } else {
+            cand[k].assign(records.begin() + (size_t)k * capacity, records.begin() + (size_t)k * capacity + counts[k]);
+            std::sort(cand[k].begin(), cand[k].end());
+        }
+    }
+    // node states to gather: both ends of each candidate branch, the bus of each candidate voltage
+    std::vector<int> req;
+    std::vector<std::vector<int>> at(K);   // per scenario, per candidate: index of its first gathered node
+    for (int k = 0; k < K; k++) {
+        at[k].resize(cand[k].size());
+        for (size_t q = 0; q < cand[k].size(); q++) {
+            const int r = cand[k][q];
+            at[k][q] = (int)(req.size() / 2);
+            if (r < t->nLimits) {
+                const int e = t->branch[t->groupBranch[t->limitGroup[r]]];
+                req.insert(req.end(), {k, P.branchNode[2 * e], k, P.branchNode[2 * e + 1]});
+            } else {
+                req.insert(req.end(), {k, P.pos[(r - t->nLimits) / 2]});
+            }
+        }
+    }
+    const int nReq = (int)(req.size() / 2);
+    std::vector<double2> xv(nReq);
+    std::vector<int> iv(nReq);
+    if (nReq > 0) {
+        hipStream_t s = c->stream;
+        t->req.upload(req, s);
+        t->vals.alloc((size_t)nReq * 16);
+        t->isl.alloc((size_t)nReq * 4);
+        const DevPlan dp = c->st->dp;
+        const DevState S = c->devState();
+        withT(c->T, [&](auto TC) {
+            constexpr int T = decltype(TC)::value;
+            gatherNodesKernel<T><<<gridFor(nReq, 256), 256, 0, s>>>(dp, S, c->modeBase.as<int>(), nReq,
+                                                                    t->req.as<int2>(), t->vals.as<double2>(),
+                                                                    t->isl.as<int>());
+        });
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipMemcpyAsync(xv.data(), t->vals.p, (size_t)nReq * 16, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipMemcpyAsync(iv.data(), t->isl.p, (size_t)nReq * 4, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+    }
+    out.clear();
+    for (int k = 0; k < K; k++) {
+        int reportedGroup = -1;
+        const int *outs = &c->lastOutages[(size_t)k * MAX_OUTAGES];
+        for (size_t q = 0; q < cand[k].size(); q++) {
+            const int r = cand[k][q];
+            const int a = at[k][q];
+            if (r < t->nLimits) {
+                const int g = t->limitGroup[r];
+                if (g == reportedGroup) continue;   // the group already reported its most serious limit
+                const int e = t->branch[t->groupBranch[g]];
+                bool isOut = false;
+                for (int m = 0; m < MAX_OUTAGES; m++) isOut |= outs[m] == e;
+                if (isOut || iv[a] || iv[a + 1]) continue;   // out of service in this scenario
+                double f[6];
+                branchFlows(P, e, xv[a], xv[a + 1], f);
+                const int side = t->groupSide[g], qty = t->groupQty[g];
+                const double p = side == 1 ? f[0] : f[3], qq = side == 1 ? f[1] : f[4], v = side == 1 ? f[2] : f[5];
+                double value;
+                if (qty == OLFHIP_QUANTITY_CURRENT) value = std::hypot(p, qq) / v;
+                else if (qty == OLFHIP_QUANTITY_ACTIVE_POWER) value = p;
+                else value = std::sqrt(p * p + qq * qq);
+                if (qty == OLFHIP_QUANTITY_APPARENT_POWER && std::isnan(value)) continue;
+                const double compared = qty == OLFHIP_QUANTITY_ACTIVE_POWER ? std::fabs(value) : value;
+                if (!(compared > t->threshold[r])) continue;
+                out.push_back(olfhip_violation{k, OLFHIP_VIOLATION_BRANCH, t->groupBranch[g], side, qty, r, value,
+                                               t->threshold[r]});
+                reportedGroup = g;
+            } else {
+                if (iv[a]) continue;
+                const int i = (r - t->nLimits) / 2;
+                const double v = xv[a].y;
+                if (((r - t->nLimits) & 1) == 0) {
+                    if (v > t->busHigh[i])
+                        out.push_back(olfhip_violation{k, OLFHIP_VIOLATION_HIGH_VOLTAGE, i, 0, 0, -1, v, t->busHigh[i]});
+                } else if (v < t->busLow[i]) {
+                    out.push_back(olfhip_violation{k, OLFHIP_VIOLATION_LOW_VOLTAGE, i, 0, 0, -1, v, t->busLow[i]});
+                }
+            }
+        }
+    }
 }
 
 static void luFactorizeAndSolve(StructureImpl *st, int mode, int capacity, const double *values, const double *rhs,
@@ -889,6 +1133,46 @@ int olfhip_autotune(olfhip_structure *structure, int mode, const olfhip_reactive
         }
         *best = bestT;
         if (best_ms) *best_ms = bestMs;
+    });
+}
+
+int olfhip_create_limit_table(olfhip_structure *structure, const olfhip_limit_table_desc *desc,
+                              olfhip_limit_table **out) {
+    return guarded([&] {
+        if (!out) throw ArgumentError("out is null");
+        *out = nullptr;
+        *out = reinterpret_cast<olfhip_limit_table *>(
+            createLimitTable(reinterpret_cast<StructureImpl *>(structure), desc));
+    });
+}
+
+void olfhip_destroy_limit_table(olfhip_limit_table *table) { delete reinterpret_cast<LimitTableImpl *>(table); }
+
+int olfhip_screen_violations(olfhip_context *context, olfhip_limit_table *table, double band, int capacity,
+                             int *counts, int *records) {
+    return guarded([&] {
+        if (!context || !table || !counts || !records) throw ArgumentError("null argument");
+        ContextImpl *c = reinterpret_cast<ContextImpl *>(context);
+        std::vector<int> cv, rv;
+        screenViolations(c, reinterpret_cast<LimitTableImpl *>(table), band, capacity, cv, rv);
+        std::memcpy(counts, cv.data(), cv.size() * 4);
+        std::memcpy(records, rv.data(), rv.size() * 4);
+    });
+}
+
+int olfhip_detect_violations(olfhip_context *context, olfhip_limit_table *table, double band, int capacity,
+                             int max_violations, olfhip_violation *violations, int *n_violations) {
+    return guarded([&] {
+        if (!context || !table || !n_violations) throw ArgumentError("null argument");
+        if (max_violations < 0 || (max_violations > 0 && !violations)) throw ArgumentError("bad output buffer");
+        std::vector<olfhip_violation> v;
+        detectViolations(reinterpret_cast<ContextImpl *>(context), reinterpret_cast<LimitTableImpl *>(table), band,
+                         capacity, v);
+        *n_violations = (int)v.size();
+        if ((int)v.size() > max_violations)
+            throw ArgumentError("max_violations is smaller than the " + std::to_string(v.size()) +
+                                " violations found (their count is in n_violations)");
+        if (!v.empty()) std::memcpy(violations, v.data(), v.size() * sizeof(olfhip_violation));
     });
 }
 

@@ -335,13 +335,15 @@ __device__ __forceinline__ int splitOf(const int *levelSplit, int l) {
 // a barrier.
 template <int T, typename VT>
 __device__ void factorForward(const DevPlan &P, int tile, int tid, int nt, VT *__restrict__ lu, VT *__restrict__ z,
-                              const double *__restrict__ rhs, double sign, int *clampLane, int l0) {
+                              const double *__restrict__ rhs, double sign, int *clampLane, int l0,
+                              const int *laneOn) {
     const int nb = P.nb;
     for (int l = 0; l < l0; l++) {
         const int i0 = P.levelItemPtr[l], i1 = P.levelItemPtr[l + 1];
         const int S = splitOf<T>(P.levelSplit, l);
         for (int w = tid; w < (i1 - i0) * T * S; w += nt) {
             const int t = w % T;
+            if (laneOn && !laneOn[t]) continue;
             const int s = (w / T) % S;
             const int4 it = P.itemRec[i0 + w / (T * S)];
             VT *own = lu + bIdx<T>(tile, P.nSlots, it.x, t);
@@ -411,7 +413,8 @@ __device__ void factorForward(const DevPlan &P, int tile, int tid, int nt, VT *_
 // barrier.
 template <int T, typename VT>
 __device__ void forwardOnly(const DevPlan &P, int tile, int tid, int nt, const VT *__restrict__ lu,
-                            VT *__restrict__ z, const double *__restrict__ rhs, double sign, int l0) {
+                            VT *__restrict__ z, const double *__restrict__ rhs, double sign, int l0,
+                            const int *laneOn) {
     const int nb = P.nb;
     for (int l = 0; l < l0; l++) {
         const int i0 = P.levelItemPtr[l];
@@ -419,6 +422,7 @@ __device__ void forwardOnly(const DevPlan &P, int tile, int tid, int nt, const V
         const int S = splitOf<T>(P.levelSplit, l);
         for (int w = tid; w < nNodes * T * S; w += nt) {
             const int t = w % T;
+            if (laneOn && !laneOn[t]) continue;
             const int s = (w / T) % S;
             const int4 it = P.itemRec[i0 + w / (T * S)];
             V2<VT> yk = {(VT)0, (VT)0};
@@ -448,13 +452,14 @@ __device__ void forwardOnly(const DevPlan &P, int tile, int tid, int nt, const V
 // S threads as in the factorization. Ends with a barrier.
 template <int T, typename VT>
 __device__ void backward(const DevPlan &P, int tile, int tid, int nt, const VT *__restrict__ lu,
-                         const VT *__restrict__ z, VT *__restrict__ xo, int l0) {
+                         const VT *__restrict__ z, VT *__restrict__ xo, int l0, const int *laneOn) {
     const int nb = P.nb;
     for (int l = l0 - 1; l >= 0; l--) {
         const int n0 = P.levelNodePtr[l], n1 = P.levelNodePtr[l + 1];
         const int S = splitOf<T>(P.levelSplitBwd, l);
         for (int w = tid; w < (n1 - n0) * T * S; w += nt) {
             const int t = w % T;
+            if (laneOn && !laneOn[t]) continue;
             const int s = (w / T) % S;
             const int4 br = P.bwdRec[n0 + w / (T * S)];
             V2<VT> acc = {(VT)0, (VT)0};
@@ -506,7 +511,7 @@ __host__ __device__ constexpr size_t topLdsBytes(int T, int n, size_t vtBytes) {
 // nodes below the block: the items of the top levels summed over their contributors j of levels < l0.
 template <int T, typename VT, bool FACTOR>
 __device__ void topFromBelow(const DevPlan &P, const DevTop &top, const TopLds<T, VT> &L, int tile, int tid, int nt,
-                             const VT *__restrict__ lu, const VT *__restrict__ z) {
+                             const VT *__restrict__ lu, const VT *__restrict__ z, const int *laneOn) {
     const int nb = P.nb;
     const int i0 = P.levelItemPtr[top.l0];
     const int nItems = FACTOR ? P.levelItemPtr[P.nLevels] - i0 : 0;
@@ -515,6 +520,7 @@ __device__ void topFromBelow(const DevPlan &P, const DevTop &top, const TopLds<T
     const int units = FACTOR ? nItems : top.n;
     for (int w = tid; w < units * T * S; w += nt) {
         const int t = w % T;
+        if (laneOn && !laneOn[t]) continue;
         const int s = (w / T) % S;
         int item;
         if (FACTOR) {
@@ -596,13 +602,14 @@ __device__ void topFromBelow(const DevPlan &P, const DevTop &top, const TopLds<T
 template <int T, typename VT>
 __device__ void topFactor(const DevPlan &P, const DevTop &top, const TopLds<T, VT> &L, int tile, int tid, int nt,
                           const VT *__restrict__ lu, const VT *__restrict__ z, const double *__restrict__ rhs,
-                          double sign, int *clampLane) {
+                          double sign, int *clampLane, const int *laneOn) {
     const int nb = P.nb, n = top.n;
     const Blk<VT> zero = {(VT)0, (VT)0, (VT)0, (VT)0};
     for (int w = tid; w < n * (n + 1) * T; w += nt) stBlk<VT>(L.base + (size_t)w * 4, zero);
     __syncthreads();
     for (int w = tid; w < (top.nSlots + n) * T; w += nt) {
         const int t = w % T, q = w / T;
+        if (laneOn && !laneOn[t]) continue;
         if (q < top.nSlots) {
             const int4 sr = top.slots[q];
             stBlk<VT>(L.M(sr.y, sr.z, t), ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, sr.x, t)));
@@ -614,8 +621,8 @@ __device__ void topFactor(const DevPlan &P, const DevTop &top, const TopLds<T, V
         }
     }
     __syncthreads();
-    topFromBelow<T, VT, true>(P, top, L, tile, tid, nt, lu, z);
-    if (tid < T) {
+    topFromBelow<T, VT, true>(P, top, L, tile, tid, nt, lu, z, laneOn);
+    if (tid < T && (!laneOn || laneOn[tid])) {
         bool cl;
         stBlk<VT>(L.Dinv(0, tid), inv2(ldBlk<VT>(L.M(0, 0, tid)), cl));
         if (cl && clampLane) atomicAdd(&clampLane[tid], 1);
@@ -623,13 +630,14 @@ __device__ void topFactor(const DevPlan &P, const DevTop &top, const TopLds<T, V
     __syncthreads();
     for (int a = 0; a < n; a++) {
         const int m = n - 1 - a;   // rows below a; columns a+1 .. n (n: right-hand side)
-        if (tid < T) {
+        if (tid < T && (!laneOn || laneOn[tid])) {
             const Blk<VT> Di = ldBlk<VT>(L.Dinv(a, tid));
             const Blk<VT> ya = ldBlk<VT>(L.M(a, n, tid));
             stV<VT>(L.Z(a, tid), V2<VT>{Di.a * ya.a + Di.b * ya.c, Di.c * ya.a + Di.d * ya.c});
         }
         for (int w = tid; w < m * (m + 1) * T; w += nt) {
             const int t = w % T, e = w / T;
+            if (laneOn && !laneOn[t]) continue;
             const int b = a + 1 + e / (m + 1), c = a + 1 + e % (m + 1);
             const Blk<VT> Di = ldBlk<VT>(L.Dinv(a, t));
             VT *mb = L.M(b, c, t);
@@ -652,19 +660,21 @@ __device__ void topFactor(const DevPlan &P, const DevTop &top, const TopLds<T, V
 template <int T, typename VT>
 __device__ void topForward(const DevPlan &P, const DevTop &top, const TopLds<T, VT> &L, int tile, int tid, int nt,
                            const VT *__restrict__ lu, const VT *__restrict__ z, const double *__restrict__ rhs,
-                           double sign) {
+                           double sign, const int *laneOn) {
     const int nb = P.nb, n = top.n;
     for (int w = tid; w < n * T; w += nt) {
         const int t = w % T, a = w / T;
+        if (laneOn && !laneOn[t]) continue;
         const int k = P.bwdRec[top.start + a].w;
         const double2 b = *reinterpret_cast<const double2 *>(rhs + vIdx<T>(tile, nb, k, t));
         stBlk<VT>(L.M(a, n, t), Blk<VT>{(VT)(sign * b.x), (VT)0, (VT)(sign * b.y), (VT)0});
     }
     __syncthreads();
-    topFromBelow<T, VT, false>(P, top, L, tile, tid, nt, lu, z);
+    topFromBelow<T, VT, false>(P, top, L, tile, tid, nt, lu, z, laneOn);
     for (int a = 0; a < n; a++) {
         for (int w = tid; w < (n - a) * T; w += nt) {
             const int t = w % T, b = a + w / T;   // b = a: store z(a); b > a: eliminate
+            if (laneOn && !laneOn[t]) continue;
             const Blk<VT> Di = ldBlk<VT>(L.Dinv(a, t));
             const Blk<VT> ya = ldBlk<VT>(L.M(a, n, t));
             const V2<VT> za = {Di.a * ya.a + Di.b * ya.c, Di.c * ya.a + Di.d * ya.c};
@@ -687,13 +697,14 @@ __device__ void topForward(const DevPlan &P, const DevTop &top, const TopLds<T, 
 // barrier per node; x of the top nodes goes to xo for the sparse levels below. Ends with a barrier.
 template <int T, typename VT>
 __device__ void topBackward(const DevPlan &P, const DevTop &top, const TopLds<T, VT> &L, int tile, int tid, int nt,
-                            VT *__restrict__ xo) {
+                            VT *__restrict__ xo, const int *laneOn) {
     const int nb = P.nb, n = top.n;
     for (int w = tid; w < n * T; w += nt) stV<VT>(L.Acc(w / T, w % T), V2<VT>{(VT)0, (VT)0});
     __syncthreads();
     for (int a = n - 1; a >= 0; a--) {
         for (int w = tid; w < (a + 1) * T; w += nt) {
             const int t = w % T, b = w / T;   // b = a: store x(a); b < a: accumulate U(b,a) x(a)
+            if (laneOn && !laneOn[t]) continue;
             const Blk<VT> Di = ldBlk<VT>(L.Dinv(a, t));
             const V2<VT> za = ldV<VT>(L.Z(a, t));
             const V2<VT> ac = ldV<VT>(L.Acc(a, t));
@@ -717,21 +728,23 @@ __device__ void topBackward(const DevPlan &P, const DevTop &top, const TopLds<T,
 template <int T, typename VT>
 __device__ __forceinline__ void luFactorForward(const DevPlan &P, const DevTop &top, const TopLds<T, VT> &L, int tile,
                                                 int tid, int nt, VT *lu, VT *z, const double *rhs, double sign,
-                                                int *clampLane) {
-    factorForward<T, VT>(P, tile, tid, nt, lu, z, rhs, sign, clampLane, top.l0);
-    if (top.n > 0) topFactor<T, VT>(P, top, L, tile, tid, nt, lu, z, rhs, sign, clampLane);
+                                                int *clampLane, const int *laneOn) {
+    factorForward<T, VT>(P, tile, tid, nt, lu, z, rhs, sign, clampLane, top.l0, laneOn);
+    if (top.n > 0) topFactor<T, VT>(P, top, L, tile, tid, nt, lu, z, rhs, sign, clampLane, laneOn);
 }
 template <int T, typename VT>
 __device__ __forceinline__ void luForward(const DevPlan &P, const DevTop &top, const TopLds<T, VT> &L, int tile,
-                                          int tid, int nt, const VT *lu, VT *z, const double *rhs, double sign) {
-    forwardOnly<T, VT>(P, tile, tid, nt, lu, z, rhs, sign, top.l0);
-    if (top.n > 0) topForward<T, VT>(P, top, L, tile, tid, nt, lu, z, rhs, sign);
+                                          int tid, int nt, const VT *lu, VT *z, const double *rhs, double sign,
+                                          const int *laneOn) {
+    forwardOnly<T, VT>(P, tile, tid, nt, lu, z, rhs, sign, top.l0, laneOn);
+    if (top.n > 0) topForward<T, VT>(P, top, L, tile, tid, nt, lu, z, rhs, sign, laneOn);
 }
 template <int T, typename VT>
 __device__ __forceinline__ void luBackward(const DevPlan &P, const DevTop &top, const TopLds<T, VT> &L, int tile,
-                                           int tid, int nt, const VT *lu, const VT *z, VT *xo) {
-    if (top.n > 0) topBackward<T, VT>(P, top, L, tile, tid, nt, xo);
-    backward<T, VT>(P, tile, tid, nt, lu, z, xo, top.l0);
+                                           int tid, int nt, const VT *lu, const VT *z, VT *xo,
+                                           const int *laneOn) {
+    if (top.n > 0) topBackward<T, VT>(P, top, L, tile, tid, nt, xo, laneOn);
+    backward<T, VT>(P, tile, tid, nt, lu, z, xo, top.l0, laneOn);
 }
 
 // Mismatch and Jacobian fill of the lanes flagged in fillLane; per-thread partial max|F| and finiteness of the
@@ -992,9 +1005,9 @@ __global__ __launch_bounds__(MAX_THREADS) void nrStepKernel(DevPlan P, DevState 
     if (sFlag[2]) {
         // ---- 4. factorization with the forward solve of -F, then the backward solve
         OLFHIP_MARK(2);
-        luFactorForward<T, VT>(P, top, TL, tile, tid, nt, lu, y, S.F, -1.0, sClamp);
+        luFactorForward<T, VT>(P, top, TL, tile, tid, nt, lu, y, S.F, -1.0, sClamp, sSolve);
         OLFHIP_MARK(3);
-        luBackward<T, VT>(P, top, TL, tile, tid, nt, lu, y, dx1);
+        luBackward<T, VT>(P, top, TL, tile, tid, nt, lu, y, dx1, sSolve);
         OLFHIP_MARK(4);
         // ---- 5. step in FP64 (+ one FP64-residual refinement step on the FP32 factors)
         for (int w = tid; w < nb * T; w += nt) {
@@ -1009,8 +1022,8 @@ __global__ __launch_bounds__(MAX_THREADS) void nrStepKernel(DevPlan P, DevState 
             applyJPass<T, VT>(P, S, tile, tid, nt, sSolve, S.dx, S.F, 1.0, S.r, pm);
             __syncthreads();
             OLFHIP_MARK(5);
-            luForward<T, VT>(P, top, TL, tile, tid, nt, lu, y, S.r, -1.0);
-            luBackward<T, VT>(P, top, TL, tile, tid, nt, lu, y, dx1);
+            luForward<T, VT>(P, top, TL, tile, tid, nt, lu, y, S.r, -1.0, sSolve);
+            luBackward<T, VT>(P, top, TL, tile, tid, nt, lu, y, dx1, sSolve);
             OLFHIP_MARK(6);
             for (int w = tid; w < nb * T; w += nt) {
                 const int k = w / T, t = w % T;
@@ -1253,8 +1266,8 @@ __global__ __launch_bounds__(MAX_THREADS) void luSolveKernel(DevPlan P, VT *lu, 
     const int tile = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
     if (tid < T) sClamp[tid] = 0;
     __syncthreads();
-    luFactorForward<T, VT>(P, top, TL, tile, tid, nt, lu, y, rhs, 1.0, sClamp);
-    luBackward<T, VT>(P, top, TL, tile, tid, nt, lu, y, xo);
+    luFactorForward<T, VT>(P, top, TL, tile, tid, nt, lu, y, rhs, 1.0, sClamp, nullptr);
+    luBackward<T, VT>(P, top, TL, tile, tid, nt, lu, y, xo, nullptr);
     if (tid < T) clamps[tile * T + tid] = sClamp[tid];
 }
 

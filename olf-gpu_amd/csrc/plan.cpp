@@ -241,6 +241,25 @@ Plan buildPlan(int n, int nBranch, const double *p8, const int *idx12, int nShun
         P.levelSplitBwd[l] = splitFor(widest);
     }
 
+    // --- per-branch flow coefficients (the same half-branch forms as the fill)
+    P.branchCoef.assign(8 * (size_t)nBranch, 0.0);
+    P.branchNode.assign(2 * (size_t)nBranch, 0);
+    for (int e = 0; e < nBranch; e++) {
+        const double *p = p8 + 8 * e;
+        const double y = p[0], g1 = p[1], g2 = p[2], b1 = p[3], b2 = p[4], ksi = p[5], r1 = p[6], a1 = p[7];
+        const double sk = std::sin(ksi), ck = std::cos(ksi);
+        double *c = &P.branchCoef[8 * (size_t)e];
+        c[0] = r1 * r1 * (g1 + y * sk);
+        c[1] = y * r1;
+        c[2] = r1 * r1 * (-b1 + y * ck);
+        c[3] = ksi - a1;
+        c[4] = g2 + y * sk;
+        c[5] = -b2 + y * ck;
+        c[6] = ksi + a1;
+        P.branchNode[2 * e] = P.pos[bus1[e]];
+        P.branchNode[2 * e + 1] = P.pos[bus2[e]];
+    }
+
     // --- fill gather tables
     std::vector<std::vector<int>> inc(nb);
     for (int e = 0; e < nBranch; e++) {

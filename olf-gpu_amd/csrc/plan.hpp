@@ -66,6 +66,10 @@ struct Plan {
     std::vector<int> grpSlot;                      // block slot of (k, far): U(k,far) or L(k,far)
     std::vector<int> fillPtr, fillSlot;            // pairs of node k absent from the pattern (zeroed per fill)
     std::vector<double> shuntG, shuntB;            // per node, pu
+    // per closed branch, for flows at both ends: A1 B C1 psi1 A2 C2 psi2 0 of the own/far form (side 1 at bus1,
+    // side 2 at bus2) and the two end nodes
+    std::vector<double> branchCoef;
+    std::vector<int> branchNode;
 
     // --- scalar CSR pattern of the stable-row Jacobian (the reference's superset pattern) and its map to slots
     std::vector<int> rowPtr, colIdx;

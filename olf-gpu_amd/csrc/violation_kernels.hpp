@@ -1,0 +1,131 @@
+// Device screen of limit violations over the states a batched solve left on the device (reference
+// native/cuda/sa/olf_gpu_violations.cu:1-27,170-300, semantics of GpuLimitScanner.java:20-130).
+//
+// The screen does not decide: it compares against a threshold relaxed by a relative band and keeps emitting a group's
+// exceeded limits until one is exceeded beyond the band (certain to be confirmed, and a group reports its first
+// confirmed limit only), so it can only hand the host more candidates than the exact decision reports. The host then
+// re-tests every candidate with the exact comparison on its own FP64 evaluation of the same state (detectViolations
+// in api.hip), which makes the reported set independent of the device arithmetic (FMA contraction, sin/cos).
+//
+// Records: branch limit l -> l; bus i high voltage -> nLimits + 2i, low voltage -> nLimits + 2i + 1.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "nr_kernels.hpp"
+
+namespace olfhip {
+
+constexpr int QTY_I = 0, QTY_P = 1, QTY_S = 2;
+
+struct DevLimits {
+    int nBranches, nGroups, nLimits, nBuses;
+    const int *branch;        // table branch -> closed branch
+    const int *groupStart;    // [nBranches + 1]
+    const int *groupSide;     // 1 or 2
+    const int *groupQty;
+    const int *limitStart;    // [nGroups + 1]
+    const double *threshold;  // [nLimits]
+    const double *busHigh, *busLow;   // [nBuses], NaN: none
+    const double *coef;       // per closed branch: A1 B C1 psi1 A2 C2 psi2 0
+    const int *branchNode;    // per closed branch: end nodes
+};
+
+__device__ __forceinline__ void emitRecord(int *records, int *counts, int scenario, int capacity, int record) {
+    const int i = atomicAdd(counts + scenario, 1);
+    if (i < capacity) records[(size_t)scenario * capacity + i] = record;
+}
+
+// Whether node k of lane t is cut off from the slack in this scenario: its angle row is an identity row here but an
+// equation in the base case (the island freeze).
+template <int T>
+__device__ __forceinline__ bool islandedNode(const DevPlan &P, const DevState &S, const int *modeBase, int tile, int k,
+                                             int t) {
+    return S.mode[vIdx<T>(tile, P.nb, k, t)] == MODE_FIXED && modeBase[2 * P.perm[k]] != MODE_FIXED;
+}
+
+// One thread per (scenario, table branch). Scenarios whose flag is 0 (not converged) emit nothing.
+template <int T>
+__global__ void screenBranchesKernel(DevPlan P, DevState S, DevLimits L, const int *modeBase, const int *screenLane,
+                                     int nScenarios, double band, int capacity, int *records, int *counts) {
+    const size_t total = (size_t)nScenarios * L.nBranches;
+    for (size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x; w < total; w += (size_t)gridDim.x * blockDim.x) {
+        const int b = (int)(w % L.nBranches);
+        const int s = (int)(w / L.nBranches);
+        if (!screenLane[s]) continue;
+        const int g0 = L.groupStart[b], g1 = L.groupStart[b + 1];
+        if (g0 == g1) continue;
+        const int e = L.branch[b];
+        const int tile = s / T, t = s % T;
+        bool out = false;
+        for (int q = 0; q < S.maxOut; q++) out |= S.outages[(size_t)s * S.maxOut + q] == e;
+        if (out) continue;
+        const int k1 = L.branchNode[2 * e], k2 = L.branchNode[2 * e + 1];
+        if (islandedNode<T>(P, S, modeBase, tile, k1, t) || islandedNode<T>(P, S, modeBase, tile, k2, t)) continue;
+        const double2 x1 = *reinterpret_cast<const double2 *>(S.x + vIdx<T>(tile, P.nb, k1, t));
+        const double2 x2 = *reinterpret_cast<const double2 *>(S.x + vIdx<T>(tile, P.nb, k2, t));
+        const double *c = L.coef + 8 * (size_t)e;
+        const double v1 = x1.y, v2 = x2.y;
+        double s1, c1, s2, c2;
+        sincos(c[3] - x1.x + x2.x, &s1, &c1);
+        sincos(c[6] - x2.x + x1.x, &s2, &c2);
+        const double bvv = c[1] * v1 * v2;
+        const double p1 = c[0] * v1 * v1 - bvv * s1, q1 = c[2] * v1 * v1 - bvv * c1;
+        const double p2 = c[4] * v2 * v2 - bvv * s2, q2 = c[5] * v2 * v2 - bvv * c2;
+        for (int g = g0; g < g1; g++) {
+            const bool one = L.groupSide[g] == 1;
+            const double p = one ? p1 : p2, q = one ? q1 : q2, v = one ? v1 : v2;
+            double value;
+            switch (L.groupQty[g]) {
+                case QTY_I: value = hypot(p, q) / v; break;
+                case QTY_P: value = fabs(p); break;
+                default: value = sqrt(p * p + q * q); break;
+            }
+            if (isnan(value)) continue;
+            for (int l = L.limitStart[g]; l < L.limitStart[g + 1]; l++) {
+                const double th = L.threshold[l];
+                if (value > th - fabs(th) * band) {
+                    emitRecord(records, counts, s, capacity, l);
+                    if (value > th + fabs(th) * band) break;   // certain to confirm: the group reports this one
+                }
+            }
+        }
+    }
+}
+
+// One thread per (scenario, bus with limits). A NaN limit compares false (never fires).
+template <int T>
+__global__ void screenBusesKernel(DevPlan P, DevState S, DevLimits L, const int *modeBase, const int *screenLane,
+                                  int nScenarios, double band, int capacity, int *records, int *counts) {
+    const size_t total = (size_t)nScenarios * L.nBuses;
+    for (size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x; w < total; w += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(w % L.nBuses);
+        const int s = (int)(w / L.nBuses);
+        if (!screenLane[s]) continue;
+        const int tile = s / T, t = s % T;
+        const int k = P.pos[i];
+        if (islandedNode<T>(P, S, modeBase, tile, k, t)) continue;
+        const double v = S.x[vIdx<T>(tile, P.nb, k, t) + 1];
+        const double hi = L.busHigh[i], lo = L.busLow[i];
+        if (v > hi - fabs(hi) * band) emitRecord(records, counts, s, capacity, L.nLimits + 2 * i);
+        if (v < lo + fabs(lo) * band) emitRecord(records, counts, s, capacity, L.nLimits + 2 * i + 1);
+    }
+}
+
+// Gathers the (phi, V) pairs of the requested (scenario, node) pairs and whether each node is islanded.
+template <int T>
+__global__ void gatherNodesKernel(DevPlan P, DevState S, const int *modeBase, int n, const int2 *req, double2 *out,
+                                  int *islanded) {
+    for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < n; w += gridDim.x * blockDim.x) {
+        const int2 r = req[w];
+        const int tile = r.x / T, t = r.x % T;
+        out[w] = *reinterpret_cast<const double2 *>(S.x + vIdx<T>(tile, P.nb, r.y, t));
+        islanded[w] = islandedNode<T>(P, S, modeBase, tile, r.y, t) ? 1 : 0;
+    }
+}
+
+__global__ void zeroIntKernel(int *p, int n) {
+    for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < n; w += gridDim.x * blockDim.x) p[w] = 0;
+}
+
+}  // namespace olfhip

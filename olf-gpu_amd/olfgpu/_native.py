@@ -28,6 +28,7 @@ EXPORTS = [
     "olfhip_neighbourhoods", "olfhip_mlp_create", "olfhip_mlp_destroy", "olfhip_mlp_set_normalization",
     "olfhip_mlp_set_base_states", "olfhip_mlp_train_step", "olfhip_mlp_forward", "olfhip_mlp_gradients",
     "olfhip_mlp_parameters", "olfhip_mlp_predict_warm_start", "olfhip_mlp_gemm_test",
+    "olfhip_create_limit_table", "olfhip_destroy_limit_table", "olfhip_screen_violations", "olfhip_detect_violations",
 ]
 
 
@@ -68,6 +69,20 @@ class SolveOutputs(C.Structure):
 _LIB = None
 
 
+class LimitTableDesc(C.Structure):
+    _fields_ = [("n_branches", C.c_int), ("branch", C.c_void_p), ("group_start", C.c_void_p),
+                ("group_side", C.c_void_p), ("group_quantity", C.c_void_p), ("limit_start", C.c_void_p),
+                ("limit_threshold", C.c_void_p), ("n_buses", C.c_int), ("bus_high", C.c_void_p),
+                ("bus_low", C.c_void_p)]
+
+
+# olfhip_violation as a numpy record
+VIOLATION_DTYPE = np.dtype([("scenario", np.int32), ("kind", np.int32), ("element", np.int32), ("side", np.int32),
+                            ("quantity", np.int32), ("limit", np.int32), ("value", np.float64),
+                            ("threshold", np.float64)])
+VIOLATION_KINDS = {0: "BRANCH", 1: "HIGH_VOLTAGE", 2: "LOW_VOLTAGE"}
+
+
 def lib():
     """The loaded library; raises if it was not built (no fallback)."""
     global _LIB
@@ -97,6 +112,11 @@ def lib():
         L.olfhip_lu_factorize_and_solve.argtypes = [vp, i, i, vp, vp, i, vp, vp, vp]
         L.olfhip_autotune.argtypes = [vp, i, C.POINTER(ReactiveLimits), C.POINTER(SolveInputs), C.POINTER(Tuning),
                                       C.POINTER(d)]
+        L.olfhip_create_limit_table.argtypes = [vp, C.POINTER(LimitTableDesc), C.POINTER(vp)]
+        L.olfhip_destroy_limit_table.argtypes = [vp]
+        L.olfhip_destroy_limit_table.restype = None
+        L.olfhip_screen_violations.argtypes = [vp, vp, d, i, vp, vp]
+        L.olfhip_detect_violations.argtypes = [vp, vp, d, i, i, vp, C.POINTER(i)]
         _LIB = L
     return _LIB
 
@@ -297,6 +317,54 @@ class BatchContext:
         x = np.zeros((k, self.structure.n))
         check(lib().olfhip_read_states(self.h, k, ptr(x)))
         return x
+
+
+class LimitTable:
+    """Owner of an olfhip_limit_table (the reference's ViolationScreenNative.uploadViolationTable) built from the
+    arrays of a table dict (gridgen.limit_table layout)."""
+
+    def __init__(self, structure: Structure, table: dict):
+        self.structure = structure
+        self._keep = {k: np.ascontiguousarray(table[k], np.int32) for k in
+                      ("branch", "group_start", "group_side", "group_quantity", "limit_start")}
+        self._keep.update({k: np.ascontiguousarray(table[k], np.float64) for k in ("threshold", "bus_high",
+                                                                                   "bus_low")})
+        k = self._keep
+        self.n_limits = len(k["threshold"])
+        self.n_buses = len(k["bus_high"])
+        desc = LimitTableDesc(len(k["branch"]), ptr(k["branch"]), ptr(k["group_start"]), ptr(k["group_side"]),
+                              ptr(k["group_quantity"]), ptr(k["limit_start"]), ptr(k["threshold"]),
+                              len(k["bus_high"]), ptr(k["bus_high"]), ptr(k["bus_low"]))
+        h = C.c_void_p()
+        check(lib().olfhip_create_limit_table(structure.h, C.byref(desc), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().olfhip_destroy_limit_table(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def screen(self, context: "BatchContext", k: int, band=1e-7, capacity=1024):
+        """The device screen's candidates for the last solve of `context` (k scenarios): (counts [k],
+        records [k][capacity])."""
+        counts = np.zeros(k, np.int32)
+        records = np.zeros((k, capacity), np.int32)
+        check(lib().olfhip_screen_violations(context.h, self.h, band, capacity, ptr(counts), ptr(records)))
+        return counts, records
+
+    def detect(self, context: "BatchContext", band=1e-7, capacity=1024, max_violations=1 << 20):
+        """Screen + exact host decisions: a VIOLATION_DTYPE array ordered by scenario, branches, buses."""
+        out = np.zeros(max_violations, VIOLATION_DTYPE)
+        n = C.c_int()
+        check(lib().olfhip_detect_violations(context.h, self.h, band, capacity, max_violations, ptr(out),
+                                             C.byref(n)))
+        return out[:n.value].copy()
 
 
 def autotune(structure: Structure, inputs: SolveInputs, mode=LU_FP64, reactive_limits=None):

@@ -314,3 +314,64 @@ WORKLOADS = {
 def workload(name: str, load_scale: float | None = None) -> Network:
     nb, ne, seed, scale = WORKLOADS[name]
     return generate(nb, ne, seed, scale if load_scale is None else load_scale)
+
+
+def branch_flows(net: Network, x: np.ndarray):
+    """Textbook pi-branch flows at a state (stable rows): S1, S2 (pu, complex) and |V1|, |V2| per branch, with the
+    ideal transformer of ratio r on side 1."""
+    x = np.asarray(x, np.float64)
+    th, v = x[0::2], x[1::2]
+    a, b = net.bus1, net.bus2
+    Va = v[a] * np.exp(1j * th[a]) * net.ratio
+    Vb = v[b] * np.exp(1j * th[b])
+    ys = 1.0 / (net.r + 1j * net.x)
+    bc = 1j * net.b / 2
+    S1 = Va * np.conj((Va - Vb) * ys + Va * bc)
+    S2 = Vb * np.conj((Vb - Va) * ys + Vb * bc)
+    return S1, S2, v[a], v[b]
+
+
+def limit_table(net: Network, x_base: np.ndarray, seed: int = 7) -> dict:
+    """Synthetic operational limits for violation detection (the workload definition carries none), rated from the
+    base case so that outages overload some neighbours:
+
+    * 10 % of the branches unmonitored; every other branch has a current group on each side with thresholds
+      [1.5, 1.25, 1.0] x PATL, most severe first, PATL = max(1.1 - 1.5 x the larger base-case end current, 0.02) pu;
+    * 5 % of the monitored branches also an active-power group on side 1 ([1.3, 1.0] x a rating of the same kind on
+      |P1|), every monitored transformer an apparent-power group on side 1 ([1.0] x a rating on |S1|);
+    * every bus a low limit U(0.90, 0.95) and a high limit U(1.05, 1.10) pu, 10 % of each NaN (no limit).
+    Returns the table as arrays in the C ABI's layout (olfhip_limit_table_desc)."""
+    rng = np.random.default_rng(seed)
+    S1, S2, v1, v2 = branch_flows(net, x_base)
+    i_base = np.maximum(np.abs(S1) / v1, np.abs(S2) / v2)
+    monitored = rng.random(net.n_branch) >= 0.1
+    factor = rng.uniform(1.1, 1.5, net.n_branch)
+    with_p = rng.random(net.n_branch) < 0.05
+    is_tr = net.ratio != 1.0
+    group_start = [0]
+    group_side, group_qty, limit_start, threshold = [], [], [0], []
+
+    def group(side, qty, rating, mults):
+        group_side.append(side)
+        group_qty.append(qty)
+        threshold.extend(m * rating for m in mults)
+        limit_start.append(len(threshold))
+
+    for e in range(net.n_branch):
+        if monitored[e]:
+            patl = max(factor[e] * i_base[e], 0.02)
+            group(1, 0, patl, (1.5, 1.25, 1.0))
+            group(2, 0, patl, (1.5, 1.25, 1.0))
+            if with_p[e]:
+                group(1, 1, max(factor[e] * abs(S1[e].real), 0.02), (1.3, 1.0))
+            if is_tr[e]:
+                group(1, 2, max(factor[e] * abs(S1[e]), 0.02), (1.0,))
+        group_start.append(len(group_side))
+    low = rng.uniform(0.90, 0.95, net.n_bus)
+    high = rng.uniform(1.05, 1.10, net.n_bus)
+    low[rng.random(net.n_bus) < 0.1] = np.nan
+    high[rng.random(net.n_bus) < 0.1] = np.nan
+    return dict(branch=np.arange(net.n_branch, dtype=np.int32), group_start=np.array(group_start, np.int32),
+                group_side=np.array(group_side, np.int32), group_quantity=np.array(group_qty, np.int32),
+                limit_start=np.array(limit_start, np.int32), threshold=np.array(threshold, np.float64),
+                bus_high=high, bus_low=low)

@@ -164,6 +164,7 @@ class BatchScenarioResult:
     chunks: int
     device_seconds: float
     warm_start_fallbacks: int = 0
+    violations: np.ndarray | None = None   # N.VIOLATION_DTYPE, scenario = contingency index (with a limit table)
 
 
 class GpuBatchedSecurityAnalysisSolver:
@@ -213,10 +214,12 @@ class GpuBatchedSecurityAnalysisSolver:
 
     def solve_batch_n1(self, x_base, target_base, row_mode_base, limit_types_base, contingencies,
                        x0_per_scenario=None, states_to_host=True, warm_start=None,
-                       warm_start_features=None) -> BatchScenarioResult:
+                       warm_start_features=None, limit_table: N.LimitTable | None = None,
+                       violation_band=1e-7) -> BatchScenarioResult:
         """solveBatchN1: contingencies is [K] or [K, m] closed-branch ids (-1 padded). With a warm_start network
         (olfgpu.warmstart.WarmStartMLP, single outages) every scenario starts from its prediction, and any that does
-        not converge is solved again from the base-case start."""
+        not converge is solved again from the base-case start. With a limit_table the violations of every converged
+        contingency are detected on its final state (device screen, exact host decisions)."""
         import time
 
         ctg = np.asarray(contingencies, dtype=np.int32)
@@ -258,6 +261,16 @@ class GpuBatchedSecurityAnalysisSolver:
             chunks += 1
             return r, cnt
 
+        viol = []
+
+        def detect(rows):
+            """violations of the last solve, scenario indices mapped through rows"""
+            if limit_table is None:
+                return
+            v = limit_table.detect(self._ctx, band=violation_band)
+            v["scenario"] = rows[v["scenario"]]
+            viol.append(v)
+
         for c0 in range(0, k, chunk):
             c1 = min(k, c0 + chunk)
             part = ctg[c0:c1]
@@ -272,16 +285,22 @@ class GpuBatchedSecurityAnalysisSolver:
             nm[c0:c1] = r["norm"]
             if states_to_host:
                 xs[c0:c1] = r["x"]
+            detect(np.arange(c0, c1))
             if warm_start is not None:
                 # a warm start that did not converge: solve again from the base-case start, whose answer stands
                 redo = np.nonzero(r["status"] != 0)[0]
                 if redo.size:
                     fallback += redo.size
                     r2, _ = solve_part(part[redo])
+                    detect(c0 + redo)
                     st[c0 + redo] = r2["status"]
                     it[c0 + redo] = r2["iterations"]
                     tot[c0 + redo] += r2["total_iterations"]
                     nm[c0 + redo] = r2["norm"]
                     if states_to_host:
                         xs[c0 + redo] = r2["x"]
-        return BatchScenarioResult(st, it, tot, nm, xs, islanded, chunks, device_s, fallback)
+        violations = None
+        if limit_table is not None:
+            violations = np.concatenate(viol) if viol else np.zeros(0, N.VIOLATION_DTYPE)
+            violations = violations[np.argsort(violations["scenario"], kind="stable")]
+        return BatchScenarioResult(st, it, tot, nm, xs, islanded, chunks, device_s, fallback, violations)

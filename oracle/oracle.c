@@ -794,3 +794,59 @@ int oracle_n1(const oracle_sys *s, const oracle_nr_params *prm, const oracle_rl 
     }
     return used;
 }
+
+/* ------------------------------------------------------------------------------------------------ */
+/* Limit violations: OLF's detection scan (sa/gpu/GpuLimitScanner.java:95-131,394-410).             */
+/* ------------------------------------------------------------------------------------------------ */
+int oracle_violations(const oracle_sys *s, const double *x, const int *out_br, int n_out,
+                      const unsigned char *islanded, const oracle_limits *lim, oracle_violation *out, int max_out)
+{
+    int count = 0;
+    double o[28];
+    for (int b = 0; b < lim->n_branches; b++) {
+        const int g0 = lim->group_start[b], g1 = lim->group_start[b + 1];
+        if (g0 == g1) continue;
+        const int e = lim->branch[b];
+        if (n_out > 0 && is_out(e, out_br, n_out)) continue;
+        const int a = s->bus1[e], c = s->bus2[e];
+        if (islanded && (islanded[a] || islanded[c])) continue;
+        oracle_closed_branch(s->p8 + 8 * e, x[2 * a + 1], x[2 * a], x[2 * c + 1], x[2 * c], o);
+        /* outputs 0, 7, 14, 21: p1, q1, p2, q2 */
+        for (int g = g0; g < g1; g++) {
+            const int one = lim->group_side[g] == 1;
+            const double p = one ? o[0] : o[14], q = one ? o[7] : o[21], v = one ? x[2 * a + 1] : x[2 * c + 1];
+            double value;
+            switch (lim->group_quantity[g]) {
+                case 0: value = hypot(p, q) / v; break;
+                case 1: value = p; break;
+                default: value = sqrt(p * p + q * q); break;
+            }
+            if (lim->group_quantity[g] == 2 && isnan(value)) continue;
+            const double compared = lim->group_quantity[g] == 1 ? fabs(value) : value;
+            for (int l = lim->limit_start[g]; l < lim->limit_start[g + 1]; l++) {
+                if (compared > lim->threshold[l]) {
+                    if (count < max_out) {
+                        oracle_violation v0 = {0, b, lim->group_side[g], lim->group_quantity[g], l, value,
+                                               lim->threshold[l]};
+                        out[count] = v0;
+                    }
+                    count++;
+                    break;     /* the most serious one only */
+                }
+            }
+        }
+    }
+    for (int i = 0; i < lim->n_buses; i++) {
+        if (islanded && islanded[i]) continue;
+        const double v = x[2 * i + 1];
+        if (v > lim->bus_high[i]) {
+            if (count < max_out) { oracle_violation v0 = {1, i, 0, 0, -1, v, lim->bus_high[i]}; out[count] = v0; }
+            count++;
+        }
+        if (v < lim->bus_low[i]) {
+            if (count < max_out) { oracle_violation v0 = {2, i, 0, 0, -1, v, lim->bus_low[i]}; out[count] = v0; }
+            count++;
+        }
+    }
+    return count;
+}

@@ -102,6 +102,33 @@ int oracle_n1(const oracle_sys *s, const oracle_nr_params *prm, const oracle_rl 
               int n_threads, double *x_out, int *status, int *iterations, int *total_iterations,
               double *norm);
 
+/* Limit table (the reference's GpuLimitTable, flattened): per monitored closed branch its groups (side 1|2,
+ * quantity 0 current |S|/V, 1 active power, 2 apparent power) with thresholds most severe first; per bus a high
+ * and a low voltage limit (NaN: none). */
+typedef struct {
+    int n_branches;
+    const int *branch, *group_start, *group_side, *group_quantity, *limit_start;
+    const double *threshold;
+    int n_buses;
+    const double *bus_high, *bus_low;
+} oracle_limits;
+
+typedef struct {
+    int kind;           /* 0 branch, 1 high voltage, 2 low voltage */
+    int element;        /* table branch or bus */
+    int side, quantity, limit;
+    double value, threshold;
+} oracle_violation;
+
+/* OLF's full detection scan on one state (reference sa/gpu/GpuLimitScanner.java:95-131,394-410, which follows
+ * LimitViolationManager.detectViolations): every monitored branch in service (not outaged, both ends not
+ * islanded) in table order, each group's first exceeded threshold (active power compares |p|, reports p;
+ * a NaN apparent power is skipped); then every bus in service, high then low (NaN limits never fire).
+ * Flows from oracle_closed_branch. Writes at most max_out violations, returns the count found.
+ * Pinned by tests/test_oracle_violations.py (textbook flows, hand cases, brute-force rescan). */
+int oracle_violations(const oracle_sys *s, const double *x, const int *out_br, int n_out,
+                      const unsigned char *islanded, const oracle_limits *lim, oracle_violation *out, int max_out);
+
 #ifdef __cplusplus
 }
 #endif

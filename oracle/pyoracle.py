@@ -32,6 +32,17 @@ class Rl(C.Structure):
                 ("max_q", C.c_void_p), ("load_q", C.c_void_p), ("target_v", C.c_void_p)]
 
 
+class Limits(C.Structure):
+    _fields_ = [("n_branches", C.c_int), ("branch", C.c_void_p), ("group_start", C.c_void_p),
+                ("group_side", C.c_void_p), ("group_quantity", C.c_void_p), ("limit_start", C.c_void_p),
+                ("threshold", C.c_void_p), ("n_buses", C.c_int), ("bus_high", C.c_void_p), ("bus_low", C.c_void_p)]
+
+
+class Violation(C.Structure):
+    _fields_ = [("kind", C.c_int), ("element", C.c_int), ("side", C.c_int), ("quantity", C.c_int),
+                ("limit", C.c_int), ("value", C.c_double), ("threshold", C.c_double)]
+
+
 class Result(C.Structure):
     _fields_ = [("status", C.c_int), ("iterations", C.c_int), ("total_iterations", C.c_int), ("solves", C.c_int),
                 ("outer_switches", C.c_int), ("norm", C.c_double)]
@@ -56,6 +67,9 @@ def lib():
         L.oracle_nr_solve.argtypes = [C.c_void_p, C.POINTER(NrParams), C.POINTER(Rl), _ip, C.c_int, C.c_void_p,
                                       _dp, _ip, _dp, _bp, _bp, C.POINTER(Result)]
         L.oracle_islands.argtypes = [C.c_void_p, C.c_int, _ip, C.c_int, _ubp]
+        L.oracle_violations.restype = C.c_int
+        L.oracle_violations.argtypes = [C.c_void_p, _dp, _ip, C.c_int, C.c_void_p, C.POINTER(Limits),
+                                        C.POINTER(Violation), C.c_int]
         L.oracle_n1.restype = C.c_int
         L.oracle_n1.argtypes = [C.c_void_p, C.POINTER(NrParams), C.POINTER(Rl), C.c_int, _dp, _ip, _dp, _bp,
                                 C.c_int, C.c_int, _ip, C.c_int, _dp, _ip, _ip, _ip, _dp]
@@ -173,3 +187,28 @@ class OracleSystem:
                                np.ascontiguousarray(target_base, np.float64), lt, k, mo, ctg, threads, x, st, it,
                                tot, nm)
         return dict(x=x, status=st, iterations=it, total_iterations=tot, norm=nm, threads=used)
+
+
+def limits_struct(table):
+    """An oracle_limits over a limit-table dict (gridgen.limit_table); returns (struct, kept arrays)."""
+    keep = {k: np.ascontiguousarray(table[k], np.int32) for k in
+            ("branch", "group_start", "group_side", "group_quantity", "limit_start")}
+    keep.update({k: np.ascontiguousarray(table[k], np.float64) for k in ("threshold", "bus_high", "bus_low")})
+    st = Limits(len(keep["branch"]), keep["branch"].ctypes.data, keep["group_start"].ctypes.data,
+                keep["group_side"].ctypes.data, keep["group_quantity"].ctypes.data, keep["limit_start"].ctypes.data,
+                keep["threshold"].ctypes.data, len(keep["bus_high"]), keep["bus_high"].ctypes.data,
+                keep["bus_low"].ctypes.data)
+    return st, keep
+
+
+def violations(system, x, table, outages=(), islanded=None, max_out=100000):
+    """OLF's detection scan on one state: list of (kind, element, side, quantity, limit, value, threshold)."""
+    st, keep = limits_struct(table)
+    out = (Violation * max_out)()
+    ob = np.ascontiguousarray(np.asarray(outages, dtype=np.int32).reshape(-1))
+    isl = None if islanded is None else np.ascontiguousarray(islanded, np.uint8)
+    n = lib().oracle_violations(system.h, np.ascontiguousarray(x, np.float64), ob, len(ob),
+                                isl.ctypes.data if isl is not None else None, C.byref(st), out, max_out)
+    if n > max_out:
+        raise RuntimeError("more violations than max_out")
+    return [(v.kind, v.element, v.side, v.quantity, v.limit, v.value, v.threshold) for v in out[:n]]

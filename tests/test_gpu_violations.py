@@ -1,0 +1,96 @@
+"""GPU tests of violation detection on N-1 states (device screen + exact host decisions, through the C ABI):
+
+* on the device's own states, the detected list equals the oracle's full OLF scan of those states, and every
+  reported limit was among the screen's candidates (the screen may only over-emit);
+* a capacity too small for the candidates falls back to a full scan with the same answer;
+* end to end, the violation lists of the batched N-1 equal those of the oracle's N-1 (its own FP64 states): same
+  contingencies, same (kind, element, side, quantity, limit) in the same order, values within 1e-5 relative; any
+  entry whose value is within 1e-6 of its threshold would be reported as ambiguous (states agree to 1e-6)."""
+import numpy as np
+import pytest
+
+import pyoracle as po
+from olfgpu import _native as N
+from olfgpu import gridgen
+from olfgpu.solver import GpuBatchedSecurityAnalysisSolver, GpuLoadFlowParameters, GpuNewtonRaphson
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(net, linear_solver, count, bridge_fraction, seed):
+    st = N.Structure.from_network(net)
+    rl = net.reactive_limits()
+    x0, tg, md, _ = net.system()
+    base = GpuNewtonRaphson(st, rl).run(x0, tg, md)
+    assert base.status == "CONVERGED"
+    table = gridgen.limit_table(net, base.x)
+    ctg = gridgen.sample_contingencies(net, count, bridge_fraction, seed)
+    sa = GpuBatchedSecurityAnalysisSolver(st, rl, GpuLoadFlowParameters(linearSolver=linear_solver))
+    return st, base, table, ctg, sa
+
+
+def _by_scenario(v, k):
+    out = [[] for _ in range(k)]
+    for r in v:
+        out[int(r["scenario"])].append((int(r["kind"]), int(r["element"]), int(r["side"]), int(r["quantity"]),
+                                        int(r["limit"]), float(r["value"])))
+    return out
+
+
+@pytest.mark.parametrize("linear_solver", ["BATCHED_LU", "BATCHED_LU_FP32_REFINED"])
+def test_detection_equals_the_oracle_scan_of_the_device_states(grid, linear_solver):
+    net = grid("grid2000")
+    print(f"[viol] grid2000 {linear_solver} device states", flush=True)
+    st, base, table, ctg, sa = _setup(net, linear_solver, 48, 0.25, 6)
+    lt = N.LimitTable(st, table)
+    r = sa.solve_batch_n1(base.x, base.target, base.row_mode, base.reactive_limit_types, ctg, limit_table=lt)
+    got = _by_scenario(r.violations, len(ctg))
+    counts, records = lt.screen(sa._ctx, len(ctg))
+    o = po.OracleSystem(net)
+    total = 0
+    for j, e in enumerate(ctg):
+        if r.status[j] != 0:
+            assert got[j] == []
+            continue
+        isl = o.islands([e])
+        ref = po.violations(o, r.x[j], table, outages=[e], islanded=isl)
+        assert [g[:5] for g in got[j]] == [q[:5] for q in ref], (j, e)
+        if ref:
+            np.testing.assert_allclose([g[5] for g in got[j]], [q[5] for q in ref], rtol=1e-12)
+        cand = set(records[j, :counts[j]].tolist())
+        assert counts[j] <= records.shape[1]
+        for g in got[j]:
+            rec = g[4] if g[0] == 0 else lt.n_limits + 2 * g[1] + (g[0] - 1)
+            assert rec in cand
+        total += len(ref)
+    assert total > 10
+    # a capacity of one record: scenarios overflow and are scanned in full, with the same answer
+    small = lt.detect(sa._ctx, capacity=1)
+    assert _by_scenario(small, len(ctg)) == got
+
+
+@pytest.mark.parametrize("name, count, bridges", [("grid2000", 48, 0.25), ("grid9241", 40, 0.1)])
+def test_violation_lists_match_the_oracle_n1(grid, name, count, bridges):
+    net = grid(name)
+    print(f"[viol] {name} N-1 vs oracle", flush=True)
+    st, base, table, ctg, sa = _setup(net, "BATCHED_LU_FP32_REFINED", count, bridges, 8)
+    lt = N.LimitTable(st, table)
+    r = sa.solve_batch_n1(base.x, base.target, base.row_mode, base.reactive_limit_types, ctg, limit_table=lt)
+    got = _by_scenario(r.violations, len(ctg))
+    o = po.OracleSystem(net)
+    ref = o.n1(base.x, base.row_mode, base.target, base.reactive_limit_types, ctg, threads=0)
+    assert np.array_equal(ref["status"], r.status)
+    total = ambiguous = 0
+    for j, e in enumerate(ctg):
+        if ref["status"][j] != 0:
+            continue
+        want = po.violations(o, ref["x"][j], table, outages=[e], islanded=o.islands([e]))
+        near = [q for q in want if abs(abs(q[5]) - q[6]) <= 1e-6 * abs(q[6])]
+        ambiguous += len(near)
+        assert [g[:5] for g in got[j]] == [q[:5] for q in want], (j, e, near)
+        if want:
+            np.testing.assert_allclose([g[5] for g in got[j]], [q[5] for q in want], rtol=1e-5)
+        total += len(want)
+    print(f"[viol] {name}: {total} violations over {len(ctg)} contingencies, {ambiguous} within 1e-6 of a limit",
+          flush=True)
+    assert total > 10

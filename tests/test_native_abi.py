@@ -116,3 +116,23 @@ def test_outage_neighbourhoods_match_the_oracle(grid):
     with pytest.raises(N.OlfHipError) as e:
         W.neighbourhoods(net.n_bus, net.bus1, net.bus2, net.x, net.n_bus + 1)
     assert e.value.code == N.E_ARGUMENT
+
+
+def test_limit_table_contract_violations_return_codes(grid):
+    """The limit table's arguments are checked before any device use."""
+    net = grid("grid118")
+    st = N.Structure.from_network(net)
+    x0 = net.system()[0]
+    good = gridgen.limit_table(net, x0)
+    for key, bad in (("group_side", 3), ("group_quantity", 7), ("branch", net.n_branch)):
+        t = {k: np.array(v, copy=True) for k, v in good.items()}
+        t[key][0] = bad
+        with pytest.raises(N.OlfHipError) as e:
+            N.LimitTable(st, t)
+        assert e.value.code == N.E_ARGUMENT, key
+    t = {k: np.array(v, copy=True) for k, v in good.items()}
+    t["bus_high"] = np.zeros(net.n_bus + 1)
+    t["bus_low"] = np.zeros(net.n_bus + 1)
+    with pytest.raises(N.OlfHipError) as e:
+        N.LimitTable(st, t)
+    assert e.value.code == N.E_ARGUMENT
