@@ -63,7 +63,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--contingencies", type=int, default=2000)
     ap.add_argument("--plain-sample", type=int, default=20)
-    ap.add_argument("--cpu-sample", type=int, default=300)
+    ap.add_argument("--cpu-sample", type=int, default=2000)
+    ap.add_argument("--cpu-sample-1t", type=int, default=60, help="contingencies for the one-thread CPU figure")
+    ap.add_argument("--no-violations", action="store_true", help="skip the violation-detection measurement")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--lanes-per-tile", type=int, default=0)
     ap.add_argument("--threads-per-tile", type=int, default=0)
@@ -76,6 +78,8 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU: the N-1 batch shards by contingency across GPUs "
                                    "(see the sharding design); only --gpus 1 is measured"}))
         sys.exit(2)
+    # the copy program runs first, as a child process, before this process touches the GPU
+    copy_gbs = measured_copy_bandwidth()
     if N.device_count() == 0:
         print(json.dumps({"error": "no GPU visible"}))
         sys.exit(1)
@@ -200,6 +204,26 @@ def main():
     plain_value = len(sample) / plain_s
     plain_match = bool(np.array_equal(np.array(plain_status), status[: len(sample)]))
 
+    # violation detection on the last step's states (reported beside the value, not inside it: the metric is the
+    # Newton solve, as for the plain FP64 arm)
+    viol_info = None
+    if not args.no_violations:
+        table = gridgen.limit_table(net, xb)
+        ltab = N.LimitTable(structure, table)
+        if mlp is not None:
+            inputs.x0_per_scenario_device = mlp.predict_warm_start(xb, ctg, feats)
+        ctx.solve(inputs, want_states=False)
+        v = ltab.detect(ctx)
+        device_synchronize()
+        tv = time.perf_counter()
+        for _ in range(3):
+            v = ltab.detect(ctx)
+        dv = (time.perf_counter() - tv) / 3
+        viol_info = {"violations": int(len(v)), "contingencies_with_violations": int(len(np.unique(v["scenario"]))),
+                     "by_kind": {N.VIOLATION_KINDS[k]: int((v["kind"] == k).sum()) for k in range(3)},
+                     "ms_per_step": round(dv * 1e3, 2), "monitored_groups": int(len(table["group_side"])),
+                     "screen_band": 1e-7}
+
     # roofline of the Newton-step kernel
     ab, per_iter = algorithmic_bytes(stats, net.n, N.LU_FP32_REFINED, lane_iters, K)
     step_ms_per_solve = step_ms / args.steps
@@ -211,10 +235,17 @@ def main():
                 "avg_launch_ms": round(step_ms_per_solve / max(launches_per_solve, 1), 4),
                 "algorithmic_bytes_per_launch": int(ab / max(launches_per_solve, 1)),
                 "bytes_per_lane_iteration": int(per_iter), "lane_iterations_per_step": lane_iters}
+    if copy_gbs:
+        roofline["measured_copy_gbs"] = copy_gbs
+        roofline["frac_of_measured_copy"] = round(achieved / copy_gbs, 4)
+    traffic = committed_traffic()
+    if traffic:
+        roofline["traffic"] = traffic["bytes_per_launch"]
+        roofline["traffic_source"] = traffic["source"]
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(net, xb, mb, tb, lt, ctg[: args.cpu_sample])
+        cpu = cpu_baseline(net, xb, mb, tb, lt, ctg[: args.cpu_sample], ctg[: args.cpu_sample_1t])
 
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "contingencies/s", "n_gpus": 1, "steps": args.steps,
@@ -231,6 +262,7 @@ def main():
         "plain_fp64_gpu": {"value": round(plain_value, 2), "unit": "contingencies/s", "sample": len(sample),
                            "statuses_match_batched": plain_match},
         "speedup_vs_plain_fp64_gpu": round(value / plain_value, 2),
+        "violation_detection": viol_info,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "setup_s": round(setup_s, 2),
@@ -243,8 +275,9 @@ def ctx_lanes(ctx):
             ctx._tuning.threads_per_tile or "default"}
 
 
-def cpu_baseline(net, xb, mb, tb, lt, sample):
-    """The FP64 CPU oracle (own sparse LU, OpenMP over contingencies) on a bounded sample, all host cores."""
+def cpu_baseline(net, xb, mb, tb, lt, sample, sample_1t):
+    """The FP64 CPU oracle (own sparse LU, OpenMP over contingencies) on a bounded sample, all host cores, and on a
+    smaller sample with one thread."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import pyoracle as po
 
@@ -252,9 +285,43 @@ def cpu_baseline(net, xb, mb, tb, lt, sample):
     t = time.perf_counter()
     r = osys.n1(xb, mb, tb, lt, sample)
     dt = time.perf_counter() - t
+    t = time.perf_counter()
+    osys.n1(xb, mb, tb, lt, sample_1t, threads=1)
+    d1 = time.perf_counter() - t
     return {"value": round(len(sample) / dt, 2), "unit": "contingencies/s", "cores": int(r["threads"]),
             "kind": "port", "sample": f"first {len(sample)} of the 2000 contingencies, same base state and rules, "
-                                       f"{dt:.1f} s wall on {int(r['threads'])} threads"}
+                                       f"{dt:.1f} s wall on {int(r['threads'])} threads",
+            "one_thread_value": round(len(sample_1t) / d1, 2),
+            "one_thread_sample": f"first {len(sample_1t)} contingencies, {d1:.1f} s on 1 thread"}
+
+
+def measured_copy_bandwidth():
+    """Best device-to-device copy bandwidth of the plain copy program built beside the library (2 x 1 GiB moved per
+    dispatch), GB/s; None when it is not built."""
+    import subprocess
+
+    exe = os.path.join(REPO, "olf-gpu_amd", "hbm_copy")
+    if not os.path.exists(exe):
+        return None
+    res = subprocess.run([exe, "1024", "5"], capture_output=True, text=True, timeout=120)
+    if res.returncode != 0:
+        return None
+    try:
+        return float(res.stdout.strip().split(",")[-1].split()[0])
+    except (ValueError, IndexError):
+        return None
+
+
+def committed_traffic():
+    """HBM bytes per Newton-step launch from the committed rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE of the
+    step kernel on this workload, scaled by the counters' reading of a copy of known size), if present."""
+    path = os.path.join(REPO, "profiles", "nrstep_grid9241_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    return {"bytes_per_launch": t["bytes_per_launch"], "source": "profiles/nrstep_grid9241_traffic.json (" +
+            t.get("command_summary", "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE") + ")"}
 
 
 if __name__ == "__main__":
