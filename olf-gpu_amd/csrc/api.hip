@@ -222,8 +222,9 @@ struct ContextImpl {
     int lastSteps = 0;
     double lastStepMs = 0.0;                 // event-timed Newton-step launches of the last solve
     hipEvent_t evStart = nullptr, evStop = nullptr;
-    DBuf prof;                               // opt-in section stamps of tile 0 (OLFHIP_PROFILE=1)
+    DBuf prof;                               // opt-in section stamps of one tile (OLFHIP_PROFILE=1)
     int profCap = 0;
+    int profTile = 0;                        // OLFHIP_PROFILE_TILE, default 0
     TopPlan top;                             // the dense top block of the step kernel
     std::vector<int> lastStatus;             // statuses of the last solve's scenarios
     std::vector<int> lastOutages;            // its outage lists, MAX_OUTAGES per lane (-1 padded)
@@ -254,6 +255,7 @@ struct ContextImpl {
         S.maxOut = MAX_OUTAGES;
         S.prof = prof.as<unsigned long long>();
         S.profCap = profCap;
+        S.profTile = profTile;
         S.rlType = rlType.as<signed char>();
         S.rlCount = rlCount.as<signed char>();
         S.rlPass = rlPass.as<signed char>();
@@ -429,6 +431,8 @@ static void solve(ContextImpl *c, const olfhip_solve_inputs *in, olfhip_solve_ou
     const bool profile = std::getenv("OLFHIP_PROFILE") && std::strcmp(std::getenv("OLFHIP_PROFILE"), "0") != 0;
     if (profile) {
         c->profCap = maxSteps;
+        const char *pt = std::getenv("OLFHIP_PROFILE_TILE");
+        c->profTile = pt ? std::max(0, std::min(c->nTiles - 1, std::atoi(pt))) : 0;
         const size_t bytes = (size_t)PROF_SECTIONS * (maxSteps + 1) * 8;
         c->prof.alloc(bytes);
         HIP_OK(hipMemsetAsync(c->prof.p, 0, bytes, s));   // before any capture: no memset node in the graph
@@ -481,7 +485,8 @@ static void solve(ContextImpl *c, const olfhip_solve_inputs *in, olfhip_solve_ou
             float ms = 0.0f;
             HIP_OK(hipEventElapsedTime(&ms, c->evStart, c->evStop));
             c->lastStepMs += ms;
-            if (traceOn()) std::fprintf(stderr, "OLFHIP step %d running tiles %d\n", steps, *c->pinnedCount);
+            if (traceOn())
+                std::fprintf(stderr, "OLFHIP step %d %.3f ms, tiles still running %d\n", steps, ms, *c->pinnedCount);
             if (*c->pinnedCount == 0) {
                 steps++;
                 break;
@@ -502,7 +507,7 @@ static void solve(ContextImpl *c, const olfhip_solve_inputs *in, olfhip_solve_ou
         const unsigned long long launches = std::min<unsigned long long>(h[0], (unsigned long long)maxSteps);
         for (unsigned long long k = 0; k < launches; k++) {
             const unsigned long long *r = h.data() + PROF_SECTIONS * (k + 1);
-            std::fprintf(stderr, "OLFHIP_PROFILE T=%d NT=%d launch %llu:", c->T, c->NT, k);
+            std::fprintf(stderr, "OLFHIP_PROFILE T=%d NT=%d tile %d launch %llu:", c->T, c->NT, c->profTile, k);
             unsigned long long prev = r[0];
             for (int i = 1; i <= 8; i++) {
                 if (r[i] == 0) continue;

@@ -69,6 +69,7 @@ struct DevState {
     // launch (wall clock) from prof[PROF_SECTIONS] on, at most profCap launches
     unsigned long long *prof;
     int profCap;
+    int profTile;   // the tile whose sections are stamped
 };
 constexpr int PROF_SECTIONS = 16;
 
@@ -824,7 +825,7 @@ __global__ __launch_bounds__(MAX_THREADS) void nrStepKernel(DevPlan P, DevState 
 
     if (tid == 0) {
         sProf = nullptr;
-        if (S.prof && tile == 0) {
+        if (S.prof && tile == S.profTile) {
             const unsigned long long k = atomicAdd(S.prof, 1ull);
             if (k < (unsigned long long)S.profCap) sProf = S.prof + PROF_SECTIONS * (k + 1);
         }
