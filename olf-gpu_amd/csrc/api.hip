@@ -99,8 +99,12 @@ struct StructureImpl {
         dp.itemRec = reinterpret_cast<const int4 *>(put(P.itemRec));
         dp.contribRec = reinterpret_cast<const int4 *>(put(P.contribRec));
         dp.bwdRec = reinterpret_cast<const int4 *>(put(P.bwdRec));
-        dp.levelSplit = put(P.levelSplit);
-        dp.levelSplitBwd = put(P.levelSplitBwd);
+        dp.bucketRec = reinterpret_cast<const int4 *>(put(P.bucketRec));
+        dp.bwdBucketRec = reinterpret_cast<const int4 *>(put(P.bwdBucketRec));
+        dp.levelBucketPtr = put(P.levelBucketPtr);
+        dp.levelBwdBucketPtr = put(P.levelBwdBucketPtr);
+        dp.levelNodes = put(P.levelNodes);
+        dp.diagItemOf = put(P.diagItemOf);
         dp.nodeBase = put(P.nodeBase);
         dp.sPtr = put(P.sPtr);
         dp.sIdx = put(P.sIdx);
@@ -131,7 +135,7 @@ struct StructureImpl {
 struct TopPlan {
     DevTop top{};
     size_t ldsBytes = 0;
-    DBuf slots;
+    DBuf slots, items, diagItems, buckets, diagBuckets;
 };
 
 static void planTop(const StructureImpl *st, const void *kernel, int T, int NT, size_t vtBytes, TopPlan &tp) {
@@ -155,7 +159,6 @@ static void planTop(const StructureImpl *st, const void *kernel, int T, int NT, 
     tp.top.l0 = n > 0 ? l0 : P.nLevels;
     tp.top.n = n;
     tp.top.start = n > 0 ? P.levelNodePtr[l0] : P.nb;
-    tp.top.S = n > 0 ? P.topSplit(l0) : 1;
     tp.ldsBytes = n > 0 ? topLdsBytes(T, n, vtBytes) : 0;
     if (n > 0) {
         std::vector<int> rec;
@@ -173,6 +176,19 @@ static void planTop(const StructureImpl *st, const void *kernel, int T, int NT, 
         tp.slots.upload(rec);
         tp.top.slots = tp.slots.as<int4>();
         tp.top.nSlots = (int)(rec.size() / 4);
+        std::vector<int> items, buckets;
+        P.topItems(l0, false, items, buckets);
+        tp.items.upload(items);
+        tp.buckets.upload(buckets);
+        tp.top.items = tp.items.as<int>();
+        tp.top.buckets = tp.buckets.as<int4>();
+        tp.top.nBuckets = (int)(buckets.size() / 4);
+        P.topItems(l0, true, items, buckets);
+        tp.diagItems.upload(items);
+        tp.diagBuckets.upload(buckets);
+        tp.top.diagItems = tp.diagItems.as<int>();
+        tp.top.diagBuckets = tp.diagBuckets.as<int4>();
+        tp.top.nDiagBuckets = (int)(buckets.size() / 4);
         if (tp.ldsBytes > 65536)
             HIP_OK(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)tp.ldsBytes));
     }
@@ -1138,6 +1154,21 @@ int olfhip_autotune(olfhip_structure *structure, int mode, const olfhip_reactive
         }
         *best = bestT;
         if (best_ms) *best_ms = bestMs;
+    });
+}
+
+int olfhip_structure_schedule(const olfhip_structure *structure, int which, int *count, int *values) {
+    return guarded([&] {
+        if (!structure || !count) throw ArgumentError("null argument");
+        const Plan &P = reinterpret_cast<const StructureImpl *>(structure)->plan;
+        const std::vector<int> *arrays[] = {&P.itemRec,     &P.bucketRec,    &P.levelBucketPtr, &P.bwdRec,
+                                            &P.bwdBucketRec, &P.levelBwdBucketPtr, &P.levelItemPtr,
+                                            &P.levelNodePtr, &P.diagItemOf,   &P.contribRec,   &P.levelNodes};
+        if (which < 0 || which >= (int)(sizeof(arrays) / sizeof(arrays[0])))
+            throw ArgumentError("which selects one of the 11 schedule arrays");
+        const std::vector<int> &v = *arrays[which];
+        *count = (int)v.size();
+        if (values && !v.empty()) std::memcpy(values, v.data(), v.size() * 4);
     });
 }
 

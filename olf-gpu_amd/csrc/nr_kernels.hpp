@@ -30,7 +30,9 @@ struct DevPlan {
     int nb, nSlots, nLevels;
     const int *levelNodePtr, *levelItemPtr;
     const int4 *itemRec, *contribRec, *bwdRec;
-    const int *levelSplit, *levelSplitBwd;
+    const int4 *bucketRec, *bwdBucketRec;       // buckets of one thread split (first, end, split, diagonal flag)
+    const int *levelBucketPtr, *levelBwdBucketPtr;
+    const int *levelNodes, *diagItemOf;
     const int *nodeBase, *sPtr, *sIdx;
     const int *hbPtr, *hbBranch;
     const double4 *hbCoef;
@@ -49,9 +51,13 @@ struct DevPlan {
 // top nodes' LU slots with their dense coordinates (slot, row, column, 0); S is the thread split of the contribution
 // sums from below the block. n = 0: no top block (l0 = nLevels).
 struct DevTop {
-    int l0, n, start, S;
+    int l0, n, start;
     const int4 *slots;
     int nSlots;
+    // the top items (all, or the diagonal ones) ordered by the split of their contributions from below, and buckets
+    const int *items, *diagItems;
+    const int4 *buckets, *diagBuckets;
+    int nBuckets, nDiagBuckets;
 };
 
 struct DevState {
@@ -322,11 +328,41 @@ __device__ __forceinline__ void groupSumBlk(Blk<VT> &b, int S) {
     b.d = groupSum<T>(b.d, S);
 }
 
-// Threads per item at level l: the plan's split, capped so that an item's threads stay in one wave.
+// Work units of a run of buckets laid end to end: unit u is lane t and thread s of the S threads sharing record rec.
+// The split is capped at 64 / T so that a record's threads stay in one wave; buckets come largest split first, so
+// every record's T * S units start at a multiple of T * S and its threads can reduce with lane shuffles.
+struct WorkUnit {
+    int rec, t, s, S;
+};
 template <int T>
-__device__ __forceinline__ int splitOf(const int *levelSplit, int l) {
-    const int s = levelSplit[l];
+__device__ __forceinline__ int capSplit(int s) {
     return s < 64 / T ? s : 64 / T;
+}
+template <int T>
+__device__ __forceinline__ int bucketUnits(const int4 *bk, int b0, int b1, bool diagOnly = false) {
+    int n = 0;
+    for (int b = b0; b < b1; b++) {
+        const int4 r = bk[b];
+        if (diagOnly && r.w != 1) continue;
+        n += (r.y - r.x) * T * capSplit<T>(r.z);
+    }
+    return n;
+}
+// unit u of the run (with diagOnly, of its diagonal-item buckets alone)
+template <int T>
+__device__ __forceinline__ WorkUnit unitOf(const int4 *bk, int b0, int b1, int u, bool diagOnly = false) {
+    for (int b = b0; b < b1; b++) {
+        const int4 r = bk[b];
+        if (diagOnly && r.w != 1) continue;
+        const int S = capSplit<T>(r.z);
+        const int n = (r.y - r.x) * T * S;
+        if (u < n) {
+            const int sh = __builtin_ctz(T * S);
+            return WorkUnit{r.x + (u >> sh), u % T, (u / T) & (S - 1), S};
+        }
+        u -= n;
+    }
+    return WorkUnit{-1, 0, 0, 1};
 }
 
 // The factorization, left-looking by elimination levels, one phase per level, with the forward solve of
@@ -340,13 +376,13 @@ __device__ void factorForward(const DevPlan &P, int tile, int tid, int nt, VT *_
                               const int *laneOn) {
     const int nb = P.nb;
     for (int l = 0; l < l0; l++) {
-        const int i0 = P.levelItemPtr[l], i1 = P.levelItemPtr[l + 1];
-        const int S = splitOf<T>(P.levelSplit, l);
-        for (int w = tid; w < (i1 - i0) * T * S; w += nt) {
-            const int t = w % T;
+        const int b0 = P.levelBucketPtr[l], b1 = P.levelBucketPtr[l + 1];
+        const int units = bucketUnits<T>(P.bucketRec, b0, b1);
+        for (int w = tid; w < units; w += nt) {
+            const WorkUnit un = unitOf<T>(P.bucketRec, b0, b1, w);
+            const int t = un.t, s = un.s, S = un.S;
             if (laneOn && !laneOn[t]) continue;
-            const int s = (w / T) % S;
-            const int4 it = P.itemRec[i0 + w / (T * S)];
+            const int4 it = P.itemRec[un.rec];
             VT *own = lu + bIdx<T>(tile, P.nSlots, it.x, t);
             if (it.w >= 0) {
                 // diagonal item of node it.w: pivot block, its inverse, and the forward-solve entry
@@ -418,14 +454,13 @@ __device__ void forwardOnly(const DevPlan &P, int tile, int tid, int nt, const V
                             const int *laneOn) {
     const int nb = P.nb;
     for (int l = 0; l < l0; l++) {
-        const int i0 = P.levelItemPtr[l];
-        const int nNodes = P.levelNodePtr[l + 1] - P.levelNodePtr[l];   // diagonal items come first
-        const int S = splitOf<T>(P.levelSplit, l);
-        for (int w = tid; w < nNodes * T * S; w += nt) {
-            const int t = w % T;
+        const int b0 = P.levelBucketPtr[l], b1 = P.levelBucketPtr[l + 1];
+        const int units = bucketUnits<T>(P.bucketRec, b0, b1, true);
+        for (int w = tid; w < units; w += nt) {
+            const WorkUnit un = unitOf<T>(P.bucketRec, b0, b1, w, true);
+            const int t = un.t, s = un.s, S = un.S;
             if (laneOn && !laneOn[t]) continue;
-            const int s = (w / T) % S;
-            const int4 it = P.itemRec[i0 + w / (T * S)];
+            const int4 it = P.itemRec[un.rec];
             V2<VT> yk = {(VT)0, (VT)0};
 #pragma unroll 4
             for (int c = it.y + s; c < it.z; c += S) {
@@ -456,13 +491,13 @@ __device__ void backward(const DevPlan &P, int tile, int tid, int nt, const VT *
                          const VT *__restrict__ z, VT *__restrict__ xo, int l0, const int *laneOn) {
     const int nb = P.nb;
     for (int l = l0 - 1; l >= 0; l--) {
-        const int n0 = P.levelNodePtr[l], n1 = P.levelNodePtr[l + 1];
-        const int S = splitOf<T>(P.levelSplitBwd, l);
-        for (int w = tid; w < (n1 - n0) * T * S; w += nt) {
-            const int t = w % T;
+        const int b0 = P.levelBwdBucketPtr[l], b1 = P.levelBwdBucketPtr[l + 1];
+        const int units = bucketUnits<T>(P.bwdBucketRec, b0, b1);
+        for (int w = tid; w < units; w += nt) {
+            const WorkUnit un = unitOf<T>(P.bwdBucketRec, b0, b1, w);
+            const int t = un.t, s = un.s, S = un.S;
             if (laneOn && !laneOn[t]) continue;
-            const int s = (w / T) % S;
-            const int4 br = P.bwdRec[n0 + w / (T * S)];
+            const int4 br = P.bwdRec[un.rec];
             V2<VT> acc = {(VT)0, (VT)0};
 #pragma unroll 4
             for (int e = br.y + s; e < br.z; e += S) {
@@ -514,25 +549,16 @@ template <int T, typename VT, bool FACTOR>
 __device__ void topFromBelow(const DevPlan &P, const DevTop &top, const TopLds<T, VT> &L, int tile, int tid, int nt,
                              const VT *__restrict__ lu, const VT *__restrict__ z, const int *laneOn) {
     const int nb = P.nb;
-    const int i0 = P.levelItemPtr[top.l0];
-    const int nItems = FACTOR ? P.levelItemPtr[P.nLevels] - i0 : 0;
-    const int S = top.S < 64 / T ? top.S : 64 / T;
-    // without FACTOR only the diagonal items (one per top node, first in each level) matter: walk the nodes
-    const int units = FACTOR ? nItems : top.n;
-    for (int w = tid; w < units * T * S; w += nt) {
-        const int t = w % T;
+    // without FACTOR only the diagonal items (one per top node) matter
+    const int4 *bk = FACTOR ? top.buckets : top.diagBuckets;
+    const int *list = FACTOR ? top.items : top.diagItems;
+    const int nbk = FACTOR ? top.nBuckets : top.nDiagBuckets;
+    const int units = bucketUnits<T>(bk, 0, nbk);
+    for (int w = tid; w < units; w += nt) {
+        const WorkUnit un = unitOf<T>(bk, 0, nbk, w);
+        const int t = un.t, s = un.s, S = un.S;
         if (laneOn && !laneOn[t]) continue;
-        const int s = (w / T) % S;
-        int item;
-        if (FACTOR) {
-            item = i0 + w / (T * S);
-        } else {
-            const int a = w / (T * S);
-            const int k = P.bwdRec[top.start + a].w;
-            // the diagonal item of node k: levels list their diagonal items first, in node order
-            const int l = P.nodeLevel[k];
-            item = P.levelItemPtr[l] + (top.start + a - P.levelNodePtr[l]);
-        }
+        const int item = list[un.rec];
         const int4 it = P.itemRec[item];
         const int2 ki = P.itemKI[item];
         const int a = P.levelPos[ki.x] - top.start;
@@ -616,7 +642,7 @@ __device__ void topFactor(const DevPlan &P, const DevTop &top, const TopLds<T, V
             stBlk<VT>(L.M(sr.y, sr.z, t), ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, sr.x, t)));
         } else {
             const int a = q - top.nSlots;
-            const int k = P.bwdRec[top.start + a].w;
+            const int k = P.levelNodes[top.start + a];
             const double2 b = *reinterpret_cast<const double2 *>(rhs + vIdx<T>(tile, nb, k, t));
             stBlk<VT>(L.M(a, n, t), Blk<VT>{(VT)(sign * b.x), (VT)0, (VT)(sign * b.y), (VT)0});
         }
@@ -666,7 +692,7 @@ __device__ void topForward(const DevPlan &P, const DevTop &top, const TopLds<T, 
     for (int w = tid; w < n * T; w += nt) {
         const int t = w % T, a = w / T;
         if (laneOn && !laneOn[t]) continue;
-        const int k = P.bwdRec[top.start + a].w;
+        const int k = P.levelNodes[top.start + a];
         const double2 b = *reinterpret_cast<const double2 *>(rhs + vIdx<T>(tile, nb, k, t));
         stBlk<VT>(L.M(a, n, t), Blk<VT>{(VT)(sign * b.x), (VT)0, (VT)(sign * b.y), (VT)0});
     }
@@ -711,7 +737,7 @@ __device__ void topBackward(const DevPlan &P, const DevTop &top, const TopLds<T,
             const V2<VT> ac = ldV<VT>(L.Acc(a, t));
             const V2<VT> xa = {za.p - (Di.a * ac.p + Di.b * ac.q), za.q - (Di.c * ac.p + Di.d * ac.q)};
             if (b == a) {
-                const int k = P.bwdRec[top.start + a].w;
+                const int k = P.levelNodes[top.start + a];
                 stV<VT>(xo + vIdx<T>(tile, nb, k, t), xa);
             } else {
                 const Blk<VT> U = ldBlk<VT>(L.M(b, a, t));

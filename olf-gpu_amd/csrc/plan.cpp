@@ -52,6 +52,13 @@ void minimumDegree(int nb, const std::vector<int> &adjPtr, const std::vector<int
     }
 }
 
+// Threads that share one contribution list: about two contributions per thread, a power of two, at most 64.
+int splitFor(int length) {
+    int s = 1;
+    while (s < 64 && 2 * s < length) s *= 2;
+    return s;
+}
+
 static int indexIn(const std::vector<int> &idx, int lo, int hi, int value) {
     auto it = std::lower_bound(idx.begin() + lo, idx.begin() + hi, value);
     if (it == idx.begin() + hi || *it != value) return -1;
@@ -170,11 +177,16 @@ Plan buildPlan(int n, int nBranch, const double *p8, const int *idx12, int nShun
             }
         }
     }
-    // level-ordered packed records: a level's diagonal items first (one per node), then its pair items
+    // level-ordered packed records: a level's items (one diagonal item per node, then its pair items) are grouped
+    // into buckets of one thread split and one kind, largest split first and diagonal before pair on equal splits
+    // (so that every group of threads sharing an item stays aligned inside one wave when buckets are laid end to
+    // end, and so is any subsequence of them, such as the diagonal buckets alone)
     std::vector<std::vector<int>> byLevel(P.nLevels);
     for (int k = 0; k < nb; k++) byLevel[level[k]].push_back(k);
     P.levelNodePtr.assign(P.nLevels + 1, 0);
     P.levelItemPtr.assign(P.nLevels + 1, 0);
+    P.levelBucketPtr.assign(P.nLevels + 1, 0);
+    P.levelBwdBucketPtr.assign(P.nLevels + 1, 0);
     auto pushContrib = [&](int a, int b, int d, int j) {
         P.contribRec.push_back(a);
         P.contribRec.push_back(b);
@@ -183,8 +195,34 @@ Plan buildPlan(int n, int nBranch, const double *p8, const int *idx12, int nShun
     };
     P.nodeLevel = level;
     P.levelPos.assign(nb, 0);
-    int nItems = 0, nContrib = 0;
+    P.diagItemOf.assign(nb, 0);
+    struct Item { int slot, c0, c1, node, k, i; };
+    auto splitOf = [](const Item &u) { return splitFor(u.c1 - u.c0); };
+    auto emit = [&](std::vector<Item> &items) {
+        std::stable_sort(items.begin(), items.end(), [&](const Item &u, const Item &v) {
+            const int su = splitOf(u), sv = splitOf(v);
+            if (su != sv) return su > sv;
+            return (u.node >= 0) > (v.node >= 0);
+        });
+        for (size_t q = 0; q < items.size(); q++) {
+            const Item &it = items[q];
+            const int idx = (int)(P.itemRec.size() / 4);
+            const int diag = it.node >= 0 ? 1 : 0;
+            if (q == 0 || splitOf(it) != splitOf(items[q - 1]) || diag != (items[q - 1].node >= 0 ? 1 : 0))
+                P.bucketRec.insert(P.bucketRec.end(), {idx, idx, splitOf(it), diag});
+            P.bucketRec[P.bucketRec.size() - 3]++;   // the open bucket's end
+            P.itemRec.insert(P.itemRec.end(), {it.slot, it.c0, it.c1, it.node});
+            P.itemKI.insert(P.itemKI.end(), {it.k, it.i});
+            if (diag) P.diagItemOf[it.k] = idx;
+        }
+    };
+    int nContrib = 0;
+    std::vector<Item> diagItems, pairItems;
+    std::vector<std::pair<int, int>> bwd;   // (split, node)
     for (int l = 0; l < P.nLevels; l++) {
+        diagItems.clear();
+        pairItems.clear();
+        bwd.clear();
         for (int k : byLevel[l]) {
             P.levelPos[k] = (int)P.levelNodes.size();
             P.levelNodes.push_back(k);
@@ -193,10 +231,8 @@ Plan buildPlan(int n, int nBranch, const double *p8, const int *idx12, int nShun
                 pushContrib(c.first, c.second, P.nodeBase[c.second], c.second);
                 nContrib++;
             }
-            P.itemRec.insert(P.itemRec.end(), {P.nodeBase[k], c0, nContrib, k});
-            P.itemKI.insert(P.itemKI.end(), {k, k});
-            nItems++;
-            P.bwdRec.insert(P.bwdRec.end(), {P.nodeBase[k], P.sPtr[k], P.sPtr[k + 1], k});
+            diagItems.push_back({P.nodeBase[k], c0, nContrib, k, k, k});
+            bwd.push_back({splitFor(P.sPtr[k + 1] - P.sPtr[k]), k});
             P.contributions += (long long)diagC[k].size();
             P.factorMads += 2LL * (long long)diagC[k].size() + 1;   // W = D^-1 U, D -= L W; the inverse
         }
@@ -212,33 +248,29 @@ Plan buildPlan(int n, int nBranch, const double *p8, const int *idx12, int nShun
                     pushContrib(c.first, c.second, P.nodeBase[j], j);
                     nContrib++;
                 }
-                P.itemRec.insert(P.itemRec.end(), {lslot(k, e), c0, nContrib, -1});
-                P.itemKI.insert(P.itemKI.end(), {k, P.sIdx[P.sPtr[k] + e]});
-                nItems++;
+                pairItems.push_back({lslot(k, e), c0, nContrib, -1, k, P.sIdx[P.sPtr[k] + e]});
                 P.contributions += (long long)lst.size();
                 P.factorMads += 4LL * (long long)lst.size();   // two W = D^-1 U and two updates
             }
         }
+        diagItems.insert(diagItems.end(), pairItems.begin(), pairItems.end());
+        emit(diagItems);
+        // backward records of the level's nodes, grouped by split, largest first
+        std::stable_sort(bwd.begin(), bwd.end(), [](const std::pair<int, int> &u, const std::pair<int, int> &v) {
+            return u.first > v.first;
+        });
+        for (size_t q = 0; q < bwd.size(); q++) {
+            const int k = bwd[q].second;
+            const int idx = (int)(P.bwdRec.size() / 4);
+            if (q == 0 || bwd[q].first != bwd[q - 1].first)
+                P.bwdBucketRec.insert(P.bwdBucketRec.end(), {idx, idx, bwd[q].first, 0});
+            P.bwdBucketRec[P.bwdBucketRec.size() - 3]++;
+            P.bwdRec.insert(P.bwdRec.end(), {P.nodeBase[k], P.sPtr[k], P.sPtr[k + 1], k});
+        }
         P.levelNodePtr[l + 1] = (int)P.levelNodes.size();
-        P.levelItemPtr[l + 1] = nItems;
-    }
-    // split factors: about two contributions per thread on the level's longest list, at most 64 threads
-    auto splitFor = [](int longest) {
-        int s = 1;
-        while (s < 64 && 2 * s < longest) s *= 2;
-        return s;
-    };
-    P.levelSplit.assign(P.nLevels, 1);
-    P.levelSplitBwd.assign(P.nLevels, 1);
-    for (int l = 0; l < P.nLevels; l++) {
-        int longest = 0;
-        for (int i = P.levelItemPtr[l]; i < P.levelItemPtr[l + 1]; i++)
-            longest = std::max(longest, P.itemRec[4 * i + 2] - P.itemRec[4 * i + 1]);
-        P.levelSplit[l] = splitFor(longest);
-        int widest = 0;
-        for (int q = P.levelNodePtr[l]; q < P.levelNodePtr[l + 1]; q++)
-            widest = std::max(widest, P.bwdRec[4 * q + 2] - P.bwdRec[4 * q + 1]);
-        P.levelSplitBwd[l] = splitFor(widest);
+        P.levelItemPtr[l + 1] = (int)(P.itemRec.size() / 4);
+        P.levelBucketPtr[l + 1] = (int)(P.bucketRec.size() / 4);
+        P.levelBwdBucketPtr[l + 1] = (int)(P.bwdBucketRec.size() / 4);
     }
 
     // --- per-branch flow coefficients (the same half-branch forms as the fill)
@@ -359,16 +391,24 @@ int Plan::topLevel(int maxNodes, int *nodes) const {
     return l0;
 }
 
-int Plan::topSplit(int l0) const {
-    int longest = 0;
+void Plan::topItems(int l0, bool diagOnly, std::vector<int> &items, std::vector<int> &buckets) const {
+    std::vector<std::pair<int, int>> v;   // (split of the contributions from below l0, item)
     for (int i = levelItemPtr[l0]; i < levelItemPtr[nLevels]; i++) {
+        if (diagOnly && itemRec[4 * i + 3] < 0) continue;
         int c = 0;
         for (int q = itemRec[4 * i + 1]; q < itemRec[4 * i + 2]; q++) c += contribRec[4 * q + 3] < l0;
-        longest = std::max(longest, c);
+        v.push_back({splitFor(c), i});
     }
-    int s = 1;
-    while (s < 64 && 2 * s < longest) s *= 2;
-    return s;
+    std::stable_sort(v.begin(), v.end(),
+                     [](const std::pair<int, int> &a, const std::pair<int, int> &b) { return a.first > b.first; });
+    items.clear();
+    buckets.clear();
+    for (size_t q = 0; q < v.size(); q++) {
+        if (q == 0 || v[q].first != v[q - 1].first)
+            buckets.insert(buckets.end(), {(int)q, (int)q, v[q].first, 0});
+        buckets[buckets.size() - 3]++;
+        items.push_back(v[q].second);
+    }
 }
 
 size_t Plan::bytesPerScenario(int luValueBytes) const {

@@ -51,10 +51,15 @@ struct Plan {
                                                    // of (j,i), diag slot of j, level of j (U(j,x) sits right after
                                                    // L(x,j))
     std::vector<int> bwdRec;                       // 4 per node in level order: diag slot, S begin, S end, node
-    // Threads that share one item's contribution list (a power of two, from the level's longest list): long lists
-    // at the top of the tree are summed by several threads of a wave and reduced across lanes. The kernels cap it
-    // at 64 / T so that an item's threads stay in one wave.
-    std::vector<int> levelSplit, levelSplitBwd;
+    std::vector<int> diagItemOf;                   // diagonal item of each node
+    // Threads that share one item's contribution list (splitFor of its length): long lists at the top of the tree
+    // are summed by several threads of a wave and reduced across lanes; the kernels cap the split at 64 / T so that
+    // an item's threads stay in one wave. Within a level the items are grouped in buckets of one split and one
+    // kind, largest split first (diagonal before pair on equal splits): 4 ints per bucket (first item, end, split,
+    // 1 for diagonal items), the level's buckets in [levelBucketPtr[l], levelBucketPtr[l+1]). The backward records
+    // likewise (split of |S(k)|).
+    std::vector<int> bucketRec, levelBucketPtr;
+    std::vector<int> bwdBucketRec, levelBwdBucketPtr;
 
     // --- fill gather tables, per node, half-branches grouped by the far node (ascending)
     std::vector<int> hbPtr;                        // half-branches of node k
@@ -85,9 +90,14 @@ struct Plan {
     // maxNodes nodes (an upper set of the elimination tree, so every member's S set stays inside it). Returns l0
     // (nLevels when even the root level does not fit) and the node count in *nodes.
     int topLevel(int maxNodes, int *nodes) const;
-    // Threads per top-block item for the contributions from below the top block (the longest such list).
-    int topSplit(int l0) const;
+    // The items of the top levels l0.. (diagonal items only with diagOnly) ordered by the split of their
+    // contributions from below l0, largest first, and their buckets (4 ints: first, end, split, 0 over `items`).
+    void topItems(int l0, bool diagOnly, std::vector<int> &items, std::vector<int> &buckets) const;
 };
+
+// Threads that share one contribution list of the given length: a power of two, about two contributions each,
+// at most 64.
+int splitFor(int length);
 
 // Builds the plan from the reference's closed-branch pack (8 doubles, 12 ints per branch) and optional shunts.
 // Throws std::invalid_argument on inputs this path does not model.

@@ -24,7 +24,7 @@ EXPORTS = [
     "olfhip_structure_statistics", "olfhip_structure_order", "olfhip_recommend_batch_size",
     "olfhip_create_batch_context", "olfhip_destroy_batch_context", "olfhip_solve_batch_context",
     "olfhip_read_states", "olfhip_islanded_buses", "olfhip_lu_factorize_and_solve", "olfhip_structure_pattern",
-    "olfhip_autotune", "olfhip_context_timing", "olfhip_read_outer_loop_state",
+    "olfhip_autotune", "olfhip_context_timing", "olfhip_read_outer_loop_state", "olfhip_structure_schedule",
     "olfhip_neighbourhoods", "olfhip_mlp_create", "olfhip_mlp_destroy", "olfhip_mlp_set_normalization",
     "olfhip_mlp_set_base_states", "olfhip_mlp_train_step", "olfhip_mlp_forward", "olfhip_mlp_gradients",
     "olfhip_mlp_parameters", "olfhip_mlp_predict_warm_start", "olfhip_mlp_gemm_test",
@@ -99,6 +99,7 @@ def lib():
         L.olfhip_structure_statistics.argtypes = [vp, vp]
         L.olfhip_structure_order.argtypes = [vp, vp]
         L.olfhip_structure_pattern.argtypes = [vp, C.POINTER(i), vp, vp]
+        L.olfhip_structure_schedule.argtypes = [vp, i, C.POINTER(i), vp]
         L.olfhip_recommend_batch_size.argtypes = [vp, i, i, C.POINTER(i)]
         L.olfhip_create_batch_context.argtypes = [vp, i, i, C.POINTER(ReactiveLimits), C.POINTER(Tuning),
                                                   C.POINTER(vp)]
@@ -182,6 +183,20 @@ class Structure:
         ci = np.zeros(nnz.value, np.int32)
         check(lib().olfhip_structure_pattern(self.h, C.byref(nnz), ptr(rp), ptr(ci)))
         return rp, ci
+
+    SCHEDULE = ("item", "bucket", "level_bucket_ptr", "bwd", "bwd_bucket", "level_bwd_bucket_ptr", "level_item_ptr",
+                "level_node_ptr", "diag_item_of", "contrib", "level_nodes")
+
+    def schedule(self):
+        """The LU level schedule (olfhip_structure_schedule), records reshaped to [.., 4]."""
+        out = {}
+        for which, name in enumerate(self.SCHEDULE):
+            cnt = C.c_int()
+            check(lib().olfhip_structure_schedule(self.h, which, C.byref(cnt), None))
+            v = np.zeros(cnt.value, np.int32)
+            check(lib().olfhip_structure_schedule(self.h, which, C.byref(cnt), ptr(v)))
+            out[name] = v.reshape(-1, 4) if name in ("item", "bucket", "bwd", "bwd_bucket", "contrib") else v
+        return out
 
     def islanded_buses(self, outaged, slack_bus=0) -> np.ndarray:
         o = np.ascontiguousarray(np.asarray(outaged, dtype=np.int32).reshape(-1))
