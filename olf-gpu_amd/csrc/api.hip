@@ -105,6 +105,8 @@ struct StructureImpl {
         dp.levelBwdBucketPtr = put(P.levelBwdBucketPtr);
         dp.levelNodes = put(P.levelNodes);
         dp.diagItemOf = put(P.diagItemOf);
+        dp.nBuckets = (int)(P.bucketRec.size() / 4);
+        dp.nBwdBuckets = (int)(P.bwdBucketRec.size() / 4);
         dp.nodeBase = put(P.nodeBase);
         dp.sPtr = put(P.sPtr);
         dp.sIdx = put(P.sIdx);
@@ -148,7 +150,8 @@ static void planTop(const StructureImpl *st, const void *kernel, int T, int NT, 
     const int vgpr = std::max(8, (fa.numRegs + 7) / 8 * 8);
     const int wavesPerSimd = std::max(1, std::min(8, 512 / vgpr));
     const int blocksPerCU = std::max(1, (4 * wavesPerSimd) / std::max(1, NT / 64));
-    long budget = std::min<long>(perBlock, perCU / blocksPerCU) - (long)fa.sharedSizeBytes - 256;
+    const size_t metaBytes = levelMetaBytes(P.nLevels, (int)(P.bucketRec.size() / 4), (int)(P.bwdBucketRec.size() / 4));
+    long budget = std::min<long>(perBlock, perCU / blocksPerCU) - (long)fa.sharedSizeBytes - (long)metaBytes - 256;
     int maxNodes = 0;
     while (budget > 0 && topLdsBytes(T, maxNodes + 1, vtBytes) <= (size_t)budget) maxNodes++;
     if (const char *e = std::getenv("OLFHIP_TOP_NODES")) maxNodes = std::min(maxNodes, std::max(0, std::atoi(e)));
@@ -159,7 +162,8 @@ static void planTop(const StructureImpl *st, const void *kernel, int T, int NT, 
     tp.top.l0 = n > 0 ? l0 : P.nLevels;
     tp.top.n = n;
     tp.top.start = n > 0 ? P.levelNodePtr[l0] : P.nb;
-    tp.ldsBytes = n > 0 ? topLdsBytes(T, n, vtBytes) : 0;
+    tp.top.metaBytes = (int)metaBytes;
+    tp.ldsBytes = metaBytes + (n > 0 ? topLdsBytes(T, n, vtBytes) : 0);
     if (n > 0) {
         std::vector<int> rec;
         for (int a = 0; a < n; a++) {

@@ -33,6 +33,7 @@ struct DevPlan {
     const int4 *bucketRec, *bwdBucketRec;       // buckets of one thread split (first, end, split, diagonal flag)
     const int *levelBucketPtr, *levelBwdBucketPtr;
     const int *levelNodes, *diagItemOf;
+    int nBuckets, nBwdBuckets;
     const int *nodeBase, *sPtr, *sIdx;
     const int *hbPtr, *hbBranch;
     const double4 *hbCoef;
@@ -58,6 +59,7 @@ struct DevTop {
     const int *items, *diagItems;
     const int4 *buckets, *diagBuckets;
     int nBuckets, nDiagBuckets;
+    int metaBytes;   // LDS before the top block: the level schedule's bucket records
 };
 
 struct DevState {
@@ -365,27 +367,59 @@ __device__ __forceinline__ WorkUnit unitOf(const int4 *bk, int b0, int b1, int u
     return WorkUnit{-1, 0, 0, 1};
 }
 
+// The level schedule's bucket records, copied to LDS once per launch so that each level's bucket lookups cost LDS
+// reads instead of a chain of dependent global loads.
+struct LevelMeta {
+    const int4 *bucketRec, *bwdBucketRec;
+    const int *levelBucketPtr, *levelBwdBucketPtr;
+};
+__host__ __device__ constexpr size_t levelMetaBytes(int nLevels, int nBuckets, int nBwdBuckets) {
+    return ((size_t)16 * (nBuckets + nBwdBuckets) + (size_t)8 * (nLevels + 1) + 15) / 16 * 16;
+}
+// Copies the records into the LDS at base (levelMetaBytes of it). Ends with a barrier.
+__device__ __forceinline__ LevelMeta loadLevelMeta(const DevPlan &P, unsigned char *base, int tid, int nt) {
+    int4 *b = reinterpret_cast<int4 *>(base);
+    int4 *bb = b + P.nBuckets;
+    int *lp = reinterpret_cast<int *>(bb + P.nBwdBuckets);
+    int *lbp = lp + P.nLevels + 1;
+    for (int i = tid; i < P.nBuckets; i += nt) b[i] = P.bucketRec[i];
+    for (int i = tid; i < P.nBwdBuckets; i += nt) bb[i] = P.bwdBucketRec[i];
+    for (int i = tid; i <= P.nLevels; i += nt) {
+        lp[i] = P.levelBucketPtr[i];
+        lbp[i] = P.levelBwdBucketPtr[i];
+    }
+    __syncthreads();
+    return LevelMeta{b, bb, lp, lbp};
+}
+
 // The factorization, left-looking by elimination levels, one phase per level, with the forward solve of
 // b = sign * rhs fused into the diagonal items. Stores D(k)^-1 in the diagonal slot, the unscaled L~(i,k) and U(k,i)
 // in the pair slots, and z(k) = D(k)^-1 y(k) (y = L^-1 b) in z. An item's contribution list is summed by S threads
 // (stride S) and reduced across lanes. Counts clamped pivots per lane into clampLane (shared, may be null). Ends with
 // a barrier.
 template <int T, typename VT>
-__device__ void factorForward(const DevPlan &P, int tile, int tid, int nt, VT *__restrict__ lu, VT *__restrict__ z,
-                              const double *__restrict__ rhs, double sign, int *clampLane, int l0,
+__device__ void factorForward(const DevPlan &P, const LevelMeta &M, int tile, int tid, int nt, VT *__restrict__ lu,
+                              VT *__restrict__ z, const double *__restrict__ rhs, double sign, int *clampLane, int l0,
                               const int *laneOn) {
     const int nb = P.nb;
     for (int l = 0; l < l0; l++) {
-        const int b0 = P.levelBucketPtr[l], b1 = P.levelBucketPtr[l + 1];
-        const int units = bucketUnits<T>(P.bucketRec, b0, b1);
+        const int b0 = M.levelBucketPtr[l], b1 = M.levelBucketPtr[l + 1];
+        const int units = bucketUnits<T>(M.bucketRec, b0, b1);
         for (int w = tid; w < units; w += nt) {
-            const WorkUnit un = unitOf<T>(P.bucketRec, b0, b1, w);
+            const WorkUnit un = unitOf<T>(M.bucketRec, b0, b1, w);
             const int t = un.t, s = un.s, S = un.S;
             if (laneOn && !laneOn[t]) continue;
             const int4 it = P.itemRec[un.rec];
             VT *own = lu + bIdx<T>(tile, P.nSlots, it.x, t);
             if (it.w >= 0) {
-                // diagonal item of node it.w: pivot block, its inverse, and the forward-solve entry
+                // diagonal item of node it.w: pivot block, its inverse, and the forward-solve entry; the own block
+                // and right-hand side are loaded first so that they overlap the contribution loads
+                Blk<VT> A = {(VT)0, (VT)0, (VT)0, (VT)0};
+                double2 b = make_double2(0.0, 0.0);
+                if (s == 0) {
+                    A = ldBlk<VT>(own);
+                    b = *reinterpret_cast<const double2 *>(rhs + vIdx<T>(tile, nb, it.w, t));
+                }
                 Blk<VT> D = {(VT)0, (VT)0, (VT)0, (VT)0};
                 V2<VT> yk = {(VT)0, (VT)0};
 #pragma unroll 2
@@ -404,9 +438,7 @@ __device__ void factorForward(const DevPlan &P, int tile, int tid, int nt, VT *_
                     yk.q = groupSum<T>(yk.q, S);
                 }
                 if (s == 0) {
-                    const Blk<VT> A = ldBlk<VT>(own);
                     const Blk<VT> Dk = {A.a + D.a, A.b + D.b, A.c + D.c, A.d + D.d};
-                    const double2 b = *reinterpret_cast<const double2 *>(rhs + vIdx<T>(tile, nb, it.w, t));
                     const V2<VT> y = {(VT)(sign * b.x) + yk.p, (VT)(sign * b.y) + yk.q};
                     bool cl;
                     const Blk<VT> Dinv = inv2(Dk, cl);
@@ -416,7 +448,12 @@ __device__ void factorForward(const DevPlan &P, int tile, int tid, int nt, VT *_
                     stV<VT>(z + vIdx<T>(tile, nb, it.w, t), zk);
                 }
             } else {
-                // pair item: L~(i,k) at it.x, U(k,i) right after
+                // pair item: L~(i,k) at it.x, U(k,i) right after (own blocks loaded first)
+                Blk<VT> A = {(VT)0, (VT)0, (VT)0, (VT)0}, B = {(VT)0, (VT)0, (VT)0, (VT)0};
+                if (s == 0) {
+                    A = ldBlk<VT>(own);
+                    B = ldBlk<VT>(own + 4 * T);
+                }
                 Blk<VT> L = {(VT)0, (VT)0, (VT)0, (VT)0};
                 Blk<VT> U = {(VT)0, (VT)0, (VT)0, (VT)0};
 #pragma unroll 2
@@ -435,8 +472,6 @@ __device__ void factorForward(const DevPlan &P, int tile, int tid, int nt, VT *_
                     groupSumBlk<T>(U, S);
                 }
                 if (s == 0) {
-                    const Blk<VT> A = ldBlk<VT>(own);
-                    const Blk<VT> B = ldBlk<VT>(own + 4 * T);
                     stBlk<VT>(own, Blk<VT>{A.a + L.a, A.b + L.b, A.c + L.c, A.d + L.d});
                     stBlk<VT>(own + 4 * T, Blk<VT>{B.a + U.a, B.b + U.b, B.c + U.c, B.d + U.d});
                 }
@@ -449,18 +484,24 @@ __device__ void factorForward(const DevPlan &P, int tile, int tid, int nt, VT *_
 // Forward solve with existing factors: z(k) = D(k)^-1 (b(k) - sum L~(k,j) z(j)), b = sign * rhs. Ends with a
 // barrier.
 template <int T, typename VT>
-__device__ void forwardOnly(const DevPlan &P, int tile, int tid, int nt, const VT *__restrict__ lu,
-                            VT *__restrict__ z, const double *__restrict__ rhs, double sign, int l0,
-                            const int *laneOn) {
+__device__ void forwardOnly(const DevPlan &P, const LevelMeta &M, int tile, int tid, int nt,
+                            const VT *__restrict__ lu, VT *__restrict__ z, const double *__restrict__ rhs, double sign,
+                            int l0, const int *laneOn) {
     const int nb = P.nb;
     for (int l = 0; l < l0; l++) {
-        const int b0 = P.levelBucketPtr[l], b1 = P.levelBucketPtr[l + 1];
-        const int units = bucketUnits<T>(P.bucketRec, b0, b1, true);
+        const int b0 = M.levelBucketPtr[l], b1 = M.levelBucketPtr[l + 1];
+        const int units = bucketUnits<T>(M.bucketRec, b0, b1, true);
         for (int w = tid; w < units; w += nt) {
-            const WorkUnit un = unitOf<T>(P.bucketRec, b0, b1, w, true);
+            const WorkUnit un = unitOf<T>(M.bucketRec, b0, b1, w, true);
             const int t = un.t, s = un.s, S = un.S;
             if (laneOn && !laneOn[t]) continue;
             const int4 it = P.itemRec[un.rec];
+            double2 b = make_double2(0.0, 0.0);
+            Blk<VT> Dinv = {(VT)0, (VT)0, (VT)0, (VT)0};
+            if (s == 0) {
+                b = *reinterpret_cast<const double2 *>(rhs + vIdx<T>(tile, nb, it.w, t));
+                Dinv = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, it.x, t));
+            }
             V2<VT> yk = {(VT)0, (VT)0};
 #pragma unroll 4
             for (int c = it.y + s; c < it.z; c += S) {
@@ -473,9 +514,7 @@ __device__ void forwardOnly(const DevPlan &P, int tile, int tid, int nt, const V
                 yk.q = groupSum<T>(yk.q, S);
             }
             if (s == 0) {
-                const double2 b = *reinterpret_cast<const double2 *>(rhs + vIdx<T>(tile, nb, it.w, t));
                 const V2<VT> y = {(VT)(sign * b.x) + yk.p, (VT)(sign * b.y) + yk.q};
-                const Blk<VT> Dinv = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, it.x, t));
                 const V2<VT> zk = {Dinv.a * y.p + Dinv.b * y.q, Dinv.c * y.p + Dinv.d * y.q};
                 stV<VT>(z + vIdx<T>(tile, nb, it.w, t), zk);
             }
@@ -487,17 +526,23 @@ __device__ void forwardOnly(const DevPlan &P, int tile, int tid, int nt, const V
 // Backward solve: x(k) = z(k) - D(k)^-1 sum U(k,i) x(i), levels from the root down; a node's sum is split across
 // S threads as in the factorization. Ends with a barrier.
 template <int T, typename VT>
-__device__ void backward(const DevPlan &P, int tile, int tid, int nt, const VT *__restrict__ lu,
+__device__ void backward(const DevPlan &P, const LevelMeta &M, int tile, int tid, int nt, const VT *__restrict__ lu,
                          const VT *__restrict__ z, VT *__restrict__ xo, int l0, const int *laneOn) {
     const int nb = P.nb;
     for (int l = l0 - 1; l >= 0; l--) {
-        const int b0 = P.levelBwdBucketPtr[l], b1 = P.levelBwdBucketPtr[l + 1];
-        const int units = bucketUnits<T>(P.bwdBucketRec, b0, b1);
+        const int b0 = M.levelBwdBucketPtr[l], b1 = M.levelBwdBucketPtr[l + 1];
+        const int units = bucketUnits<T>(M.bwdBucketRec, b0, b1);
         for (int w = tid; w < units; w += nt) {
-            const WorkUnit un = unitOf<T>(P.bwdBucketRec, b0, b1, w);
+            const WorkUnit un = unitOf<T>(M.bwdBucketRec, b0, b1, w);
             const int t = un.t, s = un.s, S = un.S;
             if (laneOn && !laneOn[t]) continue;
             const int4 br = P.bwdRec[un.rec];
+            Blk<VT> Dinv = {(VT)0, (VT)0, (VT)0, (VT)0};
+            V2<VT> zk = {(VT)0, (VT)0};
+            if (s == 0) {
+                Dinv = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, br.x, t));
+                zk = ldV<VT>(z + vIdx<T>(tile, nb, br.w, t));
+            }
             V2<VT> acc = {(VT)0, (VT)0};
 #pragma unroll 4
             for (int e = br.y + s; e < br.z; e += S) {
@@ -509,8 +554,6 @@ __device__ void backward(const DevPlan &P, int tile, int tid, int nt, const VT *
                 acc.q = groupSum<T>(acc.q, S);
             }
             if (s == 0) {
-                const Blk<VT> Dinv = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, br.x, t));
-                const V2<VT> zk = ldV<VT>(z + vIdx<T>(tile, nb, br.w, t));
                 const V2<VT> r = {zk.p + Dinv.a * acc.p + Dinv.b * acc.q, zk.q + Dinv.c * acc.p + Dinv.d * acc.q};
                 stV<VT>(xo + vIdx<T>(tile, nb, br.w, t), r);
             }
@@ -753,25 +796,25 @@ __device__ void topBackward(const DevPlan &P, const DevTop &top, const TopLds<T,
 
 // The whole block LU: sparse levels below the top block, then the top block (n = 0: sparse only).
 template <int T, typename VT>
-__device__ __forceinline__ void luFactorForward(const DevPlan &P, const DevTop &top, const TopLds<T, VT> &L, int tile,
-                                                int tid, int nt, VT *lu, VT *z, const double *rhs, double sign,
-                                                int *clampLane, const int *laneOn) {
-    factorForward<T, VT>(P, tile, tid, nt, lu, z, rhs, sign, clampLane, top.l0, laneOn);
+__device__ __forceinline__ void luFactorForward(const DevPlan &P, const LevelMeta &M, const DevTop &top,
+                                                const TopLds<T, VT> &L, int tile, int tid, int nt, VT *lu, VT *z,
+                                                const double *rhs, double sign, int *clampLane, const int *laneOn) {
+    factorForward<T, VT>(P, M, tile, tid, nt, lu, z, rhs, sign, clampLane, top.l0, laneOn);
     if (top.n > 0) topFactor<T, VT>(P, top, L, tile, tid, nt, lu, z, rhs, sign, clampLane, laneOn);
 }
 template <int T, typename VT>
-__device__ __forceinline__ void luForward(const DevPlan &P, const DevTop &top, const TopLds<T, VT> &L, int tile,
-                                          int tid, int nt, const VT *lu, VT *z, const double *rhs, double sign,
-                                          const int *laneOn) {
-    forwardOnly<T, VT>(P, tile, tid, nt, lu, z, rhs, sign, top.l0, laneOn);
+__device__ __forceinline__ void luForward(const DevPlan &P, const LevelMeta &M, const DevTop &top,
+                                          const TopLds<T, VT> &L, int tile, int tid, int nt, const VT *lu, VT *z,
+                                          const double *rhs, double sign, const int *laneOn) {
+    forwardOnly<T, VT>(P, M, tile, tid, nt, lu, z, rhs, sign, top.l0, laneOn);
     if (top.n > 0) topForward<T, VT>(P, top, L, tile, tid, nt, lu, z, rhs, sign, laneOn);
 }
 template <int T, typename VT>
-__device__ __forceinline__ void luBackward(const DevPlan &P, const DevTop &top, const TopLds<T, VT> &L, int tile,
-                                           int tid, int nt, const VT *lu, const VT *z, VT *xo,
-                                           const int *laneOn) {
+__device__ __forceinline__ void luBackward(const DevPlan &P, const LevelMeta &M, const DevTop &top,
+                                           const TopLds<T, VT> &L, int tile, int tid, int nt, const VT *lu,
+                                           const VT *z, VT *xo, const int *laneOn) {
     if (top.n > 0) topBackward<T, VT>(P, top, L, tile, tid, nt, xo, laneOn);
-    backward<T, VT>(P, tile, tid, nt, lu, z, xo, top.l0, laneOn);
+    backward<T, VT>(P, M, tile, tid, nt, lu, z, xo, top.l0, laneOn);
 }
 
 // Mismatch and Jacobian fill of the lanes flagged in fillLane; per-thread partial max|F| and finiteness of the
@@ -834,7 +877,8 @@ __global__ __launch_bounds__(MAX_THREADS) void nrStepKernel(DevPlan P, DevState 
     const int myLane = tid % T;
     if (S.tileRunning[tile] == 0) return;   // uniform per block
     extern __shared__ float4 olfhipTopLds[];
-    const TopLds<T, VT> TL = {reinterpret_cast<VT *>(olfhipTopLds), top.n};
+    const TopLds<T, VT> TL = {reinterpret_cast<VT *>(reinterpret_cast<unsigned char *>(olfhipTopLds) + top.metaBytes),
+                              top.n};
 
     __shared__ int sStatus[T], sIt[T], sSolves[T], sOuter[T], sTotal[T], sClamp[T];
     __shared__ int sFill[T], sSolve[T], sConv[T], sRl[T], sRemain[T], sSwitched[T], sFin[T];
@@ -848,6 +892,7 @@ __global__ __launch_bounds__(MAX_THREADS) void nrStepKernel(DevPlan P, DevState 
     VT *lu = reinterpret_cast<VT *>(S.lu);
     VT *y = reinterpret_cast<VT *>(S.y);
     VT *dx1 = reinterpret_cast<VT *>(S.dx1);
+    const LevelMeta LM = loadLevelMeta(P, reinterpret_cast<unsigned char *>(olfhipTopLds), tid, nt);
 
     if (tid == 0) {
         sProf = nullptr;
@@ -1032,9 +1077,9 @@ __global__ __launch_bounds__(MAX_THREADS) void nrStepKernel(DevPlan P, DevState 
     if (sFlag[2]) {
         // ---- 4. factorization with the forward solve of -F, then the backward solve
         OLFHIP_MARK(2);
-        luFactorForward<T, VT>(P, top, TL, tile, tid, nt, lu, y, S.F, -1.0, sClamp, sSolve);
+        luFactorForward<T, VT>(P, LM, top, TL, tile, tid, nt, lu, y, S.F, -1.0, sClamp, sSolve);
         OLFHIP_MARK(3);
-        luBackward<T, VT>(P, top, TL, tile, tid, nt, lu, y, dx1, sSolve);
+        luBackward<T, VT>(P, LM, top, TL, tile, tid, nt, lu, y, dx1, sSolve);
         OLFHIP_MARK(4);
         // ---- 5. step in FP64 (+ one FP64-residual refinement step on the FP32 factors)
         for (int w = tid; w < nb * T; w += nt) {
@@ -1049,8 +1094,8 @@ __global__ __launch_bounds__(MAX_THREADS) void nrStepKernel(DevPlan P, DevState 
             applyJPass<T, VT>(P, S, tile, tid, nt, sSolve, S.dx, S.F, 1.0, S.r, pm);
             __syncthreads();
             OLFHIP_MARK(5);
-            luForward<T, VT>(P, top, TL, tile, tid, nt, lu, y, S.r, -1.0, sSolve);
-            luBackward<T, VT>(P, top, TL, tile, tid, nt, lu, y, dx1, sSolve);
+            luForward<T, VT>(P, LM, top, TL, tile, tid, nt, lu, y, S.r, -1.0, sSolve);
+            luBackward<T, VT>(P, LM, top, TL, tile, tid, nt, lu, y, dx1, sSolve);
             OLFHIP_MARK(6);
             for (int w = tid; w < nb * T; w += nt) {
                 const int k = w / T, t = w % T;
@@ -1289,12 +1334,14 @@ __global__ __launch_bounds__(MAX_THREADS) void luSolveKernel(DevPlan P, VT *lu, 
                                                              int *clamps, DevTop top) {
     __shared__ int sClamp[T];
     extern __shared__ float4 olfhipTopLds[];
-    const TopLds<T, VT> TL = {reinterpret_cast<VT *>(olfhipTopLds), top.n};
+    const TopLds<T, VT> TL = {reinterpret_cast<VT *>(reinterpret_cast<unsigned char *>(olfhipTopLds) + top.metaBytes),
+                              top.n};
     const int tile = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
     if (tid < T) sClamp[tid] = 0;
     __syncthreads();
-    luFactorForward<T, VT>(P, top, TL, tile, tid, nt, lu, y, rhs, 1.0, sClamp, nullptr);
-    luBackward<T, VT>(P, top, TL, tile, tid, nt, lu, y, xo, nullptr);
+    const LevelMeta LM = loadLevelMeta(P, reinterpret_cast<unsigned char *>(olfhipTopLds), tid, nt);
+    luFactorForward<T, VT>(P, LM, top, TL, tile, tid, nt, lu, y, rhs, 1.0, sClamp, nullptr);
+    luBackward<T, VT>(P, LM, top, TL, tile, tid, nt, lu, y, xo, nullptr);
     if (tid < T) clamps[tile * T + tid] = sClamp[tid];
 }
 
