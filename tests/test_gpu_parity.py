@@ -195,3 +195,27 @@ def test_states_left_on_device_read_back(grid):
     b = ctx.solve(inp, want_states=False)
     assert np.array_equal(ctx.read_states(2), a["x"])
     assert np.array_equal(a["status"], b["status"])
+
+
+def test_n1_with_max_voltage_change_scaling_matches_oracle(grid):
+    """MAX_VOLTAGE_CHANGE step scaling (state_vector_scaling.cu:57-97) with limits small enough to act on most steps:
+    statuses, iterations and states equal the oracle's with the same scaling."""
+    net = grid("grid2000")
+    print("[n1] grid2000 MAX_VOLTAGE_CHANGE", flush=True)
+    osys, xb, mb, tb, ltb = base_case(net, "grid2000")
+    ctg = gridgen.sample_contingencies(net, 32, 0.25, 21)
+    dv, dphi = 0.004, 0.02
+    ref = osys.n1(xb, mb, tb, ltb, ctg, prm=po.params(scaling_mode=2, max_dv=dv, max_dphi=dphi,
+                                                       batch_of_one=False))
+    nrp = NewtonRaphsonParameters(stateVectorScalingMode="MAX_VOLTAGE_CHANGE", maxVoltageChangeV=dv,
+                                  maxVoltageChangePhi=dphi)
+    sa = GpuBatchedSecurityAnalysisSolver(N.Structure.from_network(net), net.reactive_limits(),
+                                          GpuLoadFlowParameters(linearSolver="BATCHED_LU"), nrp)
+    res = sa.solve_batch_n1(xb, tb, mb, ltb, ctg)
+    assert np.array_equal(res.status, ref["status"])
+    assert np.array_equal(res.total_iterations, ref["total_iterations"])
+    # the limits made the solves take more iterations than unscaled ones
+    plain = osys.n1(xb, mb, tb, ltb, ctg)
+    assert ref["total_iterations"].sum() > plain["total_iterations"].sum()
+    conv = res.status == 0
+    compare_states(res.x[conv], ref["x"][conv], "n1 grid2000 MAX_VOLTAGE_CHANGE")
