@@ -72,7 +72,7 @@ def main():
     ap.add_argument("--autotune", action="store_true", help="time every tile shape first and use the fastest")
     ap.add_argument("--no-warm-start", action="store_true", help="base-case start only (no learned warm start)")
     ap.add_argument("--train-samples", type=int, default=20000)
-    ap.add_argument("--train-epochs", type=int, default=60)
+    ap.add_argument("--train-epochs", type=int, default=150)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: the N-1 batch shards by contingency across GPUs "
