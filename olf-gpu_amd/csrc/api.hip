@@ -232,7 +232,7 @@ struct ContextImpl {
     int nRL = 0;
     DBuf rlNode, rlMinQ, rlMaxQ, rlLoadQ, rlTargetV;
     // per-lane state
-    DBuf x, F, dx, r, target, qinj, trig, mode_, lu, y, dx1;
+    DBuf x, F, dx, r, target, qinj, mode_, lu, y, dx1;
     DBuf it, status, solves, outer, total, clamps, norm, ratio, outages, tileRunning, count;
     DBuf rlType, rlCount, rlPass;
     // install inputs
@@ -259,7 +259,6 @@ struct ContextImpl {
         S.r = r.as<double>();
         S.target = target.as<double>();
         S.qinj = qinj.as<double>();
-        S.trig = trig.as<double>();
         S.mode = mode_.as<signed char>();
         S.lu = lu.p;
         S.y = y.p;
@@ -327,7 +326,6 @@ static ContextImpl *createContext(StructureImpl *st, int nScenarios, int mode, c
     c->r.alloc(nv * 8);
     c->target.alloc(nv * 8);
     c->qinj.alloc(lanes * P.nb * 8);
-    c->trig.alloc(nv * 8);
     c->mode_.alloc(nv);
     c->lu.alloc(lanes * (size_t)P.nSlots * 4 * luBytes(mode));
     c->y.alloc(nv * luBytes(mode));
@@ -482,7 +480,6 @@ static void solve(ContextImpl *c, const olfhip_solve_inputs *in, olfhip_solve_ou
             c->modeBase.as<int>());
         HIP_OK(hipGetLastError());
         auto enqueueStep = [&]() {
-            nodeTrigKernel<T><<<gridFor((size_t)c->nTiles * P.nb * T, 256), 256, 0, s>>>(dp, S);
             if (c->mode == OLFHIP_LU_FP32_REFINED)
                 nrStepKernel<T, float><<<c->nTiles, c->NT, c->top.ldsBytes, s>>>(dp, S, R, prm, c->top.top);
             else

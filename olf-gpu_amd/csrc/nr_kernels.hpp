@@ -65,7 +65,6 @@ struct DevTop {
 struct DevState {
     int nTiles, nInstalled;
     double *x, *F, *dx, *r, *target, *qinj;
-    double *trig;          // (sin phi, cos phi) of every node, per lane, for the current x
     signed char *mode;
     void *lu, *y, *dx1;
     int *it, *status, *solves, *outer, *total, *clamps;
@@ -210,19 +209,15 @@ __device__ __forceinline__ bool isOut(int br, const int *o) {
 // rows as the reference's applyRowModeBatch does (native/cuda/equations/fill_launch.cu:82-115). The closed-branch
 // formulas are the reference's (native/cuda/equations/generated/closed_branch.cuh:31-229, OLF
 // ClosedBranchSide1/2*FlowEquationTerm) in own-bus / far-bus form:
-//   p = A vo^2 - vo vf u,  q = C vo^2 - vo vf w,  u = B sin(th), w = B cos(th),  th = psi + d,  d = phi_f - phi_o,
-// with u = (B sin psi) cos d + (B cos psi) sin d and w = (B cos psi) cos d - (B sin psi) sin d, sin d and cos d from
-// the nodes' own sin/cos of phi (S.trig, one evaluation per node instead of two per branch).
+//   p = A vo^2 - B vo vf sin(th),  q = C vo^2 - B vo vf cos(th),  th = psi - phi_o + phi_f.
 template <int T, typename VT, bool WRITE_J, bool APPLY_VEC>
 __device__ __forceinline__ void nodeTerms(const DevPlan &P, int tile, int k, int t, const double *__restrict__ x,
-                                          const double *__restrict__ trig, const int *out, VT *__restrict__ lu,
-                                          const double *__restrict__ v,
+                                          const int *out, VT *__restrict__ lu, const double *__restrict__ v,
                                           signed char modeP, signed char modeQ, double &sumP, double &sumQ,
                                           double &jvP, double &jvQ) {
     const int nb = P.nb;
     const double2 xo = *reinterpret_cast<const double2 *>(x + vIdx<T>(tile, nb, k, t));
-    const double2 tro = *reinterpret_cast<const double2 *>(trig + vIdx<T>(tile, nb, k, t));
-    const double vo = xo.y;
+    const double pho = xo.x, vo = xo.y;
     double dPdth = 0, dPdv = 0, dQdth = 0, dQdv = 0;
     double sP = 0, sQ = 0, jP = 0, jQ = 0;
     int h = P.hbPtr[k];
@@ -230,27 +225,26 @@ __device__ __forceinline__ void nodeTerms(const DevPlan &P, int tile, int k, int
     for (int g = g0; g < g1; g++) {
         const int f = P.grpFar[g];
         const int hEnd = P.grpEnd[g];
-        const double vf = reinterpret_cast<const double2 *>(x + vIdx<T>(tile, nb, f, t))->y;
-        const double2 trf = *reinterpret_cast<const double2 *>(trig + vIdx<T>(tile, nb, f, t));
-        const double sd = trf.x * tro.y - trf.y * tro.x;   // sin(phi_f - phi_o)
-        const double cd = trf.y * tro.y + trf.x * tro.x;   // cos(phi_f - phi_o)
+        const double2 xf = *reinterpret_cast<const double2 *>(x + vIdx<T>(tile, nb, f, t));
+        const double phf = xf.x, vf = xf.y;
         double oPth = 0, oPv = 0, oQth = 0, oQv = 0;
         for (; h < hEnd; h++) {
             if (isOut(P.hbBranch[h], out)) continue;
-            const double4 cf = P.hbCoef[h];   // A, C, B sin(psi), B cos(psi)
-            const double u = cf.z * cd + cf.w * sd;   // B sin(th)
-            const double w = cf.w * cd - cf.z * sd;   // B cos(th)
-            const double vv = vo * vf;
-            sP += cf.x * vo * vo - vv * u;
-            sQ += cf.y * vo * vo - vv * w;
-            dPdv += 2.0 * cf.x * vo - vf * u;
-            dPdth += vv * w;
-            dQdv += 2.0 * cf.y * vo - vf * w;
-            dQdth -= vv * u;
-            oPv -= vo * u;
-            oPth -= vv * w;
-            oQv -= vo * w;
-            oQth += vv * u;
+            const double4 cf = P.hbCoef[h];
+            const double th = cf.w - pho + phf;
+            double s, c;
+            sincos(th, &s, &c);
+            const double Bvv = cf.y * vo * vf;
+            sP += cf.x * vo * vo - Bvv * s;
+            sQ += cf.z * vo * vo - Bvv * c;
+            dPdv += 2.0 * cf.x * vo - cf.y * vf * s;
+            dPdth += Bvv * c;
+            dQdv += 2.0 * cf.z * vo - cf.y * vf * c;
+            dQdth -= Bvv * s;
+            oPv -= cf.y * vo * s;
+            oPth -= Bvv * c;
+            oQv -= cf.y * vo * c;
+            oQth += Bvv * s;
         }
         if (modeP == MODE_FIXED) {
             oPth = 0;
@@ -839,7 +833,7 @@ __device__ void fillPass(const DevPlan &P, const DevState &S, int tile, int tid,
         const size_t vi = vIdx<T>(tile, nb, k, t);
         const signed char mP = S.mode[vi], mQ = S.mode[vi + 1];
         double sP, sQ, jP, jQ;
-        nodeTerms<T, VT, true, false>(P, tile, k, t, S.x, S.trig, o, lu, nullptr, mP, mQ, sP, sQ, jP, jQ);
+        nodeTerms<T, VT, true, false>(P, tile, k, t, S.x, o, lu, nullptr, mP, mQ, sP, sQ, jP, jQ);
         const double2 tg = *reinterpret_cast<const double2 *>(S.target + vi);
         const double2 xk = *reinterpret_cast<const double2 *>(S.x + vi);
         const double fP = (mP == MODE_FIXED) ? xk.x - tg.x : sP - tg.x;
@@ -864,8 +858,7 @@ __device__ void applyJPass(const DevPlan &P, const DevState &S, int tile, int ti
         const int k = w / T;
         const size_t vi = vIdx<T>(tile, nb, k, t);
         double sP, sQ, jP, jQ;
-        nodeTerms<T, VT, false, true>(P, tile, k, t, S.x, S.trig, o, nullptr, v, S.mode[vi], S.mode[vi + 1], sP, sQ,
-                                      jP, jQ);
+        nodeTerms<T, VT, false, true>(P, tile, k, t, S.x, o, nullptr, v, S.mode[vi], S.mode[vi + 1], sP, sQ, jP, jQ);
         const double2 a = *reinterpret_cast<const double2 *>(add + vi);
         const double rP = jP + addSign * a.x, rQ = jQ + addSign * a.y;
         if (out) *reinterpret_cast<double2 *>(out + vi) = make_double2(rP, rQ);
@@ -1190,21 +1183,6 @@ __global__ __launch_bounds__(MAX_THREADS) void nrStepKernel(DevPlan P, DevState 
         int any = 0;
         for (int t = 0; t < T; t++) any |= sStatus[t] == LANE_RUNNING;
         S.tileRunning[tile] = any;
-    }
-}
-
-// ---- sin and cos of every node's angle for the running tiles' current states (before each Newton step launch)
-template <int T>
-__global__ void nodeTrigKernel(DevPlan P, DevState S) {
-    const int nb = P.nb;
-    const size_t total = (size_t)S.nTiles * nb * T;
-    for (size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x; w < total; w += (size_t)gridDim.x * blockDim.x) {
-        const int tile = (int)(w / ((size_t)nb * T));
-        if (!S.tileRunning[tile]) continue;
-        const size_t vi = 2 * w;   // vIdx(tile, nb, k, t) with w = (tile * nb + k) * T + t
-        double sn, cs;
-        sincos(S.x[vi], &sn, &cs);
-        *reinterpret_cast<double2 *>(S.trig + vi) = make_double2(sn, cs);
     }
 }
 

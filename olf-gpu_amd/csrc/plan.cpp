@@ -324,9 +324,9 @@ Plan buildPlan(int n, int nBranch, const double *p8, const int *idx12, int nShun
         for (size_t h = 0; h < hs.size(); h++) {
             P.hbBranch.push_back(hs[h].branch);
             P.hbCoef.push_back(hs[h].A);
+            P.hbCoef.push_back(hs[h].B);
             P.hbCoef.push_back(hs[h].C);
-            P.hbCoef.push_back(hs[h].B * std::sin(hs[h].psi));
-            P.hbCoef.push_back(hs[h].B * std::cos(hs[h].psi));
+            P.hbCoef.push_back(hs[h].psi);
             bool last = h + 1 == hs.size() || hs[h + 1].far != hs[h].far;
             if (last) {
                 const int f = hs[h].far;

@@ -64,7 +64,7 @@ struct Plan {
     // --- fill gather tables, per node, half-branches grouped by the far node (ascending)
     std::vector<int> hbPtr;                        // half-branches of node k
     std::vector<int> hbBranch;                     // branch id (for the per-scenario outage lists)
-    std::vector<double> hbCoef;                    // 4 per half-branch: A, C, B sin(psi), B cos(psi) (see fill)
+    std::vector<double> hbCoef;                    // 4 per half-branch: A, B, C, psi (see fill)
     std::vector<int> grpPtr;                       // neighbour groups of node k
     std::vector<int> grpEnd;                       // end of the group's half-branches in hbBranch
     std::vector<int> grpFar;                       // far node
