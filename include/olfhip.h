@@ -174,7 +174,8 @@ int olfhip_solve_batch_context(olfhip_context *context, const olfhip_solve_input
                                olfhip_solve_outputs *outputs);
 
 /* Device time of the last solve's Newton-step launches, measured with HIP events on the context's stream around each
- * step (the step kernel, plus the few-microsecond running-lane count and its 4-byte copy), and the step count.
+ * launch (the step kernel, plus the few-microsecond running-tile count and its 4-byte copy), and the launch count.
+ * One launch carries every tile to its end unless OLFHIP_STEPS_PER_LAUNCH=k caps it at k Newton steps per tile.
  * No reference counterpart: the reference's phase timers are print-only (OLF_GPU_PHASE_TIME,
  * native/cuda/jni/batch_solve_jni.cu:12-26). */
 int olfhip_context_timing(const olfhip_context *context, double *step_ms, int *steps);

@@ -445,6 +445,9 @@ static void solve(ContextImpl *c, const olfhip_solve_inputs *in, olfhip_solve_ou
     prm.refine = c->mode == OLFHIP_LU_FP32_REFINED;
     prm.batchOfOne = K == 1;
     const int maxSteps = (prm.maxIter + 1) * (prm.maxOuter + 1) + 1;
+    // one launch carries every tile to its end (OLFHIP_STEPS_PER_LAUNCH=k: at most k Newton steps per launch)
+    prm.stepsPerLaunch = maxSteps;
+    if (const char *e = std::getenv("OLFHIP_STEPS_PER_LAUNCH")) prm.stepsPerLaunch = std::max(1, std::atoi(e));
     const bool profile = std::getenv("OLFHIP_PROFILE") && std::strcmp(std::getenv("OLFHIP_PROFILE"), "0") != 0;
     if (profile) {
         c->profCap = maxSteps;
@@ -493,7 +496,8 @@ static void solve(ContextImpl *c, const olfhip_solve_inputs *in, olfhip_solve_ou
             HIP_OK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
         }
         c->lastStepMs = 0.0;
-        for (steps = 0; steps < maxSteps; steps++) {
+        const int maxLaunches = (maxSteps + prm.stepsPerLaunch - 1) / prm.stepsPerLaunch;
+        for (steps = 0; steps < maxLaunches; steps++) {
             HIP_OK(hipEventRecord(c->evStart, s));
             if (useGraph) HIP_OK(hipGraphLaunch(exec, s));
             else enqueueStep();

@@ -73,8 +73,8 @@ struct DevState {
     int maxOut;
     signed char *rlType, *rlCount, *rlPass;
     int *tileRunning;
-    // opt-in section timing of one tile: prof[0] counts the launches that recorded, then PROF_SECTIONS stamps per
-    // launch (wall clock) from prof[PROF_SECTIONS] on, at most profCap launches
+    // opt-in section timing of one tile: prof[0] counts the Newton steps that recorded, then PROF_SECTIONS stamps per
+    // step (wall clock) from prof[PROF_SECTIONS] on, at most profCap steps
     unsigned long long *prof;
     int profCap;
     int profTile;   // the tile whose sections are stamped
@@ -98,6 +98,7 @@ struct NrParams {
     double maxDv, maxDphi;
     int refine;
     int batchOfOne;
+    int stepsPerLaunch;   // Newton steps one launch may run per tile
 };
 
 template <typename VT>
@@ -866,8 +867,12 @@ __device__ void applyJPass(const DevPlan &P, const DevState &S, int tile, int ti
     }
 }
 
-// One batched Newton step for the tile of blockIdx.x. Reference loop body: native/cuda/solver/newton_iteration.cu:116-203
-// (test half :447-495, testScenario :276-397, solveUniformBatch :580-660), reactive limits
+// Batched Newton steps for the tile of blockIdx.x: up to prm.stepsPerLaunch of them, ending as soon as no lane of the
+// tile runs, so that one launch carries every tile to its end and the hardware starts further tiles as earlier ones
+// finish (a tile in a long reactive-limit outer loop overlaps the others instead of holding launches of its own).
+// Every lane's step count is bounded by its iteration and outer-loop budgets and the loop by prm.stepsPerLaunch, so
+// every wave exits. Each step is the reference loop body: native/cuda/solver/newton_iteration.cu:116-203 (test half
+// :447-495, testScenario :276-397, solveUniformBatch :580-660), reactive limits
 // native/cuda/outerloop/reactive_limits.cu:321-371 plus the forced-PV election, step scaling
 // native/cuda/solver/state_vector_scaling.cu:57-97.
 template <int T, typename VT>
@@ -893,7 +898,8 @@ __global__ __launch_bounds__(MAX_THREADS) void nrStepKernel(DevPlan P, DevState 
     VT *y = reinterpret_cast<VT *>(S.y);
     VT *dx1 = reinterpret_cast<VT *>(S.dx1);
     const LevelMeta LM = loadLevelMeta(P, reinterpret_cast<unsigned char *>(olfhipTopLds), tid, nt);
-
+    __shared__ int sRunning;
+    for (int step = 0; step < prm.stepsPerLaunch; step++) {
     if (tid == 0) {
         sProf = nullptr;
         if (S.prof && tile == S.profTile) {
@@ -1183,6 +1189,10 @@ __global__ __launch_bounds__(MAX_THREADS) void nrStepKernel(DevPlan P, DevState 
         int any = 0;
         for (int t = 0; t < T; t++) any |= sStatus[t] == LANE_RUNNING;
         S.tileRunning[tile] = any;
+        sRunning = any;
+    }
+    __syncthreads();
+    if (!sRunning) break;
     }
 }
 
