@@ -19,7 +19,8 @@ struct Plan {
     int nBranch = 0;
     int nShunt = 0;
 
-    // --- order: node k is bus perm[k]; bus b is node pos[b]
+    // --- order: node k is bus perm[k]; bus b is node pos[b] (minimum degree, renumbered in a postorder of its
+    // elimination tree: same fill, subtrees contiguous)
     std::vector<int> perm, pos;
 
     // --- block LU structure in node numbering
