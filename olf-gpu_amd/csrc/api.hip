@@ -138,7 +138,67 @@ struct TopPlan {
     DevTop top{};
     size_t ldsBytes = 0;
     DBuf slots, items, diagItems, buckets, diagBuckets;
+    DBuf contribPad, bwdPad, bucketPad, bwdBucketPad;
 };
+
+// The sparse levels' records in the unit-major order the kernels read for T lanes per tile (the split of a bucket
+// capped at 64 / T as in the kernels): for bucket b with split S, rounds R = the largest ceil(length / S) of its
+// items, thread s of item q reads its r-th record at base + (r * nItems + q) * S + s, the same contribution the
+// item-major schedule gives it (index s + r S of the item's list), so sums run in the same order. Unused places
+// hold x = -1.
+static void planPadded(const Plan &P, int T, TopPlan &tp) {
+    const int cap = 64 / T;
+    std::vector<int> pad, bpad, epad, ebpad;
+    for (size_t b = 0; b < P.bucketRec.size() / 4; b++) {
+        const int first = P.bucketRec[4 * b], end = P.bucketRec[4 * b + 1];
+        const int S = std::min(P.bucketRec[4 * b + 2], cap), nq = end - first;
+        int R = 0;
+        for (int q = 0; q < nq; q++) {
+            const int len = P.itemRec[4 * (first + q) + 2] - P.itemRec[4 * (first + q) + 1];
+            R = std::max(R, (len + S - 1) / S);
+        }
+        const size_t base = pad.size() / 4;
+        pad.resize(pad.size() + (size_t)4 * R * nq * S, -1);
+        for (int q = 0; q < nq; q++) {
+            const int c0 = P.itemRec[4 * (first + q) + 1], len = P.itemRec[4 * (first + q) + 2] - c0;
+            for (int c = 0; c < len; c++) {
+                const size_t at = base + ((size_t)(c / S) * nq + q) * S + c % S;
+                for (int f = 0; f < 4; f++) pad[4 * at + f] = P.contribRec[4 * (size_t)(c0 + c) + f];
+            }
+        }
+        bpad.insert(bpad.end(), {(int)base, R});
+    }
+    for (size_t b = 0; b < P.bwdBucketRec.size() / 4; b++) {
+        const int first = P.bwdBucketRec[4 * b], end = P.bwdBucketRec[4 * b + 1];
+        const int S = std::min(P.bwdBucketRec[4 * b + 2], cap), nq = end - first;
+        int R = 0;
+        for (int q = 0; q < nq; q++) {
+            const int len = P.bwdRec[4 * (first + q) + 2] - P.bwdRec[4 * (first + q) + 1];
+            R = std::max(R, (len + S - 1) / S);
+        }
+        const size_t base = epad.size() / 2;
+        epad.resize(epad.size() + (size_t)2 * R * nq * S, -1);
+        for (int q = 0; q < nq; q++) {
+            const int *br = &P.bwdRec[4 * (size_t)(first + q)];
+            for (int e = 0; e < br[2] - br[1]; e++) {
+                const size_t at = base + ((size_t)(e / S) * nq + q) * S + e % S;
+                epad[2 * at] = br[0] + 2 + 2 * e;         // U(k, i): after L~(i, k) in node k's pair e
+                epad[2 * at + 1] = P.sIdx[br[1] + e];     // i
+            }
+        }
+        ebpad.insert(ebpad.end(), {(int)base, R});
+    }
+    if (pad.empty()) pad.assign(4, -1);
+    if (epad.empty()) epad.assign(2, -1);
+    tp.contribPad.upload(pad);
+    tp.bwdPad.upload(epad);
+    tp.bucketPad.upload(bpad.empty() ? std::vector<int>{0, 0} : bpad);
+    tp.bwdBucketPad.upload(ebpad.empty() ? std::vector<int>{0, 0} : ebpad);
+    tp.top.contribPad = tp.contribPad.as<int4>();
+    tp.top.bwdPad = tp.bwdPad.as<int2>();
+    tp.top.bucketPad = tp.bucketPad.as<int2>();
+    tp.top.bwdBucketPad = tp.bwdBucketPad.as<int2>();
+}
 
 static void planTop(const StructureImpl *st, const void *kernel, int T, int NT, size_t vtBytes, TopPlan &tp) {
     const Plan &P = st->plan;
@@ -159,6 +219,7 @@ static void planTop(const StructureImpl *st, const void *kernel, int T, int NT, 
     const int l0 = maxNodes > 0 ? P.topLevel(maxNodes, &n) : P.nLevels;
     if (l0 >= P.nLevels) n = 0;
     tp.top = DevTop{};
+    planPadded(P, T, tp);
     tp.top.l0 = n > 0 ? l0 : P.nLevels;
     tp.top.n = n;
     tp.top.start = n > 0 ? P.levelNodePtr[l0] : P.nb;

@@ -13,6 +13,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "trig.hpp"
+
 namespace olfhip {
 
 constexpr int LANE_RUNNING = 100;   // still solving
@@ -60,6 +62,13 @@ struct DevTop {
     const int4 *buckets, *diagBuckets;
     int nBuckets, nDiagBuckets;
     int metaBytes;   // LDS before the top block: the level schedule's bucket records
+    // The sparse levels' records in unit-major order for this context's lane count: bucket b's contribution records
+    // (and the backward pass's (U slot, column node) pairs) for round r, item q and thread s of its split S sit at
+    // base_b + (r * nItems_b + q) * S + s, padded with x = -1, so that a thread finds its records from its work unit
+    // alone and loads them together with the item record instead of after it. bucketPad / bwdBucketPad hold
+    // (base, rounds) per bucket.
+    const int4 *contribPad;
+    const int2 *bwdPad, *bucketPad, *bwdBucketPad;
 };
 
 struct DevState {
@@ -234,7 +243,7 @@ __device__ __forceinline__ void nodeTerms(const DevPlan &P, int tile, int k, int
             const double4 cf = P.hbCoef[h];
             const double th = cf.w - pho + phf;
             double s, c;
-            sincos(th, &s, &c);
+            sincosReduced(th, &s, &c);
             const double Bvv = cf.y * vo * vf;
             sP += cf.x * vo * vo - Bvv * s;
             sQ += cf.z * vo * vo - Bvv * c;
@@ -336,6 +345,7 @@ __device__ __forceinline__ void groupSumBlk(Blk<VT> &b, int S) {
 // every record's T * S units start at a multiple of T * S and its threads can reduce with lane shuffles.
 struct WorkUnit {
     int rec, t, s, S;
+    int b, q;   // bucket and the item's position in it
 };
 template <int T>
 __device__ __forceinline__ int capSplit(int s) {
@@ -361,11 +371,11 @@ __device__ __forceinline__ WorkUnit unitOf(const int4 *bk, int b0, int b1, int u
         const int n = (r.y - r.x) * T * S;
         if (u < n) {
             const int sh = __builtin_ctz(T * S);
-            return WorkUnit{r.x + (u >> sh), u % T, (u / T) & (S - 1), S};
+            return WorkUnit{r.x + (u >> sh), u % T, (u / T) & (S - 1), S, b, u >> sh};
         }
         u -= n;
     }
-    return WorkUnit{-1, 0, 0, 1};
+    return WorkUnit{-1, 0, 0, 1, 0, 0};
 }
 
 // The level schedule's bucket records, copied to LDS once per launch so that each level's bucket lookups cost LDS
@@ -373,24 +383,34 @@ __device__ __forceinline__ WorkUnit unitOf(const int4 *bk, int b0, int b1, int u
 struct LevelMeta {
     const int4 *bucketRec, *bwdBucketRec;
     const int *levelBucketPtr, *levelBwdBucketPtr;
+    const int2 *bucketPad, *bwdBucketPad;   // (base, rounds) of each bucket's unit-major records
 };
 __host__ __device__ constexpr size_t levelMetaBytes(int nLevels, int nBuckets, int nBwdBuckets) {
-    return ((size_t)16 * (nBuckets + nBwdBuckets) + (size_t)8 * (nLevels + 1) + 15) / 16 * 16;
+    return ((size_t)24 * (nBuckets + nBwdBuckets) + (size_t)8 * (nLevels + 1) + 15) / 16 * 16;
 }
 // Copies the records into the LDS at base (levelMetaBytes of it). Ends with a barrier.
-__device__ __forceinline__ LevelMeta loadLevelMeta(const DevPlan &P, unsigned char *base, int tid, int nt) {
+__device__ __forceinline__ LevelMeta loadLevelMeta(const DevPlan &P, const DevTop &top, unsigned char *base, int tid,
+                                                   int nt) {
     int4 *b = reinterpret_cast<int4 *>(base);
     int4 *bb = b + P.nBuckets;
-    int *lp = reinterpret_cast<int *>(bb + P.nBwdBuckets);
+    int2 *bp = reinterpret_cast<int2 *>(bb + P.nBwdBuckets);
+    int2 *bbp = bp + P.nBuckets;
+    int *lp = reinterpret_cast<int *>(bbp + P.nBwdBuckets);
     int *lbp = lp + P.nLevels + 1;
-    for (int i = tid; i < P.nBuckets; i += nt) b[i] = P.bucketRec[i];
-    for (int i = tid; i < P.nBwdBuckets; i += nt) bb[i] = P.bwdBucketRec[i];
+    for (int i = tid; i < P.nBuckets; i += nt) {
+        b[i] = P.bucketRec[i];
+        bp[i] = top.bucketPad[i];
+    }
+    for (int i = tid; i < P.nBwdBuckets; i += nt) {
+        bb[i] = P.bwdBucketRec[i];
+        bbp[i] = top.bwdBucketPad[i];
+    }
     for (int i = tid; i <= P.nLevels; i += nt) {
         lp[i] = P.levelBucketPtr[i];
         lbp[i] = P.levelBwdBucketPtr[i];
     }
     __syncthreads();
-    return LevelMeta{b, bb, lp, lbp};
+    return LevelMeta{b, bb, lp, lbp, bp, bbp};
 }
 
 // The factorization, left-looking by elimination levels, one phase per level, with the forward solve of
@@ -399,9 +419,9 @@ __device__ __forceinline__ LevelMeta loadLevelMeta(const DevPlan &P, unsigned ch
 // (stride S) and reduced across lanes. Counts clamped pivots per lane into clampLane (shared, may be null). Ends with
 // a barrier.
 template <int T, typename VT>
-__device__ void factorForward(const DevPlan &P, const LevelMeta &M, int tile, int tid, int nt, VT *__restrict__ lu,
-                              VT *__restrict__ z, const double *__restrict__ rhs, double sign, int *clampLane, int l0,
-                              const int *laneOn) {
+__device__ void factorForward(const DevPlan &P, const LevelMeta &M, const DevTop &top, int tile, int tid, int nt,
+                              VT *__restrict__ lu, VT *__restrict__ z, const double *__restrict__ rhs, double sign,
+                              int *clampLane, int l0, const int *laneOn) {
     const int nb = P.nb;
     for (int l = 0; l < l0; l++) {
         const int b0 = M.levelBucketPtr[l], b1 = M.levelBucketPtr[l + 1];
@@ -410,9 +430,15 @@ __device__ void factorForward(const DevPlan &P, const LevelMeta &M, int tile, in
             const WorkUnit un = unitOf<T>(M.bucketRec, b0, b1, w);
             const int t = un.t, s = un.s, S = un.S;
             if (laneOn && !laneOn[t]) continue;
+            // the item record and the unit's contribution records are independent loads
+            const int4 bk = M.bucketRec[un.b];
+            const int2 pad = M.bucketPad[un.b];
+            const int4 *cp = top.contribPad + pad.x + (size_t)un.q * S + s;
+            const size_t roundStride = (size_t)(bk.y - bk.x) * S;
+            int4 next = pad.y > 0 ? cp[0] : make_int4(-1, -1, -1, -1);   // issued before the item record
             const int4 it = P.itemRec[un.rec];
             VT *own = lu + bIdx<T>(tile, P.nSlots, it.x, t);
-            if (it.w >= 0) {
+            if (bk.w) {
                 // diagonal item of node it.w: pivot block, its inverse, and the forward-solve entry; the own block
                 // and right-hand side are loaded first so that they overlap the contribution loads
                 Blk<VT> A = {(VT)0, (VT)0, (VT)0, (VT)0};
@@ -424,8 +450,10 @@ __device__ void factorForward(const DevPlan &P, const LevelMeta &M, int tile, in
                 Blk<VT> D = {(VT)0, (VT)0, (VT)0, (VT)0};
                 V2<VT> yk = {(VT)0, (VT)0};
 #pragma unroll 2
-                for (int c = it.y + s; c < it.z; c += S) {
-                    const int4 cr = P.contribRec[c];
+                for (int r = 0; r < pad.y; r++) {
+                    const int4 cr = next;
+                    if (cr.x < 0) break;
+                    if (r + 1 < pad.y) next = cp[(r + 1) * roundStride];
                     const Blk<VT> Lkj = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, cr.x, t));
                     const Blk<VT> Ujk = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, cr.x + 1, t));
                     const Blk<VT> Dj = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, cr.z, t));
@@ -458,8 +486,10 @@ __device__ void factorForward(const DevPlan &P, const LevelMeta &M, int tile, in
                 Blk<VT> L = {(VT)0, (VT)0, (VT)0, (VT)0};
                 Blk<VT> U = {(VT)0, (VT)0, (VT)0, (VT)0};
 #pragma unroll 2
-                for (int c = it.y + s; c < it.z; c += S) {
-                    const int4 c0 = P.contribRec[c];
+                for (int r = 0; r < pad.y; r++) {
+                    const int4 c0 = next;
+                    if (c0.x < 0) break;
+                    if (r + 1 < pad.y) next = cp[(r + 1) * roundStride];
                     const Blk<VT> Lkj = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, c0.x, t));
                     const Blk<VT> Ujk = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, c0.x + 1, t));
                     const Blk<VT> Lij = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, c0.y, t));
@@ -485,7 +515,7 @@ __device__ void factorForward(const DevPlan &P, const LevelMeta &M, int tile, in
 // Forward solve with existing factors: z(k) = D(k)^-1 (b(k) - sum L~(k,j) z(j)), b = sign * rhs. Ends with a
 // barrier.
 template <int T, typename VT>
-__device__ void forwardOnly(const DevPlan &P, const LevelMeta &M, int tile, int tid, int nt,
+__device__ void forwardOnly(const DevPlan &P, const LevelMeta &M, const DevTop &top, int tile, int tid, int nt,
                             const VT *__restrict__ lu, VT *__restrict__ z, const double *__restrict__ rhs, double sign,
                             int l0, const int *laneOn) {
     const int nb = P.nb;
@@ -496,6 +526,11 @@ __device__ void forwardOnly(const DevPlan &P, const LevelMeta &M, int tile, int 
             const WorkUnit un = unitOf<T>(M.bucketRec, b0, b1, w, true);
             const int t = un.t, s = un.s, S = un.S;
             if (laneOn && !laneOn[t]) continue;
+            const int4 bk = M.bucketRec[un.b];
+            const int2 pad = M.bucketPad[un.b];
+            const int4 *cp = top.contribPad + pad.x + (size_t)un.q * S + s;
+            const size_t roundStride = (size_t)(bk.y - bk.x) * S;
+            int4 next = pad.y > 0 ? cp[0] : make_int4(-1, -1, -1, -1);   // issued before the item record
             const int4 it = P.itemRec[un.rec];
             double2 b = make_double2(0.0, 0.0);
             Blk<VT> Dinv = {(VT)0, (VT)0, (VT)0, (VT)0};
@@ -505,8 +540,10 @@ __device__ void forwardOnly(const DevPlan &P, const LevelMeta &M, int tile, int 
             }
             V2<VT> yk = {(VT)0, (VT)0};
 #pragma unroll 4
-            for (int c = it.y + s; c < it.z; c += S) {
-                const int4 cr = P.contribRec[c];
+            for (int r = 0; r < pad.y; r++) {
+                const int4 cr = next;
+                if (cr.x < 0) break;
+                if (r + 1 < pad.y) next = cp[(r + 1) * roundStride];
                 const Blk<VT> Lkj = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, cr.x, t));
                 mulSubV(yk, Lkj, ldV<VT>(z + vIdx<T>(tile, nb, cr.y, t)));
             }
@@ -527,8 +564,9 @@ __device__ void forwardOnly(const DevPlan &P, const LevelMeta &M, int tile, int 
 // Backward solve: x(k) = z(k) - D(k)^-1 sum U(k,i) x(i), levels from the root down; a node's sum is split across
 // S threads as in the factorization. Ends with a barrier.
 template <int T, typename VT>
-__device__ void backward(const DevPlan &P, const LevelMeta &M, int tile, int tid, int nt, const VT *__restrict__ lu,
-                         const VT *__restrict__ z, VT *__restrict__ xo, int l0, const int *laneOn) {
+__device__ void backward(const DevPlan &P, const LevelMeta &M, const DevTop &top, int tile, int tid, int nt,
+                         const VT *__restrict__ lu, const VT *__restrict__ z, VT *__restrict__ xo, int l0,
+                         const int *laneOn) {
     const int nb = P.nb;
     for (int l = l0 - 1; l >= 0; l--) {
         const int b0 = M.levelBwdBucketPtr[l], b1 = M.levelBwdBucketPtr[l + 1];
@@ -537,6 +575,11 @@ __device__ void backward(const DevPlan &P, const LevelMeta &M, int tile, int tid
             const WorkUnit un = unitOf<T>(M.bwdBucketRec, b0, b1, w);
             const int t = un.t, s = un.s, S = un.S;
             if (laneOn && !laneOn[t]) continue;
+            const int4 bk = M.bwdBucketRec[un.b];
+            const int2 pad = M.bwdBucketPad[un.b];
+            const int2 *ep = top.bwdPad + pad.x + (size_t)un.q * S + s;
+            const size_t roundStride = (size_t)(bk.y - bk.x) * S;
+            int2 next = pad.y > 0 ? ep[0] : make_int2(-1, -1);   // issued before the node record
             const int4 br = P.bwdRec[un.rec];
             Blk<VT> Dinv = {(VT)0, (VT)0, (VT)0, (VT)0};
             V2<VT> zk = {(VT)0, (VT)0};
@@ -546,9 +589,12 @@ __device__ void backward(const DevPlan &P, const LevelMeta &M, int tile, int tid
             }
             V2<VT> acc = {(VT)0, (VT)0};
 #pragma unroll 4
-            for (int e = br.y + s; e < br.z; e += S) {
-                const Blk<VT> U = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, br.x + 2 + 2 * (e - br.y), t));
-                mulSubV(acc, U, ldV<VT>(xo + vIdx<T>(tile, nb, P.sIdx[e], t)));
+            for (int r = 0; r < pad.y; r++) {
+                const int2 e = next;
+                if (e.x < 0) break;
+                if (r + 1 < pad.y) next = ep[(r + 1) * roundStride];
+                const Blk<VT> U = ldBlk<VT>(lu + bIdx<T>(tile, P.nSlots, e.x, t));
+                mulSubV(acc, U, ldV<VT>(xo + vIdx<T>(tile, nb, e.y, t)));
             }
             if (S > 1) {
                 acc.p = groupSum<T>(acc.p, S);
@@ -800,14 +846,14 @@ template <int T, typename VT>
 __device__ __forceinline__ void luFactorForward(const DevPlan &P, const LevelMeta &M, const DevTop &top,
                                                 const TopLds<T, VT> &L, int tile, int tid, int nt, VT *lu, VT *z,
                                                 const double *rhs, double sign, int *clampLane, const int *laneOn) {
-    factorForward<T, VT>(P, M, tile, tid, nt, lu, z, rhs, sign, clampLane, top.l0, laneOn);
+    factorForward<T, VT>(P, M, top, tile, tid, nt, lu, z, rhs, sign, clampLane, top.l0, laneOn);
     if (top.n > 0) topFactor<T, VT>(P, top, L, tile, tid, nt, lu, z, rhs, sign, clampLane, laneOn);
 }
 template <int T, typename VT>
 __device__ __forceinline__ void luForward(const DevPlan &P, const LevelMeta &M, const DevTop &top,
                                           const TopLds<T, VT> &L, int tile, int tid, int nt, const VT *lu, VT *z,
                                           const double *rhs, double sign, const int *laneOn) {
-    forwardOnly<T, VT>(P, M, tile, tid, nt, lu, z, rhs, sign, top.l0, laneOn);
+    forwardOnly<T, VT>(P, M, top, tile, tid, nt, lu, z, rhs, sign, top.l0, laneOn);
     if (top.n > 0) topForward<T, VT>(P, top, L, tile, tid, nt, lu, z, rhs, sign, laneOn);
 }
 template <int T, typename VT>
@@ -815,7 +861,7 @@ __device__ __forceinline__ void luBackward(const DevPlan &P, const LevelMeta &M,
                                            const TopLds<T, VT> &L, int tile, int tid, int nt, const VT *lu,
                                            const VT *z, VT *xo, const int *laneOn) {
     if (top.n > 0) topBackward<T, VT>(P, top, L, tile, tid, nt, xo, laneOn);
-    backward<T, VT>(P, M, tile, tid, nt, lu, z, xo, top.l0, laneOn);
+    backward<T, VT>(P, M, top, tile, tid, nt, lu, z, xo, top.l0, laneOn);
 }
 
 // Mismatch and Jacobian fill of the lanes flagged in fillLane; per-thread partial max|F| and finiteness of the
@@ -897,7 +943,7 @@ __global__ __launch_bounds__(MAX_THREADS) void nrStepKernel(DevPlan P, DevState 
     VT *lu = reinterpret_cast<VT *>(S.lu);
     VT *y = reinterpret_cast<VT *>(S.y);
     VT *dx1 = reinterpret_cast<VT *>(S.dx1);
-    const LevelMeta LM = loadLevelMeta(P, reinterpret_cast<unsigned char *>(olfhipTopLds), tid, nt);
+    const LevelMeta LM = loadLevelMeta(P, top, reinterpret_cast<unsigned char *>(olfhipTopLds), tid, nt);
     __shared__ int sRunning;
     for (int step = 0; step < prm.stepsPerLaunch; step++) {
     if (tid == 0) {
@@ -1349,7 +1395,7 @@ __global__ __launch_bounds__(MAX_THREADS) void luSolveKernel(DevPlan P, VT *lu, 
     const int tile = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
     if (tid < T) sClamp[tid] = 0;
     __syncthreads();
-    const LevelMeta LM = loadLevelMeta(P, reinterpret_cast<unsigned char *>(olfhipTopLds), tid, nt);
+    const LevelMeta LM = loadLevelMeta(P, top, reinterpret_cast<unsigned char *>(olfhipTopLds), tid, nt);
     luFactorForward<T, VT>(P, LM, top, TL, tile, tid, nt, lu, y, rhs, 1.0, sClamp, nullptr);
     luBackward<T, VT>(P, LM, top, TL, tile, tid, nt, lu, y, xo, nullptr);
     if (tid < T) clamps[tile * T + tid] = sClamp[tid];

@@ -67,8 +67,8 @@ __global__ void screenBranchesKernel(DevPlan P, DevState S, DevLimits L, const i
         const double *c = L.coef + 8 * (size_t)e;
         const double v1 = x1.y, v2 = x2.y;
         double s1, c1, s2, c2;
-        sincos(c[3] - x1.x + x2.x, &s1, &c1);
-        sincos(c[6] - x2.x + x1.x, &s2, &c2);
+        sincosReduced(c[3] - x1.x + x2.x, &s1, &c1);
+        sincosReduced(c[6] - x2.x + x1.x, &s2, &c2);
         const double bvv = c[1] * v1 * v2;
         const double p1 = c[0] * v1 * v1 - bvv * s1, q1 = c[2] * v1 * v1 - bvv * c1;
         const double p2 = c[4] * v2 * v2 - bvv * s2, q2 = c[5] * v2 * v2 - bvv * c2;
