@@ -113,6 +113,27 @@ int olfhip_create_batch_context(olfhip_structure *structure, int n_scenarios, in
  * (BatchSolveNative.java:51). */
 void olfhip_destroy_batch_context(olfhip_context *context);
 
+/* Distributed slack of a context: the participating buses (never the slack bus), their participation factors
+ * (1 - their sum is the slack's retained share), each one's P-row target at the anchor the water-fill restarts
+ * from and its room up (>= 0) and down (<= 0) from it, pu; the slack bus and its own P target. The reference takes
+ * these in createBatchContext (BatchSolveNative.java:44-47: distributedSlackBusPhiRow, distributedSlackBusFactor,
+ * slackBusPhiRow, slackBusTargetP, distributedSlackBusBaseTargetP / MaxDelta / MinDelta; factor mode
+ * PROPORTIONAL, no margins); the check is native/cuda/outerloop/distributed_slack.cu:131-195,215-600. */
+typedef struct {
+    int n_buses;
+    const int *bus;
+    const double *factor;
+    const double *base_target_p;
+    const double *max_delta;
+    const double *min_delta;
+    int slack_bus;
+    double slack_target_p;
+} olfhip_distributed_slack;
+
+/* Sets (or with NULL clears) the context's distributed slack; olfhip_solve_inputs.distributed_slack turns the
+ * loop on per solve. */
+int olfhip_set_distributed_slack(olfhip_context *context, const olfhip_distributed_slack *distributed_slack);
+
 /* Inputs of one batched solve: the hot-path subset of BatchSolveNative.solveBatchContext
  * (BatchSolveNative.java:106-125), same meaning and order conventions:
  *   x0_base, target_base, base_row_mode: [n] shared by every scenario (tiled on the device);
@@ -154,6 +175,13 @@ typedef struct {
     /* NULL or a DEVICE pointer to [n_scenarios][n] start states (e.g. olfhip_mlp_predict_warm_start's output);
      * exclusive with x0_per_scenario */
     const double *x0_per_scenario_device;
+    /* the distributed-slack outer loop (needs olfhip_set_distributed_slack), run before the reactive limits; its
+     * gate slackBusPMaxMismatch in pu (slackDistTol, BatchSolveNative.java:117); reject_slack_distribution_failure:
+     * a scenario whose distribution left more than 1e-5 pu unplaced ends REJECTED (the reference's
+     * rejectSlackDistributionFailure, slackDistributionFailureBehavior other than LEAVE_ON_SLACK_BUS) */
+    int distributed_slack;
+    double slack_dist_tol;
+    int reject_slack_distribution_failure;
 } olfhip_solve_inputs;
 
 /* Outputs (any may be NULL): results [n_scenarios][n+3] = (x, iterations of the last solve, ||F||inf, status) in

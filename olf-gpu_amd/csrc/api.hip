@@ -293,6 +293,11 @@ struct ContextImpl {
     DBuf x, F, dx, r, target, qinj, mode_, lu, y, dx1;
     DBuf it, status, solves, outer, total, clamps, norm, ratio, outages, tileRunning, count;
     DBuf rlType, rlCount, rlPass;
+    // distributed slack (olfhip_set_distributed_slack): participants in node numbering, per-lane loop sequence
+    int dsN = 0, slackNode = -1;
+    double slackTargetP = 0.0, dsFactorSum = 0.0;
+    DBuf dsNode, dsFactor, dsBase, dsMaxD, dsMinD, dsDesc;
+    DBuf loopAt, sweepChanged, dsFailed, slackP, dsRetain, dsPrev;
     // install inputs
     DBuf x0Base, x0Per, targetBase, modeBase, modeEntries, targetIdx, targetVal, seed, states;
     int lastInstalled = 0;
@@ -338,11 +343,23 @@ struct ContextImpl {
         S.rlCount = rlCount.as<signed char>();
         S.rlPass = rlPass.as<signed char>();
         S.tileRunning = tileRunning.as<int>();
+        S.loopAt = loopAt.as<signed char>();
+        S.sweepChanged = sweepChanged.as<signed char>();
+        S.dsFailed = dsFailed.as<signed char>();
+        S.slackP = slackP.as<double>();
+        S.dsRetain = dsRetain.as<double>();
+        S.dsPrev = dsPrev.as<double>();
         return S;
     }
     DevRL devRL() const {
-        return DevRL{nRL, rlNode.as<int>(), rlMinQ.as<double>(), rlMaxQ.as<double>(), rlLoadQ.as<double>(),
-                     rlTargetV.as<double>()};
+        return DevRL{nRL,
+                     rlNode.as<int>(),
+                     rlMinQ.as<double>(),
+                     rlMaxQ.as<double>(),
+                     rlLoadQ.as<double>(),
+                     rlTargetV.as<double>(),
+                     slackNode >= 0 ? dsDesc.as<DevDS>() : nullptr,
+                     slackNode};
     }
     ~ContextImpl() {
         if (pinnedCount) (void)hipHostFree(pinnedCount);
@@ -394,6 +411,10 @@ static ContextImpl *createContext(StructureImpl *st, int nScenarios, int mode, c
     c->outages.alloc(lanes * MAX_OUTAGES * 4);
     c->tileRunning.alloc((size_t)c->nTiles * 4);
     c->count.alloc(16);
+    for (DBuf *b : {&c->loopAt, &c->sweepChanged, &c->dsFailed}) b->alloc(lanes);
+    c->slackP.alloc(lanes * 8);
+    c->dsRetain.alloc(lanes * 8);
+    c->dsPrev.alloc(8);
     // reactive limits, entry buses in node numbering
     if (rl && rl->n_entries > 0) {
         if (!rl->voltage_row || !rl->min_q || !rl->max_q || !rl->load_q || !rl->target_v)
@@ -505,6 +526,12 @@ static void solve(ContextImpl *c, const olfhip_solve_inputs *in, olfhip_solve_ou
     prm.maxDphi = in->max_dphi;
     prm.refine = c->mode == OLFHIP_LU_FP32_REFINED;
     prm.batchOfOne = K == 1;
+    if (in->distributed_slack && c->slackNode < 0)
+        throw ArgumentError("distributed_slack is set but the context has no distributed slack (olfhip_set_distributed_slack)");
+    if (in->distributed_slack && !(in->slack_dist_tol >= 0.0)) throw ArgumentError("slack_dist_tol must be >= 0");
+    prm.distributedSlack = in->distributed_slack != 0;
+    prm.dsTol = in->slack_dist_tol;
+    prm.dsReject = in->reject_slack_distribution_failure != 0;
     const int maxSteps = (prm.maxIter + 1) * (prm.maxOuter + 1) + 1;
     // one launch carries every tile to its end (OLFHIP_STEPS_PER_LAUNCH=k: at most k Newton steps per launch)
     prm.stepsPerLaunch = maxSteps;
@@ -1097,6 +1124,56 @@ int olfhip_create_batch_context(olfhip_structure *structure, int n_scenarios, in
 }
 
 void olfhip_destroy_batch_context(olfhip_context *context) { delete reinterpret_cast<ContextImpl *>(context); }
+
+int olfhip_set_distributed_slack(olfhip_context *context, const olfhip_distributed_slack *ds) {
+    return guarded([&] {
+        ContextImpl *c = reinterpret_cast<ContextImpl *>(context);
+        if (!c) throw ArgumentError("context is null");
+        const Plan &P = c->st->plan;
+        if (!ds) {
+            c->dsN = 0;
+            c->slackNode = -1;
+            return;
+        }
+        if (ds->n_buses < 0) throw ArgumentError("n_buses must be >= 0");
+        if (ds->slack_bus < 0 || ds->slack_bus >= P.nb) throw ArgumentError("slack_bus out of range");
+        if (!std::isfinite(ds->slack_target_p)) throw ArgumentError("slack_target_p must be finite");
+        const int m = ds->n_buses;
+        if (m > 0 && (!ds->bus || !ds->factor || !ds->base_target_p || !ds->max_delta || !ds->min_delta))
+            throw ArgumentError("distributed-slack arrays are null");
+        std::vector<int> node(m);
+        std::vector<char> seen(P.nb, 0);
+        double sum = 0.0;
+        for (int i = 0; i < m; i++) {
+            const int b = ds->bus[i];
+            if (b < 0 || b >= P.nb) throw ArgumentError("participant bus out of range");
+            if (b == ds->slack_bus) throw ArgumentError("the slack bus cannot be a participant (its share is 1 - sum of factors)");
+            if (seen[b]) throw ArgumentError("participant bus listed twice");
+            seen[b] = 1;
+            if (!(ds->factor[i] >= 0.0) || !std::isfinite(ds->factor[i])) throw ArgumentError("factors must be finite and >= 0");
+            if (!(ds->max_delta[i] >= 0.0) || !(ds->min_delta[i] <= 0.0) || !std::isfinite(ds->base_target_p[i]))
+                throw ArgumentError("participant " + std::to_string(i) + ": need max_delta >= 0 >= min_delta and a finite anchor");
+            sum += ds->factor[i];
+            node[i] = P.pos[b];
+        }
+        if (sum > 1.0 + 1e-12) throw ArgumentError("participation factors sum above 1");
+        c->dsN = m;
+        c->slackNode = P.pos[ds->slack_bus];
+        c->slackTargetP = ds->slack_target_p;
+        c->dsFactorSum = sum;
+        c->dsNode.upload(node);
+        c->dsFactor.upload(std::vector<double>(ds->factor, ds->factor + m));
+        c->dsBase.upload(std::vector<double>(ds->base_target_p, ds->base_target_p + m));
+        c->dsMaxD.upload(std::vector<double>(ds->max_delta, ds->max_delta + m));
+        c->dsMinD.upload(std::vector<double>(ds->min_delta, ds->min_delta + m));
+        c->dsPrev.alloc((size_t)c->nTiles * c->T * std::max(m, 1) * 8);
+        DevDS d{m, c->dsNode.as<int>(), c->dsFactor.as<double>(), c->dsBase.as<double>(), c->dsMaxD.as<double>(),
+                c->dsMinD.as<double>(), c->slackTargetP, c->dsFactorSum};
+        c->dsDesc.alloc(sizeof(DevDS));
+        HIP_OK(hipMemcpy(c->dsDesc.p, &d, sizeof(DevDS), hipMemcpyHostToDevice));
+        HIP_OK(hipDeviceSynchronize());
+    });
+}
 
 int olfhip_solve_batch_context(olfhip_context *context, const olfhip_solve_inputs *inputs,
                                olfhip_solve_outputs *outputs) {

@@ -82,6 +82,12 @@ struct DevState {
     int maxOut;
     signed char *rlType, *rlCount, *rlPass;
     int *tileRunning;
+    // outer-loop sequence per lane: the loop it is at (0 distributed slack, 1 reactive limits) and whether the current
+    // sweep changed something; the slack bus's computed P at the lane's last fill; the slack's applied share (NaN
+    // before any distribution), whether the last distribution failed, and each participant's target before the
+    // last reset ([tile][participant][lane])
+    signed char *loopAt, *sweepChanged, *dsFailed;
+    double *slackP, *dsRetain, *dsPrev;
     // opt-in section timing of one tile: prof[0] counts the Newton steps that recorded, then PROF_SECTIONS stamps per
     // step (wall clock) from prof[PROF_SECTIONS] on, at most profCap steps
     unsigned long long *prof;
@@ -90,10 +96,22 @@ struct DevState {
 };
 constexpr int PROF_SECTIONS = 16;
 
+// Distributed slack: participating nodes, their factors, anchor targets and room up (>= 0) and down (<= 0) from the
+// anchor; the slack's own P target and the sum of the factors (1 - it is the slack's retained share). Kept in device
+// memory and read through one pointer, so that the kernel does not hold its fields in registers.
+struct DevDS {
+    int n;
+    const int *node;
+    const double *factor, *base, *maxD, *minD;
+    double slackTargetP, factorSum;
+};
+
 struct DevRL {
     int n;
     const int *node;
     const double *minQ, *maxQ, *loadQ, *targetV;
+    const DevDS *ds;   // null without distributed slack
+    int slackNode;     // -1 without distributed slack
 };
 
 struct NrParams {
@@ -108,6 +126,9 @@ struct NrParams {
     int refine;
     int batchOfOne;
     int stepsPerLaunch;   // Newton steps one launch may run per tile
+    int distributedSlack; // run the distributed-slack outer loop (before the reactive limits)
+    double dsTol;         // its gate, slackBusPMaxMismatch in pu
+    int dsReject;         // a failed distribution ends the scenario REJECTED
 };
 
 template <typename VT>
@@ -868,7 +889,7 @@ __device__ __forceinline__ void luBackward(const DevPlan &P, const LevelMeta &M,
 // thread's lane (tid % T) are returned.
 template <int T, typename VT>
 __device__ void fillPass(const DevPlan &P, const DevState &S, int tile, int tid, int nt, const int *fillLane,
-                         double &pmax, int &pfin) {
+                         double &pmax, int &pfin, int slackNode) {
     const int nb = P.nb;
     VT *lu = reinterpret_cast<VT *>(S.lu);
     const int t = tid % T;   // fixed per thread: the stride nt is a multiple of T
@@ -887,6 +908,7 @@ __device__ void fillPass(const DevPlan &P, const DevState &S, int tile, int tid,
         const double fQ = (mQ == MODE_FIXED) ? xk.y - tg.y : sQ - tg.y;
         *reinterpret_cast<double2 *>(S.F + vi) = make_double2(fP, fQ);
         S.qinj[qIdx<T>(tile, nb, k, t)] = sQ;
+        if (k == slackNode) S.slackP[tile * T + t] = sP;
         if (!isfinite(fP) || !isfinite(fQ)) pfin = 0;
         pmax = fmax(pmax, fmax(fabs(fP), fabs(fQ)));
     }
@@ -913,6 +935,163 @@ __device__ void applyJPass(const DevPlan &P, const DevState &S, int tile, int ti
     }
 }
 
+constexpr double P_RESIDUE_EPS = 1e-5;   // OLF ActivePowerDistribution.P_RESIDUE_EPS (outer_loop_driver.cu:204)
+
+// One distributed-slack check (DistributedSlackOuterLoop, the reference's per-bus water-fill) of the lanes flagged in
+// on[]; every thread of the block calls it. Per lane: the slack's retained share of the total from the anchor
+// (native/cuda/outerloop/distributed_slack.cu:131-195), the gate on the participants' share (:215-305), then, when
+// it fires, every participant reset to its anchor and the whole total water-filled from there, the factors
+// renormalised over the unsaturated participants each round (:395-420, :530-566), and OLF's verdict: unstable when
+// the targets and the slack's applied share moved by more than 0.9 P_RESIDUE_EPS, failed when more than
+// P_RESIDUE_EPS stayed unplaced (:568-600). A participant whose P row is not an equation (islanded) takes no part.
+// Sums over the participants are per-lane block reductions; participant i of lane t is always handled by the thread
+// with tid % T == t, which the strides keep.
+template <int T>
+__device__ void dsCheck(const DevPlan &P, const DevState &S, const DevRL &R, const NrParams &prm, int tile, int tid,
+                        int nt, const int *on, int *unstable, int *failed, double *red) {
+    __shared__ double sRem[T], sPrevRetain[T], sSum[T];
+    __shared__ int sFire[T], sGo[T];
+    const int nb = P.nb;
+    const int t = tid % T;
+    const DevDS D = *R.ds;
+    // (1) the active factors and what the participants hold above their anchors
+    double fa = 0.0, ab = 0.0;
+    if (on[t]) {
+        for (int w = tid; w < D.n * T; w += nt) {
+            const int i = w / T;
+            const size_t vi = vIdx<T>(tile, nb, D.node[i], t);
+            if (S.mode[vi] != MODE_EQUATION) continue;
+            fa += D.factor[i];
+            ab += S.target[vi] - D.base[i];
+        }
+    }
+    red[tid] = fa;
+    laneReduce<T, false>(red, tid, nt);
+    if (tid < T) sSum[tid] = red[tid];
+    __syncthreads();
+    red[tid] = ab;
+    laneReduce<T, false>(red, tid, nt);
+    // (2) the gate
+    if (tid < T) {
+        sFire[tid] = 0;
+        sGo[tid] = 0;
+        if (on[tid]) {
+            const int lane = tile * T + tid;
+            unstable[tid] = 0;
+            failed[tid] = 0;
+            const double fsActive = sSum[tid], absorbed = red[tid];
+            const double gate = fmax(prm.dsTol, P_RESIDUE_EPS);
+            const double m = S.slackP[lane] - D.slackTargetP;
+            const double fsAll = D.factorSum, rho = 1.0 - fsAll;
+            const double rhoEff = fsActive != fsAll ? (fsActive + rho > 0.0 ? rho / (fsActive + rho) : 0.0) : rho;
+            const double retain = rhoEff * (m + absorbed);
+            const double rem = m - retain;
+            const double fsGlobal =
+                fsActive == fsAll ? fsActive : (fsActive + rho > 0.0 ? fsActive / (fsActive + rho) : 0.0);
+            const bool noCapacity = !(fsActive > 0.0) && rho < 1e-12;
+            const double applied = S.dsRetain[lane];
+            if (fsGlobal <= 0.0) {
+                if (noCapacity) {
+                    if (fabs(m) > prm.dsTol && fabs(m) > P_RESIDUE_EPS) failed[tid] = 1;
+                } else {
+                    // only the slack participates: it takes the whole mismatch, and the loop re-solves when that
+                    // changed
+                    const double before = isnan(applied) ? 0.0 : applied;
+                    if (fabs(m - before) > gate) {
+                        S.dsRetain[lane] = m;
+                        unstable[tid] = 1;
+                    }
+                }
+            } else if (fabs(rem) > fsGlobal * gate) {
+                sFire[tid] = 1;
+                sGo[tid] = 1;
+                sPrevRetain[tid] = isnan(applied) ? 0.0 : applied;
+                S.dsRetain[lane] = retain;
+                sRem[tid] = fsGlobal * (m + absorbed);
+            }
+        }
+    }
+    __syncthreads();
+    // (3) reset to the anchors
+    if (sFire[t]) {
+        for (int w = tid; w < D.n * T; w += nt) {
+            const int i = w / T;
+            const size_t vi = vIdx<T>(tile, nb, D.node[i], t);
+            if (S.mode[vi] != MODE_EQUATION) continue;
+            S.dsPrev[((size_t)tile * D.n + i) * T + t] = S.target[vi];
+            S.target[vi] = D.base[i];
+        }
+    }
+    __syncthreads();
+    // (4) the water-fill, at most one round per participant
+    for (int iter = 0; iter < D.n; iter++) {
+        if (tid < T && sGo[tid] && !(fabs(sRem[tid]) > P_RESIDUE_EPS)) sGo[tid] = 0;
+        __syncthreads();
+        int any = 0;
+        for (int u = 0; u < T; u++) any |= sGo[u];
+        if (!any) break;
+        double fs = 0.0;
+        if (sGo[t]) {
+            const double rem = sRem[t];
+            for (int w = tid; w < D.n * T; w += nt) {
+                const int i = w / T;
+                const size_t vi = vIdx<T>(tile, nb, D.node[i], t);
+                if (S.mode[vi] != MODE_EQUATION) continue;
+                const double cum = S.target[vi] - D.base[i];
+                const bool sat = rem > 0 ? cum >= D.maxD[i] : cum <= D.minD[i];
+                if (!sat) fs += D.factor[i];
+            }
+        }
+        red[tid] = fs;
+        laneReduce<T, false>(red, tid, nt);
+        if (tid < T) {
+            sSum[tid] = red[tid];
+            if (sGo[tid] && !(red[tid] > 0.0)) sGo[tid] = 0;   // every participant saturated: the rest stays
+        }
+        __syncthreads();
+        double done = 0.0;
+        if (sGo[t]) {
+            const double rem = sRem[t], fsum = sSum[t];
+            for (int w = tid; w < D.n * T; w += nt) {
+                const int i = w / T;
+                const size_t vi = vIdx<T>(tile, nb, D.node[i], t);
+                if (S.mode[vi] != MODE_EQUATION) continue;
+                const double cum = S.target[vi] - D.base[i];
+                const bool sat = rem > 0 ? cum >= D.maxD[i] : cum <= D.minD[i];
+                if (sat) continue;
+                const double next = fmin(fmax(cum + rem * (D.factor[i] / fsum), D.minD[i]), D.maxD[i]);
+                if (next != cum) {
+                    S.target[vi] = D.base[i] + next;
+                    done += next - cum;
+                }
+            }
+        }
+        red[tid] = done;
+        laneReduce<T, false>(red, tid, nt);
+        if (tid < T && sGo[tid]) sRem[tid] -= red[tid];
+        __syncthreads();
+    }
+    // (5) the verdict
+    double mv = 0.0;
+    if (sFire[t]) {
+        for (int w = tid; w < D.n * T; w += nt) {
+            const int i = w / T;
+            const size_t vi = vIdx<T>(tile, nb, D.node[i], t);
+            if (S.mode[vi] != MODE_EQUATION) continue;
+            mv += fabs(S.target[vi] - S.dsPrev[((size_t)tile * D.n + i) * T + t]);
+        }
+    }
+    red[tid] = mv;
+    laneReduce<T, false>(red, tid, nt);
+    if (tid < T && sFire[tid]) {
+        const int lane = tile * T + tid;
+        const double moved = red[tid] + fabs(S.dsRetain[lane] - sPrevRetain[tid]);
+        unstable[tid] = moved > 0.9 * P_RESIDUE_EPS;
+        failed[tid] = fabs(sRem[tid]) > P_RESIDUE_EPS;
+    }
+    __syncthreads();
+}
+
 // Batched Newton steps for the tile of blockIdx.x: up to prm.stepsPerLaunch of them, ending as soon as no lane of the
 // tile runs, so that one launch carries every tile to its end and the hardware starts further tiles as earlier ones
 // finish (a tile in a long reactive-limit outer loop overlaps the others instead of holding launches of its own).
@@ -933,6 +1112,7 @@ __global__ __launch_bounds__(MAX_THREADS) void nrStepKernel(DevPlan P, DevState 
 
     __shared__ int sStatus[T], sIt[T], sSolves[T], sOuter[T], sTotal[T], sClamp[T];
     __shared__ int sFill[T], sSolve[T], sConv[T], sRl[T], sRemain[T], sSwitched[T], sFin[T];
+    __shared__ int sChk[T], sLoop[T], sSweep[T], sResolve[T], sReject[T], sDs[T], sDsUnst[T], sDsFail[T];
     __shared__ double sNorm[T], sRatio[T], sFactor[T];
     __shared__ int sFlag[4];
     __shared__ double red[MAX_THREADS];
@@ -944,6 +1124,8 @@ __global__ __launch_bounds__(MAX_THREADS) void nrStepKernel(DevPlan P, DevState 
     VT *y = reinterpret_cast<VT *>(S.y);
     VT *dx1 = reinterpret_cast<VT *>(S.dx1);
     const LevelMeta LM = loadLevelMeta(P, top, reinterpret_cast<unsigned char *>(olfhipTopLds), tid, nt);
+    const bool dsOn = prm.distributedSlack && R.slackNode >= 0;
+    const int slackNode = dsOn ? R.slackNode : -1;
     __shared__ int sRunning;
     for (int step = 0; step < prm.stepsPerLaunch; step++) {
     if (tid == 0) {
@@ -981,7 +1163,7 @@ __global__ __launch_bounds__(MAX_THREADS) void nrStepKernel(DevPlan P, DevState 
     {
         double pmax = 0.0;
         int pfin = 1;
-        fillPass<T, VT>(P, S, tile, tid, nt, sFill, pmax, pfin);
+        fillPass<T, VT>(P, S, tile, tid, nt, sFill, pmax, pfin, slackNode);
         red[tid] = pmax;
         redI[tid] = pfin;
         laneReduce<T, true>(red, tid, nt);
@@ -1012,11 +1194,54 @@ __global__ __launch_bounds__(MAX_THREADS) void nrStepKernel(DevPlan P, DevState 
     }
     __syncthreads();
 
-    // ---- 3. outer loop (reactive limits) of the lanes whose solve converged
+    // ---- 3. outer loops of the lanes whose solve converged: distributed slack, then reactive limits, each driven
+    // to STABLE with a re-solve after every unstable check, the sweep restarted from the first loop while it changed
+    // something (OLF's AcloadFlowEngine.runOuterLoops as the reference's driver runs it,
+    // native/cuda/solver/outer_loop_driver.cu:300-363), at most maxOuter re-solves. Without a re-solve a lane's
+    // checks are at most: distributed slack, reactive limits, and once more both after a sweep that changed
+    // something, so two rounds decide every lane.
     if (sFlag[0]) {
         if (tid < T) {
-            sRl[tid] = sConv[tid] && prm.reactiveLimits && R.n > 0 && sOuter[tid] < prm.maxOuter;
+            const int lane = tile * T + tid;
+            sChk[tid] = sConv[tid];
+            sLoop[tid] = S.loopAt[lane];
+            sSweep[tid] = S.sweepChanged[lane];
+            sResolve[tid] = 0;
+            sReject[tid] = 0;
             sFill[tid] = 0;
+        }
+        __syncthreads();
+        for (int round = 0; round < 2; round++) {
+        if (tid < T) {
+            if (sChk[tid] && sOuter[tid] >= prm.maxOuter) sChk[tid] = 0;   // budget spent: the lane ends here
+            sDs[tid] = sChk[tid] && sLoop[tid] == 0 && dsOn;
+        }
+        __syncthreads();
+        {
+            int anyDs = 0;
+            for (int u = 0; u < T; u++) anyDs |= sDs[u];
+            if (anyDs) dsCheck<T>(P, S, R, prm, tile, tid, nt, sDs, sDsUnst, sDsFail, red);
+        }
+        if (tid < T && sChk[tid] && sLoop[tid] == 0) {
+            if (sDs[tid]) {
+                S.dsFailed[tile * T + tid] = (signed char)sDsFail[tid];
+                if (sDsFail[tid] && prm.dsReject) {
+                    sReject[tid] = 1;
+                    sChk[tid] = 0;
+                } else if (sDsUnst[tid]) {
+                    sResolve[tid] = 1;
+                    sChk[tid] = 0;
+                } else {
+                    sLoop[tid] = 1;
+                }
+            } else {
+                sLoop[tid] = 1;
+            }
+        }
+        if (tid < T) {
+            sRl[tid] = sChk[tid] && sLoop[tid] == 1 && prm.reactiveLimits && R.n > 0;
+            sRemain[tid] = 0;
+            sSwitched[tid] = 0;
         }
         __syncthreads();
         for (int w = tid; w < R.n * T; w += nt) {
@@ -1088,15 +1313,41 @@ __global__ __launch_bounds__(MAX_THREADS) void nrStepKernel(DevPlan P, DevState 
             }
         }
         __syncthreads();
+        if (tid < T && sChk[tid]) {
+            if (sLoop[tid] == 1) {
+                if (sRl[tid] && sSwitched[tid]) {
+                    sResolve[tid] = 1;
+                    sChk[tid] = 0;
+                } else {
+                    sLoop[tid] = 2;
+                }
+            }
+            if (sChk[tid] && sLoop[tid] == 2) {   // end of the sweep
+                if (sSweep[tid] && dsOn) {
+                    sSweep[tid] = 0;
+                    sLoop[tid] = 0;
+                } else {
+                    sChk[tid] = 0;
+                }
+            }
+        }
+        __syncthreads();
+        }
         if (tid < T && sConv[tid]) {
+            const int lane = tile * T + tid;
             sSolves[tid]++;
-            if (sRl[tid] && sSwitched[tid]) {
+            if (sResolve[tid]) {
                 sOuter[tid]++;
                 sIt[tid] = 0;
                 sFill[tid] = 1;     // restarted at its new modes and targets: refill, no test at iteration 0
+                sSweep[tid] = 1;
+            } else if (sReject[tid]) {
+                sStatus[tid] = ST_REJECTED;
             } else {
                 sStatus[tid] = ST_CONVERGED;
             }
+            S.loopAt[lane] = (signed char)sLoop[tid];
+            S.sweepChanged[lane] = (signed char)sSweep[tid];
         }
         if (tid == 0) {
             int any = 0;
@@ -1107,7 +1358,7 @@ __global__ __launch_bounds__(MAX_THREADS) void nrStepKernel(DevPlan P, DevState 
         if (sFlag[1]) {
             double pmax = 0.0;
             int pfin = 1;
-            fillPass<T, VT>(P, S, tile, tid, nt, sFill, pmax, pfin);
+            fillPass<T, VT>(P, S, tile, tid, nt, sFill, pmax, pfin, slackNode);
             red[tid] = pmax;
             redI[tid] = pfin;
             laneReduce<T, true>(red, tid, nt);
@@ -1307,6 +1558,11 @@ __global__ void laneInitKernel(DevPlan P, DevState S, DevRL R, const signed char
             S.clamps[lane] = 0;
             S.norm[lane] = 0.0;
             S.ratio[lane] = 0.0;
+            S.loopAt[lane] = 0;
+            S.sweepChanged[lane] = 0;
+            S.dsFailed[lane] = 0;
+            S.dsRetain[lane] = __builtin_nan("");
+            S.slackP[lane] = 0.0;
             if (lane % T == 0) S.tileRunning[lane / T] = lane < S.nInstalled;
         } else {
             const size_t q = w - lanes;

@@ -29,6 +29,7 @@ EXPORTS = [
     "olfhip_mlp_set_base_states", "olfhip_mlp_train_step", "olfhip_mlp_forward", "olfhip_mlp_gradients",
     "olfhip_mlp_parameters", "olfhip_mlp_predict_warm_start", "olfhip_mlp_gemm_test",
     "olfhip_create_limit_table", "olfhip_destroy_limit_table", "olfhip_screen_violations", "olfhip_detect_violations",
+    "olfhip_set_distributed_slack",
 ]
 
 
@@ -58,7 +59,15 @@ class SolveInputs(C.Structure):
                 ("max_iter", C.c_int), ("tol", C.c_double), ("reactive_limit_tol", C.c_double),
                 ("state_vector_scaling_mode", C.c_int), ("max_dv", C.c_double), ("max_dphi", C.c_double),
                 ("max_outer_iterations", C.c_int), ("max_pq_pv_switch", C.c_int), ("skip_reactive_limits", C.c_int),
-                ("states_to_host", C.c_int), ("x0_per_scenario_device", C.c_void_p)]
+                ("states_to_host", C.c_int), ("x0_per_scenario_device", C.c_void_p),
+                ("distributed_slack", C.c_int), ("slack_dist_tol", C.c_double),
+                ("reject_slack_distribution_failure", C.c_int)]
+
+
+class DistributedSlackDesc(C.Structure):
+    _fields_ = [("n_buses", C.c_int), ("bus", C.c_void_p), ("factor", C.c_void_p), ("base_target_p", C.c_void_p),
+                ("max_delta", C.c_void_p), ("min_delta", C.c_void_p), ("slack_bus", C.c_int),
+                ("slack_target_p", C.c_double)]
 
 
 class SolveOutputs(C.Structure):
@@ -108,6 +117,7 @@ def lib():
         L.olfhip_solve_batch_context.argtypes = [vp, C.POINTER(SolveInputs), C.POINTER(SolveOutputs)]
         L.olfhip_read_states.argtypes = [vp, i, vp]
         L.olfhip_context_timing.argtypes = [vp, C.POINTER(d), C.POINTER(i)]
+        L.olfhip_set_distributed_slack.argtypes = [vp, C.c_void_p]
         L.olfhip_read_outer_loop_state.argtypes = [vp, i, vp, vp, vp]
         L.olfhip_islanded_buses.argtypes = [vp, i, i, vp, vp, C.POINTER(i)]
         L.olfhip_lu_factorize_and_solve.argtypes = [vp, i, i, vp, vp, i, vp, vp, vp]
@@ -263,7 +273,8 @@ class BatchContext:
                     outaged_branches=None, x0_per_scenario=None, reactive_limits_seed_state=None, max_iter=15,
                     tol=1e-4, reactive_limit_tol=1e-4, state_vector_scaling_mode=SCALING_NONE, max_dv=0.1,
                     max_dphi=0.174533, max_outer_iterations=20, max_pq_pv_switch=3, skip_reactive_limits=False,
-                    states_to_host=True, x0_per_scenario_device=None):
+                    states_to_host=True, x0_per_scenario_device=None, distributed_slack=False, slack_dist_tol=0.0,
+                    reject_slack_distribution_failure=False):
         keep = []
 
         def arr(a, dt):
@@ -296,8 +307,23 @@ class BatchContext:
         si = SolveInputs(n_scenarios, ptr(x0), ptr(tg), ptr(md), 0 if rme is None else rme.shape[0], ptr(rme),
                          0 if ti is None else ti.size, ptr(ti), ptr(tv), mo, ptr(ob), ptr(xp), ptr(sd), max_iter, tol,
                          reactive_limit_tol, state_vector_scaling_mode, max_dv, max_dphi, max_outer_iterations,
-                         max_pq_pv_switch, int(skip_reactive_limits), int(states_to_host), x0_per_scenario_device)
+                         max_pq_pv_switch, int(skip_reactive_limits), int(states_to_host), x0_per_scenario_device,
+                         int(distributed_slack), float(slack_dist_tol), int(reject_slack_distribution_failure))
         return si, keep
+
+    def set_distributed_slack(self, ds):
+        """olfhip_set_distributed_slack: ds is a dict (bus, factor, base_target_p, max_delta, min_delta, slack_bus,
+        slack_target_p; pu), or None to clear."""
+        if ds is None:
+            check(lib().olfhip_set_distributed_slack(self.h, None))
+            self._ds_keep = None
+            return
+        keep = [np.ascontiguousarray(ds["bus"], np.int32)] + [
+            np.ascontiguousarray(ds[k], np.float64) for k in ("factor", "base_target_p", "max_delta", "min_delta")]
+        d = DistributedSlackDesc(len(keep[0]), *(ptr(a) for a in keep), int(ds["slack_bus"]),
+                                 float(ds["slack_target_p"]))
+        check(lib().olfhip_set_distributed_slack(self.h, C.byref(d)))
+        self._ds_keep = keep
 
     def solve(self, inputs: SolveInputs, want_states=True):
         k = inputs.n_scenarios

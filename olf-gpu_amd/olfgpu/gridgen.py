@@ -375,3 +375,22 @@ def limit_table(net: Network, x_base: np.ndarray, seed: int = 7) -> dict:
                 group_side=np.array(group_side, np.int32), group_quantity=np.array(group_qty, np.int32),
                 limit_start=np.array(limit_start, np.int32), threshold=np.array(threshold, np.float64),
                 bus_high=high, bus_low=low)
+
+
+def distributed_slack(net: Network, target: np.ndarray | None = None, headroom: float = 1.25) -> dict:
+    """Distributed slack over the PV generators, OLF's default balance type PROPORTIONAL_TO_GENERATION_P_MAX: each
+    generator's factor is its Pmax over the Pmax of every generator, the slack's included, so the slack keeps its own
+    share (1 - the participants' sum). The workload definition gives no active limits: Pmax = headroom x the
+    generator's dispatch, Pmin = 0. The anchor is the participants' P-row targets in `target` (default: the
+    dispatch of `net.system()`), the slack's own P target its dispatch minus its load; pu on SB."""
+    if target is None:
+        target = net.system()[1]
+    gens = np.nonzero(net.bus_type != PQ)[0]
+    pv = np.nonzero(net.bus_type == PV)[0].astype(np.int32)
+    pmax = headroom * net.pgen
+    base = np.asarray(target, dtype=np.float64)[2 * pv].copy()
+    dispatch = base * SB + net.pload[pv]
+    slack = int(np.nonzero(net.bus_type == SLACK)[0][0])
+    return dict(bus=pv, factor=pmax[pv] / pmax[gens].sum(), base_target_p=base,
+                max_delta=(pmax[pv] - dispatch) / SB, min_delta=(0.0 - dispatch) / SB, slack_bus=slack,
+                slack_target_p=(net.pgen[slack] - net.pload[slack]) / SB, base_mva=SB)

@@ -102,6 +102,22 @@ class NewtonRaphsonParameters:
     maxReactivePowerMismatch: float = 1e-4    # pu
     maxOuterLoopIterations: int = 20
     maxPqPvSwitch: int = 3
+    # distributed slack (OLF LoadFlowParameters.distributedSlack, OpenLoadFlowParameters.slackBusPMaxMismatch in MW,
+    # slackDistributionFailureBehavior); it runs when a solver is given a distributed-slack description
+    # (gridgen.distributed_slack) and distributedSlack is set
+    distributedSlack: bool = True
+    slackBusPMaxMismatch: float = 1.0
+    slackDistributionFailureBehavior: str = "LEAVE_ON_SLACK_BUS"
+
+    def slack_inputs(self, ds: dict | None) -> dict:
+        """make_inputs keywords of the distributed-slack loop for a description ds (None: off)."""
+        if ds is None or not self.distributedSlack:
+            return dict(distributed_slack=False)
+        if self.slackDistributionFailureBehavior not in ("LEAVE_ON_SLACK_BUS", "FAIL"):
+            raise ValueError(f"slackDistributionFailureBehavior {self.slackDistributionFailureBehavior!r} is not "
+                             "modelled (LEAVE_ON_SLACK_BUS or FAIL)")
+        return dict(distributed_slack=True, slack_dist_tol=self.slackBusPMaxMismatch / float(ds["base_mva"]),
+                    reject_slack_distribution_failure=self.slackDistributionFailureBehavior == "FAIL")
 
     def scaling_code(self) -> int:
         if self.stateVectorScalingMode == "NONE":
@@ -128,12 +144,15 @@ class GpuNewtonRaphson:
 
     def __init__(self, structure: N.Structure, reactive_limits: dict | None = None,
                  gpu_parameters: GpuLoadFlowParameters | None = None,
-                 nr_parameters: NewtonRaphsonParameters | None = None):
+                 nr_parameters: NewtonRaphsonParameters | None = None, distributed_slack: dict | None = None):
         self.structure = structure
         self.gpu_parameters = gpu_parameters or GpuLoadFlowParameters()
         self.nr = nr_parameters or NewtonRaphsonParameters()
         self.reactive_limits = reactive_limits
+        self.distributed_slack = distributed_slack
         self.context = N.BatchContext(structure, 1, self.gpu_parameters.lu_mode, reactive_limits)
+        if distributed_slack is not None:
+            self.context.set_distributed_slack(distributed_slack)
 
     def run(self, x0, target, row_mode, seed_limit_types=None) -> AcSolverResult:
         nr = self.nr
@@ -143,7 +162,7 @@ class GpuNewtonRaphson:
             reactive_limit_tol=nr.maxReactivePowerMismatch, state_vector_scaling_mode=nr.scaling_code(),
             max_dv=nr.maxVoltageChangeV, max_dphi=nr.maxVoltageChangePhi,
             max_outer_iterations=nr.maxOuterLoopIterations, max_pq_pv_switch=nr.maxPqPvSwitch,
-            skip_reactive_limits=not nr.reactiveLimits)
+            skip_reactive_limits=not nr.reactiveLimits, **nr.slack_inputs(self.distributed_slack))
         r = self.context.solve(inp)
         n_rl = 0 if self.reactive_limits is None else len(self.reactive_limits["voltage_row"])
         m, t, lt = self.context.read_outer_loop_state(1, n_rl)
@@ -174,8 +193,9 @@ class GpuBatchedSecurityAnalysisSolver:
     def __init__(self, structure: N.Structure, reactive_limits: dict | None = None,
                  gpu_parameters: GpuLoadFlowParameters | None = None,
                  nr_parameters: NewtonRaphsonParameters | None = None, slack_bus: int = 0, max_chunk: int = 4096,
-                 lanes_per_tile: int = 0, threads_per_tile: int = 0):
+                 lanes_per_tile: int = 0, threads_per_tile: int = 0, distributed_slack: dict | None = None):
         self.structure = structure
+        self.distributed_slack = distributed_slack
         self.reactive_limits = reactive_limits
         self.gpu_parameters = gpu_parameters or GpuLoadFlowParameters()
         self.nr = nr_parameters or NewtonRaphsonParameters()
@@ -210,6 +230,8 @@ class GpuBatchedSecurityAnalysisSolver:
                 self._ctx.close()
             self._ctx = N.BatchContext(self.structure, chunk, self.gpu_parameters.lu_mode, self.reactive_limits,
                                        self.lanes_per_tile, self.threads_per_tile)
+            if self.distributed_slack is not None:
+                self._ctx.set_distributed_slack(self.distributed_slack)
         return self._ctx
 
     def solve_batch_n1(self, x_base, target_base, row_mode_base, limit_types_base, contingencies,
@@ -253,7 +275,8 @@ class GpuBatchedSecurityAnalysisSolver:
                 state_vector_scaling_mode=nr.scaling_code(), max_dv=nr.maxVoltageChangeV,
                 max_dphi=nr.maxVoltageChangePhi, max_outer_iterations=nr.maxOuterLoopIterations,
                 max_pq_pv_switch=nr.maxPqPvSwitch, skip_reactive_limits=not nr.reactiveLimits,
-                states_to_host=states_to_host, x0_per_scenario_device=x0_dev)
+                states_to_host=states_to_host, x0_per_scenario_device=x0_dev,
+                **nr.slack_inputs(self.distributed_slack))
             ctx = self.context(len(part))
             t0 = time.perf_counter()
             r = ctx.solve(inp, want_states=states_to_host)

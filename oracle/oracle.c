@@ -620,7 +620,107 @@ static int rl_check(const oracle_rl *rl, const oracle_nr_params *prm, const unsi
     return 0;
 }
 
-void oracle_nr_solve(const oracle_sys *s, const oracle_nr_params *prm, const oracle_rl *rl,
+double oracle_bus_p(const oracle_sys *s, const double *x, const int *out_br, int n_out, int bus)
+{
+    double p = 0.0, o[28];
+    for (int e = 0; e < s->n_branch; e++) {
+        const int a = s->bus1[e], b = s->bus2[e];
+        if (a != bus && b != bus) continue;
+        if (n_out > 0 && is_out(e, out_br, n_out)) continue;
+        oracle_closed_branch(s->p8 + 8 * e, x[2 * a + 1], x[2 * a], x[2 * b + 1], x[2 * b], o);
+        p += a == bus ? o[0] : o[14];
+    }
+    return p;
+}
+
+#define P_RESIDUE_EPS 1e-5 /* OLF ActivePowerDistribution.P_RESIDUE_EPS (outer_loop_driver.cu:204) */
+
+/* One DistributedSlackOuterLoop check, the reference's aggregate (per-bus) path: the slack's retained share
+ * (distributed_slack.cu:131-195, computeSlackRetain), the gate on the participants' share (:215-305), the reset of
+ * every participant to its anchor and the water-fill of the whole total from there, factors renormalised over the
+ * unsaturated participants each round (:395-420, 530-566), and OLF's verdict: UNSTABLE when the targets (and the
+ * slack's applied share) moved by more than 0.9 P_RESIDUE_EPS (:568-600). A participant whose P row is not an
+ * equation (islanded) takes no part. *retain_applied is NaN before the first distribution. Returns 1 when unstable;
+ * *failed when more than P_RESIDUE_EPS stayed unplaced. */
+static int ds_check(const oracle_ds *ds, double p_slack, const int *mode, double *target, double *retain_applied,
+                    double *prev, int *failed)
+{
+    const double gate = fmax(ds->tol, P_RESIDUE_EPS);
+    const double m = p_slack - ds->slack_target_p;
+    double fs_all = 0.0, fs_active = 0.0, absorbed = 0.0;
+    for (int i = 0; i < ds->n; i++) {
+        fs_all += ds->factor[i];
+        const int row = 2 * ds->bus[i];
+        if (mode[row] != ORC_EQUATION) continue;
+        fs_active += ds->factor[i];
+        absorbed += target[row] - ds->base_target_p[i];
+    }
+    const double rho = 1.0 - fs_all;
+    const double rho_eff = fs_active != fs_all ? (fs_active + rho > 0 ? rho / (fs_active + rho) : 0.0) : rho;
+    const double retain = rho_eff * (m + absorbed);
+    double rem = m - retain;
+    const double fs_global = fs_active == fs_all ? fs_active : (fs_active + rho > 0 ? fs_active / (fs_active + rho) : 0.0);
+    const int no_capacity = !(fs_active > 0.0) && rho < 1e-12;
+    *failed = 0;
+    if (fs_global <= 0.0) {
+        if (no_capacity) {
+            if (fabs(m) > ds->tol && fabs(m) > P_RESIDUE_EPS) *failed = 1;
+            return 0;
+        }
+        /* only the slack participates: it takes the whole mismatch, and the loop re-solves when that changed */
+        const double before = isnan(*retain_applied) ? 0.0 : *retain_applied;
+        if (fabs(m - before) > gate) {
+            *retain_applied = m;
+            return 1;
+        }
+        return 0;
+    }
+    if (fabs(rem) <= fs_global * gate) return 0;
+    const double prev_retain = isnan(*retain_applied) ? 0.0 : *retain_applied;
+    *retain_applied = retain;
+    for (int i = 0; i < ds->n; i++) {
+        const int row = 2 * ds->bus[i];
+        if (mode[row] != ORC_EQUATION) continue;
+        prev[i] = target[row];
+        target[row] = ds->base_target_p[i];
+    }
+    rem = fs_global * (m + absorbed);
+    for (int iter = 0; iter < ds->n && fabs(rem) > P_RESIDUE_EPS; iter++) {
+        double fsum = 0.0;
+        for (int i = 0; i < ds->n; i++) {
+            const int row = 2 * ds->bus[i];
+            if (mode[row] != ORC_EQUATION) continue;
+            const double cum = target[row] - ds->base_target_p[i];
+            const int sat = rem > 0 ? cum >= ds->max_delta[i] : cum <= ds->min_delta[i];
+            if (!sat) fsum += ds->factor[i];
+        }
+        if (fsum <= 0.0) break;
+        double done = 0.0;
+        for (int i = 0; i < ds->n; i++) {
+            const int row = 2 * ds->bus[i];
+            if (mode[row] != ORC_EQUATION) continue;
+            const double cum = target[row] - ds->base_target_p[i];
+            const int sat = rem > 0 ? cum >= ds->max_delta[i] : cum <= ds->min_delta[i];
+            if (sat) continue;
+            const double next = fmin(fmax(cum + rem * (ds->factor[i] / fsum), ds->min_delta[i]), ds->max_delta[i]);
+            if (next != cum) {
+                target[row] = ds->base_target_p[i] + next;
+                done += next - cum;
+            }
+        }
+        rem -= done;
+    }
+    double moved = fabs(*retain_applied - prev_retain);
+    for (int i = 0; i < ds->n; i++) {
+        const int row = 2 * ds->bus[i];
+        if (mode[row] != ORC_EQUATION) continue;
+        moved += fabs(target[row] - prev[i]);
+    }
+    if (fabs(rem) > P_RESIDUE_EPS) *failed = 1;
+    return moved > 0.9 * P_RESIDUE_EPS;
+}
+
+void oracle_nr_solve(const oracle_sys *s, const oracle_nr_params *prm, const oracle_rl *rl, const oracle_ds *ds,
                      const int *out_br, int n_out, const unsigned char *islanded,
                      double *x, int *mode, double *target, signed char *lt, signed char *sc, oracle_result *res)
 {
@@ -632,8 +732,14 @@ void oracle_nr_solve(const oracle_sys *s, const oracle_nr_params *prm, const ora
     double *J = (double *)malloc(sizeof(double) * (size_t)s->nnz);
     double *qinj = (double *)malloc(sizeof(double) * (size_t)s->n_bus);
     signed char *pass = (signed char *)malloc((size_t)(rl && rl->n > 0 ? rl->n : 1));
+    double *prev = (double *)malloc(sizeof(double) * (size_t)(ds && ds->n > 0 ? ds->n : 1));
     memset(res, 0, sizeof(*res));
+    const int ds_on = ds != NULL;
+    const int rl_on = rl && rl->n > 0 && prm->reactive_limits;
+    double retain_applied = NAN;
     int outer = 0;
+    int loop = 0;             /* 0 distributed slack, 1 reactive limits, 2 end of the sweep */
+    int sweep_changed = 0;
     for (;;) {
         int it = 0;
         double last_ratio = 0.0;
@@ -691,17 +797,43 @@ void oracle_nr_solve(const oracle_sys *s, const oracle_nr_params *prm, const ora
         }
         if (ended) break;
         res->solves++;
-        if (converged && rl && rl->n > 0 && prm->reactive_limits && outer < prm->max_outer) {
-            if (rl_check(rl, prm, islanded, x, qinj, mode, target, lt, sc, pass)) {
-                outer++;
-                res->outer_switches = outer;
-                continue;
+        (void)converged;
+        /* the outer loops, in order, on the converged state */
+        int resolve = 0, rejected = 0;
+        while (outer < prm->max_outer) {
+            if (loop == 0) {
+                if (ds_on) {
+                    int failed = 0;
+                    const double ps = oracle_bus_p(s, x, out_br, n_out, ds->slack_bus);
+                    const int unstable = ds_check(ds, ps, mode, target, &retain_applied, prev, &failed);
+                    res->slack_distribution_failed = failed;
+                    if (failed && ds->reject_failure) { rejected = 1; break; }
+                    if (unstable) { resolve = 1; break; }
+                }
+                loop = 1;
+            } else if (loop == 1) {
+                if (rl_on && rl_check(rl, prm, islanded, x, qinj, mode, target, lt, sc, pass)) {
+                    resolve = 1;
+                    break;
+                }
+                loop = 2;
+            } else {
+                if (!sweep_changed || !ds_on) break;
+                sweep_changed = 0;
+                loop = 0;
             }
+        }
+        if (rejected) { res->status = ORC_REJECTED; break; }
+        if (resolve) {
+            outer++;
+            sweep_changed = 1;
+            res->outer_switches = outer;
+            continue;
         }
         res->status = ORC_CONVERGED;
         break;
     }
-    free(F); free(b); free(dx); free(r); free(J); free(qinj); free(pass);
+    free(F); free(b); free(dx); free(r); free(J); free(qinj); free(pass); free(prev);
 }
 
 int oracle_islands(const oracle_sys *s, int slack_bus, const int *out_br, int n_out, unsigned char *islanded)
@@ -733,10 +865,11 @@ int oracle_islands(const oracle_sys *s, int slack_bus, const int *out_br, int n_
     return nb - tail;
 }
 
-int oracle_n1(const oracle_sys *s, const oracle_nr_params *prm, const oracle_rl *rl, int slack_bus,
-              const double *x_base, const int *mode_base, const double *target_base,
+int oracle_n1(const oracle_sys *s, const oracle_nr_params *prm, const oracle_rl *rl, const oracle_ds *ds,
+              int slack_bus, const double *x_base, const int *mode_base, const double *target_base,
               const signed char *limit_type_base, int n_ctg, int max_out, const int *ctg,
-              int n_threads, double *x_out, int *status, int *iterations, int *total_iterations, double *norm)
+              int n_threads, double *x_out, int *status, int *iterations, int *total_iterations, double *norm,
+              double *target_out)
 {
     const int n = s->n;
     const int nrl = rl ? rl->n : 0;
@@ -784,7 +917,8 @@ int oracle_n1(const oracle_sys *s, const oracle_nr_params *prm, const oracle_rl 
             oracle_result res;
             oracle_nr_params p = *prm;
             p.batch_of_one = 0;
-            oracle_nr_solve(s, &p, rl, outs, n_out, ni > 0 ? isl : NULL, x, mode, target, lt, sc, &res);
+            oracle_nr_solve(s, &p, rl, ds, outs, n_out, ni > 0 ? isl : NULL, x, mode, target, lt, sc, &res);
+            if (target_out) memcpy(target_out + (size_t)c * n, target, sizeof(double) * (size_t)n);
             status[c] = res.status;
             iterations[c] = res.iterations;
             total_iterations[c] = res.total_iterations;

@@ -77,16 +77,39 @@ typedef struct {
 typedef struct {
     int status, iterations, total_iterations, solves, outer_switches;
     double norm;
+    int slack_distribution_failed;   /* the last distributed-slack check left more than P_RESIDUE_EPS unplaced */
 } oracle_result;
 
-/* One AC load flow with the reactive-limit outer loop, in place on x/mode/target/limit_type/switch_count.
- * islanded[bus] (may be NULL) marks buses whose entries leave the outer loop.
+/* Distributed slack (the reference's per-bus water-fill, outerloop/distributed_slack.cu:131-195,215-305,530-600,
+ * OLF's DistributedSlackOuterLoop + ActivePowerDistribution): participating buses (never the slack bus) with
+ * factors (1 - their sum is the slack's retained share), their P-row targets at the anchor and their room up
+ * (>= 0) and down (<= 0) from it; the slack bus and its own P target; the gate (slackBusPMaxMismatch, pu) and
+ * whether a failed distribution rejects the case (slackDistributionFailureBehavior != LEAVE_ON_SLACK_BUS). */
+typedef struct {
+    int n;
+    const int *bus;
+    const double *factor, *base_target_p, *max_delta, *min_delta;
+    int slack_bus;
+    double slack_target_p;
+    double tol;
+    int reject_failure;
+} oracle_ds;
+
+/* One AC load flow with its outer loops, in place on x/mode/target/limit_type/switch_count: distributed slack (ds,
+ * may be NULL) then the reactive limits, each driven to STABLE with a re-solve after every unstable check, the
+ * sweep restarted from the first loop while it changed something (OLF's AcloadFlowEngine.runOuterLoops as the
+ * reference's driver runs it, solver/outer_loop_driver.cu:300-363), at most max_outer re-solves.
+ * islanded[bus] (may be NULL) marks buses whose entries leave the outer loops.
  * Follows solver/newton_iteration.cu:93-203,276-397,580-660 and outerloop/reactive_limits.cu:321-371.
- * Pinned by tests/test_oracle_nr.py (two-bus closed form, Y-bus mismatch, power balance). */
-void oracle_nr_solve(const oracle_sys *s, const oracle_nr_params *prm, const oracle_rl *rl,
+ * Pinned by tests/test_oracle_nr.py (two-bus closed form, Y-bus mismatch, power balance) and
+ * tests/test_oracle_distributed_slack.py (lossless closed form, water-fill invariants). */
+void oracle_nr_solve(const oracle_sys *s, const oracle_nr_params *prm, const oracle_rl *rl, const oracle_ds *ds,
                      const int *out_br, int n_out, const unsigned char *islanded,
                      double *x, int *mode, double *target, signed char *limit_type,
                      signed char *switch_count, oracle_result *res);
+
+/* The slack bus's computed active injection (sum of its in-service branches' p at its side), pu. */
+double oracle_bus_p(const oracle_sys *s, const double *x, const int *out_br, int n_out, int bus);
 
 /* Buses cut off from the slack's component once the listed branches are out (1 = islanded).
  * Reference batch/plan/IslandedBuses.java:50-60. Returns the islanded count. Pinned by
@@ -95,12 +118,14 @@ int oracle_islands(const oracle_sys *s, int slack_bus, const int *out_br, int n_
 
 /* Batched N-1 on the CPU: every contingency starts from the base state/modes/targets/limit types,
  * islanded buses are frozen at 1<0 (identity rows), OpenMP over contingencies with n_threads threads.
- * ctg is n_ctg x max_out branch ids, -1 padded. Outputs per contingency. Returns the thread count used. */
-int oracle_n1(const oracle_sys *s, const oracle_nr_params *prm, const oracle_rl *rl, int slack_bus,
-              const double *x_base, const int *mode_base, const double *target_base,
+ * ctg is n_ctg x max_out branch ids, -1 padded. Outputs per contingency (target_out, may be NULL: the final
+ * targets, n per contingency). A case whose distribution failed under ds->reject_failure ends REJECTED.
+ * Returns the thread count used. */
+int oracle_n1(const oracle_sys *s, const oracle_nr_params *prm, const oracle_rl *rl, const oracle_ds *ds,
+              int slack_bus, const double *x_base, const int *mode_base, const double *target_base,
               const signed char *limit_type_base, int n_ctg, int max_out, const int *ctg,
               int n_threads, double *x_out, int *status, int *iterations, int *total_iterations,
-              double *norm);
+              double *norm, double *target_out);
 
 /* Limit table (the reference's GpuLimitTable, flattened): per monitored closed branch its groups (side 1|2,
  * quantity 0 current |S|/V, 1 active power, 2 apparent power) with thresholds most severe first; per bus a high

@@ -45,7 +45,22 @@ class Violation(C.Structure):
 
 class Result(C.Structure):
     _fields_ = [("status", C.c_int), ("iterations", C.c_int), ("total_iterations", C.c_int), ("solves", C.c_int),
-                ("outer_switches", C.c_int), ("norm", C.c_double)]
+                ("outer_switches", C.c_int), ("norm", C.c_double), ("slack_distribution_failed", C.c_int)]
+
+
+class Ds(C.Structure):
+    _fields_ = [("n", C.c_int), ("bus", C.c_void_p), ("factor", C.c_void_p), ("base_target_p", C.c_void_p),
+                ("max_delta", C.c_void_p), ("min_delta", C.c_void_p), ("slack_bus", C.c_int),
+                ("slack_target_p", C.c_double), ("tol", C.c_double), ("reject_failure", C.c_int)]
+
+
+def ds_struct(ds, tol, reject_failure=False):
+    """An oracle_ds over a distributed-slack dict (gridgen.distributed_slack); returns (struct, kept arrays)."""
+    keep = [np.ascontiguousarray(ds["bus"], np.int32)] + [
+        np.ascontiguousarray(ds[k], np.float64) for k in ("factor", "base_target_p", "max_delta", "min_delta")]
+    st = Ds(len(keep[0]), *(a.ctypes.data for a in keep), int(ds["slack_bus"]), float(ds["slack_target_p"]),
+            float(tol), int(reject_failure))
+    return st, keep
 
 
 def lib():
@@ -64,15 +79,17 @@ def lib():
         L.oracle_fill.argtypes = [C.c_void_p, _dp, _ip, _dp, _ip, C.c_int, _dp, _dp, C.c_void_p]
         L.oracle_lu_solve.argtypes = [C.c_int, _ip, _ip, _dp, _dp, _dp]
         L.oracle_min_degree.argtypes = [C.c_int, _ip, _ip, _ip]
-        L.oracle_nr_solve.argtypes = [C.c_void_p, C.POINTER(NrParams), C.POINTER(Rl), _ip, C.c_int, C.c_void_p,
-                                      _dp, _ip, _dp, _bp, _bp, C.POINTER(Result)]
+        L.oracle_nr_solve.argtypes = [C.c_void_p, C.POINTER(NrParams), C.POINTER(Rl), C.c_void_p, _ip, C.c_int,
+                                      C.c_void_p, _dp, _ip, _dp, _bp, _bp, C.POINTER(Result)]
+        L.oracle_bus_p.restype = C.c_double
+        L.oracle_bus_p.argtypes = [C.c_void_p, _dp, _ip, C.c_int, C.c_int]
         L.oracle_islands.argtypes = [C.c_void_p, C.c_int, _ip, C.c_int, _ubp]
         L.oracle_violations.restype = C.c_int
         L.oracle_violations.argtypes = [C.c_void_p, _dp, _ip, C.c_int, C.c_void_p, C.POINTER(Limits),
                                         C.POINTER(Violation), C.c_int]
         L.oracle_n1.restype = C.c_int
-        L.oracle_n1.argtypes = [C.c_void_p, C.POINTER(NrParams), C.POINTER(Rl), C.c_int, _dp, _ip, _dp, _bp,
-                                C.c_int, C.c_int, _ip, C.c_int, _dp, _ip, _ip, _ip, _dp]
+        L.oracle_n1.argtypes = [C.c_void_p, C.POINTER(NrParams), C.POINTER(Rl), C.c_void_p, C.c_int, _dp, _ip, _dp,
+                                _bp, C.c_int, C.c_int, _ip, C.c_int, _dp, _ip, _ip, _ip, _dp, C.c_void_p]
         _LIB = L
     return _LIB
 
@@ -143,8 +160,15 @@ class OracleSystem:
                           J.ctypes.data if J is not None else None)
         return F, q, J
 
-    def solve(self, x0, mode, target, limit_type=None, prm: NrParams | None = None, outages=(), islanded=None):
-        """One load flow; returns (x, mode, target, limit_type, Result)."""
+    def bus_p(self, x, bus, outages=()):
+        """The computed active injection of one bus (its in-service branches' p at its side), pu."""
+        out = np.ascontiguousarray(np.asarray(outages, dtype=np.int32).reshape(-1))
+        return lib().oracle_bus_p(self.h, np.ascontiguousarray(x, np.float64), out, len(out), int(bus))
+
+    def solve(self, x0, mode, target, limit_type=None, prm: NrParams | None = None, outages=(), islanded=None,
+              ds=None, ds_tol=0.0, ds_reject=False):
+        """One load flow (distributed slack when ds is a gridgen.distributed_slack dict); returns
+        (x, mode, target, limit_type, Result)."""
         prm = prm or params()
         x = np.ascontiguousarray(x0, np.float64).copy()
         mode = np.ascontiguousarray(mode, np.int32).copy()
@@ -155,8 +179,10 @@ class OracleSystem:
         out = np.ascontiguousarray(np.asarray(outages, dtype=np.int32).reshape(-1))
         isl = None if islanded is None else np.ascontiguousarray(islanded, np.uint8)
         res = Result()
-        lib().oracle_nr_solve(self.h, C.byref(prm), C.byref(self.rl), out, len(out),
-                              isl.ctypes.data if isl is not None else None, x, mode, target, lt, sc, C.byref(res))
+        dst, dkeep = ds_struct(ds, ds_tol, ds_reject) if ds is not None else (None, None)
+        lib().oracle_nr_solve(self.h, C.byref(prm), C.byref(self.rl), C.byref(dst) if dst is not None else None,
+                              out, len(out), isl.ctypes.data if isl is not None else None, x, mode, target, lt, sc,
+                              C.byref(res))
         return x, mode, target, lt[:nrl], res
 
     def islands(self, outages, slack=0):
@@ -166,8 +192,10 @@ class OracleSystem:
         return isl.astype(bool)
 
     def n1(self, x_base, mode_base, target_base, limit_type_base, contingencies, prm: NrParams | None = None,
-           threads=0, slack=0):
-        """Batched N-1 on the CPU. contingencies: [K] or [K, max_out] branch ids (-1 pad)."""
+           threads=0, slack=0, ds=None, ds_tol=0.0, ds_reject=False, want_targets=False):
+        """Batched N-1 on the CPU. contingencies: [K] or [K, max_out] branch ids (-1 pad). With ds (a
+        gridgen.distributed_slack dict) the distributed-slack loop runs first; want_targets returns each
+        contingency's final targets."""
         prm = prm or params(batch_of_one=False)
         ctg = np.asarray(contingencies, dtype=np.int32)
         if ctg.ndim == 1:
@@ -182,11 +210,16 @@ class OracleSystem:
         lt = np.ascontiguousarray(np.asarray(limit_type_base, dtype=np.int8).reshape(-1))
         if lt.size == 0:
             lt = np.zeros(1, np.int8)
-        used = lib().oracle_n1(self.h, C.byref(prm), C.byref(self.rl), slack,
-                               np.ascontiguousarray(x_base, np.float64), np.ascontiguousarray(mode_base, np.int32),
-                               np.ascontiguousarray(target_base, np.float64), lt, k, mo, ctg, threads, x, st, it,
-                               tot, nm)
-        return dict(x=x, status=st, iterations=it, total_iterations=tot, norm=nm, threads=used)
+        tg = np.zeros((k, self.n)) if want_targets else None
+        dst, dkeep = ds_struct(ds, ds_tol, ds_reject) if ds is not None else (None, None)
+        used = lib().oracle_n1(self.h, C.byref(prm), C.byref(self.rl), C.byref(dst) if dst is not None else None,
+                               slack, np.ascontiguousarray(x_base, np.float64),
+                               np.ascontiguousarray(mode_base, np.int32), np.ascontiguousarray(target_base, np.float64),
+                               lt, k, mo, ctg, threads, x, st, it, tot, nm, tg.ctypes.data if tg is not None else None)
+        out = dict(x=x, status=st, iterations=it, total_iterations=tot, norm=nm, threads=used)
+        if want_targets:
+            out["target"] = tg
+        return out
 
 
 def limits_struct(table):
