@@ -41,6 +41,7 @@ extern "C" {
 
 /* state vector scaling: reference GpuStateVectorScalingMode */
 #define OLFHIP_SCALING_NONE 0
+#define OLFHIP_SCALING_LINE_SEARCH 1
 #define OLFHIP_SCALING_MAX_VOLTAGE_CHANGE 2
 
 typedef struct olfhip_structure olfhip_structure;
@@ -182,6 +183,10 @@ typedef struct {
     int distributed_slack;
     double slack_dist_tol;
     int reject_slack_distribution_failure;
+    /* LINE_SEARCH scaling: backtracks per test (<= 0: 10) and step fold (<= 1: 4/3), the reference's
+     * NewtonSettingsNative.setBatchLineSearch (NewtonSettingsNative.java:56) */
+    int line_search_max_iteration;
+    double line_search_step_fold;
 } olfhip_solve_inputs;
 
 /* Outputs (any may be NULL): results [n_scenarios][n+3] = (x, iterations of the last solve, ||F||inf, status) in

@@ -298,6 +298,7 @@ struct ContextImpl {
     double slackTargetP = 0.0, dsFactorSum = 0.0;
     DBuf dsNode, dsFactor, dsBase, dsMaxD, dsMinD, dsDesc;
     DBuf loopAt, sweepChanged, dsFailed, slackP, dsRetain, dsPrev;
+    DBuf xPrev, lsLast, lsHas;   // LINE_SEARCH state (xPrev allocated by the first solve that asks for it)
     // install inputs
     DBuf x0Base, x0Per, targetBase, modeBase, modeEntries, targetIdx, targetVal, seed, states;
     int lastInstalled = 0;
@@ -349,6 +350,9 @@ struct ContextImpl {
         S.slackP = slackP.as<double>();
         S.dsRetain = dsRetain.as<double>();
         S.dsPrev = dsPrev.as<double>();
+        S.xPrev = xPrev.as<double>();
+        S.lsLast = lsLast.as<double>();
+        S.lsHas = lsHas.as<signed char>();
         return S;
     }
     DevRL devRL() const {
@@ -415,6 +419,8 @@ static ContextImpl *createContext(StructureImpl *st, int nScenarios, int mode, c
     c->slackP.alloc(lanes * 8);
     c->dsRetain.alloc(lanes * 8);
     c->dsPrev.alloc(8);
+    c->lsLast.alloc(lanes * 8);
+    c->lsHas.alloc(lanes);
     // reactive limits, entry buses in node numbering
     if (rl && rl->n_entries > 0) {
         if (!rl->voltage_row || !rl->min_q || !rl->max_q || !rl->load_q || !rl->target_v)
@@ -466,9 +472,9 @@ static void solve(ContextImpl *c, const olfhip_solve_inputs *in, olfhip_solve_ou
     if (in->n_row_mode_entries > 0 && !in->row_mode_entries) throw ArgumentError("row_mode_entries is null");
     if (in->n_target_overrides > 0 && (!in->target_override_index || !in->target_override_value))
         throw ArgumentError("target override arrays are null");
-    if (in->state_vector_scaling_mode != OLFHIP_SCALING_NONE &&
+    if (in->state_vector_scaling_mode != OLFHIP_SCALING_NONE && in->state_vector_scaling_mode != OLFHIP_SCALING_LINE_SEARCH &&
         in->state_vector_scaling_mode != OLFHIP_SCALING_MAX_VOLTAGE_CHANGE)
-        throw UnsupportedError("state vector scaling: only NONE and MAX_VOLTAGE_CHANGE are modelled");
+        throw UnsupportedError("state vector scaling: NONE, LINE_SEARCH and MAX_VOLTAGE_CHANGE are modelled");
     for (int i = 0; i < n; i++)
         if (in->base_row_mode[i] != OLFHIP_ROW_FIXED_VARIABLE && in->base_row_mode[i] != OLFHIP_ROW_EQUATION)
             throw UnsupportedError("base row mode " + std::to_string(in->base_row_mode[i]) + " is not modelled");
@@ -529,6 +535,9 @@ static void solve(ContextImpl *c, const olfhip_solve_inputs *in, olfhip_solve_ou
     if (in->distributed_slack && c->slackNode < 0)
         throw ArgumentError("distributed_slack is set but the context has no distributed slack (olfhip_set_distributed_slack)");
     if (in->distributed_slack && !(in->slack_dist_tol >= 0.0)) throw ArgumentError("slack_dist_tol must be >= 0");
+    prm.lsMax = in->line_search_max_iteration > 0 ? in->line_search_max_iteration : 10;
+    prm.lsFold = in->line_search_step_fold > 1.0 ? in->line_search_step_fold : 4.0 / 3.0;
+    if (prm.scaling == OLFHIP_SCALING_LINE_SEARCH) c->xPrev.alloc((size_t)c->nTiles * c->T * P.nb * 2 * 8);
     prm.distributedSlack = in->distributed_slack != 0;
     prm.dsTol = in->slack_dist_tol;
     prm.dsReject = in->reject_slack_distribution_failure != 0;

@@ -88,6 +88,10 @@ struct DevState {
     // last reset ([tile][participant][lane])
     signed char *loopAt, *sweepChanged, *dsFailed;
     double *slackP, *dsRetain, *dsPrev;
+    // LINE_SEARCH: the state before the last step ([tile][node][lane][2]), and per lane the last tested ||F||_2^2 of
+    // the current solve and whether there is one
+    double *xPrev, *lsLast;
+    signed char *lsHas;
     // opt-in section timing of one tile: prof[0] counts the Newton steps that recorded, then PROF_SECTIONS stamps per
     // step (wall clock) from prof[PROF_SECTIONS] on, at most profCap steps
     unsigned long long *prof;
@@ -126,6 +130,8 @@ struct NrParams {
     int refine;
     int batchOfOne;
     int stepsPerLaunch;   // Newton steps one launch may run per tile
+    int lsMax;            // LINE_SEARCH (scaling 1): backtracks per test and the step fold, mu = lsFold^-b
+    double lsFold;
     int distributedSlack; // run the distributed-slack outer loop (before the reactive limits)
     double dsTol;         // its gate, slackBusPMaxMismatch in pu
     int dsReject;         // a failed distribution ends the scenario REJECTED
@@ -889,7 +895,7 @@ __device__ __forceinline__ void luBackward(const DevPlan &P, const LevelMeta &M,
 // thread's lane (tid % T) are returned.
 template <int T, typename VT>
 __device__ void fillPass(const DevPlan &P, const DevState &S, int tile, int tid, int nt, const int *fillLane,
-                         double &pmax, int &pfin, int slackNode) {
+                         double &pmax, int &pfin, double &psq, int slackNode) {
     const int nb = P.nb;
     VT *lu = reinterpret_cast<VT *>(S.lu);
     const int t = tid % T;   // fixed per thread: the stride nt is a multiple of T
@@ -911,6 +917,7 @@ __device__ void fillPass(const DevPlan &P, const DevState &S, int tile, int tid,
         if (k == slackNode) S.slackP[tile * T + t] = sP;
         if (!isfinite(fP) || !isfinite(fQ)) pfin = 0;
         pmax = fmax(pmax, fmax(fabs(fP), fabs(fQ)));
+        psq += fP * fP + fQ * fQ;
     }
 }
 
@@ -1112,7 +1119,8 @@ __global__ __launch_bounds__(MAX_THREADS) void nrStepKernel(DevPlan P, DevState 
 
     __shared__ int sStatus[T], sIt[T], sSolves[T], sOuter[T], sTotal[T], sClamp[T];
     __shared__ int sFill[T], sSolve[T], sConv[T], sRl[T], sRemain[T], sSwitched[T], sFin[T];
-    __shared__ int sChk[T], sLoop[T], sSweep[T], sResolve[T], sReject[T], sDs[T], sDsUnst[T], sDsFail[T];
+    __shared__ int sChk[T], sLoop[T], sSweep[T], sResolve[T], sReject[T], sDs[T], sDsUnst[T], sDsFail[T], sBt[T];
+    __shared__ double sSsq[T];
     __shared__ double sNorm[T], sRatio[T], sFactor[T];
     __shared__ int sFlag[4];
     __shared__ double red[MAX_THREADS];
@@ -1161,19 +1169,77 @@ __global__ __launch_bounds__(MAX_THREADS) void nrStepKernel(DevPlan P, DevState 
 
     // ---- 1. mismatch + Jacobian of the running lanes
     {
-        double pmax = 0.0;
+        double pmax = 0.0, psq = 0.0;
         int pfin = 1;
-        fillPass<T, VT>(P, S, tile, tid, nt, sFill, pmax, pfin, slackNode);
+        fillPass<T, VT>(P, S, tile, tid, nt, sFill, pmax, pfin, psq, slackNode);
         red[tid] = pmax;
         redI[tid] = pfin;
         laneReduce<T, true>(red, tid, nt);
         laneReduceAnd<T>(redI, tid, nt);
+        if (tid < T) {
+            sNorm[tid] = red[tid];
+            sFin[tid] = redI[tid];
+        }
+        if (prm.scaling == 1) {
+            // LINE_SEARCH (native/cuda/solver/newton_iteration.cu:497-530): after every step but a solve's first, a
+            // step that did not lower ||F||_2 (or made F non-finite) is walked back to x_prev + mu dx,
+            // mu = lsFold^-b, b = 1..lsMax, refilled at each point until ||F||_2 is below the last tested one
+            __syncthreads();
+            red[tid] = psq;
+            laneReduce<T, false>(red, tid, nt);
+            if (tid < T) {
+                const int lane = tile * T + tid;
+                sSsq[tid] = red[tid];
+                sBt[tid] = sFill[tid] && sIt[tid] > 0 && S.lsHas[lane] &&
+                           (!sFin[tid] || sSsq[tid] >= S.lsLast[lane]);
+            }
+            __syncthreads();
+            for (int b = 1; b <= prm.lsMax; b++) {
+                int any = 0;
+                for (int u = 0; u < T; u++) any |= sBt[u];
+                if (!any) break;
+                const double mu = 1.0 / pow(prm.lsFold, (double)b);
+                for (int w = tid; w < nb * T; w += nt) {
+                    const int k = w / T, t = w % T;
+                    if (!sBt[t]) continue;
+                    const size_t vi = vIdx<T>(tile, nb, k, t);
+                    const double2 xp = *reinterpret_cast<const double2 *>(S.xPrev + vi);
+                    const double2 d = *reinterpret_cast<const double2 *>(S.dx + vi);
+                    *reinterpret_cast<double2 *>(S.x + vi) = make_double2(xp.x + mu * d.x, xp.y + mu * d.y);
+                }
+                __syncthreads();
+                double bmax = 0.0, bsq = 0.0;
+                int bfin = 1;
+                fillPass<T, VT>(P, S, tile, tid, nt, sBt, bmax, bfin, bsq, slackNode);
+                red[tid] = bmax;
+                redI[tid] = bfin;
+                laneReduce<T, true>(red, tid, nt);
+                laneReduceAnd<T>(redI, tid, nt);
+                if (tid < T && sBt[tid]) {
+                    sNorm[tid] = red[tid];
+                    sFin[tid] = redI[tid];
+                }
+                __syncthreads();
+                red[tid] = bsq;
+                laneReduce<T, false>(red, tid, nt);
+                if (tid < T && sBt[tid]) {
+                    sSsq[tid] = red[tid];
+                    if (sFin[tid] && sSsq[tid] < S.lsLast[tile * T + tid]) sBt[tid] = 0;
+                }
+                __syncthreads();
+            }
+            if (tid < T && sFill[tid] && sIt[tid] > 0) {
+                const int lane = tile * T + tid;
+                S.lsLast[lane] = sSsq[tid];
+                S.lsHas[lane] = 1;
+            }
+        }
     }
     OLFHIP_MARK(1);
     // ---- 2. the test of each running lane at its own iteration
     if (tid < T && sFill[tid]) {
-        const double norm = red[tid];
-        const bool fin = redI[tid] != 0;
+        const double norm = sNorm[tid];
+        const bool fin = sFin[tid] != 0;
         const int it = sIt[tid];
         sNorm[tid] = norm;
         if (!fin) {
@@ -1341,6 +1407,7 @@ __global__ __launch_bounds__(MAX_THREADS) void nrStepKernel(DevPlan P, DevState 
                 sIt[tid] = 0;
                 sFill[tid] = 1;     // restarted at its new modes and targets: refill, no test at iteration 0
                 sSweep[tid] = 1;
+                S.lsHas[lane] = 0;  // a restart retargets the system: no line-search reference yet
             } else if (sReject[tid]) {
                 sStatus[tid] = ST_REJECTED;
             } else {
@@ -1356,9 +1423,9 @@ __global__ __launch_bounds__(MAX_THREADS) void nrStepKernel(DevPlan P, DevState 
         }
         __syncthreads();
         if (sFlag[1]) {
-            double pmax = 0.0;
+            double pmax = 0.0, psq = 0.0;
             int pfin = 1;
-            fillPass<T, VT>(P, S, tile, tid, nt, sFill, pmax, pfin, slackNode);
+            fillPass<T, VT>(P, S, tile, tid, nt, sFill, pmax, pfin, psq, slackNode);
             red[tid] = pmax;
             redI[tid] = pfin;
             laneReduce<T, true>(red, tid, nt);
@@ -1460,6 +1527,7 @@ __global__ __launch_bounds__(MAX_THREADS) void nrStepKernel(DevPlan P, DevState 
             const double2 d = *reinterpret_cast<const double2 *>(S.dx + vi);
             double2 *xp = reinterpret_cast<double2 *>(S.x + vi);
             const double2 xv = *xp;
+            if (prm.scaling == 1) *reinterpret_cast<double2 *>(S.xPrev + vi) = xv;
             *xp = make_double2(xv.x + f * d.x, xv.y + f * d.y);
         }
         if (tid < T && sSolve[tid]) {
@@ -1562,6 +1630,7 @@ __global__ void laneInitKernel(DevPlan P, DevState S, DevRL R, const signed char
             S.sweepChanged[lane] = 0;
             S.dsFailed[lane] = 0;
             S.dsRetain[lane] = __builtin_nan("");
+            S.lsHas[lane] = 0;
             S.slackP[lane] = 0.0;
             if (lane % T == 0) S.tileRunning[lane / T] = lane < S.nInstalled;
         } else {

@@ -16,7 +16,7 @@ OK, E_ARGUMENT, E_DEVICE, E_MEMORY, E_UNSUPPORTED = 0, -1, -2, -3, -4
 CONVERGED, MAX_ITERATION_REACHED, DIVERGED, REJECTED, LINEAR_SOLVE_FAILED = 0, 1, -1, -2, -3
 STATUS_NAMES = {0: "CONVERGED", 1: "MAX_ITERATION_REACHED", -1: "DIVERGED", -2: "REJECTED", -3: "LINEAR_SOLVE_FAILED"}
 LU_FP64, LU_FP32_REFINED = 0, 1
-SCALING_NONE, SCALING_MAX_VOLTAGE_CHANGE = 0, 2
+SCALING_NONE, SCALING_LINE_SEARCH, SCALING_MAX_VOLTAGE_CHANGE = 0, 1, 2
 
 # every entry point include/olfhip.h declares
 EXPORTS = [
@@ -61,7 +61,8 @@ class SolveInputs(C.Structure):
                 ("max_outer_iterations", C.c_int), ("max_pq_pv_switch", C.c_int), ("skip_reactive_limits", C.c_int),
                 ("states_to_host", C.c_int), ("x0_per_scenario_device", C.c_void_p),
                 ("distributed_slack", C.c_int), ("slack_dist_tol", C.c_double),
-                ("reject_slack_distribution_failure", C.c_int)]
+                ("reject_slack_distribution_failure", C.c_int), ("line_search_max_iteration", C.c_int),
+                ("line_search_step_fold", C.c_double)]
 
 
 class DistributedSlackDesc(C.Structure):
@@ -274,7 +275,8 @@ class BatchContext:
                     tol=1e-4, reactive_limit_tol=1e-4, state_vector_scaling_mode=SCALING_NONE, max_dv=0.1,
                     max_dphi=0.174533, max_outer_iterations=20, max_pq_pv_switch=3, skip_reactive_limits=False,
                     states_to_host=True, x0_per_scenario_device=None, distributed_slack=False, slack_dist_tol=0.0,
-                    reject_slack_distribution_failure=False):
+                    reject_slack_distribution_failure=False, line_search_max_iteration=10,
+                    line_search_step_fold=4.0 / 3.0):
         keep = []
 
         def arr(a, dt):
@@ -308,7 +310,8 @@ class BatchContext:
                          0 if ti is None else ti.size, ptr(ti), ptr(tv), mo, ptr(ob), ptr(xp), ptr(sd), max_iter, tol,
                          reactive_limit_tol, state_vector_scaling_mode, max_dv, max_dphi, max_outer_iterations,
                          max_pq_pv_switch, int(skip_reactive_limits), int(states_to_host), x0_per_scenario_device,
-                         int(distributed_slack), float(slack_dist_tol), int(reject_slack_distribution_failure))
+                         int(distributed_slack), float(slack_dist_tol), int(reject_slack_distribution_failure),
+                         int(line_search_max_iteration), float(line_search_step_fold))
         return si, keep
 
     def set_distributed_slack(self, ds):

@@ -95,9 +95,11 @@ class NewtonRaphsonParameters:
 
     maxIterations: int = 15
     maxMismatch: float = 1e-4                 # ||F||inf stopping threshold, pu
-    stateVectorScalingMode: str = "NONE"      # NONE or MAX_VOLTAGE_CHANGE
+    stateVectorScalingMode: str = "NONE"      # NONE, LINE_SEARCH or MAX_VOLTAGE_CHANGE
     maxVoltageChangeV: float = 0.1
     maxVoltageChangePhi: float = 0.174533
+    lineSearchStateVectorScalingMaxIteration: int = 10
+    lineSearchStateVectorScalingStepFold: float = 4.0 / 3.0
     reactiveLimits: bool = True
     maxReactivePowerMismatch: float = 1e-4    # pu
     maxOuterLoopIterations: int = 20
@@ -124,6 +126,8 @@ class NewtonRaphsonParameters:
             return N.SCALING_NONE
         if self.stateVectorScalingMode == "MAX_VOLTAGE_CHANGE":
             return N.SCALING_MAX_VOLTAGE_CHANGE
+        if self.stateVectorScalingMode == "LINE_SEARCH":
+            return N.SCALING_LINE_SEARCH
         raise ValueError(f"state vector scaling {self.stateVectorScalingMode!r} is not modelled")
 
 
@@ -162,7 +166,9 @@ class GpuNewtonRaphson:
             reactive_limit_tol=nr.maxReactivePowerMismatch, state_vector_scaling_mode=nr.scaling_code(),
             max_dv=nr.maxVoltageChangeV, max_dphi=nr.maxVoltageChangePhi,
             max_outer_iterations=nr.maxOuterLoopIterations, max_pq_pv_switch=nr.maxPqPvSwitch,
-            skip_reactive_limits=not nr.reactiveLimits, **nr.slack_inputs(self.distributed_slack))
+            skip_reactive_limits=not nr.reactiveLimits, **nr.slack_inputs(self.distributed_slack),
+            line_search_max_iteration=nr.lineSearchStateVectorScalingMaxIteration,
+            line_search_step_fold=nr.lineSearchStateVectorScalingStepFold)
         r = self.context.solve(inp)
         n_rl = 0 if self.reactive_limits is None else len(self.reactive_limits["voltage_row"])
         m, t, lt = self.context.read_outer_loop_state(1, n_rl)
@@ -276,7 +282,9 @@ class GpuBatchedSecurityAnalysisSolver:
                 max_dphi=nr.maxVoltageChangePhi, max_outer_iterations=nr.maxOuterLoopIterations,
                 max_pq_pv_switch=nr.maxPqPvSwitch, skip_reactive_limits=not nr.reactiveLimits,
                 states_to_host=states_to_host, x0_per_scenario_device=x0_dev,
-                **nr.slack_inputs(self.distributed_slack))
+                **nr.slack_inputs(self.distributed_slack),
+                line_search_max_iteration=nr.lineSearchStateVectorScalingMaxIteration,
+                line_search_step_fold=nr.lineSearchStateVectorScalingStepFold)
             ctx = self.context(len(part))
             t0 = time.perf_counter()
             r = ctx.solve(inp, want_states=states_to_host)

@@ -733,6 +733,10 @@ void oracle_nr_solve(const oracle_sys *s, const oracle_nr_params *prm, const ora
     double *qinj = (double *)malloc(sizeof(double) * (size_t)s->n_bus);
     signed char *pass = (signed char *)malloc((size_t)(rl && rl->n > 0 ? rl->n : 1));
     double *prev = (double *)malloc(sizeof(double) * (size_t)(ds && ds->n > 0 ? ds->n : 1));
+    double *xprev = (double *)malloc(sizeof(double) * (size_t)n);
+    const int ls = prm->scaling_mode == 1;
+    const int ls_max = prm->ls_max_iteration > 0 ? prm->ls_max_iteration : 10;
+    const double ls_fold = prm->ls_step_fold > 1.0 ? prm->ls_step_fold : 4.0 / 3.0;
     memset(res, 0, sizeof(*res));
     const int ds_on = ds != NULL;
     const int rl_on = rl && rl->n > 0 && prm->reactive_limits;
@@ -745,10 +749,31 @@ void oracle_nr_solve(const oracle_sys *s, const oracle_nr_params *prm, const ora
         double last_ratio = 0.0;
         int ended = 0, converged = 0;
         double norm = 0.0;
+        int has_last = 0;          /* LINE_SEARCH: the 2-norm of the last tested step, per solve */
+        double last_ssq = 0.0;
         for (;;) {
             oracle_fill(s, x, mode, target, out_br, n_out, F, qinj, J);
             int finite;
             norm = inf_norm(F, n, &finite);
+            if (ls && it > 0) {
+                /* LINE_SEARCH (newton_iteration.cu:497-530): a step that did not lower ||F||2 (or went non-finite)
+                 * is walked back to xprev + mu dx, mu = fold^-b, b = 1.. ls_max, refilling each time */
+                double ssq = 0.0;
+                for (int i = 0; i < n; i++) ssq += F[i] * F[i];
+                if (has_last && (!finite || ssq >= last_ssq)) {
+                    for (int bt = 1; bt <= ls_max; bt++) {
+                        const double mu = 1.0 / pow(ls_fold, bt);
+                        for (int i = 0; i < n; i++) x[i] = xprev[i] + mu * dx[i];
+                        oracle_fill(s, x, mode, target, out_br, n_out, F, qinj, J);
+                        norm = inf_norm(F, n, &finite);
+                        ssq = 0.0;
+                        for (int i = 0; i < n; i++) ssq += F[i] * F[i];
+                        if (finite && ssq < last_ssq) break;
+                    }
+                }
+                last_ssq = ssq;
+                has_last = 1;
+            }
             res->norm = norm;
             res->iterations = it;
             if (!finite) { res->status = ORC_DIVERGED; ended = 1; break; }
@@ -791,6 +816,7 @@ void oracle_nr_solve(const oracle_sys *s, const oracle_nr_params *prm, const ora
                 if (mdv > prm->max_dv) f = fmin(f, prm->max_dv / mdv);
                 if (mdp > prm->max_dphi) f = fmin(f, prm->max_dphi / mdp);
             }
+            if (ls) memcpy(xprev, x, sizeof(double) * (size_t)n);
             for (int i = 0; i < n; i++) x[i] += f * dx[i];
             it++;
             res->total_iterations++;
@@ -833,7 +859,7 @@ void oracle_nr_solve(const oracle_sys *s, const oracle_nr_params *prm, const ora
         res->status = ORC_CONVERGED;
         break;
     }
-    free(F); free(b); free(dx); free(r); free(J); free(qinj); free(pass); free(prev);
+    free(F); free(b); free(dx); free(r); free(J); free(qinj); free(pass); free(prev); free(xprev);
 }
 
 int oracle_islands(const oracle_sys *s, int slack_bus, const int *out_br, int n_out, unsigned char *islanded)

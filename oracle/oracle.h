@@ -63,9 +63,12 @@ typedef struct {
     double rl_tol;         /* reactive-limit tolerance, pu */
     int max_outer;         /* productive outer-loop iterations (outer_loop_driver.cu:191) */
     int max_switch;        /* maxPqPvSwitch (outer_loop_driver.cu:194) */
-    int scaling_mode;      /* 0 NONE, 2 MAX_VOLTAGE_CHANGE (state_vector_scaling.cu:57-97) */
+    int scaling_mode;      /* 0 NONE, 1 LINE_SEARCH (newton_iteration.cu:497-530), 2 MAX_VOLTAGE_CHANGE
+                              (state_vector_scaling.cu:57-97) */
     double max_dv, max_dphi;
     int batch_of_one;      /* a bad last solve is LINEAR_SOLVE_FAILED (1) or REJECTED (0) */
+    int ls_max_iteration;  /* LINE_SEARCH backtracks per test (<= 0: 10) */
+    double ls_step_fold;   /* LINE_SEARCH: mu = ls_step_fold^-b (<= 1: 4/3) */
 } oracle_nr_params;
 
 typedef struct {

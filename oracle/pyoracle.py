@@ -24,7 +24,8 @@ STATUS = {0: "CONVERGED", 1: "MAX_ITERATION_REACHED", -1: "DIVERGED", -2: "REJEC
 class NrParams(C.Structure):
     _fields_ = [("max_iter", C.c_int), ("tol", C.c_double), ("reactive_limits", C.c_int), ("rl_tol", C.c_double),
                 ("max_outer", C.c_int), ("max_switch", C.c_int), ("scaling_mode", C.c_int),
-                ("max_dv", C.c_double), ("max_dphi", C.c_double), ("batch_of_one", C.c_int)]
+                ("max_dv", C.c_double), ("max_dphi", C.c_double), ("batch_of_one", C.c_int),
+                ("ls_max_iteration", C.c_int), ("ls_step_fold", C.c_double)]
 
 
 class Rl(C.Structure):
@@ -118,9 +119,11 @@ def min_degree(adjptr, adj):
 
 
 def params(max_iter=15, tol=1e-4, reactive_limits=True, rl_tol=1e-4, max_outer=20, max_switch=3,
-           scaling_mode=0, max_dv=0.1, max_dphi=0.174533, batch_of_one=True) -> NrParams:
+           scaling_mode=0, max_dv=0.1, max_dphi=0.174533, batch_of_one=True, ls_max_iteration=10,
+           ls_step_fold=4.0 / 3.0) -> NrParams:
+    """scaling_mode: 0 NONE, 1 LINE_SEARCH, 2 MAX_VOLTAGE_CHANGE"""
     return NrParams(max_iter, tol, int(reactive_limits), rl_tol, max_outer, max_switch, scaling_mode, max_dv,
-                    max_dphi, int(batch_of_one))
+                    max_dphi, int(batch_of_one), ls_max_iteration, ls_step_fold)
 
 
 class OracleSystem:

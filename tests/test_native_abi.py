@@ -94,8 +94,9 @@ def test_parameters_keep_the_reference_surface():
     with pytest.raises(ValueError):
         GpuLoadFlowParameters.load({"noSuchParameter": 1})
     assert NewtonRaphsonParameters().scaling_code() == N.SCALING_NONE
+    assert NewtonRaphsonParameters(stateVectorScalingMode="LINE_SEARCH").scaling_code() == N.SCALING_LINE_SEARCH
     with pytest.raises(ValueError):
-        NewtonRaphsonParameters(stateVectorScalingMode="LINE_SEARCH").scaling_code()
+        NewtonRaphsonParameters(stateVectorScalingMode="NOT_A_MODE").scaling_code()
 
 
 def test_outage_neighbourhoods_match_the_oracle(grid):
