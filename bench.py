@@ -73,6 +73,8 @@ def main():
     ap.add_argument("--no-warm-start", action="store_true", help="base-case start only (no learned warm start)")
     ap.add_argument("--train-samples", type=int, default=20000)
     ap.add_argument("--train-epochs", type=int, default=150)
+    ap.add_argument("--order", choices=("cost", "given"), default="cost",
+                    help="solve order: longest first by the training solves' iteration counts, or as sampled")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: the N-1 batch shards by contingency across GPUs "
@@ -98,11 +100,36 @@ def main():
     # contingencies start from the base case's converged state, row modes, targets and reactive-limit types
     xb, mb, tb, lt = base.x, base.row_mode, base.target, base.reactive_limit_types
 
-    ctg = gridgen.sample_contingencies(net, args.contingencies, 0.05, 2)
+    ctg_given = gridgen.sample_contingencies(net, args.contingencies, 0.05, 2)
     nrp = NewtonRaphsonParameters()
     sa = GpuBatchedSecurityAnalysisSolver(structure, rl, GpuLoadFlowParameters(linearSolver="BATCHED_LU_FP32_REFINED"),
                                           nrp, lanes_per_tile=args.lanes_per_tile,
                                           threads_per_tile=args.threads_per_tile)
+
+    # learned warm start: trained on post-contingency states the solver itself computes (load scalings U(0.9, 1.1)),
+    # before the timed region
+    mlp = None
+    warm_info = None
+    data = None
+    if not args.no_warm_start:
+        from olfgpu import warmstart as W
+        tw = time.perf_counter()
+        mlp = W.WarmStartMLP(net, hidden=512, seed=0)
+        data = W.make_training_set(net, structure, mlp.hoods, n_samples=args.train_samples, levels=20, seed=3)
+        tgen = time.perf_counter() - tw
+        hist = W.train(mlp, data, epochs=args.train_epochs, batch=128, lr=2e-3, seed=0)
+        warm_info = {"training_samples": int(len(data["branch"])), "generation_s": round(tgen, 1),
+                     "training_s": round(time.perf_counter() - tw - tgen, 1), "epochs": args.train_epochs,
+                     "loss_first_last": [round(hist[0], 4), round(hist[-1], 4)], "hidden": 512,
+                     "neighbourhood_buses": mlp.n_local}
+    # solve order: longest first by the Newton iterations each outage took in the training solves (at other load
+    # scalings); results stay per contingency, the order only decides which scenarios share a tile and start first
+    order = np.arange(len(ctg_given))
+    if data is not None and args.order == "cost":
+        from olfgpu import warmstart as W
+        from olfgpu.solver import longest_first
+        order = longest_first(W.iteration_costs(data, net.n_branch)[ctg_given])
+    ctg = ctg_given[order]
     K = len(ctg)
     rme, tix, tvl, isl = sa.plan_islands(ctg[:, None])
     inputs, keep = N.BatchContext.make_inputs(
@@ -116,23 +143,9 @@ def main():
         sa.lanes_per_tile = tuned["lanes_per_tile"]
         sa.threads_per_tile = tuned["threads_per_tile"]
     ctx = sa.context(K)
-
-    # learned warm start: trained on post-contingency states the solver itself computes (load scalings U(0.9, 1.1)),
-    # before the timed region
-    mlp = None
-    warm_info = None
-    if not args.no_warm_start:
+    if mlp is not None:
         from olfgpu import warmstart as W
-        tw = time.perf_counter()
-        mlp = W.WarmStartMLP(net, hidden=512, seed=0)
-        data = W.make_training_set(net, structure, mlp.hoods, n_samples=args.train_samples, levels=20, seed=3)
-        tgen = time.perf_counter() - tw
-        hist = W.train(mlp, data, epochs=args.train_epochs, batch=128, lr=2e-3, seed=0)
         feats = W.endpoint_features(net, xb, ctg, 1.0, gridgen.bridges(net))
-        warm_info = {"training_samples": int(len(data["branch"])), "generation_s": round(tgen, 1),
-                     "training_s": round(time.perf_counter() - tw - tgen, 1), "epochs": args.train_epochs,
-                     "loss_first_last": [round(hist[0], 4), round(hist[-1], 4)], "hidden": 512,
-                     "neighbourhood_buses": mlp.n_local}
     setup_s = time.perf_counter() - t_setup
 
     def one_step():
@@ -181,7 +194,7 @@ def main():
 
     # plain FP64 one-case-at-a-time GPU NR on a sample (the same library at batch 1, FP64, base-case start)
     plain = GpuBatchedSecurityAnalysisSolver(structure, rl, GpuLoadFlowParameters(linearSolver="BATCHED_LU"), nrp)
-    sample = ctg[: args.plain_sample]
+    sample = ctg_given[: args.plain_sample]
     prme, ptix, ptvl, _ = plain.plan_islands(sample[:, None])
     pctx = plain.context(1)
     plain_inputs = []
@@ -205,7 +218,9 @@ def main():
     device_synchronize()
     plain_s = time.perf_counter() - tp
     plain_value = len(sample) / plain_s
-    plain_match = bool(np.array_equal(np.array(plain_status), status[: len(sample)]))
+    status_given = np.empty_like(status)
+    status_given[order] = status
+    plain_match = bool(np.array_equal(np.array(plain_status), status_given[: len(sample)]))
 
     # violation detection on the last step's states (reported beside the value, not inside it: the metric is the
     # Newton solve, as for the plain FP64 arm)
@@ -250,7 +265,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(net, xb, mb, tb, lt, ctg[: args.cpu_sample], ctg[: args.cpu_sample_1t])
+        cpu = cpu_baseline(net, xb, mb, tb, lt, ctg_given[: args.cpu_sample], ctg_given[: args.cpu_sample_1t])
 
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "contingencies/s", "n_gpus": 1, "steps": args.steps,
@@ -258,7 +273,9 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "fp32-lu+fp64-residual", "data": "synthetic",
         "config": {"workload": "grid9241 x1.1 load, 2000 single-branch outages (5% bridges, seed 2), FP32 block LU + "
                                "1 FP64 refinement, reactive limits on, " +
-                               ("base-case start" if mlp is None else "learned warm start (base-start fallback)"),
+                               ("base-case start" if mlp is None else "learned warm start (base-start fallback)") +
+                               (", solved longest first by the training solves' iteration counts"
+                                if data is not None and args.order == "cost" else ""),
                    "buses": net.n_bus, "branches": net.n_branch, "contingencies": K, "tol_pu": nrp.maxMismatch,
                    "tile": ctx_lanes(ctx), "lu_levels": stats["levels"], "lu_nnz": stats["lu_nnz"]},
         "statuses": {N.STATUS_NAMES[int(s)]: int((status == s).sum()) for s in np.unique(status)},

@@ -177,6 +177,14 @@ class GpuNewtonRaphson:
                               lt[0])
 
 
+def longest_first(cost) -> np.ndarray:
+    """The order in which a batch solves its scenarios, given an expected cost per scenario: most expensive first
+    (ties in the given order). Consecutive scenarios share a tile and tiles start in order as workgroups free up, so
+    this pairs scenarios of similar length in a tile and starts the long ones early (longest-processing-time
+    scheduling); results are reported in the caller's order."""
+    return np.argsort(-np.asarray(cost, dtype=np.float64), kind="stable")
+
+
 @dataclass
 class BatchScenarioResult:
     """Per-contingency result (the reference's BatchScenarioResult)."""
@@ -243,12 +251,34 @@ class GpuBatchedSecurityAnalysisSolver:
     def solve_batch_n1(self, x_base, target_base, row_mode_base, limit_types_base, contingencies,
                        x0_per_scenario=None, states_to_host=True, warm_start=None,
                        warm_start_features=None, limit_table: N.LimitTable | None = None,
-                       violation_band=1e-7) -> BatchScenarioResult:
+                       violation_band=1e-7, cost=None) -> BatchScenarioResult:
         """solveBatchN1: contingencies is [K] or [K, m] closed-branch ids (-1 padded). With a warm_start network
         (olfgpu.warmstart.WarmStartMLP, single outages) every scenario starts from its prediction, and any that does
         not converge is solved again from the base-case start. With a limit_table the violations of every converged
-        contingency are detected on its final state (device screen, exact host decisions)."""
+        contingency are detected on its final state (device screen, exact host decisions). With a cost per
+        contingency (e.g. warmstart.iteration_costs of its branch) the batch is solved longest first
+        (longest_first); the results are in the contingencies' order either way."""
         import time
+
+        if cost is not None:
+            order = longest_first(cost)
+            inv = np.empty_like(order)
+            inv[order] = np.arange(len(order))
+
+            def part(a):
+                return None if a is None else np.asarray(a)[order]
+            r = self.solve_batch_n1(x_base, target_base, row_mode_base, limit_types_base,
+                                    np.asarray(contingencies)[order], part(x0_per_scenario), states_to_host,
+                                    warm_start, part(warm_start_features), limit_table, violation_band)
+            r.status, r.iterations, r.total_iterations, r.norm = (r.status[inv], r.iterations[inv],
+                                                                  r.total_iterations[inv], r.norm[inv])
+            r.islanded_count = r.islanded_count[inv]
+            if r.x is not None:
+                r.x = r.x[inv]
+            if r.violations is not None:
+                r.violations["scenario"] = order[r.violations["scenario"]]
+                r.violations = r.violations[np.argsort(r.violations["scenario"], kind="stable")]
+            return r
 
         ctg = np.asarray(contingencies, dtype=np.int32)
         if ctg.ndim == 1:

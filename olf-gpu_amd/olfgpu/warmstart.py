@@ -240,9 +240,10 @@ def make_training_set(net, structure, hoods, n_samples=20000, levels=20, seed=3,
     U(scale_range); the outages are consecutive slices of one random permutation of all branches, so every branch is
     covered before any repeats.
 
-    Returns a dict with the base states x_base [levels, n], per sample its level, branch, endpoint features and
-    neighbourhood target (local_change / flow_scale, unnormalised) [S, 2K]. Islanded buses count as unchanged (the
-    solver freezes them whatever the start); outages whose solve does not converge are left out."""
+    Returns a dict with the base states x_base [levels, n], per sample its level, branch, endpoint features,
+    neighbourhood target (local_change / flow_scale, unnormalised) [S, 2K] and the Newton iterations its solve took
+    from the base start (all solves of its outer loop). Islanded buses count as unchanged (the solver freezes them
+    whatever the start); outages whose solve does not converge are left out."""
     from . import gridgen
     from .solver import GpuBatchedSecurityAnalysisSolver, GpuLoadFlowParameters, GpuNewtonRaphson
 
@@ -251,7 +252,7 @@ def make_training_set(net, structure, hoods, n_samples=20000, levels=20, seed=3,
     per = int(np.ceil(n_samples / levels))
     order = rng.permutation(net.n_branch)
     bridges = gridgen.bridges(net)
-    xb_all, lvl, brs, feats, targets = [], [], [], [], []
+    xb_all, lvl, brs, feats, targets, iters = [], [], [], [], [], []
     t0 = time.perf_counter()
     pos = 0
     for k, s in enumerate(scales):
@@ -281,12 +282,24 @@ def make_training_set(net, structure, hoods, n_samples=20000, levels=20, seed=3,
         brs.append(c)
         feats.append(f)
         targets.append((local_change(d, hoods, c) / flow_scale(f)[:, None]).astype(np.float32))
+        iters.append(res.total_iterations[ok].astype(np.int32))
         if verbose:
             print(f"  level {k}: scale {s:.3f}, {ok.sum()}/{len(ctg)} converged, {time.perf_counter() - t0:.1f}s",
                   flush=True)
     return dict(x_base=np.array(xb_all), level=np.concatenate(lvl), branch=np.concatenate(brs).astype(np.int32),
-                features=np.concatenate(feats), target=np.concatenate(targets), scales=scales,
-                seconds=time.perf_counter() - t0)
+                features=np.concatenate(feats), target=np.concatenate(targets), iterations=np.concatenate(iters),
+                scales=scales, seconds=time.perf_counter() - t0)
+
+
+def iteration_costs(data, n_branch):
+    """Expected Newton iterations of each branch's outage: the mean over the training set's solves of it (at their
+    load scalings), the mean of all solves for a branch the set did not cover."""
+    total = np.bincount(data["branch"], weights=data["iterations"], minlength=n_branch)
+    count = np.bincount(data["branch"], minlength=n_branch)
+    cost = np.full(n_branch, float(data["iterations"].mean()))
+    seen = count > 0
+    cost[seen] = total[seen] / count[seen]
+    return cost
 
 
 def train(mlp, data, epochs=60, batch=128, lr=2e-3, seed=0, verbose=False):

@@ -219,3 +219,20 @@ def test_n1_with_max_voltage_change_scaling_matches_oracle(grid):
     assert ref["total_iterations"].sum() > plain["total_iterations"].sum()
     conv = res.status == 0
     compare_states(res.x[conv], ref["x"][conv], "n1 grid2000 MAX_VOLTAGE_CHANGE")
+
+
+def test_longest_first_order_leaves_every_result_unchanged(grid):
+    """Solving in a cost order (longest_first) changes which scenarios share a tile and when they start, not what
+    each lane computes: per-contingency results are bitwise those of the given order."""
+    net = grid("grid2000")
+    print("[parity] grid2000 cost order", flush=True)
+    osys, xb, mb, tb, ltb = base_case(net, "grid2000")
+    st = N.Structure.from_network(net)
+    sa = GpuBatchedSecurityAnalysisSolver(st, net.reactive_limits(),
+                                          GpuLoadFlowParameters(linearSolver="BATCHED_LU_FP32_REFINED"))
+    ctg = gridgen.sample_contingencies(net, 64, 0.2, 11)
+    a = sa.solve_batch_n1(xb, tb, mb, ltb, ctg)
+    cost = np.random.default_rng(0).random(len(ctg))
+    b = sa.solve_batch_n1(xb, tb, mb, ltb, ctg, cost=cost)
+    assert np.array_equal(a.status, b.status) and np.array_equal(a.total_iterations, b.total_iterations)
+    np.testing.assert_array_equal(a.x, b.x)
