@@ -187,6 +187,11 @@ typedef struct {
      * NewtonSettingsNative.setBatchLineSearch (NewtonSettingsNative.java:56) */
     int line_search_max_iteration;
     double line_search_step_fold;
+    /* > 0: OLF's stopping criterion ||F||_2 < sqrt(conv_eps_per_eq^2 * active rows) instead of ||F||inf < tol
+     * (solveBatchContext's convEpsPerEq, BatchSolveNative.java:117); active_row_counts: NULL (every lane n) or
+     * [n_scenarios] equations of each scenario's system (NewtonSettingsNative.setBatchActiveRowCounts) */
+    double conv_eps_per_eq;
+    const int *active_row_counts;
 } olfhip_solve_inputs;
 
 /* Outputs (any may be NULL): results [n_scenarios][n+3] = (x, iterations of the last solve, ||F||inf, status) in

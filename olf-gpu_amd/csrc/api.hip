@@ -299,6 +299,7 @@ struct ContextImpl {
     DBuf dsNode, dsFactor, dsBase, dsMaxD, dsMinD, dsDesc;
     DBuf loopAt, sweepChanged, dsFailed, slackP, dsRetain, dsPrev;
     DBuf xPrev, lsLast, lsHas;   // LINE_SEARCH state (xPrev allocated by the first solve that asks for it)
+    DBuf nActive;                // active equations per lane (the 2-norm criterion)
     // install inputs
     DBuf x0Base, x0Per, targetBase, modeBase, modeEntries, targetIdx, targetVal, seed, states;
     int lastInstalled = 0;
@@ -353,6 +354,7 @@ struct ContextImpl {
         S.xPrev = xPrev.as<double>();
         S.lsLast = lsLast.as<double>();
         S.lsHas = lsHas.as<signed char>();
+        S.nActive = nActive.as<int>();
         return S;
     }
     DevRL devRL() const {
@@ -421,6 +423,7 @@ static ContextImpl *createContext(StructureImpl *st, int nScenarios, int mode, c
     c->dsPrev.alloc(8);
     c->lsLast.alloc(lanes * 8);
     c->lsHas.alloc(lanes);
+    c->nActive.alloc(lanes * 4);
     // reactive limits, entry buses in node numbering
     if (rl && rl->n_entries > 0) {
         if (!rl->voltage_row || !rl->min_q || !rl->max_q || !rl->load_q || !rl->target_v)
@@ -535,6 +538,19 @@ static void solve(ContextImpl *c, const olfhip_solve_inputs *in, olfhip_solve_ou
     if (in->distributed_slack && c->slackNode < 0)
         throw ArgumentError("distributed_slack is set but the context has no distributed slack (olfhip_set_distributed_slack)");
     if (in->distributed_slack && !(in->slack_dist_tol >= 0.0)) throw ArgumentError("slack_dist_tol must be >= 0");
+    if (!(in->conv_eps_per_eq >= 0.0)) throw ArgumentError("conv_eps_per_eq must be >= 0");
+    prm.convEps = in->conv_eps_per_eq;
+    if (prm.convEps > 0.0) {
+        std::vector<int> na(lanes, n);
+        for (int k = 0; k < K; k++) {
+            if (!in->active_row_counts) break;
+            const int a = in->active_row_counts[k];
+            if (a <= 0 || a > n) throw ArgumentError("active_row_counts[" + std::to_string(k) + "] out of range");
+            na[k] = a;
+        }
+        HIP_OK(hipMemcpyAsync(c->nActive.p, na.data(), lanes * 4, hipMemcpyHostToDevice, s));
+        HIP_OK(hipStreamSynchronize(s));
+    }
     prm.lsMax = in->line_search_max_iteration > 0 ? in->line_search_max_iteration : 10;
     prm.lsFold = in->line_search_step_fold > 1.0 ? in->line_search_step_fold : 4.0 / 3.0;
     if (prm.scaling == OLFHIP_SCALING_LINE_SEARCH) c->xPrev.alloc((size_t)c->nTiles * c->T * P.nb * 2 * 8);

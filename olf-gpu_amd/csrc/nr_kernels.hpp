@@ -92,6 +92,7 @@ struct DevState {
     // the current solve and whether there is one
     double *xPrev, *lsLast;
     signed char *lsHas;
+    const int *nActive;   // active equations per lane (the 2-norm criterion's count)
     // opt-in section timing of one tile: prof[0] counts the Newton steps that recorded, then PROF_SECTIONS stamps per
     // step (wall clock) from prof[PROF_SECTIONS] on, at most profCap steps
     unsigned long long *prof;
@@ -130,6 +131,7 @@ struct NrParams {
     int refine;
     int batchOfOne;
     int stepsPerLaunch;   // Newton steps one launch may run per tile
+    double convEps;       // > 0: converged when ||F||_2 < sqrt(convEps^2 nActive) (OLF's criterion), else ||F||inf < tol
     int lsMax;            // LINE_SEARCH (scaling 1): backtracks per test and the step fold, mu = lsFold^-b
     double lsFold;
     int distributedSlack; // run the distributed-slack outer loop (before the reactive limits)
@@ -1180,16 +1182,18 @@ __global__ __launch_bounds__(MAX_THREADS) void nrStepKernel(DevPlan P, DevState 
             sNorm[tid] = red[tid];
             sFin[tid] = redI[tid];
         }
+        if (prm.scaling == 1 || prm.convEps > 0.0) {
+            __syncthreads();
+            red[tid] = psq;
+            laneReduce<T, false>(red, tid, nt);
+            if (tid < T) sSsq[tid] = red[tid];
+        }
         if (prm.scaling == 1) {
             // LINE_SEARCH (native/cuda/solver/newton_iteration.cu:497-530): after every step but a solve's first, a
             // step that did not lower ||F||_2 (or made F non-finite) is walked back to x_prev + mu dx,
             // mu = lsFold^-b, b = 1..lsMax, refilled at each point until ||F||_2 is below the last tested one
-            __syncthreads();
-            red[tid] = psq;
-            laneReduce<T, false>(red, tid, nt);
             if (tid < T) {
                 const int lane = tile * T + tid;
-                sSsq[tid] = red[tid];
                 sBt[tid] = sFill[tid] && sIt[tid] > 0 && S.lsHas[lane] &&
                            (!sFin[tid] || sSsq[tid] >= S.lsLast[lane]);
             }
@@ -1242,9 +1246,14 @@ __global__ __launch_bounds__(MAX_THREADS) void nrStepKernel(DevPlan P, DevState 
         const bool fin = sFin[tid] != 0;
         const int it = sIt[tid];
         sNorm[tid] = norm;
+        // the criterion (native/cuda/solver/newton_iteration.cu:241-300): ||F||inf < tol, or with convEps OLF's
+        // ||F||_2 < sqrt(convEps^2 nActive)
+        const bool below = prm.convEps > 0.0
+                               ? sqrt(sSsq[tid]) < sqrt(prm.convEps * prm.convEps * (double)S.nActive[tile * T + tid])
+                               : norm < prm.tol;
         if (!fin) {
             sStatus[tid] = ST_DIVERGED;
-        } else if (it > 0 && norm < prm.tol && it < prm.maxIter) {
+        } else if (it > 0 && below && it < prm.maxIter) {
             sConv[tid] = 1;
         } else if (it > 0 && it >= prm.maxIter) {
             const bool bad = !(sRatio[tid] <= LIN_SOLVE_FAIL_RATIO);

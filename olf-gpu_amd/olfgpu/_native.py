@@ -62,7 +62,8 @@ class SolveInputs(C.Structure):
                 ("states_to_host", C.c_int), ("x0_per_scenario_device", C.c_void_p),
                 ("distributed_slack", C.c_int), ("slack_dist_tol", C.c_double),
                 ("reject_slack_distribution_failure", C.c_int), ("line_search_max_iteration", C.c_int),
-                ("line_search_step_fold", C.c_double)]
+                ("line_search_step_fold", C.c_double), ("conv_eps_per_eq", C.c_double),
+                ("active_row_counts", C.c_void_p)]
 
 
 class DistributedSlackDesc(C.Structure):
@@ -276,7 +277,7 @@ class BatchContext:
                     max_dphi=0.174533, max_outer_iterations=20, max_pq_pv_switch=3, skip_reactive_limits=False,
                     states_to_host=True, x0_per_scenario_device=None, distributed_slack=False, slack_dist_tol=0.0,
                     reject_slack_distribution_failure=False, line_search_max_iteration=10,
-                    line_search_step_fold=4.0 / 3.0):
+                    line_search_step_fold=4.0 / 3.0, conv_eps_per_eq=0.0, active_row_counts=None):
         keep = []
 
         def arr(a, dt):
@@ -306,12 +307,13 @@ class BatchContext:
             ob = arr(ob, np.int32)
         xp = arr(x0_per_scenario, np.float64)
         sd = arr(reactive_limits_seed_state, np.int8)
+        ac = arr(active_row_counts, np.int32)
         si = SolveInputs(n_scenarios, ptr(x0), ptr(tg), ptr(md), 0 if rme is None else rme.shape[0], ptr(rme),
                          0 if ti is None else ti.size, ptr(ti), ptr(tv), mo, ptr(ob), ptr(xp), ptr(sd), max_iter, tol,
                          reactive_limit_tol, state_vector_scaling_mode, max_dv, max_dphi, max_outer_iterations,
                          max_pq_pv_switch, int(skip_reactive_limits), int(states_to_host), x0_per_scenario_device,
                          int(distributed_slack), float(slack_dist_tol), int(reject_slack_distribution_failure),
-                         int(line_search_max_iteration), float(line_search_step_fold))
+                         int(line_search_max_iteration), float(line_search_step_fold), float(conv_eps_per_eq), ptr(ac))
         return si, keep
 
     def set_distributed_slack(self, ds):

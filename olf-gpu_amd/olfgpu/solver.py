@@ -95,6 +95,7 @@ class NewtonRaphsonParameters:
 
     maxIterations: int = 15
     maxMismatch: float = 1e-4                 # ||F||inf stopping threshold, pu
+    convEpsPerEq: float = 0.0                 # > 0: OLF's ||F||_2 < sqrt(convEpsPerEq^2 * equations) instead
     stateVectorScalingMode: str = "NONE"      # NONE, LINE_SEARCH or MAX_VOLTAGE_CHANGE
     maxVoltageChangeV: float = 0.1
     maxVoltageChangePhi: float = 0.174533
@@ -168,7 +169,7 @@ class GpuNewtonRaphson:
             max_outer_iterations=nr.maxOuterLoopIterations, max_pq_pv_switch=nr.maxPqPvSwitch,
             skip_reactive_limits=not nr.reactiveLimits, **nr.slack_inputs(self.distributed_slack),
             line_search_max_iteration=nr.lineSearchStateVectorScalingMaxIteration,
-            line_search_step_fold=nr.lineSearchStateVectorScalingStepFold)
+            line_search_step_fold=nr.lineSearchStateVectorScalingStepFold, conv_eps_per_eq=nr.convEpsPerEq)
         r = self.context.solve(inp)
         n_rl = 0 if self.reactive_limits is None else len(self.reactive_limits["voltage_row"])
         m, t, lt = self.context.read_outer_loop_state(1, n_rl)
@@ -314,7 +315,8 @@ class GpuBatchedSecurityAnalysisSolver:
                 states_to_host=states_to_host, x0_per_scenario_device=x0_dev,
                 **nr.slack_inputs(self.distributed_slack),
                 line_search_max_iteration=nr.lineSearchStateVectorScalingMaxIteration,
-                line_search_step_fold=nr.lineSearchStateVectorScalingStepFold)
+                line_search_step_fold=nr.lineSearchStateVectorScalingStepFold, conv_eps_per_eq=nr.convEpsPerEq,
+                active_row_counts=(n - 2 * cnt).astype(np.int32))
             ctx = self.context(len(part))
             t0 = time.perf_counter()
             r = ctx.solve(inp, want_states=states_to_host)

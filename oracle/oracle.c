@@ -778,7 +778,14 @@ void oracle_nr_solve(const oracle_sys *s, const oracle_nr_params *prm, const ora
             res->iterations = it;
             if (!finite) { res->status = ORC_DIVERGED; ended = 1; break; }
             if (it > 0) {
-                if (norm < prm->tol && it < prm->max_iter) { converged = 1; break; }
+                int below = norm < prm->tol;
+                if (prm->conv_eps_per_eq > 0.0) {
+                    double ssq = 0.0;
+                    for (int i = 0; i < n; i++) ssq += F[i] * F[i];
+                    const double na = prm->n_active > 0 ? (double)prm->n_active : (double)n;
+                    below = sqrt(ssq) < sqrt(prm->conv_eps_per_eq * prm->conv_eps_per_eq * na);
+                }
+                if (below && it < prm->max_iter) { converged = 1; break; }
                 if (it == prm->max_iter) {
                     int bad = !(last_ratio <= LIN_SOLVE_FAIL_RATIO);
                     res->status = bad ? (prm->batch_of_one ? ORC_LINEAR_SOLVE_FAILED : ORC_REJECTED)
@@ -943,6 +950,7 @@ int oracle_n1(const oracle_sys *s, const oracle_nr_params *prm, const oracle_rl 
             oracle_result res;
             oracle_nr_params p = *prm;
             p.batch_of_one = 0;
+            p.n_active = n - 2 * ni;
             oracle_nr_solve(s, &p, rl, ds, outs, n_out, ni > 0 ? isl : NULL, x, mode, target, lt, sc, &res);
             if (target_out) memcpy(target_out + (size_t)c * n, target, sizeof(double) * (size_t)n);
             status[c] = res.status;

@@ -69,6 +69,9 @@ typedef struct {
     int batch_of_one;      /* a bad last solve is LINEAR_SOLVE_FAILED (1) or REJECTED (0) */
     int ls_max_iteration;  /* LINE_SEARCH backtracks per test (<= 0: 10) */
     double ls_step_fold;   /* LINE_SEARCH: mu = ls_step_fold^-b (<= 1: 4/3) */
+    double conv_eps_per_eq;/* > 0: OLF's criterion ||F||_2 < sqrt(conv_eps_per_eq^2 n_active) instead of ||F||inf < tol
+                              (newton_iteration.cu:241-300) */
+    int n_active;          /* active equations of the system (<= 0: n; oracle_n1 uses n - 2 islanded buses) */
 } oracle_nr_params;
 
 typedef struct {

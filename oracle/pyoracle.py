@@ -25,7 +25,8 @@ class NrParams(C.Structure):
     _fields_ = [("max_iter", C.c_int), ("tol", C.c_double), ("reactive_limits", C.c_int), ("rl_tol", C.c_double),
                 ("max_outer", C.c_int), ("max_switch", C.c_int), ("scaling_mode", C.c_int),
                 ("max_dv", C.c_double), ("max_dphi", C.c_double), ("batch_of_one", C.c_int),
-                ("ls_max_iteration", C.c_int), ("ls_step_fold", C.c_double)]
+                ("ls_max_iteration", C.c_int), ("ls_step_fold", C.c_double), ("conv_eps_per_eq", C.c_double),
+                ("n_active", C.c_int)]
 
 
 class Rl(C.Structure):
@@ -120,10 +121,10 @@ def min_degree(adjptr, adj):
 
 def params(max_iter=15, tol=1e-4, reactive_limits=True, rl_tol=1e-4, max_outer=20, max_switch=3,
            scaling_mode=0, max_dv=0.1, max_dphi=0.174533, batch_of_one=True, ls_max_iteration=10,
-           ls_step_fold=4.0 / 3.0) -> NrParams:
-    """scaling_mode: 0 NONE, 1 LINE_SEARCH, 2 MAX_VOLTAGE_CHANGE"""
+           ls_step_fold=4.0 / 3.0, conv_eps_per_eq=0.0, n_active=0) -> NrParams:
+    """scaling_mode: 0 NONE, 1 LINE_SEARCH, 2 MAX_VOLTAGE_CHANGE; conv_eps_per_eq > 0: the 2-norm criterion"""
     return NrParams(max_iter, tol, int(reactive_limits), rl_tol, max_outer, max_switch, scaling_mode, max_dv,
-                    max_dphi, int(batch_of_one), ls_max_iteration, ls_step_fold)
+                    max_dphi, int(batch_of_one), ls_max_iteration, ls_step_fold, conv_eps_per_eq, n_active)
 
 
 class OracleSystem:

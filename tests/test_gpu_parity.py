@@ -236,3 +236,24 @@ def test_longest_first_order_leaves_every_result_unchanged(grid):
     b = sa.solve_batch_n1(xb, tb, mb, ltb, ctg, cost=cost)
     assert np.array_equal(a.status, b.status) and np.array_equal(a.total_iterations, b.total_iterations)
     np.testing.assert_array_equal(a.x, b.x)
+
+
+@pytest.mark.parametrize("lu", ["BATCHED_LU", "BATCHED_LU_FP32_REFINED"])
+def test_n1_with_two_norm_stopping_criterion(grid, lu):
+    """OLF's ||F||_2 < sqrt(eps^2 * equations) criterion (convEpsPerEq = 1e-4, OLF's default), with each
+    contingency's equation count lowered by its islanded buses: statuses, FP64 iteration counts and states as the
+    oracle's N-1 under the same rule."""
+    net = grid("grid2000")
+    print(f"[parity] grid2000 N-1 two-norm criterion {lu}", flush=True)
+    osys, xb, mb, tb, ltb = base_case(net, "grid2000")
+    ctg = gridgen.sample_contingencies(net, 48, 0.25, 13)
+    ref = osys.n1(xb, mb, tb, ltb, ctg, prm=po.params(batch_of_one=False, conv_eps_per_eq=1e-4, tol=0.0))
+    st = N.Structure.from_network(net)
+    sa = GpuBatchedSecurityAnalysisSolver(st, net.reactive_limits(), GpuLoadFlowParameters(linearSolver=lu),
+                                          NewtonRaphsonParameters(convEpsPerEq=1e-4))
+    r = sa.solve_batch_n1(xb, tb, mb, ltb, ctg)
+    assert np.array_equal(r.status, ref["status"])
+    if lu == "BATCHED_LU":
+        assert np.array_equal(r.total_iterations, ref["total_iterations"])
+    ok = ref["status"] == 0
+    compare_states(r.x[ok], ref["x"][ok], f"two-norm {lu}")
