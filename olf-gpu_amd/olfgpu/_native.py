@@ -164,7 +164,8 @@ class Structure:
 
     @classmethod
     def from_network(cls, net):
-        return cls(net.n, net.pi_model(), net.branch_indices())
+        sp_, si_ = net.shunts() if hasattr(net, "shunts") else (None, None)
+        return cls(net.n, net.pi_model(), net.branch_indices(), sp_, si_)
 
     def close(self):
         if getattr(self, "h", None):

@@ -62,6 +62,8 @@ class Network:
     qload: np.ndarray         # [N] MVar
     pgen: np.ndarray          # [N] MW
     pos: np.ndarray = field(repr=False, default=None)
+    shunt_g: np.ndarray | None = field(repr=False, default=None)   # [N] pu bus shunt conductance (None: no shunts)
+    shunt_b: np.ndarray | None = field(repr=False, default=None)   # [N] pu bus shunt susceptance
 
     @property
     def n_branch(self) -> int:
@@ -122,6 +124,29 @@ class Network:
         target[2 * slack] = 0.0
         mode[2 * slack] = FIXED_VARIABLE
         return x0, target, mode, var_type
+
+    def shunts(self):
+        """Bus-shunt pack for the structure: per shunt (g, b) in pu and its (P row, Q row, V column) stable rows;
+        (None, None) without shunts."""
+        if self.shunt_g is None:
+            return None, None
+        buses = np.nonzero((self.shunt_g != 0) | (self.shunt_b != 0))[0]
+        params = np.stack([self.shunt_g[buses], self.shunt_b[buses]], axis=1)
+        idx = np.stack([2 * buses, 2 * buses + 1, 2 * buses + 1], axis=1).astype(np.int32)
+        return params, idx
+
+    def with_shunts(self, fraction: float = 0.05, seed: int = 17) -> "Network":
+        """The same grid with bus shunts on a fraction of the non-slack buses (not part of the workload definition;
+        for the shunt family's parity): b ~ U(-0.15, 0.3) pu (reactors and capacitors), g ~ U(0, 0.005) pu."""
+        import dataclasses
+        rng = np.random.default_rng(seed)
+        m = int(round(fraction * (self.n_bus - 1)))
+        buses = 1 + rng.permutation(self.n_bus - 1)[:m]
+        g = np.zeros(self.n_bus)
+        b = np.zeros(self.n_bus)
+        g[buses] = rng.uniform(0.0, 0.005, m)
+        b[buses] = rng.uniform(-0.15, 0.3, m)
+        return dataclasses.replace(self, shunt_g=g, shunt_b=b)
 
     def reactive_limits(self):
         """Reactive-limit entries of the PV buses (pu): voltage row, minQ, maxQ, loadQ, targetV, bus."""

@@ -77,6 +77,7 @@ struct oracle_sys {
     int *diag;           /* CSR position of (r, r) */
     int *pos;            /* 16 CSR positions per branch: rows Pa Qa Pb Qb x cols pha va phb vb */
     int *colorder;       /* scalar column order for the LU (bus min degree, phi then V) */
+    double *shunt_g, *shunt_b;   /* per bus, pu (NULL: no shunts) */
 };
 
 static int find_col(const int *colidx, int lo, int hi, int c)
@@ -197,6 +198,7 @@ void oracle_sys_free(oracle_sys *s)
     if (!s) return;
     free(s->p8); free(s->bus1); free(s->bus2); free(s->adjptr); free(s->adj);
     free(s->rowptr); free(s->colidx); free(s->diag); free(s->pos); free(s->colorder);
+    free(s->shunt_g); free(s->shunt_b);
     free(s);
 }
 
@@ -216,6 +218,18 @@ static int is_out(int e, const int *out_br, int n_out)
 }
 
 /* reference equations/fill_launch.cu:82-126 (F = -target, family sums, identity rows). */
+void oracle_sys_set_shunts(oracle_sys *s, const double *g, const double *b)
+{
+    free(s->shunt_g);
+    free(s->shunt_b);
+    s->shunt_g = s->shunt_b = NULL;
+    if (!g || !b) return;
+    s->shunt_g = (double *)malloc(sizeof(double) * (size_t)s->n_bus);
+    s->shunt_b = (double *)malloc(sizeof(double) * (size_t)s->n_bus);
+    memcpy(s->shunt_g, g, sizeof(double) * (size_t)s->n_bus);
+    memcpy(s->shunt_b, b, sizeof(double) * (size_t)s->n_bus);
+}
+
 void oracle_fill(const oracle_sys *s, const double *x, const int *mode, const double *target,
                  const int *out_br, int n_out, double *F, double *qinj, double *J)
 {
@@ -246,6 +260,20 @@ void oracle_fill(const oracle_sys *s, const double *x, const int *mode, const do
             J[p[8]] += o[17]; J[p[9]] += o[15]; J[p[10]] += o[18]; J[p[11]] += o[16];
             /* row Qb */
             J[p[12]] += o[24]; J[p[13]] += o[22]; J[p[14]] += o[25]; J[p[15]] += o[23];
+        }
+    }
+    if (s->shunt_g) {
+        /* bus shunt (reference generated/shunt.cuh:47-60): p = g v^2, q = -b v^2 */
+        for (int a = 0; a < s->n_bus; a++) {
+            const double v = x[2 * a + 1], g = s->shunt_g[a], bb = s->shunt_b[a];
+            if (g == 0.0 && bb == 0.0) continue;
+            F[2 * a] += g * v * v;
+            F[2 * a + 1] -= bb * v * v;
+            if (qinj) qinj[a] -= bb * v * v;
+            if (J) {
+                J[s->diag[2 * a] + 1] += 2.0 * g * v;      /* row P_a, column V_a (right after column phi_a) */
+                J[s->diag[2 * a + 1]] -= 2.0 * bb * v;     /* row Q_a, column V_a */
+            }
         }
     }
     for (int r = 0; r < n; r++) {
@@ -630,6 +658,7 @@ double oracle_bus_p(const oracle_sys *s, const double *x, const int *out_br, int
         oracle_closed_branch(s->p8 + 8 * e, x[2 * a + 1], x[2 * a], x[2 * b + 1], x[2 * b], o);
         p += a == bus ? o[0] : o[14];
     }
+    if (s->shunt_g) p += s->shunt_g[bus] * x[2 * bus + 1] * x[2 * bus + 1];
     return p;
 }
 

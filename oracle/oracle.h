@@ -34,6 +34,9 @@ typedef struct oracle_sys oracle_sys;
 /* Stable-row system over n = 2*n_bus rows with the superset CSR pattern (every row carries its bus's
  * and its neighbours' phi/V columns), reference structure/csr_pattern.cu:75-140. */
 oracle_sys *oracle_sys_create(int n_bus, int n_branch, const double *p8, const int *idx12);
+/* Bus shunts, per bus conductance g and susceptance b in pu (NULL clears): p = g v^2, q = -b v^2 (reference
+ * equations/generated/shunt.cuh:47-60). */
+void oracle_sys_set_shunts(oracle_sys *s, const double *g, const double *b);
 void oracle_sys_free(oracle_sys *s);
 int oracle_sys_nnz(const oracle_sys *s);
 void oracle_sys_pattern(const oracle_sys *s, int *rowptr, int *colidx);

@@ -76,6 +76,7 @@ def lib():
         L.oracle_sys_create.restype = C.c_void_p
         L.oracle_sys_create.argtypes = [C.c_int, C.c_int, _dp, _ip]
         L.oracle_sys_free.argtypes = [C.c_void_p]
+        L.oracle_sys_set_shunts.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.oracle_sys_nnz.argtypes = [C.c_void_p]
         L.oracle_sys_pattern.argtypes = [C.c_void_p, _ip, _ip]
         L.oracle_fill.argtypes = [C.c_void_p, _dp, _ip, _dp, _ip, C.c_int, _dp, _dp, C.c_void_p]
@@ -136,6 +137,9 @@ class OracleSystem:
         self.p8 = np.ascontiguousarray(net.pi_model())
         self.idx = np.ascontiguousarray(net.branch_indices())
         self.h = lib().oracle_sys_create(net.n_bus, net.n_branch, self.p8, self.idx)
+        if getattr(net, "shunt_g", None) is not None:
+            self._shunts = (np.ascontiguousarray(net.shunt_g, np.float64), np.ascontiguousarray(net.shunt_b, np.float64))
+            lib().oracle_sys_set_shunts(self.h, self._shunts[0].ctypes.data, self._shunts[1].ctypes.data)
         self.nnz = lib().oracle_sys_nnz(self.h)
         self.rowptr = np.zeros(self.n + 1, np.int32)
         self.colidx = np.zeros(self.nnz, np.int32)

@@ -257,3 +257,29 @@ def test_n1_with_two_norm_stopping_criterion(grid, lu):
         assert np.array_equal(r.total_iterations, ref["total_iterations"])
     ok = ref["status"] == 0
     compare_states(r.x[ok], ref["x"][ok], f"two-norm {lu}")
+
+
+@pytest.mark.parametrize("lu", ["BATCHED_LU", "BATCHED_LU_FP32_REFINED"])
+def test_bus_shunts_base_and_n1(grid, lu):
+    """The bus-shunt family (the workload has none; Network.with_shunts puts reactors and capacitors on 5 % of the
+    buses): base case and N-1 against the oracle's shunt fill (pinned in tests/test_oracle_shunts.py)."""
+    net = grid("grid2000").with_shunts(0.05)
+    print(f"[parity] grid2000 with shunts {lu}", flush=True)
+    o = po.OracleSystem(net)
+    x0, tgt, md, _ = net.system()
+    xb, mb, tb, ltb, rb = o.solve(x0, md, tgt)
+    assert rb.status == 0
+    st = N.Structure.from_network(net)
+    rl = net.reactive_limits()
+    g = GpuNewtonRaphson(st, rl, GpuLoadFlowParameters(linearSolver=lu)).run(x0, tgt, md)
+    assert g.status == "CONVERGED"
+    compare_states(g.x, xb, f"shunt base {lu}")
+    if lu == "BATCHED_LU":
+        assert g.total_iterations == rb.total_iterations
+    ctg = gridgen.sample_contingencies(net, 40, 0.2, 21)
+    ref = o.n1(xb, mb, tb, ltb, ctg)
+    sa = GpuBatchedSecurityAnalysisSolver(st, rl, GpuLoadFlowParameters(linearSolver=lu))
+    r = sa.solve_batch_n1(xb, tb, mb, ltb, ctg)
+    assert np.array_equal(r.status, ref["status"])
+    ok = ref["status"] == 0
+    compare_states(r.x[ok], ref["x"][ok], f"shunt N-1 {lu}")
