@@ -71,8 +71,10 @@ def main():
     ap.add_argument("--threads-per-tile", type=int, default=0)
     ap.add_argument("--autotune", action="store_true", help="time every tile shape first and use the fastest")
     ap.add_argument("--no-warm-start", action="store_true", help="base-case start only (no learned warm start)")
-    ap.add_argument("--train-samples", type=int, default=20000)
+    ap.add_argument("--train-samples", type=int, default=40000)
     ap.add_argument("--train-epochs", type=int, default=150)
+    ap.add_argument("--dump-schedule", default=None, help="write the solve order's costs and the timed solve's "
+                    "per-contingency iterations to this .npz")
     ap.add_argument("--order", choices=("cost", "given"), default="cost",
                     help="solve order: longest first by the training solves' iteration counts, or as sampled")
     args = ap.parse_args()
@@ -189,6 +191,9 @@ def main():
     value = K * args.steps / wall
     status = last["status"]
     lane_iters = int(last["total_iterations"].sum())
+    if args.dump_schedule:
+        np.savez(args.dump_schedule, ctg=ctg, total_iterations=last["total_iterations"],
+                 cost=(W.iteration_costs(data, net.n_branch)[ctg] if data is not None else np.zeros(len(ctg))))
     if warm_info is not None:
         warm_info["fallbacks_per_step"] = fallbacks / args.steps
 
