@@ -38,6 +38,15 @@ extern "C" {
 /* linear-solve modes (reference GpuLinearSolveMode; the reference's UBATCH/ROBUST are cuDSS modes) */
 #define OLFHIP_LU_FP64 0            /* batched block LU in FP64 */
 #define OLFHIP_LU_FP32_REFINED 1    /* FP32 factors and solves, one FP64-residual refinement step */
+/* The robust single-scenario mode, what the reference retries rejected scenarios in (GpuLinearSolveMode.ROBUST = 6,
+ * GpuLinearSolveMode.java:24-27; native/cuda/common/check.cuh:38-40; GpuNewtonRaphson.java:755-775,
+ * GpuBatchPrecomputer.java:635-790): one scenario per tile, FP64 factors with static pivoting (a nearly singular
+ * pivot block is perturbed by sqrt(eps) ||J|| instead of clamped) and every Newton step's linear solve finished by
+ * restarted GMRES on the exact FP64 Jacobian, preconditioned by those factors, to the relative residual below. */
+#define OLFHIP_LU_ROBUST 2
+#define OLFHIP_ROBUST_KRYLOV_DIM 30   /* GMRES basis vectors per cycle */
+#define OLFHIP_ROBUST_RESTARTS 4      /* cycles after the first */
+#define OLFHIP_ROBUST_TOL 1e-10       /* ||J dx + F||_2 / ||F||_2 target */
 
 /* state vector scaling: reference GpuStateVectorScalingMode */
 #define OLFHIP_SCALING_NONE 0

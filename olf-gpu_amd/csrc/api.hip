@@ -73,6 +73,7 @@ struct StructureImpl {
     Plan plan;
     std::vector<int> bus1, bus2;
     std::vector<int> incPtr, inc;    // branches incident to each bus (island search)
+    double jacobianScale = 1.0;      // twice the largest bus admittance sum: the order of ||J||inf near 1 pu
     int refs = 1;
     bool onDevice = false;
     int device = -1;
@@ -312,6 +313,7 @@ struct ContextImpl {
     int profTile = 0;                        // OLFHIP_PROFILE_TILE, default 0
     TopPlan top;                             // the dense top block of the step kernel
     std::vector<int> lastStatus;             // statuses of the last solve's scenarios
+    DBuf kryV, kryH;                         // robust mode: the GMRES basis and per-tile dense state
     std::vector<int> lastOutages;            // its outage lists, MAX_OUTAGES per lane (-1 padded)
 
     DevState devState() const {
@@ -345,6 +347,8 @@ struct ContextImpl {
         S.rlCount = rlCount.as<signed char>();
         S.rlPass = rlPass.as<signed char>();
         S.tileRunning = tileRunning.as<int>();
+        S.kryV = kryV.as<double>();
+        S.kryH = kryH.as<double>();
         S.loopAt = loopAt.as<signed char>();
         S.sweepChanged = sweepChanged.as<signed char>();
         S.dsFailed = dsFailed.as<signed char>();
@@ -377,19 +381,28 @@ struct ContextImpl {
 };
 
 static size_t luBytes(int mode) { return mode == OLFHIP_LU_FP32_REFINED ? 4 : 8; }
+static bool knownMode(int mode) {
+    return mode == OLFHIP_LU_FP64 || mode == OLFHIP_LU_FP32_REFINED || mode == OLFHIP_LU_ROBUST;
+}
+// Static pivoting threshold of the robust mode: sqrt(machine epsilon) times the Jacobian's scale (Li & Demmel's
+// choice of sqrt(eps) ||A||).
+static double robustPivotDelta(const StructureImpl *st) { return 1.4901161193847656e-08 * st->jacobianScale; }
 
 static ContextImpl *createContext(StructureImpl *st, int nScenarios, int mode, const olfhip_reactive_limits *rl,
                                   const olfhip_tuning *tuning) {
     if (!st) throw ArgumentError("structure is null");
     if (nScenarios <= 0) throw ArgumentError("n_scenarios must be positive");
-    if (mode != OLFHIP_LU_FP64 && mode != OLFHIP_LU_FP32_REFINED) throw ArgumentError("unknown linear-solve mode");
+    if (!knownMode(mode)) throw ArgumentError("unknown linear-solve mode");
+    if (mode == OLFHIP_LU_ROBUST && tuning && tuning->lanes_per_tile > 1)
+        throw ArgumentError("the robust mode solves one scenario per tile (lanes_per_tile must be 1)");
     st->ensureDevice();
     std::unique_ptr<ContextImpl> c(new ContextImpl());
     st->refs++;
     c->st = st;
     c->capacity = nScenarios;
     c->mode = mode;
-    c->T = (tuning && tuning->lanes_per_tile > 0) ? tuning->lanes_per_tile : defaultLanesPerTile(nScenarios);
+    c->T = mode == OLFHIP_LU_ROBUST ? 1
+           : (tuning && tuning->lanes_per_tile > 0) ? tuning->lanes_per_tile : defaultLanesPerTile(nScenarios);
     c->NT = (tuning && tuning->threads_per_tile > 0) ? tuning->threads_per_tile : 1024;
     withT(c->T, [](auto) {});
     if (c->NT != 256 && c->NT != 512 && c->NT != 1024) throw ArgumentError("threads_per_tile must be 256, 512 or 1024");
@@ -445,13 +458,20 @@ static ContextImpl *createContext(StructureImpl *st, int nScenarios, int mode, c
     c->rlType.alloc(ne);
     c->rlCount.alloc(ne);
     c->rlPass.alloc(ne);
-    withT(c->T, [&](auto TC) {
-        constexpr int T = decltype(TC)::value;
-        if (mode == OLFHIP_LU_FP32_REFINED)
-            planTop(st, reinterpret_cast<const void *>(&nrStepKernel<T, float>), T, c->NT, 4, c->top);
-        else
-            planTop(st, reinterpret_cast<const void *>(&nrStepKernel<T, double>), T, c->NT, 8, c->top);
-    });
+    if (mode == OLFHIP_LU_ROBUST) {
+        c->kryV.alloc((size_t)(OLFHIP_ROBUST_KRYLOV_DIM + 1) * nv * 8);
+        c->kryH.alloc((size_t)c->nTiles * kryHStride(OLFHIP_ROBUST_KRYLOV_DIM) * 8);
+        planTop(st, reinterpret_cast<const void *>(&nrStepKernel<1, double, true>), 1, c->NT, 8, c->top);
+        c->top.top.pivotDelta = robustPivotDelta(st);
+    } else {
+        withT(c->T, [&](auto TC) {
+            constexpr int T = decltype(TC)::value;
+            if (mode == OLFHIP_LU_FP32_REFINED)
+                planTop(st, reinterpret_cast<const void *>(&nrStepKernel<T, float>), T, c->NT, 4, c->top);
+            else
+                planTop(st, reinterpret_cast<const void *>(&nrStepKernel<T, double>), T, c->NT, 8, c->top);
+        });
+    }
     HIP_OK(hipDeviceSynchronize());
     return c.release();
 }
@@ -535,6 +555,9 @@ static void solve(ContextImpl *c, const olfhip_solve_inputs *in, olfhip_solve_ou
     prm.maxDphi = in->max_dphi;
     prm.refine = c->mode == OLFHIP_LU_FP32_REFINED;
     prm.batchOfOne = K == 1;
+    prm.kryDim = OLFHIP_ROBUST_KRYLOV_DIM;
+    prm.kryRestarts = OLFHIP_ROBUST_RESTARTS;
+    prm.kryTol = OLFHIP_ROBUST_TOL;
     if (in->distributed_slack && c->slackNode < 0)
         throw ArgumentError("distributed_slack is set but the context has no distributed slack (olfhip_set_distributed_slack)");
     if (in->distributed_slack && !(in->slack_dist_tol >= 0.0)) throw ArgumentError("slack_dist_tol must be >= 0");
@@ -593,10 +616,14 @@ static void solve(ContextImpl *c, const olfhip_solve_inputs *in, olfhip_solve_ou
             c->modeBase.as<int>());
         HIP_OK(hipGetLastError());
         auto enqueueStep = [&]() {
-            if (c->mode == OLFHIP_LU_FP32_REFINED)
+            if (c->mode == OLFHIP_LU_FP32_REFINED) {
                 nrStepKernel<T, float><<<c->nTiles, c->NT, c->top.ldsBytes, s>>>(dp, S, R, prm, c->top.top);
-            else
+            } else if (c->mode == OLFHIP_LU_ROBUST) {
+                if constexpr (T == 1)
+                    nrStepKernel<1, double, true><<<c->nTiles, c->NT, c->top.ldsBytes, s>>>(dp, S, R, prm, c->top.top);
+            } else {
                 nrStepKernel<T, double><<<c->nTiles, c->NT, c->top.ldsBytes, s>>>(dp, S, R, prm, c->top.top);
+            }
             countRunningKernel<<<1, 256, 0, s>>>(S.tileRunning, c->nTiles, c->count.as<int>());
             HIP_OK(hipMemcpyAsync(c->pinnedCount, c->count.p, sizeof(int), hipMemcpyDeviceToHost, s));
         };
@@ -953,7 +980,8 @@ static void luFactorizeAndSolve(StructureImpl *st, int mode, int capacity, const
                                 int nLanes, const int *laneMap, double *solutions, int *clampsOut) {
     if (!st || !values || !rhs || !laneMap || !solutions) throw ArgumentError("null argument");
     if (nLanes <= 0 || capacity <= 0) throw ArgumentError("lane counts must be positive");
-    if (mode != OLFHIP_LU_FP64 && mode != OLFHIP_LU_FP32_REFINED) throw ArgumentError("unknown mode");
+    if (!knownMode(mode)) throw ArgumentError("unknown mode");
+    if (mode == OLFHIP_LU_ROBUST) throw ArgumentError("the robust mode is a Newton-solve mode (its GMRES needs J)");
     st->ensureDevice();
     const Plan &P = st->plan;
     const int nnz = (int)P.colIdx.size();
@@ -1075,6 +1103,23 @@ int olfhip_create_structure(int n, int n_closed_branch, const double *closed_bra
             s->bus2[e] = closed_branch_indices[12 * e + 2] / 2;
         }
         const int nb = n / 2;
+        {
+            // bus admittance sums |y| (tap-scaled on side 1) + |g| + |b| of the branch ends and the shunts
+            std::vector<double> ysum(nb, 0.0);
+            for (int e = 0; e < n_closed_branch; e++) {
+                const double *p8 = closed_branch_parameters + 8 * e;
+                const double r1 = p8[6] != 0.0 ? p8[6] : 1.0;
+                ysum[s->bus1[e]] += (std::fabs(p8[0]) + std::fabs(p8[1]) + std::fabs(p8[3])) * r1 * r1;
+                ysum[s->bus2[e]] += std::fabs(p8[0]) + std::fabs(p8[2]) + std::fabs(p8[4]);
+            }
+            for (int q = 0; q < n_shunt; q++) {
+                const int b = shunt_indices[3 * q] / 2;
+                if (b >= 0 && b < nb) ysum[b] += std::fabs(shunt_parameters[2 * q]) + std::fabs(shunt_parameters[2 * q + 1]);
+            }
+            double mx = 0.0;
+            for (double v : ysum) mx = std::max(mx, v);
+            s->jacobianScale = mx > 0.0 ? 2.0 * mx : 1.0;
+        }
         s->incPtr.assign(nb + 1, 0);
         for (int e = 0; e < n_closed_branch; e++) {
             s->incPtr[s->bus1[e] + 1]++;
@@ -1131,7 +1176,10 @@ int olfhip_recommend_batch_size(const olfhip_structure *structure, int max_cap, 
         const Plan &P = reinterpret_cast<const StructureImpl *>(structure)->plan;
         size_t freeB = 0, totalB = 0;
         HIP_OK(hipMemGetInfo(&freeB, &totalB));
-        const size_t per = P.bytesPerScenario((int)luBytes(mode));
+        if (!knownMode(mode)) throw ArgumentError("unknown mode");
+        size_t per = P.bytesPerScenario((int)luBytes(mode));
+        if (mode == OLFHIP_LU_ROBUST)
+            per += (size_t)(OLFHIP_ROBUST_KRYLOV_DIM + 1) * P.nb * 2 * 8 + kryHStride(OLFHIP_ROBUST_KRYLOV_DIM) * 8;
         const size_t fit = (size_t)(0.7 * (double)freeB) / per;
         *out = (int)std::max<size_t>(1, std::min<size_t>(fit, (size_t)max_cap));
     });
@@ -1303,7 +1351,7 @@ int olfhip_autotune(olfhip_structure *structure, int mode, const olfhip_reactive
         double bestMs = 1e300;
         olfhip_tuning bestT{0, 0};
         for (int T : {1, 2, 4, 8, 16, 32}) {
-            if (T > 1 && T > K) break;
+            if (T > 1 && (T > K || mode == OLFHIP_LU_ROBUST)) break;   // the robust mode has one lane per tile
             for (int NT : {256, 512, 1024}) {
                 olfhip_tuning tu{T, NT};
                 std::unique_ptr<ContextImpl> c(createContext(st, K, mode, reactive_limits, &tu));

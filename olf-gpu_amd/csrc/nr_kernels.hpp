@@ -69,6 +69,10 @@ struct DevTop {
     // (base, rounds) per bucket.
     const int4 *contribPad;
     const int2 *bwdPad, *bucketPad, *bwdBucketPad;
+    // Static pivoting of the FP64 factors (0: off). A pivot block whose smallest singular value is below about
+    // pivotDelta gets pivotDelta added to its diagonal entries (sign-matched), so that the factors are those of
+    // J + E with E small and of rank 2 per such block; the robust solve's GMRES then corrects for E.
+    double pivotDelta;
 };
 
 struct DevState {
@@ -93,6 +97,9 @@ struct DevState {
     double *xPrev, *lsLast;
     signed char *lsHas;
     const int *nActive;   // active equations per lane (the 2-norm criterion's count)
+    // robust solve: the Krylov basis (kryDim + 1 vectors, each laid out like x over the whole context) and per tile
+    // the Hessenberg matrix, right-hand side, rotations and coefficients of its GMRES cycle
+    double *kryV, *kryH;
     // opt-in section timing of one tile: prof[0] counts the Newton steps that recorded, then PROF_SECTIONS stamps per
     // step (wall clock) from prof[PROF_SECTIONS] on, at most profCap steps
     unsigned long long *prof;
@@ -137,6 +144,9 @@ struct NrParams {
     int distributedSlack; // run the distributed-slack outer loop (before the reactive limits)
     double dsTol;         // its gate, slackBusPMaxMismatch in pu
     int dsReject;         // a failed distribution ends the scenario REJECTED
+    int kryDim;           // robust solve: GMRES basis size, restarts after the first cycle, relative 2-norm target
+    int kryRestarts;
+    double kryTol;
 };
 
 template <typename VT>
@@ -215,15 +225,29 @@ __device__ __forceinline__ void mulSubV(V2<VT> &v, const Blk<VT> &X, const V2<VT
 }
 
 // Inverse of a diagonal block with a static-pivot clamp (the reference clamps near-zero scalar pivots to +-tiny
-// rather than failing, native/cuda/lu/lu_pivot.cuh:12-21; here on the block determinant).
+// rather than failing, native/cuda/lu/lu_pivot.cuh:12-21; here on the block determinant). With delta > 0 (FP64
+// factors of the robust solve) a nearly singular block is perturbed first, Li & Demmel's static pivoting (GESP,
+// "Making sparse Gaussian elimination scalable by static pivoting", SC'98) on the 2x2 block: |det| <= delta ||D||_1
+// bounds the smallest singular value by about delta, and then delta goes onto each diagonal entry with its sign.
 template <typename VT>
-__device__ __forceinline__ Blk<VT> inv2(const Blk<VT> &D, bool &clamped) {
+__device__ __forceinline__ Blk<VT> inv2(Blk<VT> D, bool &clamped, VT delta = (VT)0) {
     VT det = D.a * D.d - D.b * D.c;
+    if (delta > (VT)0 && !(fabs(det) > delta * (fabs(D.a) + fabs(D.b) + fabs(D.c) + fabs(D.d)))) {
+        D.a += D.a < (VT)0 ? -delta : delta;
+        D.d += D.d < (VT)0 ? -delta : delta;
+        det = D.a * D.d - D.b * D.c;
+    }
     const VT tiny = (VT)1e-26;
     clamped = !(fabs(det) > tiny);
     if (clamped) det = det < (VT)0 ? -tiny : tiny;
     VT id = (VT)1 / det;
     return {D.d * id, -D.b * id, -D.c * id, D.a * id};
+}
+// The static-pivot perturbation of a context's factors: FP64 only (the FP32 factors keep the plain clamp).
+template <typename VT>
+__device__ __forceinline__ VT pivotDeltaOf(const DevTop &top) {
+    if constexpr (sizeof(VT) == 8) return top.pivotDelta;
+    else return (VT)0;
 }
 
 template <int T>
@@ -499,7 +523,7 @@ __device__ void factorForward(const DevPlan &P, const LevelMeta &M, const DevTop
                     const Blk<VT> Dk = {A.a + D.a, A.b + D.b, A.c + D.c, A.d + D.d};
                     const V2<VT> y = {(VT)(sign * b.x) + yk.p, (VT)(sign * b.y) + yk.q};
                     bool cl;
-                    const Blk<VT> Dinv = inv2(Dk, cl);
+                    const Blk<VT> Dinv = inv2(Dk, cl, pivotDeltaOf<VT>(top));
                     if (cl && clampLane) atomicAdd(&clampLane[t], 1);
                     stBlk<VT>(own, Dinv);
                     const V2<VT> zk = {Dinv.a * y.p + Dinv.b * y.q, Dinv.c * y.p + Dinv.d * y.q};
@@ -770,7 +794,7 @@ __device__ void topFactor(const DevPlan &P, const DevTop &top, const TopLds<T, V
     topFromBelow<T, VT, true>(P, top, L, tile, tid, nt, lu, z, laneOn);
     if (tid < T && (!laneOn || laneOn[tid])) {
         bool cl;
-        stBlk<VT>(L.Dinv(0, tid), inv2(ldBlk<VT>(L.M(0, 0, tid)), cl));
+        stBlk<VT>(L.Dinv(0, tid), inv2(ldBlk<VT>(L.M(0, 0, tid)), cl, pivotDeltaOf<VT>(top)));
         if (cl && clampLane) atomicAdd(&clampLane[tid], 1);
     }
     __syncthreads();
@@ -793,7 +817,7 @@ __device__ void topFactor(const DevPlan &P, const DevTop &top, const TopLds<T, V
             if (b == a + 1 && c == a + 1) {
                 // the next pivot is final: invert it for the next step
                 bool cl;
-                stBlk<VT>(L.Dinv(b, t), inv2(acc, cl));
+                stBlk<VT>(L.Dinv(b, t), inv2(acc, cl, pivotDeltaOf<VT>(top)));
                 if (cl && clampLane) atomicAdd(&clampLane[t], 1);
             }
         }
@@ -942,6 +966,140 @@ __device__ void applyJPass(const DevPlan &P, const DevState &S, int tile, int ti
         if (out) *reinterpret_cast<double2 *>(out + vi) = make_double2(rP, rQ);
         pmax = fmax(pmax, fmax(fabs(rP), fabs(rQ)));
     }
+}
+
+// Per-tile words of the robust solve's small dense GMRES state: the (m + 1) x m Hessenberg matrix (column-major), the
+// right-hand side g (m + 1), the rotations' cosines and sines (m each) and the coefficients y (m).
+__host__ __device__ inline size_t kryHStride(int m) { return (size_t)(m + 1) * m + (m + 1) + 3 * (size_t)m; }
+
+// The robust solve's correction of dx, the factors' solution of J dx = -F (one lane per tile): restarted GMRES on
+// the FP64 Jacobian, applied matrix-free at the lane's state, right-preconditioned by the lane's FP64 factors M
+// (Saad & Schultz 1986; the right-preconditioned form of Saad, Iterative Methods for Sparse Linear Systems, 2nd ed.,
+// Alg. 9.5), with modified Gram-Schmidt and Givens rotations. It solves J M^-1 u = -(J dx + F) and adds M^-1 u to dx
+// until ||J dx + F||_2 <= kryTol ||F||_2, the basis (kryDim vectors) is exhausted kryRestarts + 1 times, or the
+// basis breaks down. Where the factors were perturbed by static pivoting, J M^-1 is the identity plus a term of low
+// rank, which GMRES removes in a few iterations. Thread tid owns the nodes k = tid (mod nt) in every vector loop, the
+// same nodes applyJPass gives it, so a thread reads back what it wrote without a barrier. Ends with a barrier.
+template <int T>
+__device__ void gmresRefine(const DevPlan &P, const LevelMeta &LM, const DevTop &top, const TopLds<T, double> &TL,
+                            const DevState &S, const NrParams &prm, int tile, int tid, int nt, double *lu, double *y,
+                            double *z, const int *laneOn, double *red) {
+    static_assert(T == 1, "the robust solve runs one lane per tile");
+    __shared__ double sRes;
+    __shared__ int sK;
+    if (!laneOn[0]) return;   // uniform per block
+    const int nb = P.nb, m = prm.kryDim;
+    const size_t vs = (size_t)S.nTiles * nb * 2;
+    double *H = S.kryH + (size_t)tile * kryHStride(m);
+    double *g = H + (size_t)(m + 1) * m, *cs = g + (m + 1), *sn = cs + m, *yc = sn + m;
+    // vec(j), F, dx, zz and u point at this tile's part and take the node index; the solves and applyJPass take
+    // context-wide vectors and index them with vIdx themselves
+    auto vec = [&](int j) -> double2 * {
+        return reinterpret_cast<double2 *>(S.kryV + (size_t)j * vs + vIdx<1>(tile, nb, 0, 0));
+    };
+    const double2 *F = reinterpret_cast<const double2 *>(S.F + vIdx<1>(tile, nb, 0, 0));
+    double2 *dx = reinterpret_cast<double2 *>(S.dx + vIdx<1>(tile, nb, 0, 0));
+    double2 *zz = reinterpret_cast<double2 *>(z + vIdx<1>(tile, nb, 0, 0));
+    double2 *u = reinterpret_cast<double2 *>(S.r + vIdx<1>(tile, nb, 0, 0));
+    double ss = 0.0;
+    for (int k = tid; k < nb; k += nt) ss += F[k].x * F[k].x + F[k].y * F[k].y;
+    red[tid] = ss;
+    laneReduce<1, false>(red, tid, nt);
+    const double target = prm.kryTol * sqrt(red[0]);
+    for (int cycle = 0; cycle <= prm.kryRestarts; cycle++) {
+        __syncthreads();
+        // v0 = -(J dx + F) / beta
+        double pm = 0.0;
+        applyJPass<1, double>(P, S, tile, tid, nt, laneOn, S.dx, S.F, 1.0, S.kryV, pm);   // indexes from the base
+        double2 *v0 = vec(0);
+        ss = 0.0;
+        for (int k = tid; k < nb; k += nt) ss += v0[k].x * v0[k].x + v0[k].y * v0[k].y;
+        red[tid] = ss;
+        laneReduce<1, false>(red, tid, nt);
+        const double beta = sqrt(red[0]);
+        __syncthreads();
+        if (!(beta > target)) break;   // converged, or a non-finite residual that the guard reports
+        for (int k = tid; k < nb; k += nt) v0[k] = make_double2(-v0[k].x / beta, -v0[k].y / beta);
+        if (tid == 0) {
+            g[0] = beta;
+            sK = 0;
+        }
+        for (int j = 0; j < m; j++) {
+            __syncthreads();   // v_j complete before the solves read it anywhere
+            luForward<1, double>(P, LM, top, TL, tile, tid, nt, lu, y, S.kryV + (size_t)j * vs, 1.0, laneOn);
+            luBackward<1, double>(P, LM, top, TL, tile, tid, nt, lu, y, z, laneOn);
+            __syncthreads();
+            // w = J M^-1 v_j into v_{j+1}, orthogonalised against v_0..v_j
+            applyJPass<1, double>(P, S, tile, tid, nt, laneOn, z, S.F, 0.0, S.kryV + (size_t)(j + 1) * vs, pm);
+            double2 *w = vec(j + 1);
+            for (int i = 0; i <= j; i++) {
+                const double2 *vi = vec(i);
+                double d = 0.0;
+                for (int k = tid; k < nb; k += nt) d += w[k].x * vi[k].x + w[k].y * vi[k].y;
+                red[tid] = d;
+                laneReduce<1, false>(red, tid, nt);
+                const double h = red[0];
+                __syncthreads();
+                for (int k = tid; k < nb; k += nt) w[k] = make_double2(w[k].x - h * vi[k].x, w[k].y - h * vi[k].y);
+                if (tid == 0) H[(size_t)j * (m + 1) + i] = h;
+            }
+            double d = 0.0;
+            for (int k = tid; k < nb; k += nt) d += w[k].x * w[k].x + w[k].y * w[k].y;
+            red[tid] = d;
+            laneReduce<1, false>(red, tid, nt);
+            const double hn = sqrt(red[0]);
+            if (tid == 0) {
+                // the earlier rotations on column j, then the one that zeroes its subdiagonal
+                double *h = H + (size_t)j * (m + 1);
+                h[j + 1] = hn;
+                for (int i = 0; i < j; i++) {
+                    const double a = cs[i] * h[i] + sn[i] * h[i + 1];
+                    h[i + 1] = -sn[i] * h[i] + cs[i] * h[i + 1];
+                    h[i] = a;
+                }
+                const double den = hypot(h[j], h[j + 1]);
+                const double c = den > 0.0 ? h[j] / den : 1.0, sv = den > 0.0 ? h[j + 1] / den : 0.0;
+                cs[j] = c;
+                sn[j] = sv;
+                h[j] = den;
+                h[j + 1] = 0.0;
+                g[j + 1] = -sv * g[j];
+                g[j] = c * g[j];
+                sRes = fabs(g[j + 1]);
+                sK = j + 1;
+            }
+            if (hn > 0.0)
+                for (int k = tid; k < nb; k += nt) w[k] = make_double2(w[k].x / hn, w[k].y / hn);
+            __syncthreads();
+            if (!(sRes > target) || !(hn > 0.0)) break;
+        }
+        // y = R^-1 g over the sK columns built, then dx += M^-1 (V y)
+        const int kk = sK;
+        if (tid == 0) {
+            for (int i = kk - 1; i >= 0; i--) {
+                double acc = g[i];
+                for (int l = i + 1; l < kk; l++) acc -= H[(size_t)l * (m + 1) + i] * yc[l];
+                const double rii = H[(size_t)i * (m + 1) + i];
+                yc[i] = rii != 0.0 ? acc / rii : 0.0;
+            }
+        }
+        __syncthreads();
+        for (int k = tid; k < nb; k += nt) {
+            double2 a = make_double2(0.0, 0.0);
+            for (int i = 0; i < kk; i++) {
+                const double2 vv = vec(i)[k];
+                a.x += yc[i] * vv.x;
+                a.y += yc[i] * vv.y;
+            }
+            u[k] = a;
+        }
+        __syncthreads();
+        luForward<1, double>(P, LM, top, TL, tile, tid, nt, lu, y, S.r, 1.0, laneOn);
+        luBackward<1, double>(P, LM, top, TL, tile, tid, nt, lu, y, z, laneOn);
+        __syncthreads();
+        for (int k = tid; k < nb; k += nt) dx[k] = make_double2(dx[k].x + zz[k].x, dx[k].y + zz[k].y);
+    }
+    __syncthreads();
 }
 
 constexpr double P_RESIDUE_EPS = 1e-5;   // OLF ActivePowerDistribution.P_RESIDUE_EPS (outer_loop_driver.cu:204)
@@ -1108,8 +1266,10 @@ __device__ void dsCheck(const DevPlan &P, const DevState &S, const DevRL &R, con
 // every wave exits. Each step is the reference loop body: native/cuda/solver/newton_iteration.cu:116-203 (test half
 // :447-495, testScenario :276-397, solveUniformBatch :580-660), reactive limits
 // native/cuda/outerloop/reactive_limits.cu:321-371 plus the forced-PV election, step scaling
-// native/cuda/solver/state_vector_scaling.cu:57-97.
-template <int T, typename VT>
+// native/cuda/solver/state_vector_scaling.cu:57-97. ROBUST (one lane per tile, FP64): the linear solve runs on
+// statically pivoted factors corrected by GMRES (gmresRefine), the role of the reference's ROBUST single-matrix mode
+// (native/cuda/common/check.cuh:38-40).
+template <int T, typename VT, bool ROBUST = false>
 __global__ __launch_bounds__(MAX_THREADS) void nrStepKernel(DevPlan P, DevState S, DevRL R, NrParams prm, DevTop top) {
     const int tile = blockIdx.x;
     const int tid = threadIdx.x, nt = blockDim.x;
@@ -1460,7 +1620,15 @@ __global__ __launch_bounds__(MAX_THREADS) void nrStepKernel(DevPlan P, DevState 
         OLFHIP_MARK(3);
         luBackward<T, VT>(P, LM, top, TL, tile, tid, nt, lu, y, dx1, sSolve);
         OLFHIP_MARK(4);
-        // ---- 5. step in FP64 (+ one FP64-residual refinement step on the FP32 factors)
+        // a lane whose factorization met a singular pivot block (clamped) was not factored: the unpivoted batch
+        // factorization leaves it, REJECTED in a batch and LINEAR_SOLVE_FAILED alone (the reference routes such
+        // lanes out of its fast engine, native/cuda/lu/lu_pivot.cuh:12-21), for a robust solve to take up
+        if (tid < T && sSolve[tid] && sClamp[tid] > 0) {
+            sStatus[tid] = prm.batchOfOne ? ST_LIN_FAILED : ST_REJECTED;
+            sSolve[tid] = 0;
+        }
+        __syncthreads();
+        // ---- 5. step in FP64 (+ one FP64-residual refinement step on the FP32 factors, or the robust solve's GMRES)
         for (int w = tid; w < nb * T; w += nt) {
             const int k = w / T, t = w % T;
             const V2<VT> d = ldV<VT>(dx1 + vIdx<T>(tile, nb, k, t));
@@ -1484,6 +1652,10 @@ __global__ __launch_bounds__(MAX_THREADS) void nrStepKernel(DevPlan P, DevState 
                 *dp = make_double2(dv.x + (double)d.p, dv.y + (double)d.q);
             }
             __syncthreads();
+        }
+        if constexpr (ROBUST) {
+            static_assert(T == 1 && sizeof(VT) == 8, "the robust solve is FP64, one lane per tile");
+            gmresRefine<T>(P, LM, top, TL, S, prm, tile, tid, nt, lu, y, dx1, sSolve, red);
         }
         // ---- 6. linear-solve guard at the 2nd iteration of the first solve, and the residual of the solve before
         //         the budget (reference newton_iteration.cu:596-652, :369-389)

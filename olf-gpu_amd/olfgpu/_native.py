@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(os.path.dirname(HERE), "libolfhip.so")
 OK, E_ARGUMENT, E_DEVICE, E_MEMORY, E_UNSUPPORTED = 0, -1, -2, -3, -4
 CONVERGED, MAX_ITERATION_REACHED, DIVERGED, REJECTED, LINEAR_SOLVE_FAILED = 0, 1, -1, -2, -3
 STATUS_NAMES = {0: "CONVERGED", 1: "MAX_ITERATION_REACHED", -1: "DIVERGED", -2: "REJECTED", -3: "LINEAR_SOLVE_FAILED"}
-LU_FP64, LU_FP32_REFINED = 0, 1
+LU_FP64, LU_FP32_REFINED, LU_ROBUST = 0, 1, 2   # LU_ROBUST: one scenario per tile, static pivoting + GMRES
 SCALING_NONE, SCALING_LINE_SEARCH, SCALING_MAX_VOLTAGE_CHANGE = 0, 1, 2
 
 # every entry point include/olfhip.h declares

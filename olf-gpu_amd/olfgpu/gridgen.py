@@ -148,6 +148,32 @@ class Network:
         b[buses] = rng.uniform(-0.15, 0.3, m)
         return dataclasses.replace(self, shunt_g=g, shunt_b=b)
 
+    def with_series_resonance(self, branches, x: float = 0.05) -> "Network":
+        """The same grid with, beside each listed branch (bus a to bus b), a path through a new load-free PQ bus i:
+        a lossless branch a-i of reactance x and one i-b of reactance -x, no charging (not part of the workload
+        definition; a test network for pivoting). The two reactances cancel, so the path ties a and b together
+        and bus i's own 2x2 Jacobian block is singular at the flat start and at the solution, while the whole
+        Jacobian is not: an elimination that takes bus i before a and b without pivoting meets a zero pivot block."""
+        import dataclasses
+        branches = np.asarray(branches, dtype=np.int64).reshape(-1)
+        m = branches.size
+        new = self.n_bus + np.arange(m)
+
+        def grow(a, fill):
+            return None if a is None else np.concatenate([a, np.full(m, fill, dtype=a.dtype)])
+        bus1 = np.concatenate([self.bus1, self.bus1[branches], new]).astype(self.bus1.dtype)
+        bus2 = np.concatenate([self.bus2, new, self.bus2[branches]]).astype(self.bus2.dtype)
+        e = np.zeros(2 * m)
+        pos = None if self.pos is None else np.concatenate(
+            [self.pos, 0.5 * (self.pos[self.bus1[branches]] + self.pos[self.bus2[branches]])])
+        return dataclasses.replace(
+            self, n_bus=self.n_bus + m, bus1=bus1, bus2=bus2, r=np.concatenate([self.r, e]),
+            x=np.concatenate([self.x, np.full(m, x), np.full(m, -x)]), b=np.concatenate([self.b, e]),
+            ratio=np.concatenate([self.ratio, np.ones(2 * m)]), bus_type=grow(self.bus_type, PQ),
+            vset=grow(self.vset, 1.0), qmin=grow(self.qmin, 0.0), qmax=grow(self.qmax, 0.0),
+            pload=grow(self.pload, 0.0), qload=grow(self.qload, 0.0), pgen=grow(self.pgen, 0.0), pos=pos,
+            shunt_g=grow(self.shunt_g, 0.0), shunt_b=grow(self.shunt_b, 0.0))
+
     def reactive_limits(self):
         """Reactive-limit entries of the PV buses (pu): voltage row, minQ, maxQ, loadQ, targetV, bus."""
         pv = np.nonzero(self.bus_type == PV)[0].astype(np.int32)

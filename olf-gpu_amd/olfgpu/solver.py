@@ -145,21 +145,41 @@ class AcSolverResult:
 
 
 class GpuNewtonRaphson:
-    """Batch-of-one device load flow on a cached structure and context (GpuNewtonRaphson.run/runImpl)."""
+    """Batch-of-one device load flow on a cached structure and context (GpuNewtonRaphson.run/runImpl). A solve whose
+    linear solve failed (LINEAR_SOLVE_FAILED: the unpivoted factorization met a singular pivot block, or the guard
+    rejected the step) is solved again on the device in the robust mode (N.LU_ROBUST), which the solver then keeps
+    for the rest of its life (retryOrRejectOnDevice, GpuNewtonRaphson.java:751-775); robust_retry=False turns that
+    off."""
 
     def __init__(self, structure: N.Structure, reactive_limits: dict | None = None,
                  gpu_parameters: GpuLoadFlowParameters | None = None,
-                 nr_parameters: NewtonRaphsonParameters | None = None, distributed_slack: dict | None = None):
+                 nr_parameters: NewtonRaphsonParameters | None = None, distributed_slack: dict | None = None,
+                 robust_retry: bool = True):
         self.structure = structure
         self.gpu_parameters = gpu_parameters or GpuLoadFlowParameters()
         self.nr = nr_parameters or NewtonRaphsonParameters()
         self.reactive_limits = reactive_limits
         self.distributed_slack = distributed_slack
-        self.context = N.BatchContext(structure, 1, self.gpu_parameters.lu_mode, reactive_limits)
-        if distributed_slack is not None:
-            self.context.set_distributed_slack(distributed_slack)
+        self.robust_retry = robust_retry
+        self.robust_mode = False
+        self.context = self._make_context(self.gpu_parameters.lu_mode)
+
+    def _make_context(self, mode):
+        ctx = N.BatchContext(self.structure, 1, mode, self.reactive_limits)
+        if self.distributed_slack is not None:
+            ctx.set_distributed_slack(self.distributed_slack)
+        return ctx
 
     def run(self, x0, target, row_mode, seed_limit_types=None) -> AcSolverResult:
+        r = self._run(x0, target, row_mode, seed_limit_types)
+        if r.status == "LINEAR_SOLVE_FAILED" and self.robust_retry and not self.robust_mode:
+            self.context.close()
+            self.context = self._make_context(N.LU_ROBUST)
+            self.robust_mode = True
+            r = self._run(x0, target, row_mode, seed_limit_types)
+        return r
+
+    def _run(self, x0, target, row_mode, seed_limit_types=None) -> AcSolverResult:
         nr = self.nr
         inp, keep = N.BatchContext.make_inputs(
             1, x0, target, row_mode, reactive_limits_seed_state=None if seed_limit_types is None else
@@ -199,17 +219,28 @@ class BatchScenarioResult:
     device_seconds: float
     warm_start_fallbacks: int = 0
     violations: np.ndarray | None = None   # N.VIOLATION_DTYPE, scenario = contingency index (with a limit table)
+    robust_attempted: int = 0              # scenarios the robust retry pass solved again
+    robust_recovered: int = 0              # of which it kept the answer (CONVERGED or MAX_ITERATION_REACHED)
 
 
 class GpuBatchedSecurityAnalysisSolver:
     """Batched N-1 from a converged base case: islands frozen at 1<0 in their scenario, chunks sized from device
-    memory, every chunk one batched solve."""
+    memory, every chunk one batched solve. Scenarios the batch rejects (REJECTED or LINEAR_SOLVE_FAILED: a singular
+    pivot block in the unpivoted factorization, a bad linear solve at the iteration budget) are solved again on the
+    device in the robust mode (N.LU_ROBUST), whose answer is kept when it is CONVERGED or MAX_ITERATION_REACHED
+    (retryRejectedOnDevice, GpuBatchPrecomputer.java:635-830); robust_retry=False turns that pass off. force_mode
+    builds every context in one N.LU_* mode instead of the parameters' (GpuBatchContextCache.forceModeForTests,
+    GpuBatchContextCache.java:144)."""
 
     def __init__(self, structure: N.Structure, reactive_limits: dict | None = None,
                  gpu_parameters: GpuLoadFlowParameters | None = None,
                  nr_parameters: NewtonRaphsonParameters | None = None, slack_bus: int = 0, max_chunk: int = 4096,
-                 lanes_per_tile: int = 0, threads_per_tile: int = 0, distributed_slack: dict | None = None):
+                 lanes_per_tile: int = 0, threads_per_tile: int = 0, distributed_slack: dict | None = None,
+                 robust_retry: bool = True, force_mode: int | None = None):
         self.structure = structure
+        self.robust_retry = robust_retry
+        self.force_mode = force_mode
+        self._robust_ctx = None
         self.distributed_slack = distributed_slack
         self.reactive_limits = reactive_limits
         self.gpu_parameters = gpu_parameters or GpuLoadFlowParameters()
@@ -243,11 +274,26 @@ class GpuBatchedSecurityAnalysisSolver:
         if self._ctx is None or self._ctx.capacity < chunk:
             if self._ctx is not None:
                 self._ctx.close()
-            self._ctx = N.BatchContext(self.structure, chunk, self.gpu_parameters.lu_mode, self.reactive_limits,
-                                       self.lanes_per_tile, self.threads_per_tile)
+            mode = self.lu_mode
+            self._ctx = N.BatchContext(self.structure, chunk, mode, self.reactive_limits,
+                                       1 if mode == N.LU_ROBUST else self.lanes_per_tile, self.threads_per_tile)
             if self.distributed_slack is not None:
                 self._ctx.set_distributed_slack(self.distributed_slack)
         return self._ctx
+
+    @property
+    def lu_mode(self) -> int:
+        return self.gpu_parameters.lu_mode if self.force_mode is None else self.force_mode
+
+    def robust_context(self, chunk: int) -> N.BatchContext:
+        if self._robust_ctx is None or self._robust_ctx.capacity < chunk:
+            if self._robust_ctx is not None:
+                self._robust_ctx.close()
+            self._robust_ctx = N.BatchContext(self.structure, chunk, N.LU_ROBUST, self.reactive_limits, 1,
+                                              self.threads_per_tile)
+            if self.distributed_slack is not None:
+                self._robust_ctx.set_distributed_slack(self.distributed_slack)
+        return self._robust_ctx
 
     def solve_batch_n1(self, x_base, target_base, row_mode_base, limit_types_base, contingencies,
                        x0_per_scenario=None, states_to_host=True, warm_start=None,
@@ -285,7 +331,7 @@ class GpuBatchedSecurityAnalysisSolver:
         if ctg.ndim == 1:
             ctg = ctg[:, None]
         k = ctg.shape[0]
-        chunk = min(k, self.max_chunk, self.structure.recommend_batch_size(self.max_chunk, self.gpu_parameters.lu_mode))
+        chunk = min(k, self.max_chunk, self.structure.recommend_batch_size(self.max_chunk, self.lu_mode))
         nr = self.nr
         n = self.structure.n
         st = np.zeros(k, np.int32)
@@ -299,8 +345,10 @@ class GpuBatchedSecurityAnalysisSolver:
         device_s = 0.0
         chunks = 0
         fallback = 0
+        robust_attempted = 0
+        robust_recovered = 0
 
-        def solve_part(part, x0_host=None, x0_dev=None):
+        def solve_part(part, x0_host=None, x0_dev=None, robust=False):
             nonlocal device_s, chunks, seed
             rme, tix, tvl, cnt = self.plan_islands(part)
             if lt is not None and lt.size:
@@ -317,7 +365,7 @@ class GpuBatchedSecurityAnalysisSolver:
                 line_search_max_iteration=nr.lineSearchStateVectorScalingMaxIteration,
                 line_search_step_fold=nr.lineSearchStateVectorScalingStepFold, conv_eps_per_eq=nr.convEpsPerEq,
                 active_row_counts=(n - 2 * cnt).astype(np.int32))
-            ctx = self.context(len(part))
+            ctx = self.robust_context(len(part)) if robust else self.context(len(part))
             t0 = time.perf_counter()
             r = ctx.solve(inp, want_states=states_to_host)
             device_s += time.perf_counter() - t0
@@ -326,11 +374,11 @@ class GpuBatchedSecurityAnalysisSolver:
 
         viol = []
 
-        def detect(rows):
+        def detect(rows, ctx=None):
             """violations of the last solve, scenario indices mapped through rows"""
             if limit_table is None:
                 return
-            v = limit_table.detect(self._ctx, band=violation_band)
+            v = limit_table.detect(ctx or self._ctx, band=violation_band)
             v["scenario"] = rows[v["scenario"]]
             viol.append(v)
 
@@ -362,8 +410,25 @@ class GpuBatchedSecurityAnalysisSolver:
                     nm[c0 + redo] = r2["norm"]
                     if states_to_host:
                         xs[c0 + redo] = r2["x"]
+            if self.robust_retry and self.lu_mode != N.LU_ROBUST:
+                # the scenarios the batch rejected, once more in the robust mode from the base-case start
+                redo = np.nonzero((st[c0:c1] == N.REJECTED) | (st[c0:c1] == N.LINEAR_SOLVE_FAILED))[0]
+                if redo.size:
+                    robust_attempted += redo.size
+                    r3, _ = solve_part(part[redo], robust=True)
+                    keep = np.isin(r3["status"], (N.CONVERGED, N.MAX_ITERATION_REACHED))
+                    rows = c0 + redo[keep]
+                    robust_recovered += rows.size
+                    detect(c0 + redo, self._robust_ctx)   # converged lanes only, all of them kept
+                    st[rows] = r3["status"][keep]
+                    it[rows] = r3["iterations"][keep]
+                    tot[rows] += r3["total_iterations"][keep]
+                    nm[rows] = r3["norm"][keep]
+                    if states_to_host:
+                        xs[rows] = r3["x"][keep]
         violations = None
         if limit_table is not None:
             violations = np.concatenate(viol) if viol else np.zeros(0, N.VIOLATION_DTYPE)
             violations = violations[np.argsort(violations["scenario"], kind="stable")]
-        return BatchScenarioResult(st, it, tot, nm, xs, islanded, chunks, device_s, fallback, violations)
+        return BatchScenarioResult(st, it, tot, nm, xs, islanded, chunks, device_s, fallback, violations,
+                                   robust_attempted, robust_recovered)

@@ -79,6 +79,22 @@ def test_unsupported_and_invalid_inputs_return_codes(grid):
     assert "even" in N.lib().olfhip_last_error().decode()
 
 
+def test_robust_mode_contract(grid):
+    """The robust mode solves one scenario per tile and is a Newton-solve mode only; both are refused before any
+    device work."""
+    net = grid("grid118")
+    st = N.Structure.from_network(net)
+    with pytest.raises(N.OlfHipError) as e:
+        N.BatchContext(st, 4, N.LU_ROBUST, net.reactive_limits(), lanes_per_tile=2)
+    assert e.value.code == N.E_ARGUMENT and "one scenario per tile" in str(e.value)
+    with pytest.raises(N.OlfHipError) as e:
+        st.lu_factorize_and_solve(np.zeros((1, st.statistics()["pattern_nnz"])), np.zeros((1, net.n)), mode=N.LU_ROBUST)
+    assert e.value.code == N.E_ARGUMENT and "Newton-solve mode" in str(e.value)
+    with pytest.raises(N.OlfHipError) as e:
+        N.BatchContext(st, 4, 7, net.reactive_limits())
+    assert e.value.code == N.E_ARGUMENT
+
+
 def test_parameters_keep_the_reference_surface():
     p = GpuLoadFlowParameters()
     assert p.NAME == "gpu-load-flow-parameters"
