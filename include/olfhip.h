@@ -28,8 +28,8 @@ extern "C" {
 #define OLFHIP_CONVERGED 0
 #define OLFHIP_MAX_ITERATION_REACHED 1
 #define OLFHIP_DIVERGED -1
-#define OLFHIP_REJECTED -2
-#define OLFHIP_LINEAR_SOLVE_FAILED -3
+#define OLFHIP_REJECTED -2             /* batch: a clamped (singular) pivot block, a bad solve at the budget, … */
+#define OLFHIP_LINEAR_SOLVE_FAILED -3  /* the step guard, or (batch of one) a clamped pivot block */
 
 /* row modes: reference GpuRowMode */
 #define OLFHIP_ROW_FIXED_VARIABLE 0
