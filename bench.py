@@ -66,6 +66,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=2000)
     ap.add_argument("--cpu-sample-1t", type=int, default=60, help="contingencies for the one-thread CPU figure")
     ap.add_argument("--no-violations", action="store_true", help="skip the violation-detection measurement")
+    ap.add_argument("--robust-sample", type=int, default=64,
+                    help="contingencies of the robust-retry measurement on the grid with series-resonant paths (0: off)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--lanes-per-tile", type=int, default=0)
     ap.add_argument("--threads-per-tile", type=int, default=0)
@@ -247,6 +249,8 @@ def main():
                      "ms_per_step": round(dv * 1e3, 2), "monitored_groups": int(len(table["group_side"])),
                      "screen_band": 1e-7}
 
+    robust_info = robust_retry_measure(net, nrp, args.robust_sample) if args.robust_sample > 0 else None
+
     # roofline of the Newton-step kernel: the event-timed launches of the batched solve against the lane iterations
     # they ran (the fallback re-solves run in another context and are in neither figure)
     ab, per_iter = algorithmic_bytes(stats, net.n, N.LU_FP32_REFINED, timed_iters / args.steps, K)
@@ -290,11 +294,56 @@ def main():
                            "statuses_match_batched": plain_match},
         "speedup_vs_plain_fp64_gpu": round(value / plain_value, 2),
         "violation_detection": viol_info,
+        "robust_retry": robust_info,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "setup_s": round(setup_s, 2),
     }
     print(json.dumps(out), flush=True)
+
+
+def robust_retry_measure(net, nrp, k):
+    """The retry of rejected scenarios in the robust mode, on the same grid with two series-resonant paths beside
+    two of its branches (a load-free bus between reactances x and -x, whose 2x2 Jacobian block is singular): the
+    batch (FP32 block LU, no pivoting across buses) rejects the scenarios it cannot factor, and the robust mode
+    (FP64 static pivoting + GMRES, one scenario per workgroup) solves them again. Reported beside the value: the
+    metric's workload needs no pivoting."""
+    deg = np.bincount(np.concatenate([net.bus1, net.bus2]), minlength=net.n_bus)
+    slack = set(np.nonzero(net.bus_type == gridgen.SLACK)[0].tolist())
+    beside = [e for e in range(net.n_branch) if deg[net.bus1[e]] >= 3 and deg[net.bus2[e]] >= 3
+              and int(net.bus1[e]) not in slack and int(net.bus2[e]) not in slack][:2]
+    rnet = net.with_series_resonance(beside, x=0.05)
+    rst = N.Structure.from_network(rnet)
+    rrl = rnet.reactive_limits()
+    x0, tgt, md, _ = rnet.system()
+    nr1 = GpuNewtonRaphson(rst, rrl)
+    b = nr1.run(x0, tgt, md)
+    info = {"network": f"grid9241 + series-resonant paths beside branches {beside}", "contingencies": k,
+            "base_case": b.status, "base_case_in_robust_mode": nr1.robust_mode}
+    if b.status != "CONVERGED":
+        return info
+    ctg = gridgen.sample_contingencies(rnet, k, 0.05, 4)
+    lu = GpuLoadFlowParameters(linearSolver="BATCHED_LU_FP32_REFINED")
+    first = GpuBatchedSecurityAnalysisSolver(rst, rrl, lu, nrp, robust_retry=False)
+    full = GpuBatchedSecurityAnalysisSolver(rst, rrl, lu, nrp)
+    args = (b.x, b.target, b.row_mode, b.reactive_limit_types, ctg)
+    first.solve_batch_n1(*args, states_to_host=False)   # warm-up (contexts, code objects)
+    full.solve_batch_n1(*args, states_to_host=False)
+    device_synchronize()
+    t0 = time.perf_counter()
+    r1 = first.solve_batch_n1(*args, states_to_host=False)
+    device_synchronize()
+    t1 = time.perf_counter()
+    r2 = full.solve_batch_n1(*args, states_to_host=False)
+    device_synchronize()
+    t2 = time.perf_counter()
+    rejected = int(np.isin(r1.status, (N.REJECTED, N.LINEAR_SOLVE_FAILED)).sum())
+    retry_ms = ((t2 - t1) - (t1 - t0)) * 1e3
+    info.update({"rejected_by_batch": rejected, "retried": r2.robust_attempted, "recovered": r2.robust_recovered,
+                 "statuses": {N.STATUS_NAMES[int(s)]: int((r2.status == s).sum()) for s in np.unique(r2.status)},
+                 "batch_ms": round((t1 - t0) * 1e3, 2), "batch_with_retry_ms": round((t2 - t1) * 1e3, 2),
+                 "retry_ms_per_scenario": round(retry_ms / max(r2.robust_attempted, 1), 3)})
+    return info
 
 
 def ctx_lanes(ctx):

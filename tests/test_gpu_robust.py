@@ -16,7 +16,7 @@ import pytest
 import pyoracle as po
 from olfgpu import _native as N
 from olfgpu import gridgen
-from olfgpu.solver import GpuBatchedSecurityAnalysisSolver, GpuNewtonRaphson
+from olfgpu.solver import GpuBatchedSecurityAnalysisSolver, GpuLoadFlowParameters, GpuNewtonRaphson
 
 pytestmark = pytest.mark.gpu
 
@@ -118,3 +118,26 @@ def test_robust_context_rejects_several_lanes_per_tile(resonant):
     st = N.Structure.from_network(net)
     with pytest.raises(N.OlfHipError, match="one scenario per tile"):
         N.BatchContext(st, 4, N.LU_ROBUST, net.reactive_limits(), lanes_per_tile=2)
+
+
+def test_resonant_9241_n1_robust_retry_matches_oracle(grid):
+    """The same at full size: grid9241 with series-resonant paths beside its first two branches, 16 contingencies
+    in the FP32 batch; every rejected scenario comes back from the robust retry with the oracle's status, iteration
+    count and state."""
+    base = grid("grid9241")
+    net = base.with_series_resonance([0, 1], x=0.05)
+    print("[robust] grid9241 + 2 resonant paths, N-1 over 16 contingencies", flush=True)
+    osys = po.OracleSystem(net)
+    x0, tgt, md, _ = net.system()
+    xb, mb, tb, ltb, res = osys.solve(x0, md, tgt)
+    assert po.STATUS[res.status] == "CONVERGED"
+    ctg = gridgen.sample_contingencies(net, 16, 0.05, 4)
+    ref = osys.n1(xb, mb, tb, ltb, ctg)
+    st = N.Structure.from_network(net)
+    sa = GpuBatchedSecurityAnalysisSolver(st, net.reactive_limits(),
+                                          GpuLoadFlowParameters(linearSolver="BATCHED_LU_FP32_REFINED"))
+    r = sa.solve_batch_n1(xb, tb, mb, ltb, ctg)
+    assert r.robust_attempted >= 8 and r.robust_recovered == r.robust_attempted
+    assert np.array_equal(r.status, ref["status"]), (r.status, ref["status"])
+    conv = r.status == 0
+    compare_states(r.x[conv], ref["x"][conv], "resonant n1 grid9241")
