@@ -365,10 +365,22 @@ def cpu_baseline(net, xb, mb, tb, lt, sample, sample_1t):
     osys.n1(xb, mb, tb, lt, sample_1t, threads=1)
     d1 = time.perf_counter() - t
     return {"value": round(len(sample) / dt, 2), "unit": "contingencies/s", "cores": int(r["threads"]),
-            "kind": "port", "sample": f"first {len(sample)} of the 2000 contingencies, same base state and rules, "
+            "cpu_model": host_cpu_model(), "kind": "port", "sample": f"first {len(sample)} of the 2000 contingencies, same base state and rules, "
                                        f"{dt:.1f} s wall on {int(r['threads'])} threads",
             "one_thread_value": round(len(sample_1t) / d1, 2),
             "one_thread_sample": f"first {len(sample_1t)} contingencies, {d1:.1f} s on 1 thread"}
+
+
+def host_cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), or None."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def measured_copy_bandwidth():
