@@ -118,8 +118,6 @@ struct StructureImpl {
         dp.grpEnd = put(P.grpEnd);
         dp.grpFar = put(P.grpFar);
         dp.grpSlot = put(P.grpSlot);
-        dp.fillPtr = put(P.fillPtr);
-        dp.fillSlot = put(P.fillSlot);
         dp.shuntG = put(P.shuntG);
         dp.shuntB = put(P.shuntB);
         dp.perm = put(P.perm);
@@ -235,6 +233,7 @@ static void planTop(const StructureImpl *st, const void *kernel, int T, int NT, 
                 const int b = P.levelPos[P.sIdx[q]] - tp.top.start;
                 if (b < 0 || b >= n) throw std::logic_error("top block is not closed under S");
                 const int slot = P.nodeBase[k] + 1 + 2 * (q - P.sPtr[k]);
+                if (P.fillMask[slot]) continue;               // fill-in: starts from the zeroed LDS block
                 rec.insert(rec.end(), {slot, b, a, 0});       // L~(i,k): row i, column k
                 rec.insert(rec.end(), {slot + 1, a, b, 0});   // U(k,i): row k, column i
             }

@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "plan.hpp"   // ITEM_FILL_PAIR
 #include "trig.hpp"
 
 namespace olfhip {
@@ -40,7 +41,6 @@ struct DevPlan {
     const int *hbPtr, *hbBranch;
     const double4 *hbCoef;
     const int *grpPtr, *grpEnd, *grpFar, *grpSlot;
-    const int *fillPtr, *fillSlot;
     const double *shuntG, *shuntB;
     const int *perm, *pos;
     const int *levelPos;        // position of each node in level order
@@ -343,13 +343,7 @@ __device__ __forceinline__ void nodeTerms(const DevPlan &P, int tile, int k, int
     }
     if (WRITE_J) {
         Blk<VT> d = {(VT)dPdth, (VT)dPdv, (VT)dQdth, (VT)dQdv};
-        stBlk<VT>(lu + bIdx<T>(tile, P.nSlots, P.nodeBase[k], t), d);
-        const Blk<VT> z = {(VT)0, (VT)0, (VT)0, (VT)0};
-        for (int q = P.fillPtr[k]; q < P.fillPtr[k + 1]; q++) {
-            const int s0 = P.fillSlot[q];
-            stBlk<VT>(lu + bIdx<T>(tile, P.nSlots, s0, t), z);
-            stBlk<VT>(lu + bIdx<T>(tile, P.nSlots, s0 + 1, t), z);
-        }
+        stBlk<VT>(lu + bIdx<T>(tile, P.nSlots, P.nodeBase[k], t), d);   // fill-in slots are left alone
     }
     if (APPLY_VEC) {
         const double2 vk = *reinterpret_cast<const double2 *>(v + vIdx<T>(tile, nb, k, t));
@@ -530,9 +524,10 @@ __device__ void factorForward(const DevPlan &P, const LevelMeta &M, const DevTop
                     stV<VT>(z + vIdx<T>(tile, nb, it.w, t), zk);
                 }
             } else {
-                // pair item: L~(i,k) at it.x, U(k,i) right after (own blocks loaded first)
+                // pair item: L~(i,k) at it.x, U(k,i) right after (own blocks loaded first; a fill-in pair starts
+                // from zero, the fill writes nothing there)
                 Blk<VT> A = {(VT)0, (VT)0, (VT)0, (VT)0}, B = {(VT)0, (VT)0, (VT)0, (VT)0};
-                if (s == 0) {
+                if (s == 0 && it.w != ITEM_FILL_PAIR) {
                     A = ldBlk<VT>(own);
                     B = ldBlk<VT>(own + 4 * T);
                 }

@@ -398,6 +398,18 @@ Plan buildPlan(int n, int nBranch, const double *p8, const int *idx12, int nShun
         P.shuntB[k] = bBus[b];
     }
 
+    // fill-in pairs: flagged in their items and never written by the fill
+    P.fillMask.assign(P.nSlots, 0);
+    for (int s0 : P.fillSlot) P.fillMask[s0] = P.fillMask[s0 + 1] = 1;
+    size_t fillItems = 0;
+    for (size_t q = 0; q < P.itemRec.size() / 4; q++)
+        if (P.itemRec[4 * q + 3] < 0 && P.fillMask[P.itemRec[4 * q]]) {
+            P.itemRec[4 * q + 3] = ITEM_FILL_PAIR;
+            fillItems++;
+        }
+    // every fill-in pair takes contributions, so it has an item that starts it from zero
+    if (fillItems != P.fillSlot.size()) throw std::logic_error("a fill-in pair has no factorization item");
+
     // --- scalar CSR pattern (stable rows) and its slot map
     P.rowPtr.assign(n + 1, 0);
     for (int b = 0; b < nb; b++) {

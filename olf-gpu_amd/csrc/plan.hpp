@@ -13,6 +13,10 @@
 
 namespace olfhip {
 
+// Item record kind of a pair item whose slots are fill-in: the fill writes nothing there, so the factorization
+// starts its value from zero instead of loading it.
+constexpr int ITEM_FILL_PAIR = -2;
+
 struct Plan {
     int nb = 0;          // buses (= LU nodes)
     int n = 0;           // 2 * nb stable rows
@@ -46,6 +50,7 @@ struct Plan {
     // contribution list ascends in the contributing node j.
     std::vector<int> levelItemPtr;                 // items of level l in [levelItemPtr[l], levelItemPtr[l+1])
     std::vector<int> itemRec;                      // 4 per item: slot (diag or L), contrib begin, end, node | -1
+                                                   // (pair) | ITEM_FILL_PAIR (pair absent from the pattern)
     std::vector<int> itemKI;                       // 2 per item: k and i of L~(i,k) / U(k,i) (diag: k, k)
     std::vector<int> contribRec;                   // 4 per contribution: diag item: L slot of pair (j,k), j,
                                                    // diag slot of j, level of j; pair item: L slot of (j,k), L slot
@@ -70,7 +75,8 @@ struct Plan {
     std::vector<int> grpEnd;                       // end of the group's half-branches in hbBranch
     std::vector<int> grpFar;                       // far node
     std::vector<int> grpSlot;                      // block slot of (k, far): U(k,far) or L(k,far)
-    std::vector<int> fillPtr, fillSlot;            // pairs of node k absent from the pattern (zeroed per fill)
+    std::vector<int> fillPtr, fillSlot;            // pairs of node k absent from the pattern (fill-in)
+    std::vector<char> fillMask;                    // per slot: 1 in a fill-in pair (never written by the fill)
     std::vector<double> shuntG, shuntB;            // per node, pu
     // per closed branch, for flows at both ends: A1 B C1 psi1 A2 C2 psi2 0 of the own/far form (side 1 at bus1,
     // side 2 at bus2) and the two end nodes
