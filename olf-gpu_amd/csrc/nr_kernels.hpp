@@ -609,12 +609,22 @@ __device__ void forwardOnly(const DevPlan &P, const LevelMeta &M, const DevTop &
     }
 }
 
+// The FP64 step of a node as the backward solve finalises its x: dx64 = x (acc false) or dx64 += x (acc true).
+template <int T, typename VT>
+__device__ __forceinline__ void putStep(double *dx64, bool acc, size_t vi, const V2<VT> &x) {
+    if (!dx64) return;
+    double2 *p = reinterpret_cast<double2 *>(dx64 + vi);
+    const double2 o = acc ? *p : make_double2(0.0, 0.0);
+    *p = make_double2(o.x + (double)x.p, o.y + (double)x.q);
+}
+
 // Backward solve: x(k) = z(k) - D(k)^-1 sum U(k,i) x(i), levels from the root down; a node's sum is split across
-// S threads as in the factorization. Ends with a barrier.
+// S threads as in the factorization. With dx64, each x(k) also goes to (or, acc, onto) the FP64 step there (putStep).
+// Ends with a barrier.
 template <int T, typename VT>
 __device__ void backward(const DevPlan &P, const LevelMeta &M, const DevTop &top, int tile, int tid, int nt,
                          const VT *__restrict__ lu, const VT *__restrict__ z, VT *__restrict__ xo, int l0,
-                         const int *laneOn) {
+                         const int *laneOn, double *dx64 = nullptr, bool accStep = false) {
     const int nb = P.nb;
     for (int l = l0 - 1; l >= 0; l--) {
         const int b0 = M.levelBwdBucketPtr[l], b1 = M.levelBwdBucketPtr[l + 1];
@@ -651,6 +661,7 @@ __device__ void backward(const DevPlan &P, const LevelMeta &M, const DevTop &top
             if (s == 0) {
                 const V2<VT> r = {zk.p + Dinv.a * acc.p + Dinv.b * acc.q, zk.q + Dinv.c * acc.p + Dinv.d * acc.q};
                 stV<VT>(xo + vIdx<T>(tile, nb, br.w, t), r);
+                putStep<T, VT>(dx64, accStep, vIdx<T>(tile, nb, br.w, t), r);
             }
         }
         __syncthreads();
@@ -862,7 +873,7 @@ __device__ void topForward(const DevPlan &P, const DevTop &top, const TopLds<T, 
 // barrier per node; x of the top nodes goes to xo for the sparse levels below. Ends with a barrier.
 template <int T, typename VT>
 __device__ void topBackward(const DevPlan &P, const DevTop &top, const TopLds<T, VT> &L, int tile, int tid, int nt,
-                            VT *__restrict__ xo, const int *laneOn) {
+                            VT *__restrict__ xo, const int *laneOn, double *dx64 = nullptr, bool accStep = false) {
     const int nb = P.nb, n = top.n;
     for (int w = tid; w < n * T; w += nt) stV<VT>(L.Acc(w / T, w % T), V2<VT>{(VT)0, (VT)0});
     __syncthreads();
@@ -877,6 +888,7 @@ __device__ void topBackward(const DevPlan &P, const DevTop &top, const TopLds<T,
             if (b == a) {
                 const int k = P.levelNodes[top.start + a];
                 stV<VT>(xo + vIdx<T>(tile, nb, k, t), xa);
+                putStep<T, VT>(dx64, accStep, vIdx<T>(tile, nb, k, t), xa);
             } else {
                 const Blk<VT> U = ldBlk<VT>(L.M(b, a, t));
                 V2<VT> ab = ldV<VT>(L.Acc(b, t));
@@ -907,9 +919,10 @@ __device__ __forceinline__ void luForward(const DevPlan &P, const LevelMeta &M, 
 template <int T, typename VT>
 __device__ __forceinline__ void luBackward(const DevPlan &P, const LevelMeta &M, const DevTop &top,
                                            const TopLds<T, VT> &L, int tile, int tid, int nt, const VT *lu,
-                                           const VT *z, VT *xo, const int *laneOn) {
-    if (top.n > 0) topBackward<T, VT>(P, top, L, tile, tid, nt, xo, laneOn);
-    backward<T, VT>(P, M, top, tile, tid, nt, lu, z, xo, top.l0, laneOn);
+                                           const VT *z, VT *xo, const int *laneOn, double *dx64 = nullptr,
+                                           bool accStep = false) {
+    if (top.n > 0) topBackward<T, VT>(P, top, L, tile, tid, nt, xo, laneOn, dx64, accStep);
+    backward<T, VT>(P, M, top, tile, tid, nt, lu, z, xo, top.l0, laneOn, dx64, accStep);
 }
 
 // Mismatch and Jacobian fill of the lanes flagged in fillLane; per-thread partial max|F| and finiteness of the
@@ -1613,7 +1626,7 @@ __global__ __launch_bounds__(MAX_THREADS) void nrStepKernel(DevPlan P, DevState 
         OLFHIP_MARK(2);
         luFactorForward<T, VT>(P, LM, top, TL, tile, tid, nt, lu, y, S.F, -1.0, sClamp, sSolve);
         OLFHIP_MARK(3);
-        luBackward<T, VT>(P, LM, top, TL, tile, tid, nt, lu, y, dx1, sSolve);
+        luBackward<T, VT>(P, LM, top, TL, tile, tid, nt, lu, y, dx1, sSolve, S.dx, false);   // and the FP64 step
         OLFHIP_MARK(4);
         // a lane whose factorization met a singular pivot block (clamped) was not factored: the unpivoted batch
         // factorization leaves it, REJECTED in a batch and LINEAR_SOLVE_FAILED alone (the reference routes such
@@ -1623,13 +1636,8 @@ __global__ __launch_bounds__(MAX_THREADS) void nrStepKernel(DevPlan P, DevState 
             sSolve[tid] = 0;
         }
         __syncthreads();
-        // ---- 5. step in FP64 (+ one FP64-residual refinement step on the FP32 factors, or the robust solve's GMRES)
-        for (int w = tid; w < nb * T; w += nt) {
-            const int k = w / T, t = w % T;
-            const V2<VT> d = ldV<VT>(dx1 + vIdx<T>(tile, nb, k, t));
-            *reinterpret_cast<double2 *>(S.dx + vIdx<T>(tile, nb, k, t)) = make_double2((double)d.p, (double)d.q);
-        }
-        __syncthreads();
+        // ---- 5. the step in FP64, S.dx (+ one FP64-residual refinement step on the FP32 factors, or the robust
+        //         solve's GMRES)
         if (prm.refine) {
             double pm = 0.0;
             // FP64 residual r = J dx + F, solved for the correction with sign -1: J dx2 = -r
@@ -1637,16 +1645,8 @@ __global__ __launch_bounds__(MAX_THREADS) void nrStepKernel(DevPlan P, DevState 
             __syncthreads();
             OLFHIP_MARK(5);
             luForward<T, VT>(P, LM, top, TL, tile, tid, nt, lu, y, S.r, -1.0, sSolve);
-            luBackward<T, VT>(P, LM, top, TL, tile, tid, nt, lu, y, dx1, sSolve);
+            luBackward<T, VT>(P, LM, top, TL, tile, tid, nt, lu, y, dx1, sSolve, S.dx, true);   // dx += correction
             OLFHIP_MARK(6);
-            for (int w = tid; w < nb * T; w += nt) {
-                const int k = w / T, t = w % T;
-                const V2<VT> d = ldV<VT>(dx1 + vIdx<T>(tile, nb, k, t));
-                double2 *dp = reinterpret_cast<double2 *>(S.dx + vIdx<T>(tile, nb, k, t));
-                const double2 dv = *dp;
-                *dp = make_double2(dv.x + (double)d.p, dv.y + (double)d.q);
-            }
-            __syncthreads();
         }
         if constexpr (ROBUST) {
             static_assert(T == 1 && sizeof(VT) == 8, "the robust solve is FP64, one lane per tile");
