@@ -138,7 +138,7 @@ def main():
     rme, tix, tvl, isl = sa.plan_islands(ctg[:, None])
     inputs, keep = N.BatchContext.make_inputs(
         K, xb, tb, mb, row_mode_entries=rme, target_overrides=(tix, tvl), outaged_branches=ctg[:, None],
-        reactive_limits_seed_state=np.tile(lt, (K, 1)), max_iter=nrp.maxIterations, tol=nrp.maxMismatch,
+        reactive_limits_seed_state=lt, max_iter=nrp.maxIterations, tol=nrp.maxMismatch,
         reactive_limit_tol=nrp.maxReactivePowerMismatch, states_to_host=False)
     tuned = None
     if args.autotune:

@@ -201,6 +201,9 @@ typedef struct {
      * [n_scenarios] equations of each scenario's system (NewtonSettingsNative.setBatchActiveRowCounts) */
     double conv_eps_per_eq;
     const int *active_row_counts;
+    /* 1: reactive_limits_seed_state is ONE row [n_entries] that every scenario starts from (the base case's types,
+     * the usual N-1 seed), uploaded once instead of n_scenarios times */
+    int reactive_limits_seed_shared;
 } olfhip_solve_inputs;
 
 /* Outputs (any may be NULL): results [n_scenarios][n+3] = (x, iterations of the last solve, ||F||inf, status) in

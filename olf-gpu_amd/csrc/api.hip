@@ -538,9 +538,10 @@ static void solve(ContextImpl *c, const olfhip_solve_inputs *in, olfhip_solve_ou
         c->targetIdx.upload(std::vector<long long>(in->target_override_index, in->target_override_index + in->n_target_overrides), s);
         c->targetVal.upload(std::vector<double>(in->target_override_value, in->target_override_value + in->n_target_overrides), s);
     }
+    const size_t seedRows = in->reactive_limits_seed_shared ? 1 : (size_t)K;
     if (in->reactive_limits_seed_state && c->nRL > 0) {
-        c->seed.alloc((size_t)K * c->nRL);
-        HIP_OK(hipMemcpyAsync(c->seed.p, in->reactive_limits_seed_state, (size_t)K * c->nRL, hipMemcpyHostToDevice, s));
+        c->seed.alloc(seedRows * c->nRL);
+        HIP_OK(hipMemcpyAsync(c->seed.p, in->reactive_limits_seed_state, seedRows * c->nRL, hipMemcpyHostToDevice, s));
     }
     NrParams prm{};
     prm.maxIter = in->max_iter;
@@ -612,6 +613,7 @@ static void solve(ContextImpl *c, const olfhip_solve_inputs *in, olfhip_solve_ou
                 c->targetIdx.as<long long>(), c->targetVal.as<double>());
         laneInitKernel<T><<<gridFor(lanes * (1 + (size_t)c->nRL), 256), 256, 0, s>>>(
             dp, S, R, (in->reactive_limits_seed_state && c->nRL > 0) ? c->seed.as<signed char>() : nullptr,
+            seedRows == 1 ? 0 : c->nRL,
             c->modeBase.as<int>());
         HIP_OK(hipGetLastError());
         auto enqueueStep = [&]() {

@@ -1786,7 +1786,8 @@ __global__ void overrideKernel(DevPlan P, DevState S, int nModes, const int *mod
 // lane counters and reactive-limit entry state; an entry whose bus's P row a scenario froze (an islanded bus) leaves
 // that scenario's outer loop (reference reactive_limits.cu:62-69)
 template <int T>
-__global__ void laneInitKernel(DevPlan P, DevState S, DevRL R, const signed char *seed, const int *modeBase) {
+__global__ void laneInitKernel(DevPlan P, DevState S, DevRL R, const signed char *seed, int seedStride,
+                               const int *modeBase) {
     const int nb = P.nb;
     const int lanes = S.nTiles * T;
     const size_t totalE = (size_t)lanes * R.n;
@@ -1819,7 +1820,7 @@ __global__ void laneInitKernel(DevPlan P, DevState S, DevRL R, const signed char
             const size_t ei = ((size_t)tile * R.n + e) * T + t;
             const int k = R.node[e];
             const int pRow = 2 * P.perm[k];
-            signed char lt = (seed && lane < S.nInstalled) ? seed[(size_t)lane * R.n + e] : QLIM_NONE;
+            signed char lt = (seed && lane < S.nInstalled) ? seed[(size_t)lane * seedStride + e] : QLIM_NONE;
             const signed char pm = S.mode[vIdx<T>(tile, nb, k, t)];
             if (pm == MODE_FIXED && modeBase[pRow] != MODE_FIXED) lt = QLIM_FROZEN;
             S.rlType[ei] = lt;

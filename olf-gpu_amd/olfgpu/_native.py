@@ -63,7 +63,7 @@ class SolveInputs(C.Structure):
                 ("distributed_slack", C.c_int), ("slack_dist_tol", C.c_double),
                 ("reject_slack_distribution_failure", C.c_int), ("line_search_max_iteration", C.c_int),
                 ("line_search_step_fold", C.c_double), ("conv_eps_per_eq", C.c_double),
-                ("active_row_counts", C.c_void_p)]
+                ("active_row_counts", C.c_void_p), ("reactive_limits_seed_shared", C.c_int)]
 
 
 class DistributedSlackDesc(C.Structure):
@@ -307,6 +307,13 @@ class BatchContext:
             mo = ob.shape[1]
             ob = arr(ob, np.int32)
         xp = arr(x0_per_scenario, np.float64)
+        # one row of seeds (1-d, or [1][n_entries] with more than one scenario): shared by every scenario
+        shared = 0
+        if reactive_limits_seed_state is not None:
+            rs = np.asarray(reactive_limits_seed_state)
+            if rs.ndim == 1 or (rs.shape[0] == 1 and n_scenarios > 1):
+                shared = 1
+                reactive_limits_seed_state = rs.reshape(-1)
         sd = arr(reactive_limits_seed_state, np.int8)
         ac = arr(active_row_counts, np.int32)
         si = SolveInputs(n_scenarios, ptr(x0), ptr(tg), ptr(md), 0 if rme is None else rme.shape[0], ptr(rme),
@@ -314,7 +321,8 @@ class BatchContext:
                          reactive_limit_tol, state_vector_scaling_mode, max_dv, max_dphi, max_outer_iterations,
                          max_pq_pv_switch, int(skip_reactive_limits), int(states_to_host), x0_per_scenario_device,
                          int(distributed_slack), float(slack_dist_tol), int(reject_slack_distribution_failure),
-                         int(line_search_max_iteration), float(line_search_step_fold), float(conv_eps_per_eq), ptr(ac))
+                         int(line_search_max_iteration), float(line_search_step_fold), float(conv_eps_per_eq), ptr(ac),
+                         shared)
         return si, keep
 
     def set_distributed_slack(self, ds):

@@ -352,7 +352,7 @@ class GpuBatchedSecurityAnalysisSolver:
             nonlocal device_s, chunks, seed
             rme, tix, tvl, cnt = self.plan_islands(part)
             if lt is not None and lt.size:
-                seed = np.tile(lt, (len(part), 1))
+                seed = lt   # one row, shared by every scenario of the chunk
             inp, keep = N.BatchContext.make_inputs(
                 len(part), x_base, target_base, row_mode_base, row_mode_entries=rme, target_overrides=(tix, tvl),
                 outaged_branches=part, x0_per_scenario=x0_host, reactive_limits_seed_state=seed,

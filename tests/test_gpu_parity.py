@@ -184,6 +184,31 @@ def test_statuses_forced_from_the_host(grid):
     assert (r["iterations"] == 1).all()
 
 
+def test_shared_and_per_scenario_seeds_agree(grid):
+    """One shared row of reactive-limit seeds gives bitwise the same solves as that row repeated per scenario, and
+    per-scenario rows still apply per scenario."""
+    net = grid("grid2000")
+    print("[seed] grid2000 shared vs per-scenario reactive-limit seeds", flush=True)
+    osys, xb, mb, tb, ltb = base_case(net, "grid2000")
+    st = N.Structure.from_network(net)
+    ctg = gridgen.sample_contingencies(net, 12, 0.0, 21)
+    ctx = N.BatchContext(st, len(ctg), N.LU_FP64, net.reactive_limits())
+    lt = np.asarray(ltb, np.int8)
+    res = []
+    for seed in (lt, np.tile(lt, (len(ctg), 1))):
+        inp, keep = N.BatchContext.make_inputs(len(ctg), xb, tb, mb, outaged_branches=ctg, reactive_limits_seed_state=seed)
+        assert inp.reactive_limits_seed_shared == (1 if seed.ndim == 1 else 0)
+        res.append(ctx.solve(inp))
+    assert np.array_equal(res[0]["x"], res[1]["x"]) and np.array_equal(res[0]["status"], res[1]["status"])
+    # a different seed row for one scenario changes only that scenario's start
+    per = np.tile(lt, (len(ctg), 1))
+    per[3] = 0
+    inp, keep = N.BatchContext.make_inputs(len(ctg), xb, tb, mb, outaged_branches=ctg, reactive_limits_seed_state=per)
+    r = ctx.solve(inp)
+    others = np.arange(len(ctg)) != 3
+    assert np.array_equal(r["x"][others], res[0]["x"][others])
+
+
 def test_states_left_on_device_read_back(grid):
     net = grid("grid118")
     st = N.Structure.from_network(net)
