@@ -308,3 +308,25 @@ def test_bus_shunts_base_and_n1(grid, lu):
     assert np.array_equal(r.status, ref["status"])
     ok = ref["status"] == 0
     compare_states(r.x[ok], ref["x"][ok], f"shunt N-1 {lu}")
+
+
+@pytest.mark.parametrize("lu", ["BATCHED_LU", "BATCHED_LU_FP32_REFINED"])
+@pytest.mark.parametrize("k_out", [2, 4])
+def test_multi_branch_outages_match_oracle(grid, lu, k_out):
+    """Scenarios with 2 and 4 outaged branches (the most the library models per scenario; some cut buses off):
+    same statuses and states as the oracle, and in FP64 the same iteration counts."""
+    net = grid("grid2000")
+    print(f"[n-{k_out}] grid2000 {lu}", flush=True)
+    osys, xb, mb, tb, ltb = base_case(net, "grid2000")
+    rng = np.random.default_rng(30 + k_out)
+    ctg = np.stack([rng.choice(net.n_branch, k_out, replace=False) for _ in range(24)]).astype(np.int32)
+    ref = osys.n1(xb, mb, tb, ltb, ctg)
+    sa = GpuBatchedSecurityAnalysisSolver(N.Structure.from_network(net), net.reactive_limits(),
+                                          GpuLoadFlowParameters(linearSolver=lu))
+    res = sa.solve_batch_n1(xb, tb, mb, ltb, ctg)
+    assert np.array_equal(res.status, ref["status"]), (res.status, ref["status"])
+    conv = res.status == 0
+    assert conv.sum() >= 20
+    compare_states(res.x[conv], ref["x"][conv], f"n-{k_out} grid2000 {lu}")
+    if lu == "BATCHED_LU":
+        assert np.array_equal(res.iterations, ref["iterations"])
