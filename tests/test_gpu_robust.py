@@ -141,3 +141,32 @@ def test_resonant_9241_n1_robust_retry_matches_oracle(grid):
     assert np.array_equal(r.status, ref["status"]), (r.status, ref["status"])
     conv = r.status == 0
     compare_states(r.x[conv], ref["x"][conv], "resonant n1 grid9241")
+
+
+def test_resonant_robust_with_distributed_slack_and_line_search(resonant):
+    """The robust retry under the outer loops: distributed slack and LINE_SEARCH scaling on the resonant network,
+    base case (batch of one) and N-1, against the oracle with the same settings."""
+    from olfgpu.solver import NewtonRaphsonParameters
+
+    base, net, osys, (x0, tgt, md), _ = resonant
+    print("[robust] grid118 + resonant paths, distributed slack + LINE_SEARCH", flush=True)
+    ds = gridgen.distributed_slack(net)
+    xo, mo, to, lto, ro = osys.solve(x0, md, tgt, prm=po.params(scaling_mode=1), ds=ds, ds_tol=1.0 / gridgen.SB)
+    assert po.STATUS[ro.status] == "CONVERGED"
+    st = N.Structure.from_network(net)
+    nrp = NewtonRaphsonParameters(stateVectorScalingMode="LINE_SEARCH")
+    solver = GpuNewtonRaphson(st, net.reactive_limits(), nr_parameters=nrp, distributed_slack=ds)
+    r = solver.run(x0, tgt, md)
+    assert solver.robust_mode and r.status == "CONVERGED"
+    assert r.total_iterations == ro.total_iterations
+    compare_states(r.x, xo, "resonant base, ds + line search")
+    assert np.abs(r.target[2 * ds["bus"]] - to[2 * ds["bus"]]).max() <= V_TOL
+    ctg = gridgen.sample_contingencies(net, 20, 0.1, 3)
+    ref = osys.n1(xo, mo, to, lto, ctg, prm=po.params(batch_of_one=False, scaling_mode=1), ds=ds,
+                  ds_tol=1.0 / gridgen.SB)
+    sa = GpuBatchedSecurityAnalysisSolver(st, net.reactive_limits(), nr_parameters=nrp, distributed_slack=ds)
+    g = sa.solve_batch_n1(r.x, r.target, r.row_mode, r.reactive_limit_types, ctg)
+    assert g.robust_attempted > 0 and g.robust_recovered == g.robust_attempted
+    assert np.array_equal(g.status, ref["status"]), (g.status, ref["status"])
+    conv = g.status == 0
+    compare_states(g.x[conv], ref["x"][conv], "resonant n1, ds + line search")
