@@ -131,8 +131,9 @@ def main():
     order = np.arange(len(ctg_given))
     if data is not None and args.order == "cost":
         from olfgpu import warmstart as W
-        from olfgpu.solver import longest_first
-        order = longest_first(W.iteration_costs(data, net.n_branch)[ctg_given])
+        from olfgpu.solver import longest_first, outage_flow_magnitude
+        order = longest_first(W.iteration_costs(data, net.n_branch)[ctg_given],
+                              outage_flow_magnitude(net, xb, ctg_given))
     ctg = ctg_given[order]
     K = len(ctg)
     rme, tix, tvl, isl = sa.plan_islands(ctg[:, None])
@@ -283,7 +284,8 @@ def main():
         "config": {"workload": "grid9241 x1.1 load, 2000 single-branch outages (5% bridges, seed 2), FP32 block LU + "
                                "1 FP64 refinement, reactive limits on, " +
                                ("base-case start" if mlp is None else "learned warm start (base-start fallback)") +
-                               (", solved longest first by the training solves' iteration counts"
+                               (", solved longest first by the training solves' iteration counts (ties: larger base flow of the "
+                                "outaged branch first)"
                                 if data is not None and args.order == "cost" else ""),
                    "buses": net.n_bus, "branches": net.n_branch, "contingencies": K, "tol_pu": nrp.maxMismatch,
                    "tile": ctx_lanes(ctx), "lu_levels": stats["levels"], "lu_nnz": stats["lu_nnz"]},

@@ -198,12 +198,27 @@ class GpuNewtonRaphson:
                               lt[0])
 
 
-def longest_first(cost) -> np.ndarray:
-    """The order in which a batch solves its scenarios, given an expected cost per scenario: most expensive first
-    (ties in the given order). Consecutive scenarios share a tile and tiles start in order as workgroups free up, so
-    this pairs scenarios of similar length in a tile and starts the long ones early (longest-processing-time
-    scheduling); results are reported in the caller's order."""
-    return np.argsort(-np.asarray(cost, dtype=np.float64), kind="stable")
+def longest_first(cost, tie=None) -> np.ndarray:
+    """The order in which a batch solves its scenarios, given an expected cost per scenario: most expensive first;
+    ties broken by `tie`, largest first (e.g. the outaged branch's base-case flow magnitude, outage_flow_magnitude:
+    a larger flow to redistribute is a harder start), then in the given order. Consecutive scenarios share a tile and
+    tiles start in order as workgroups free up, so this pairs scenarios of similar length in a tile and starts the
+    long ones early (longest-processing-time scheduling); results are reported in the caller's order."""
+    c = -np.asarray(cost, dtype=np.float64)
+    if tie is None:
+        return np.argsort(c, kind="stable")
+    return np.lexsort((-np.asarray(tie, dtype=np.float64), c))
+
+
+def outage_flow_magnitude(net, x_base, branches) -> np.ndarray:
+    """|S| at side 1 of each outaged branch in the base-case state (pu): the flow the outage redistributes (for
+    scenarios of several branches, the largest)."""
+    from . import gridgen
+    s1 = np.abs(gridgen.branch_flows(net, x_base)[0])
+    br = np.asarray(branches)
+    if br.ndim == 1:
+        return s1[br]
+    return np.where(br >= 0, s1[np.maximum(br, 0)], 0.0).max(axis=1)
 
 
 @dataclass
@@ -298,17 +313,18 @@ class GpuBatchedSecurityAnalysisSolver:
     def solve_batch_n1(self, x_base, target_base, row_mode_base, limit_types_base, contingencies,
                        x0_per_scenario=None, states_to_host=True, warm_start=None,
                        warm_start_features=None, limit_table: N.LimitTable | None = None,
-                       violation_band=1e-7, cost=None) -> BatchScenarioResult:
+                       violation_band=1e-7, cost=None, cost_tie=None) -> BatchScenarioResult:
         """solveBatchN1: contingencies is [K] or [K, m] closed-branch ids (-1 padded). With a warm_start network
         (olfgpu.warmstart.WarmStartMLP, single outages) every scenario starts from its prediction, and any that does
         not converge is solved again from the base-case start. With a limit_table the violations of every converged
         contingency are detected on its final state (device screen, exact host decisions). With a cost per
         contingency (e.g. warmstart.iteration_costs of its branch) the batch is solved longest first
-        (longest_first); the results are in the contingencies' order either way."""
+        (longest_first, ties broken by cost_tie when given); the results are in the contingencies' order either
+        way."""
         import time
 
         if cost is not None:
-            order = longest_first(cost)
+            order = longest_first(cost, cost_tie)
             inv = np.empty_like(order)
             inv[order] = np.arange(len(order))
 
