@@ -236,6 +236,7 @@ class BatchScenarioResult:
     violations: np.ndarray | None = None   # N.VIOLATION_DTYPE, scenario = contingency index (with a limit table)
     robust_attempted: int = 0              # scenarios the robust retry pass solved again
     robust_recovered: int = 0              # of which it kept the answer (CONVERGED or MAX_ITERATION_REACHED)
+    robust_errors: list = field(default_factory=list)   # failures of the retry pass (the first pass's answers kept)
 
 
 class GpuBatchedSecurityAnalysisSolver:
@@ -363,6 +364,7 @@ class GpuBatchedSecurityAnalysisSolver:
         fallback = 0
         robust_attempted = 0
         robust_recovered = 0
+        robust_errors = []
 
         def solve_part(part, x0_host=None, x0_dev=None, robust=False):
             nonlocal device_s, chunks, seed
@@ -431,7 +433,13 @@ class GpuBatchedSecurityAnalysisSolver:
                 redo = np.nonzero((st[c0:c1] == N.REJECTED) | (st[c0:c1] == N.LINEAR_SOLVE_FAILED))[0]
                 if redo.size:
                     robust_attempted += redo.size
-                    r3, _ = solve_part(part[redo], robust=True)
+                    try:
+                        r3, _ = solve_part(part[redo], robust=True)
+                    except N.OlfHipError as e:
+                        # an optional pass: its failure (e.g. no device memory for its FP64 Krylov basis) keeps the
+                        # first pass's answers (GpuBatchPrecomputer.java:780-786)
+                        robust_errors.append(str(e))
+                        continue
                     keep = np.isin(r3["status"], (N.CONVERGED, N.MAX_ITERATION_REACHED))
                     rows = c0 + redo[keep]
                     robust_recovered += rows.size
@@ -447,4 +455,4 @@ class GpuBatchedSecurityAnalysisSolver:
             violations = np.concatenate(viol) if viol else np.zeros(0, N.VIOLATION_DTYPE)
             violations = violations[np.argsort(violations["scenario"], kind="stable")]
         return BatchScenarioResult(st, it, tot, nm, xs, islanded, chunks, device_s, fallback, violations,
-                                   robust_attempted, robust_recovered)
+                                   robust_attempted, robust_recovered, robust_errors)

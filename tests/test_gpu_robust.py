@@ -170,3 +170,21 @@ def test_resonant_robust_with_distributed_slack_and_line_search(resonant):
     assert np.array_equal(g.status, ref["status"]), (g.status, ref["status"])
     conv = g.status == 0
     compare_states(g.x[conv], ref["x"][conv], "resonant n1, ds + line search")
+
+
+def test_failed_robust_pass_keeps_the_first_pass(resonant, monkeypatch):
+    """The retry pass is optional: when it fails (forced from the host: its context cannot be created), the first
+    pass's answers stand and the failure is reported."""
+    base, net, osys, _, (xb, mb, tb, ltb, _) = resonant
+    st = N.Structure.from_network(net)
+    ctg = gridgen.sample_contingencies(net, 8, 0.0, 3)
+    sa = GpuBatchedSecurityAnalysisSolver(st, net.reactive_limits())
+
+    def refuse(chunk):
+        raise N.OlfHipError(N.E_MEMORY, "forced: no memory for the robust context")
+    monkeypatch.setattr(sa, "robust_context", refuse)
+    r = sa.solve_batch_n1(xb, tb, mb, ltb, ctg)
+    first = GpuBatchedSecurityAnalysisSolver(st, net.reactive_limits(), robust_retry=False).solve_batch_n1(
+        xb, tb, mb, ltb, ctg)
+    assert r.robust_attempted > 0 and r.robust_recovered == 0 and len(r.robust_errors) == 1
+    assert np.array_equal(r.status, first.status)
