@@ -144,6 +144,16 @@ typedef struct {
  * loop on per solve. */
 int olfhip_set_distributed_slack(olfhip_context *context, const olfhip_distributed_slack *distributed_slack);
 
+/* OLF's PerEquationTypeStoppingCriteria as per-row thresholds on the context, the reference's
+ * NewtonSettingsNative.setBatchRowCriteria (NewtonSettingsNative.java:29-36; applied in
+ * native/cuda/solver/convergence.cu:28-60 and newton_iteration.cu:296-300): fixed_row_max_mismatch [n] for a row
+ * held as an identity (FIXED_VARIABLE) and equation_row_max_mismatch [n] for an equation row, in the system's per
+ * unit (e.g. maxActivePowerMismatch / base MVA on a P row, maxVoltageMismatch on a V row). While set, a scenario
+ * converges when every row's |F| is below its mode's threshold, instead of ||F||inf < tol or the 2-norm criterion.
+ * Both NULL clears them. Thresholds must be > 0. */
+int olfhip_set_row_criteria(olfhip_context *context, const double *fixed_row_max_mismatch,
+                            const double *equation_row_max_mismatch);
+
 /* Inputs of one batched solve: the hot-path subset of BatchSolveNative.solveBatchContext
  * (BatchSolveNative.java:106-125), same meaning and order conventions:
  *   x0_base, target_base, base_row_mode: [n] shared by every scenario (tiled on the device);

@@ -300,6 +300,8 @@ struct ContextImpl {
     DBuf loopAt, sweepChanged, dsFailed, slackP, dsRetain, dsPrev;
     DBuf xPrev, lsLast, lsHas;   // LINE_SEARCH state (xPrev allocated by the first solve that asks for it)
     DBuf nActive;                // active equations per lane (the 2-norm criterion)
+    bool rowCriteria = false;    // per-row stopping thresholds set (olfhip_set_row_criteria), node order [node][2]
+    DBuf rowThrFixed, rowThrEq;
     // install inputs
     DBuf x0Base, x0Per, targetBase, modeBase, modeEntries, targetIdx, targetVal, seed, states;
     int lastInstalled = 0;
@@ -358,6 +360,8 @@ struct ContextImpl {
         S.lsLast = lsLast.as<double>();
         S.lsHas = lsHas.as<signed char>();
         S.nActive = nActive.as<int>();
+        S.rowThrFixed = rowCriteria ? rowThrFixed.as<double>() : nullptr;
+        S.rowThrEq = rowCriteria ? rowThrEq.as<double>() : nullptr;
         return S;
     }
     DevRL devRL() const {
@@ -555,6 +559,7 @@ static void solve(ContextImpl *c, const olfhip_solve_inputs *in, olfhip_solve_ou
     prm.maxDphi = in->max_dphi;
     prm.refine = c->mode == OLFHIP_LU_FP32_REFINED;
     prm.batchOfOne = K == 1;
+    prm.rowCriteria = c->rowCriteria ? 1 : 0;
     prm.kryDim = OLFHIP_ROBUST_KRYLOV_DIM;
     prm.kryRestarts = OLFHIP_ROBUST_RESTARTS;
     prm.kryTol = OLFHIP_ROBUST_TOL;
@@ -1198,6 +1203,34 @@ int olfhip_create_batch_context(olfhip_structure *structure, int n_scenarios, in
 }
 
 void olfhip_destroy_batch_context(olfhip_context *context) { delete reinterpret_cast<ContextImpl *>(context); }
+
+int olfhip_set_row_criteria(olfhip_context *context, const double *fixed_row_max_mismatch,
+                            const double *equation_row_max_mismatch) {
+    return guarded([&] {
+        ContextImpl *c = reinterpret_cast<ContextImpl *>(context);
+        if (!c) throw ArgumentError("context is null");
+        if (!fixed_row_max_mismatch && !equation_row_max_mismatch) {
+            c->rowCriteria = false;
+            return;
+        }
+        if (!fixed_row_max_mismatch || !equation_row_max_mismatch)
+            throw ArgumentError("row criteria need both threshold arrays (or neither)");
+        const Plan &P = c->st->plan;
+        std::vector<double> fixedNode(P.n), eqNode(P.n);
+        for (int r = 0; r < P.n; r++) {
+            const double f = fixed_row_max_mismatch[r], e = equation_row_max_mismatch[r];
+            if (!(f > 0.0) || !(e > 0.0) || !std::isfinite(f) || !std::isfinite(e))
+                throw ArgumentError("row threshold " + std::to_string(r) + " must be finite and > 0");
+            const int i = 2 * P.pos[r >> 1] + (r & 1);
+            fixedNode[i] = f;
+            eqNode[i] = e;
+        }
+        c->rowThrFixed.upload(fixedNode);
+        c->rowThrEq.upload(eqNode);
+        HIP_OK(hipDeviceSynchronize());   // the uploads (null stream) complete before any solve on the context's
+        c->rowCriteria = true;
+    });
+}
 
 int olfhip_set_distributed_slack(olfhip_context *context, const olfhip_distributed_slack *ds) {
     return guarded([&] {

@@ -97,6 +97,9 @@ struct DevState {
     double *xPrev, *lsLast;
     signed char *lsHas;
     const int *nActive;   // active equations per lane (the 2-norm criterion's count)
+    // per-row stopping thresholds in node order ([node][2]: P row, Q row), for identity and for equation rows;
+    // null without the per-row criterion
+    const double *rowThrFixed, *rowThrEq;
     // robust solve: the Krylov basis (kryDim + 1 vectors, each laid out like x over the whole context) and per tile
     // the Hessenberg matrix, right-hand side, rotations and coefficients of its GMRES cycle
     double *kryV, *kryH;
@@ -144,6 +147,7 @@ struct NrParams {
     int distributedSlack; // run the distributed-slack outer loop (before the reactive limits)
     double dsTol;         // its gate, slackBusPMaxMismatch in pu
     int dsReject;         // a failed distribution ends the scenario REJECTED
+    int rowCriteria;      // converged when every row's |F| is below its threshold (S.rowThrFixed / S.rowThrEq)
     int kryDim;           // robust solve: GMRES basis size, restarts after the first cycle, relative 2-norm target
     int kryRestarts;
     double kryTol;
@@ -929,7 +933,7 @@ __device__ __forceinline__ void luBackward(const DevPlan &P, const LevelMeta &M,
 // thread's lane (tid % T) are returned.
 template <int T, typename VT>
 __device__ void fillPass(const DevPlan &P, const DevState &S, int tile, int tid, int nt, const int *fillLane,
-                         double &pmax, int &pfin, double &psq, int slackNode) {
+                         double &pmax, int &pfin, double &psq, double &pratio, int slackNode) {
     const int nb = P.nb;
     VT *lu = reinterpret_cast<VT *>(S.lu);
     const int t = tid % T;   // fixed per thread: the stride nt is a multiple of T
@@ -952,6 +956,12 @@ __device__ void fillPass(const DevPlan &P, const DevState &S, int tile, int tid,
         if (!isfinite(fP) || !isfinite(fQ)) pfin = 0;
         pmax = fmax(pmax, fmax(fabs(fP), fabs(fQ)));
         psq += fP * fP + fQ * fQ;
+        if (S.rowThrFixed) {   // the per-row criterion: max |F| / threshold of the row's mode
+            const double2 tf = *reinterpret_cast<const double2 *>(S.rowThrFixed + 2 * k);
+            const double2 te = *reinterpret_cast<const double2 *>(S.rowThrEq + 2 * k);
+            pratio = fmax(pratio, fmax(fabs(fP) / (mP == MODE_FIXED ? tf.x : te.x),
+                                       fabs(fQ) / (mQ == MODE_FIXED ? tf.y : te.y)));
+        }
     }
 }
 
@@ -1290,7 +1300,7 @@ __global__ __launch_bounds__(MAX_THREADS) void nrStepKernel(DevPlan P, DevState 
     __shared__ int sStatus[T], sIt[T], sSolves[T], sOuter[T], sTotal[T], sClamp[T];
     __shared__ int sFill[T], sSolve[T], sConv[T], sRl[T], sRemain[T], sSwitched[T], sFin[T];
     __shared__ int sChk[T], sLoop[T], sSweep[T], sResolve[T], sReject[T], sDs[T], sDsUnst[T], sDsFail[T], sBt[T];
-    __shared__ double sSsq[T];
+    __shared__ double sSsq[T], sRowR[T];
     __shared__ double sNorm[T], sRatio[T], sFactor[T];
     __shared__ int sFlag[4];
     __shared__ double red[MAX_THREADS];
@@ -1339,9 +1349,9 @@ __global__ __launch_bounds__(MAX_THREADS) void nrStepKernel(DevPlan P, DevState 
 
     // ---- 1. mismatch + Jacobian of the running lanes
     {
-        double pmax = 0.0, psq = 0.0;
+        double pmax = 0.0, psq = 0.0, pratio = 0.0;
         int pfin = 1;
-        fillPass<T, VT>(P, S, tile, tid, nt, sFill, pmax, pfin, psq, slackNode);
+        fillPass<T, VT>(P, S, tile, tid, nt, sFill, pmax, pfin, psq, pratio, slackNode);
         red[tid] = pmax;
         redI[tid] = pfin;
         laneReduce<T, true>(red, tid, nt);
@@ -1349,6 +1359,12 @@ __global__ __launch_bounds__(MAX_THREADS) void nrStepKernel(DevPlan P, DevState 
         if (tid < T) {
             sNorm[tid] = red[tid];
             sFin[tid] = redI[tid];
+        }
+        if (prm.rowCriteria) {
+            __syncthreads();
+            red[tid] = pratio;
+            laneReduce<T, true>(red, tid, nt);
+            if (tid < T) sRowR[tid] = red[tid];
         }
         if (prm.scaling == 1 || prm.convEps > 0.0) {
             __syncthreads();
@@ -1380,9 +1396,9 @@ __global__ __launch_bounds__(MAX_THREADS) void nrStepKernel(DevPlan P, DevState 
                     *reinterpret_cast<double2 *>(S.x + vi) = make_double2(xp.x + mu * d.x, xp.y + mu * d.y);
                 }
                 __syncthreads();
-                double bmax = 0.0, bsq = 0.0;
+                double bmax = 0.0, bsq = 0.0, bratio = 0.0;
                 int bfin = 1;
-                fillPass<T, VT>(P, S, tile, tid, nt, sBt, bmax, bfin, bsq, slackNode);
+                fillPass<T, VT>(P, S, tile, tid, nt, sBt, bmax, bfin, bsq, bratio, slackNode);
                 red[tid] = bmax;
                 redI[tid] = bfin;
                 laneReduce<T, true>(red, tid, nt);
@@ -1390,6 +1406,12 @@ __global__ __launch_bounds__(MAX_THREADS) void nrStepKernel(DevPlan P, DevState 
                 if (tid < T && sBt[tid]) {
                     sNorm[tid] = red[tid];
                     sFin[tid] = redI[tid];
+                }
+                if (prm.rowCriteria) {
+                    __syncthreads();
+                    red[tid] = bratio;
+                    laneReduce<T, true>(red, tid, nt);
+                    if (tid < T && sBt[tid]) sRowR[tid] = red[tid];
                 }
                 __syncthreads();
                 red[tid] = bsq;
@@ -1415,8 +1437,9 @@ __global__ __launch_bounds__(MAX_THREADS) void nrStepKernel(DevPlan P, DevState 
         const int it = sIt[tid];
         sNorm[tid] = norm;
         // the criterion (native/cuda/solver/newton_iteration.cu:241-300): ||F||inf < tol, or with convEps OLF's
-        // ||F||_2 < sqrt(convEps^2 nActive)
-        const bool below = prm.convEps > 0.0
+        // ||F||_2 < sqrt(convEps^2 nActive), or with row thresholds every |F_i| below its row's (:296-300)
+        const bool below = prm.rowCriteria ? sRowR[tid] < 1.0
+                           : prm.convEps > 0.0
                                ? sqrt(sSsq[tid]) < sqrt(prm.convEps * prm.convEps * (double)S.nActive[tile * T + tid])
                                : norm < prm.tol;
         if (!fin) {
@@ -1600,9 +1623,9 @@ __global__ __launch_bounds__(MAX_THREADS) void nrStepKernel(DevPlan P, DevState 
         }
         __syncthreads();
         if (sFlag[1]) {
-            double pmax = 0.0, psq = 0.0;
+            double pmax = 0.0, psq = 0.0, pratio = 0.0;   // restarted lanes solve without a test here
             int pfin = 1;
-            fillPass<T, VT>(P, S, tile, tid, nt, sFill, pmax, pfin, psq, slackNode);
+            fillPass<T, VT>(P, S, tile, tid, nt, sFill, pmax, pfin, psq, pratio, slackNode);
             red[tid] = pmax;
             redI[tid] = pfin;
             laneReduce<T, true>(red, tid, nt);

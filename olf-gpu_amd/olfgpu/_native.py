@@ -30,6 +30,7 @@ EXPORTS = [
     "olfhip_mlp_parameters", "olfhip_mlp_predict_warm_start", "olfhip_mlp_gemm_test",
     "olfhip_create_limit_table", "olfhip_destroy_limit_table", "olfhip_screen_violations", "olfhip_detect_violations",
     "olfhip_set_distributed_slack",
+    "olfhip_set_row_criteria",
 ]
 
 
@@ -120,6 +121,7 @@ def lib():
         L.olfhip_read_states.argtypes = [vp, i, vp]
         L.olfhip_context_timing.argtypes = [vp, C.POINTER(d), C.POINTER(i)]
         L.olfhip_set_distributed_slack.argtypes = [vp, C.c_void_p]
+        L.olfhip_set_row_criteria.argtypes = [vp, C.c_void_p, C.c_void_p]
         L.olfhip_read_outer_loop_state.argtypes = [vp, i, vp, vp, vp]
         L.olfhip_islanded_buses.argtypes = [vp, i, i, vp, vp, C.POINTER(i)]
         L.olfhip_lu_factorize_and_solve.argtypes = [vp, i, i, vp, vp, i, vp, vp, vp]
@@ -324,6 +326,19 @@ class BatchContext:
                          int(line_search_max_iteration), float(line_search_step_fold), float(conv_eps_per_eq), ptr(ac),
                          shared)
         return si, keep
+
+    def set_row_criteria(self, thresholds):
+        """olfhip_set_row_criteria: thresholds = (fixed [n], equation [n]) per-row stopping thresholds (pu), or None
+        to clear."""
+        if thresholds is None:
+            check(lib().olfhip_set_row_criteria(self.h, None, None))
+            self._rows_keep = None
+            return
+        keep = tuple(np.ascontiguousarray(a, np.float64) for a in thresholds)
+        if any(a.size != self.structure.n for a in keep):
+            raise ValueError("row thresholds need n values each")
+        check(lib().olfhip_set_row_criteria(self.h, ptr(keep[0]), ptr(keep[1])))
+        self._rows_keep = keep
 
     def set_distributed_slack(self, ds):
         """olfhip_set_distributed_slack: ds is a dict (bus, factor, base_target_p, max_delta, min_delta, slack_bus,

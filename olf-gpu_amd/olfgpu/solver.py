@@ -90,6 +90,32 @@ class GpuLoadFlowParameters:
 
 
 @dataclass
+class PerEquationTypeStoppingCriteria:
+    """OLF's PerEquationTypeStoppingCriteria (NewtonRaphsonStoppingCriteriaType.PER_EQUATION_TYPE_CRITERIA), as the
+    reference's GpuAcActivationExtractor.extractRowCriteria turns it into per-row thresholds
+    (GpuAcActivationExtractor.java:280-325): on a P row maxAngleMismatch when the row is an identity (slack angle)
+    and maxActivePowerMismatch / base when it is an equation; on a Q row maxVoltageMismatch (PV voltage) and
+    maxReactivePowerMismatch / base. Powers in MW / MVar; base_mva is the system's per-unit base (OLF's is 100 MVA,
+    the synthetic grids' gridgen.SB)."""
+
+    maxActivePowerMismatch: float = 1e-2
+    maxReactivePowerMismatch: float = 1e-2
+    maxVoltageMismatch: float = 1e-4
+    maxAngleMismatch: float = 1e-5
+    base_mva: float = 400.0
+
+    def row_thresholds(self, n: int):
+        """(fixed [n], equation [n]) on the stable rows (2b: P / angle, 2b + 1: Q / voltage)."""
+        fixed = np.empty(n)
+        eq = np.empty(n)
+        fixed[0::2] = self.maxAngleMismatch
+        fixed[1::2] = self.maxVoltageMismatch
+        eq[0::2] = self.maxActivePowerMismatch / self.base_mva
+        eq[1::2] = self.maxReactivePowerMismatch / self.base_mva
+        return fixed, eq
+
+
+@dataclass
 class NewtonRaphsonParameters:
     """The OLF Newton-Raphson parameters the AcSolver copies (GpuNewtonRaphsonFactory.java:62-79)."""
 
@@ -111,6 +137,14 @@ class NewtonRaphsonParameters:
     distributedSlack: bool = True
     slackBusPMaxMismatch: float = 1.0
     slackDistributionFailureBehavior: str = "LEAVE_ON_SLACK_BUS"
+    # OLF's stopping criterion object: None for the uniform ones (maxMismatch, or convEpsPerEq), or a
+    # PerEquationTypeStoppingCriteria (per-row thresholds installed on the device context)
+    stoppingCriteria: PerEquationTypeStoppingCriteria | None = None
+
+    def install_stopping_criteria(self, ctx: N.BatchContext) -> None:
+        """GpuNewtonRaphson.installRowCriteria (GpuNewtonRaphson.java:446-459): the per-row thresholds, or none."""
+        sc = self.stoppingCriteria
+        ctx.set_row_criteria(None if sc is None else sc.row_thresholds(ctx.structure.n))
 
     def slack_inputs(self, ds: dict | None) -> dict:
         """make_inputs keywords of the distributed-slack loop for a description ds (None: off)."""
@@ -168,6 +202,7 @@ class GpuNewtonRaphson:
         ctx = N.BatchContext(self.structure, 1, mode, self.reactive_limits)
         if self.distributed_slack is not None:
             ctx.set_distributed_slack(self.distributed_slack)
+        self.nr.install_stopping_criteria(ctx)
         return ctx
 
     def run(self, x0, target, row_mode, seed_limit_types=None) -> AcSolverResult:
@@ -295,6 +330,7 @@ class GpuBatchedSecurityAnalysisSolver:
                                        1 if mode == N.LU_ROBUST else self.lanes_per_tile, self.threads_per_tile)
             if self.distributed_slack is not None:
                 self._ctx.set_distributed_slack(self.distributed_slack)
+            self.nr.install_stopping_criteria(self._ctx)
         return self._ctx
 
     @property
@@ -309,6 +345,7 @@ class GpuBatchedSecurityAnalysisSolver:
                                               self.threads_per_tile)
             if self.distributed_slack is not None:
                 self._robust_ctx.set_distributed_slack(self.distributed_slack)
+            self.nr.install_stopping_criteria(self._robust_ctx)
         return self._robust_ctx
 
     def solve_batch_n1(self, x_base, target_base, row_mode_base, limit_types_base, contingencies,

@@ -808,7 +808,11 @@ void oracle_nr_solve(const oracle_sys *s, const oracle_nr_params *prm, const ora
             if (!finite) { res->status = ORC_DIVERGED; ended = 1; break; }
             if (it > 0) {
                 int below = norm < prm->tol;
-                if (prm->conv_eps_per_eq > 0.0) {
+                if (prm->row_thr_fixed && prm->row_thr_eq) {
+                    below = 1;
+                    for (int i = 0; i < n && below; i++)
+                        below = fabs(F[i]) < (mode[i] == ORC_FIXED_VARIABLE ? prm->row_thr_fixed[i] : prm->row_thr_eq[i]);
+                } else if (prm->conv_eps_per_eq > 0.0) {
                     double ssq = 0.0;
                     for (int i = 0; i < n; i++) ssq += F[i] * F[i];
                     const double na = prm->n_active > 0 ? (double)prm->n_active : (double)n;

@@ -75,6 +75,11 @@ typedef struct {
     double conv_eps_per_eq;/* > 0: OLF's criterion ||F||_2 < sqrt(conv_eps_per_eq^2 n_active) instead of ||F||inf < tol
                               (newton_iteration.cu:241-300) */
     int n_active;          /* active equations of the system (<= 0: n; oracle_n1 uses n - 2 islanded buses) */
+    /* OLF's PerEquationTypeStoppingCriteria as per-row thresholds (newton_iteration.cu:296-300, convergence.cu:28-
+       60): [n] for a row held as an identity (FIXED_VARIABLE) and [n] for an equation row; both set: converged when
+       every row's |F| is below its mode's threshold (instead of the two criteria above) */
+    const double *row_thr_fixed;
+    const double *row_thr_eq;
 } oracle_nr_params;
 
 typedef struct {

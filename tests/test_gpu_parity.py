@@ -330,3 +330,55 @@ def test_multi_branch_outages_match_oracle(grid, lu, k_out):
     compare_states(res.x[conv], ref["x"][conv], f"n-{k_out} grid2000 {lu}")
     if lu == "BATCHED_LU":
         assert np.array_equal(res.iterations, ref["iterations"])
+
+
+@pytest.mark.parametrize("lu", ["BATCHED_LU", "BATCHED_LU_FP32_REFINED"])
+def test_per_equation_type_criteria_match_oracle(grid, lu):
+    """OLF's PerEquationTypeStoppingCriteria (the reference's per-row thresholds, setBatchRowCriteria) with the
+    reference test's values (1e-2 MW, 1e-2 MVar, 1e-4 pu, 1e-4 rad): base case from the flat start and N-1, same
+    statuses and states as the oracle under the same rows, FP64 iteration counts equal; the rule is in force (the
+    uniform 1e-4 pu criterion stops some scenarios earlier)."""
+    from olfgpu.solver import PerEquationTypeStoppingCriteria
+
+    net = grid("grid2000")
+    print(f"[criteria] grid2000 per-equation-type {lu}", flush=True)
+    crit = PerEquationTypeStoppingCriteria(maxActivePowerMismatch=1e-2, maxReactivePowerMismatch=1e-2,
+                                           maxVoltageMismatch=1e-4, maxAngleMismatch=1e-4, base_mva=gridgen.SB)
+    thr = crit.row_thresholds(net.n)
+    osys = po.OracleSystem(net)
+    x0, tg, md, _ = net.system()
+    xo, mo, to, lto, ro = osys.solve(x0, md, tg, prm=po.params(row_thresholds=thr))
+    assert po.STATUS[ro.status] == "CONVERGED"
+    st = N.Structure.from_network(net)
+    nrp = NewtonRaphsonParameters(stoppingCriteria=crit)
+    g = GpuNewtonRaphson(st, net.reactive_limits(), GpuLoadFlowParameters(linearSolver=lu), nrp).run(x0, tg, md)
+    assert g.status == "CONVERGED"
+    compare_states(g.x, xo, f"per-row base {lu}")
+    if lu == "BATCHED_LU":
+        assert g.total_iterations == ro.total_iterations
+    ctg = gridgen.sample_contingencies(net, 32, 0.25, 17)
+    ref = osys.n1(xo, mo, to, lto, ctg, prm=po.params(batch_of_one=False, row_thresholds=thr))
+    sa = GpuBatchedSecurityAnalysisSolver(st, net.reactive_limits(), GpuLoadFlowParameters(linearSolver=lu), nrp)
+    res = sa.solve_batch_n1(xo, to, mo, lto, ctg)
+    assert np.array_equal(res.status, ref["status"]), (res.status, ref["status"])
+    conv = res.status == 0
+    compare_states(res.x[conv], ref["x"][conv], f"per-row n1 {lu}")
+    if lu == "BATCHED_LU":
+        assert np.array_equal(res.total_iterations, ref["total_iterations"])
+    uniform = osys.n1(xo, mo, to, lto, ctg, prm=po.params(batch_of_one=False))
+    assert (uniform["total_iterations"] < ref["total_iterations"]).any()
+
+
+def test_row_criteria_contract(grid):
+    net = grid("grid118")
+    st = N.Structure.from_network(net)
+    ctx = N.BatchContext(st, 2, N.LU_FP64, net.reactive_limits())
+    ones = np.ones(net.n)
+    with pytest.raises(N.OlfHipError) as e:
+        ctx.set_row_criteria((ones, -ones))
+    assert e.value.code == N.E_ARGUMENT
+    with pytest.raises(N.OlfHipError) as e:
+        N.check(N.lib().olfhip_set_row_criteria(ctx.h, N.ptr(ones), None))
+    assert e.value.code == N.E_ARGUMENT
+    ctx.set_row_criteria((ones, ones))
+    ctx.set_row_criteria(None)

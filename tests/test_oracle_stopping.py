@@ -41,3 +41,35 @@ def test_two_norm_criterion_stops_where_an_independent_loop_does():
         _, _, _, _, r2 = o.solve(x0, md, tg, prm=po.params(reactive_limits=False, conv_eps_per_eq=eps, tol=0.0,
                                                            n_active=n // 4))
         assert r2.iterations >= k
+
+
+def _abs_f(o, x, mode, target):
+    return np.abs(o.fill(x, mode, target, jacobian=False)[0])
+
+
+def test_per_row_criterion_stops_where_an_independent_loop_does():
+    """OLF's PerEquationTypeStoppingCriteria as per-row thresholds (the reference's setBatchRowCriteria,
+    native/cuda/solver/convergence.cu:28-60): the solve stops at the first iterate whose every row is below its
+    threshold, the identity rows' threshold for FIXED_VARIABLE rows and the equation rows' for the others. Row
+    thresholds of three times each equation row's |F| at iterate k (plus a floor) must stop it exactly at k; a
+    huge identity-row threshold must not leak into the equation rows (it would stop at iterate 1); a per-row
+    threshold read at the wrong row would miss the iterate."""
+    net = gridgen.workload("grid118")
+    o = po.OracleSystem(net)
+    x0, tg, md, _ = net.system()
+    hist = _norms_and_states(o, x0, md, tg, 5)
+    eq = np.asarray(md) != 0
+    for k in (2, 3):
+        fk = _abs_f(o, hist[k][2], md, tg)
+        thr_eq = 3.0 * fk + 1e-14
+        thr_fixed = np.full(o.n, 1e9)
+        # iterate k - 1 fails some equation row, iterate k passes all
+        assert (_abs_f(o, hist[k - 1][2], md, tg)[eq] >= thr_eq[eq]).any() and (fk[eq] < thr_eq[eq]).all()
+        x, _, _, _, r = o.solve(x0, md, tg, prm=po.params(reactive_limits=False, tol=0.0,
+                                                          row_thresholds=(thr_fixed, thr_eq)))
+        assert po.STATUS[r.status] == "CONVERGED" and r.iterations == k, (k, r.iterations)
+        np.testing.assert_allclose(x, hist[k][2], rtol=0, atol=1e-10)
+        # thresholds shifted by one row: no longer this iterate's
+        _, _, _, _, r2 = o.solve(x0, md, tg, prm=po.params(reactive_limits=False, tol=0.0, max_iter=k,
+                                                           row_thresholds=(thr_fixed, np.roll(thr_eq, 1))))
+        assert not (po.STATUS[r2.status] == "CONVERGED" and r2.iterations == k)
