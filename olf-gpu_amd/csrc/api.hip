@@ -621,15 +621,22 @@ static void solve(ContextImpl *c, const olfhip_solve_inputs *in, olfhip_solve_ou
             seedRows == 1 ? 0 : c->nRL,
             c->modeBase.as<int>());
         HIP_OK(hipGetLastError());
-        auto enqueueStep = [&]() {
+        // the per-row criterion runs in the kernels' ROWCRIT variants (the same code plus its test)
+        auto launchStep = [&](auto rowCrit) {
+            constexpr bool RC = decltype(rowCrit)::value;
             if (c->mode == OLFHIP_LU_FP32_REFINED) {
-                nrStepKernel<T, float><<<c->nTiles, c->NT, c->top.ldsBytes, s>>>(dp, S, R, prm, c->top.top);
+                nrStepKernel<T, float, false, RC><<<c->nTiles, c->NT, c->top.ldsBytes, s>>>(dp, S, R, prm, c->top.top);
             } else if (c->mode == OLFHIP_LU_ROBUST) {
                 if constexpr (T == 1)
-                    nrStepKernel<1, double, true><<<c->nTiles, c->NT, c->top.ldsBytes, s>>>(dp, S, R, prm, c->top.top);
+                    nrStepKernel<1, double, true, RC><<<c->nTiles, c->NT, c->top.ldsBytes, s>>>(dp, S, R, prm,
+                                                                                           c->top.top);
             } else {
-                nrStepKernel<T, double><<<c->nTiles, c->NT, c->top.ldsBytes, s>>>(dp, S, R, prm, c->top.top);
+                nrStepKernel<T, double, false, RC><<<c->nTiles, c->NT, c->top.ldsBytes, s>>>(dp, S, R, prm, c->top.top);
             }
+        };
+        auto enqueueStep = [&]() {
+            if (prm.rowCriteria) launchStep(std::true_type{});
+            else launchStep(std::false_type{});
             countRunningKernel<<<1, 256, 0, s>>>(S.tileRunning, c->nTiles, c->count.as<int>());
             HIP_OK(hipMemcpyAsync(c->pinnedCount, c->count.p, sizeof(int), hipMemcpyDeviceToHost, s));
         };

@@ -933,7 +933,7 @@ __device__ __forceinline__ void luBackward(const DevPlan &P, const LevelMeta &M,
 // thread's lane (tid % T) are returned.
 template <int T, typename VT>
 __device__ void fillPass(const DevPlan &P, const DevState &S, int tile, int tid, int nt, const int *fillLane,
-                         double &pmax, int &pfin, double &psq, double &pratio, int slackNode) {
+                         double &pmax, int &pfin, double &psq, int slackNode) {
     const int nb = P.nb;
     VT *lu = reinterpret_cast<VT *>(S.lu);
     const int t = tid % T;   // fixed per thread: the stride nt is a multiple of T
@@ -956,12 +956,25 @@ __device__ void fillPass(const DevPlan &P, const DevState &S, int tile, int tid,
         if (!isfinite(fP) || !isfinite(fQ)) pfin = 0;
         pmax = fmax(pmax, fmax(fabs(fP), fabs(fQ)));
         psq += fP * fP + fQ * fQ;
-        if (S.rowThrFixed) {   // the per-row criterion: max |F| / threshold of the row's mode
-            const double2 tf = *reinterpret_cast<const double2 *>(S.rowThrFixed + 2 * k);
-            const double2 te = *reinterpret_cast<const double2 *>(S.rowThrEq + 2 * k);
-            pratio = fmax(pratio, fmax(fabs(fP) / (mP == MODE_FIXED ? tf.x : te.x),
-                                       fabs(fQ) / (mQ == MODE_FIXED ? tf.y : te.y)));
-        }
+    }
+}
+
+// The per-row criterion's max |F_i| / threshold of row i's mode for the lanes flagged, per thread (after a fill):
+// a pass of its own, so that the fill carries nothing for it when the criterion is not set.
+template <int T>
+__device__ void rowRatioPass(const DevPlan &P, const DevState &S, int tile, int tid, int nt, const int *laneOn,
+                             double &pratio) {
+    const int nb = P.nb;
+    const int t = tid % T;
+    if (!laneOn[t]) return;
+    for (int w = tid; w < nb * T; w += nt) {
+        const int k = w / T;
+        const size_t vi = vIdx<T>(tile, nb, k, t);
+        const double2 f = *reinterpret_cast<const double2 *>(S.F + vi);
+        const double2 tf = *reinterpret_cast<const double2 *>(S.rowThrFixed + 2 * k);
+        const double2 te = *reinterpret_cast<const double2 *>(S.rowThrEq + 2 * k);
+        pratio = fmax(pratio, fmax(fabs(f.x) / (S.mode[vi] == MODE_FIXED ? tf.x : te.x),
+                                   fabs(f.y) / (S.mode[vi + 1] == MODE_FIXED ? tf.y : te.y)));
     }
 }
 
@@ -1287,7 +1300,9 @@ __device__ void dsCheck(const DevPlan &P, const DevState &S, const DevRL &R, con
 // native/cuda/solver/state_vector_scaling.cu:57-97. ROBUST (one lane per tile, FP64): the linear solve runs on
 // statically pivoted factors corrected by GMRES (gmresRefine), the role of the reference's ROBUST single-matrix mode
 // (native/cuda/common/check.cuh:38-40).
-template <int T, typename VT, bool ROBUST = false>
+// ROWCRIT: the per-row stopping criterion (prm.rowCriteria), a variant of its own so that the default kernel carries
+// no code for it.
+template <int T, typename VT, bool ROBUST = false, bool ROWCRIT = false>
 __global__ __launch_bounds__(MAX_THREADS) void nrStepKernel(DevPlan P, DevState S, DevRL R, NrParams prm, DevTop top) {
     const int tile = blockIdx.x;
     const int tid = threadIdx.x, nt = blockDim.x;
@@ -1349,9 +1364,9 @@ __global__ __launch_bounds__(MAX_THREADS) void nrStepKernel(DevPlan P, DevState 
 
     // ---- 1. mismatch + Jacobian of the running lanes
     {
-        double pmax = 0.0, psq = 0.0, pratio = 0.0;
+        double pmax = 0.0, psq = 0.0;
         int pfin = 1;
-        fillPass<T, VT>(P, S, tile, tid, nt, sFill, pmax, pfin, psq, pratio, slackNode);
+        fillPass<T, VT>(P, S, tile, tid, nt, sFill, pmax, pfin, psq, slackNode);
         red[tid] = pmax;
         redI[tid] = pfin;
         laneReduce<T, true>(red, tid, nt);
@@ -1360,7 +1375,9 @@ __global__ __launch_bounds__(MAX_THREADS) void nrStepKernel(DevPlan P, DevState 
             sNorm[tid] = red[tid];
             sFin[tid] = redI[tid];
         }
-        if (prm.rowCriteria) {
+        if constexpr (ROWCRIT) {
+            double pratio = 0.0;
+            rowRatioPass<T>(P, S, tile, tid, nt, sFill, pratio);   // the thread's own F entries: no barrier needed
             __syncthreads();
             red[tid] = pratio;
             laneReduce<T, true>(red, tid, nt);
@@ -1396,9 +1413,9 @@ __global__ __launch_bounds__(MAX_THREADS) void nrStepKernel(DevPlan P, DevState 
                     *reinterpret_cast<double2 *>(S.x + vi) = make_double2(xp.x + mu * d.x, xp.y + mu * d.y);
                 }
                 __syncthreads();
-                double bmax = 0.0, bsq = 0.0, bratio = 0.0;
+                double bmax = 0.0, bsq = 0.0;
                 int bfin = 1;
-                fillPass<T, VT>(P, S, tile, tid, nt, sBt, bmax, bfin, bsq, bratio, slackNode);
+                fillPass<T, VT>(P, S, tile, tid, nt, sBt, bmax, bfin, bsq, slackNode);
                 red[tid] = bmax;
                 redI[tid] = bfin;
                 laneReduce<T, true>(red, tid, nt);
@@ -1407,7 +1424,9 @@ __global__ __launch_bounds__(MAX_THREADS) void nrStepKernel(DevPlan P, DevState 
                     sNorm[tid] = red[tid];
                     sFin[tid] = redI[tid];
                 }
-                if (prm.rowCriteria) {
+                if constexpr (ROWCRIT) {
+                    double bratio = 0.0;
+                    rowRatioPass<T>(P, S, tile, tid, nt, sBt, bratio);
                     __syncthreads();
                     red[tid] = bratio;
                     laneReduce<T, true>(red, tid, nt);
@@ -1438,7 +1457,7 @@ __global__ __launch_bounds__(MAX_THREADS) void nrStepKernel(DevPlan P, DevState 
         sNorm[tid] = norm;
         // the criterion (native/cuda/solver/newton_iteration.cu:241-300): ||F||inf < tol, or with convEps OLF's
         // ||F||_2 < sqrt(convEps^2 nActive), or with row thresholds every |F_i| below its row's (:296-300)
-        const bool below = prm.rowCriteria ? sRowR[tid] < 1.0
+        const bool below = ROWCRIT ? sRowR[tid] < 1.0
                            : prm.convEps > 0.0
                                ? sqrt(sSsq[tid]) < sqrt(prm.convEps * prm.convEps * (double)S.nActive[tile * T + tid])
                                : norm < prm.tol;
@@ -1623,9 +1642,9 @@ __global__ __launch_bounds__(MAX_THREADS) void nrStepKernel(DevPlan P, DevState 
         }
         __syncthreads();
         if (sFlag[1]) {
-            double pmax = 0.0, psq = 0.0, pratio = 0.0;   // restarted lanes solve without a test here
+            double pmax = 0.0, psq = 0.0;
             int pfin = 1;
-            fillPass<T, VT>(P, S, tile, tid, nt, sFill, pmax, pfin, psq, pratio, slackNode);
+            fillPass<T, VT>(P, S, tile, tid, nt, sFill, pmax, pfin, psq, slackNode);
             red[tid] = pmax;
             redI[tid] = pfin;
             laneReduce<T, true>(red, tid, nt);
