@@ -140,7 +140,7 @@ def main():
     inputs, keep = N.BatchContext.make_inputs(
         K, xb, tb, mb, row_mode_entries=rme, target_overrides=(tix, tvl), outaged_branches=ctg[:, None],
         reactive_limits_seed_state=lt, max_iter=nrp.maxIterations, tol=nrp.maxMismatch,
-        reactive_limit_tol=nrp.maxReactivePowerMismatch, states_to_host=False)
+        reactive_limit_tol=nrp.reactive_limit_band(), states_to_host=False)
     tuned = None
     if args.autotune:
         tuned = N.autotune(structure, inputs, N.LU_FP32_REFINED, rl)
@@ -214,7 +214,7 @@ def main():
         pin, pk = N.BatchContext.make_inputs(
             1, xb, tb, mb, row_mode_entries=r1, target_overrides=(ptix[tsel] - s * net.n, ptvl[tsel]),
             outaged_branches=sample[s:s + 1, None], reactive_limits_seed_state=lt[None, :],
-            max_iter=nrp.maxIterations, tol=nrp.maxMismatch, reactive_limit_tol=nrp.maxReactivePowerMismatch,
+            max_iter=nrp.maxIterations, tol=nrp.maxMismatch, reactive_limit_tol=nrp.reactive_limit_band(),
             states_to_host=False)
         plain_inputs.append((pin, pk))
     pctx.solve(plain_inputs[0][0], want_states=False)   # warm-up

@@ -128,7 +128,7 @@ class NewtonRaphsonParameters:
     lineSearchStateVectorScalingMaxIteration: int = 10
     lineSearchStateVectorScalingStepFold: float = 4.0 / 3.0
     reactiveLimits: bool = True
-    maxReactivePowerMismatch: float = 1e-4    # pu
+    maxReactivePowerMismatch: float = 1e-4    # pu: the PV->PQ switch band when no other criterion sets it
     maxOuterLoopIterations: int = 20
     maxPqPvSwitch: int = 3
     # distributed slack (OLF LoadFlowParameters.distributedSlack, OpenLoadFlowParameters.slackBusPMaxMismatch in MW,
@@ -140,6 +140,19 @@ class NewtonRaphsonParameters:
     # OLF's stopping criterion object: None for the uniform ones (maxMismatch, or convEpsPerEq), or a
     # PerEquationTypeStoppingCriteria (per-row thresholds installed on the device context)
     stoppingCriteria: PerEquationTypeStoppingCriteria | None = None
+
+    def reactive_limit_band(self) -> float:
+        """The reactive-limit loop's switch band (pu), derived from the stopping criterion as OLF's
+        AbstractAcOuterLoopConfig.createReactiveLimitsOuterLoop does and the reference mirrors
+        (GpuSaRunScope.java:339-350, GpuSolveSettings.java:35-41): the per-equation-type criterion's
+        maxReactivePowerMismatch / base; under the uniform 2-norm criterion (convEpsPerEq > 0) convEpsPerEq;
+        under the plain ||F||inf criterion maxReactivePowerMismatch."""
+        sc = self.stoppingCriteria
+        if sc is not None:
+            return sc.maxReactivePowerMismatch / sc.base_mva
+        if self.convEpsPerEq > 0:
+            return self.convEpsPerEq
+        return self.maxReactivePowerMismatch
 
     def install_stopping_criteria(self, ctx: N.BatchContext) -> None:
         """GpuNewtonRaphson.installRowCriteria (GpuNewtonRaphson.java:446-459): the per-row thresholds, or none."""
@@ -219,7 +232,7 @@ class GpuNewtonRaphson:
         inp, keep = N.BatchContext.make_inputs(
             1, x0, target, row_mode, reactive_limits_seed_state=None if seed_limit_types is None else
             np.asarray(seed_limit_types, np.int8)[None, :], max_iter=nr.maxIterations, tol=nr.maxMismatch,
-            reactive_limit_tol=nr.maxReactivePowerMismatch, state_vector_scaling_mode=nr.scaling_code(),
+            reactive_limit_tol=nr.reactive_limit_band(), state_vector_scaling_mode=nr.scaling_code(),
             max_dv=nr.maxVoltageChangeV, max_dphi=nr.maxVoltageChangePhi,
             max_outer_iterations=nr.maxOuterLoopIterations, max_pq_pv_switch=nr.maxPqPvSwitch,
             skip_reactive_limits=not nr.reactiveLimits, **nr.slack_inputs(self.distributed_slack),
@@ -411,7 +424,7 @@ class GpuBatchedSecurityAnalysisSolver:
             inp, keep = N.BatchContext.make_inputs(
                 len(part), x_base, target_base, row_mode_base, row_mode_entries=rme, target_overrides=(tix, tvl),
                 outaged_branches=part, x0_per_scenario=x0_host, reactive_limits_seed_state=seed,
-                max_iter=nr.maxIterations, tol=nr.maxMismatch, reactive_limit_tol=nr.maxReactivePowerMismatch,
+                max_iter=nr.maxIterations, tol=nr.maxMismatch, reactive_limit_tol=nr.reactive_limit_band(),
                 state_vector_scaling_mode=nr.scaling_code(), max_dv=nr.maxVoltageChangeV,
                 max_dphi=nr.maxVoltageChangePhi, max_outer_iterations=nr.maxOuterLoopIterations,
                 max_pq_pv_switch=nr.maxPqPvSwitch, skip_reactive_limits=not nr.reactiveLimits,

@@ -336,8 +336,9 @@ def test_multi_branch_outages_match_oracle(grid, lu, k_out):
 def test_per_equation_type_criteria_match_oracle(grid, lu):
     """OLF's PerEquationTypeStoppingCriteria (the reference's per-row thresholds, setBatchRowCriteria) with the
     reference test's values (1e-2 MW, 1e-2 MVar, 1e-4 pu, 1e-4 rad): base case from the flat start and N-1, same
-    statuses and states as the oracle under the same rows, FP64 iteration counts equal; the rule is in force (the
-    uniform 1e-4 pu criterion stops some scenarios earlier)."""
+    statuses and states as the oracle under the same rows and the reactive-limit band the criterion sets
+    (maxReactivePowerMismatch / base), FP64 iteration counts equal; the rule is in force (the uniform 1e-4 pu
+    criterion stops some scenarios earlier)."""
     from olfgpu.solver import PerEquationTypeStoppingCriteria
 
     net = grid("grid2000")
@@ -345,19 +346,21 @@ def test_per_equation_type_criteria_match_oracle(grid, lu):
     crit = PerEquationTypeStoppingCriteria(maxActivePowerMismatch=1e-2, maxReactivePowerMismatch=1e-2,
                                            maxVoltageMismatch=1e-4, maxAngleMismatch=1e-4, base_mva=gridgen.SB)
     thr = crit.row_thresholds(net.n)
+    nrp = NewtonRaphsonParameters(stoppingCriteria=crit)
+    band = nrp.reactive_limit_band()   # the criterion's maxReactivePowerMismatch / base, as OLF derives it
+    assert band == pytest.approx(1e-2 / gridgen.SB)
     osys = po.OracleSystem(net)
     x0, tg, md, _ = net.system()
-    xo, mo, to, lto, ro = osys.solve(x0, md, tg, prm=po.params(row_thresholds=thr))
+    xo, mo, to, lto, ro = osys.solve(x0, md, tg, prm=po.params(row_thresholds=thr, rl_tol=band))
     assert po.STATUS[ro.status] == "CONVERGED"
     st = N.Structure.from_network(net)
-    nrp = NewtonRaphsonParameters(stoppingCriteria=crit)
     g = GpuNewtonRaphson(st, net.reactive_limits(), GpuLoadFlowParameters(linearSolver=lu), nrp).run(x0, tg, md)
     assert g.status == "CONVERGED"
     compare_states(g.x, xo, f"per-row base {lu}")
     if lu == "BATCHED_LU":
         assert g.total_iterations == ro.total_iterations
     ctg = gridgen.sample_contingencies(net, 32, 0.25, 17)
-    ref = osys.n1(xo, mo, to, lto, ctg, prm=po.params(batch_of_one=False, row_thresholds=thr))
+    ref = osys.n1(xo, mo, to, lto, ctg, prm=po.params(batch_of_one=False, row_thresholds=thr, rl_tol=band))
     sa = GpuBatchedSecurityAnalysisSolver(st, net.reactive_limits(), GpuLoadFlowParameters(linearSolver=lu), nrp)
     res = sa.solve_batch_n1(xo, to, mo, lto, ctg)
     assert np.array_equal(res.status, ref["status"]), (res.status, ref["status"])
@@ -365,7 +368,7 @@ def test_per_equation_type_criteria_match_oracle(grid, lu):
     compare_states(res.x[conv], ref["x"][conv], f"per-row n1 {lu}")
     if lu == "BATCHED_LU":
         assert np.array_equal(res.total_iterations, ref["total_iterations"])
-    uniform = osys.n1(xo, mo, to, lto, ctg, prm=po.params(batch_of_one=False))
+    uniform = osys.n1(xo, mo, to, lto, ctg, prm=po.params(batch_of_one=False, rl_tol=band))
     assert (uniform["total_iterations"] < ref["total_iterations"]).any()
 
 
