@@ -391,10 +391,14 @@ class BatchContext:
 
 class LimitTable:
     """Owner of an olfhip_limit_table (the reference's ViolationScreenNative.uploadViolationTable) built from the
-    arrays of a table dict (gridgen.limit_table layout)."""
+    arrays of a table dict (gridgen.limit_table layout). An optional "bus_nominal_kv" entry (scalar or per bus, the
+    reference's GpuLimitTable.busScale) puts the voltage comparisons of olfgpu.violations.manage in kV."""
 
     def __init__(self, structure: Structure, table: dict):
         self.structure = structure
+        self.n_branches = len(table["branch"])
+        kv = table.get("bus_nominal_kv")
+        self.bus_nominal_kv = None if kv is None else np.asarray(kv, np.float64)
         self._keep = {k: np.ascontiguousarray(table[k], np.int32) for k in
                       ("branch", "group_start", "group_side", "group_quantity", "limit_start")}
         self._keep.update({k: np.ascontiguousarray(table[k], np.float64) for k in ("threshold", "bus_high",

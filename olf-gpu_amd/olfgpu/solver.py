@@ -241,9 +241,24 @@ class GpuNewtonRaphson:
         r = self.context.solve(inp)
         n_rl = 0 if self.reactive_limits is None else len(self.reactive_limits["voltage_row"])
         m, t, lt = self.context.read_outer_loop_state(1, n_rl)
+        self._last_status = int(r["status"][0])
         return AcSolverResult(N.STATUS_NAMES[int(r["status"][0])], int(r["iterations"][0]),
                               int(r["total_iterations"][0]), float(r["norm"][0]), r["x"][0].copy(), m[0], t[0],
                               lt[0])
+
+
+    def detect_violations(self, limit_table: N.LimitTable, band=1e-7, managed=True) -> np.ndarray:
+        """The violations of the last run's converged state (N.VIOLATION_DTYPE, scenario 0): the base case's, against
+        which solve_batch_n1 compares the contingencies' (the reference's pre-contingency detection,
+        GpuBaseViolations.java:164-190). managed: as OLF's manager keeps them (one per key, the last one fed);
+        otherwise every violation the scan feeds it."""
+        if getattr(self, "_last_status", None) != N.CONVERGED:
+            raise RuntimeError("violations are detected on a converged run's state")
+        v = limit_table.detect(self.context, band=band)
+        if not managed:
+            return v
+        from .violations import manage
+        return manage(v, limit_table.n_branches, bus_nominal_kv=limit_table.bus_nominal_kv)
 
 
 def longest_first(cost, tie=None) -> np.ndarray:
@@ -364,11 +379,16 @@ class GpuBatchedSecurityAnalysisSolver:
     def solve_batch_n1(self, x_base, target_base, row_mode_base, limit_types_base, contingencies,
                        x0_per_scenario=None, states_to_host=True, warm_start=None,
                        warm_start_features=None, limit_table: N.LimitTable | None = None,
-                       violation_band=1e-7, cost=None, cost_tie=None) -> BatchScenarioResult:
+                       violation_band=1e-7, cost=None, cost_tie=None, base_violations=None,
+                       increased_violations=None) -> BatchScenarioResult:
         """solveBatchN1: contingencies is [K] or [K, m] closed-branch ids (-1 padded). With a warm_start network
         (olfgpu.warmstart.WarmStartMLP, single outages) every scenario starts from its prediction, and any that does
         not converge is solved again from the base-case start. With a limit_table the violations of every converged
-        contingency are detected on its final state (device screen, exact host decisions). With a cost per
+        contingency are detected on its final state (device screen, exact host decisions); given the base case's
+        violations as well (GpuNewtonRaphson.detect_violations), each contingency's list is the one OLF's
+        LimitViolationManager keeps against them (olfgpu.violations.manage: one violation per key, those weaker than
+        or equivalent to the base's dropped under increased_violations, default powsybl-core's thresholds; the
+        reference's post-contingency detection, GpuBaseViolations.java:27-41). With a cost per
         contingency (e.g. warmstart.iteration_costs of its branch) the batch is solved longest first
         (longest_first, ties broken by cost_tie when given); the results are in the contingencies' order either
         way."""
@@ -383,7 +403,8 @@ class GpuBatchedSecurityAnalysisSolver:
                 return None if a is None else np.asarray(a)[order]
             r = self.solve_batch_n1(x_base, target_base, row_mode_base, limit_types_base,
                                     np.asarray(contingencies)[order], part(x0_per_scenario), states_to_host,
-                                    warm_start, part(warm_start_features), limit_table, violation_band)
+                                    warm_start, part(warm_start_features), limit_table, violation_band,
+                                    base_violations=base_violations, increased_violations=increased_violations)
             r.status, r.iterations, r.total_iterations, r.norm = (r.status[inv], r.iterations[inv],
                                                                   r.total_iterations[inv], r.norm[inv])
             r.islanded_count = r.islanded_count[inv]
@@ -504,5 +525,9 @@ class GpuBatchedSecurityAnalysisSolver:
         if limit_table is not None:
             violations = np.concatenate(viol) if viol else np.zeros(0, N.VIOLATION_DTYPE)
             violations = violations[np.argsort(violations["scenario"], kind="stable")]
+            if base_violations is not None:
+                from .violations import manage
+                violations = manage(violations, limit_table.n_branches, base_violations, increased_violations,
+                                    limit_table.bus_nominal_kv)
         return BatchScenarioResult(st, it, tot, nm, xs, islanded, chunks, device_s, fallback, violations,
                                    robust_attempted, robust_recovered, robust_errors)

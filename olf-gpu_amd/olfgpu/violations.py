@@ -1,0 +1,135 @@
+"""OLF's LimitViolationManager over the violations the scan feeds it, and its comparison with the base case.
+
+The device screen plus the exact host decisions (LimitTable.detect) give every violation the scan finds, in the
+order the scan feeds them to OLF's manager (GpuLimitScanner.java:93-135): per scenario, branches in table order (a
+group's first exceeded limit), then buses (high before low). OLF's manager then does two things with that feed,
+which the reference leaves to it and restates for its device screen (GpuBaseViolations.java:27-41,100-170):
+
+* it keeps one violation per key, the last one fed: a branch's key is (branch, side, operational limits group), so
+  the current, active-power and apparent-power groups of one side share a key (the synthetic tables carry one limits
+  group per side); a bus's key is the bus, high and low sharing it;
+* after a contingency, it drops a violation that is weaker than or equivalent to the base case's violation under the
+  same key (violationWeakenedOrEquivalent): same type and a smaller limit, or the same limit and a value within the
+  increased-violations thresholds (powsybl-core's SecurityAnalysisParameters.IncreasedViolationsParameters):
+
+      flows:        |v2| <= |v1| (1 + flowProportionalThreshold + eps)
+      high voltage:  v2  <=  v1 + min(highVoltageAbsoluteThreshold, v1 highVoltageProportionalThreshold) + |v1| eps
+      low voltage:   v2  >=  v1 - min(lowVoltageAbsoluteThreshold,  v1 lowVoltageProportionalThreshold)  - |v1| eps
+
+  with v1 the base value, v2 the post-contingency one, voltages in kV (the table's per-unit value times the bus's
+  nominal voltage) and eps = 1e-12, OLF's EQUALITY_EPSILON (GpuBaseViolations.java:129-140). The voltage
+  thresholds default to zero, so with the defaults a pre-existing voltage violation comes back as soon as its value
+  moves past the base value by more than that relative margin; the two legs are combined with min, so an absolute
+  threshold alone leaves a zero band (docs/increased-violations-thresholds.md:20-48).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _native as N
+
+EQUALITY_EPSILON = 1e-12
+
+KIND_BRANCH, KIND_HIGH_VOLTAGE, KIND_LOW_VOLTAGE = 0, 1, 2
+
+
+@dataclass
+class IncreasedViolationsParameters:
+    """powsybl-core's SecurityAnalysisParameters.IncreasedViolationsParameters, with its defaults. Voltage thresholds
+    absolute in kV, proportional as fractions; flowProportionalThreshold a fraction of the base value."""
+
+    flowProportionalThreshold: float = 0.1
+    lowVoltageProportionalThreshold: float = 0.0
+    lowVoltageAbsoluteThreshold: float = 0.0
+    highVoltageProportionalThreshold: float = 0.0
+    highVoltageAbsoluteThreshold: float = 0.0
+
+
+def _keys(v: np.ndarray, n_table_branches: int) -> np.ndarray:
+    """OLF's manager key of each record, as an integer: 2 element + side - 1 for a branch side, 2 n + bus for a bus."""
+    branch = v["kind"] == KIND_BRANCH
+    return np.where(branch, 2 * v["element"].astype(np.int64) + v["side"] - 1,
+                    2 * np.int64(n_table_branches) + v["element"])
+
+
+def _type(v: np.ndarray) -> np.ndarray:
+    """OLF's LimitViolationType of each record as an integer: 0-2 the branch quantities, 3 high, 4 low voltage."""
+    return np.where(v["kind"] == KIND_BRANCH, v["quantity"], 2 + v["kind"])
+
+
+def _last_per_key(keys: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    """For an array of keys in feed order: (the index of each key's last occurrence, of its first occurrence), both
+    listed in the order of the first occurrences (a dict's insertion order)."""
+    if keys.size == 0:
+        return np.zeros(0, np.int64), np.zeros(0, np.int64)
+    uniq, first = np.unique(keys, return_index=True)
+    _, last_rev = np.unique(keys[::-1], return_index=True)
+    last = keys.size - 1 - last_rev
+    order = np.argsort(first, kind="stable")
+    return last[order], first[order]
+
+
+def weakened_or_equivalent(ref: np.ndarray, rep: np.ndarray, params: IncreasedViolationsParameters,
+                           ref_kv: np.ndarray, rep_kv: np.ndarray) -> np.ndarray:
+    """violationWeakenedOrEquivalent(reference, reported) element-wise over matched record arrays; ref_kv / rep_kv:
+    the nominal voltage that turns each record's value into kV (1 for a branch). GpuBaseViolations.java:143-160."""
+    same_type = _type(ref) == _type(rep)
+    smaller = rep["threshold"] < ref["threshold"]
+    same = rep["threshold"] == ref["threshold"]
+    v1 = ref["value"] * ref_kv
+    v2 = rep["value"] * rep_kv
+    eps = EQUALITY_EPSILON
+    flow = np.abs(v2) <= np.abs(v1) * (1.0 + params.flowProportionalThreshold + eps)
+    high = v2 <= v1 + np.minimum(params.highVoltageAbsoluteThreshold, v1 * params.highVoltageProportionalThreshold) \
+        + np.abs(v1) * eps
+    low = v2 >= v1 - np.minimum(params.lowVoltageAbsoluteThreshold, v1 * params.lowVoltageProportionalThreshold) \
+        - np.abs(v1) * eps
+    within = np.where(rep["kind"] == KIND_BRANCH, flow, np.where(rep["kind"] == KIND_HIGH_VOLTAGE, high, low))
+    return same_type & (smaller | (same & within))
+
+
+def manage(violations: np.ndarray, n_table_branches: int, base: np.ndarray | None = None,
+           params: IncreasedViolationsParameters | None = None, bus_nominal_kv=None) -> np.ndarray:
+    """OLF's LimitViolationManager fed `violations` (N.VIOLATION_DTYPE in the scan's feed order, any number of
+    scenarios): per scenario, the records it keeps, in its map's order (a key's first place, its last kept record).
+
+    base: the base case's violations (one scenario, the scan's records of the base state), or None for the base case
+    itself (no filtering); params: the increased-violations thresholds (default: powsybl-core's); bus_nominal_kv:
+    per-bus nominal voltage in kV (scalar or [n_buses]; None: 1, the voltage thresholds then in per unit)."""
+    v = np.asarray(violations, N.VIOLATION_DTYPE)
+    params = params or IncreasedViolationsParameters()
+    kv = None if bus_nominal_kv is None else np.asarray(bus_nominal_kv, np.float64)
+
+    def to_kv(r):
+        if kv is None:
+            return np.ones(r.size)
+        bus_kv = kv if kv.ndim == 0 else kv[np.where(r["kind"] == KIND_BRANCH, 0, r["element"])]
+        return np.where(r["kind"] == KIND_BRANCH, 1.0, bus_kv)
+
+    keys = _keys(v, n_table_branches)
+    keep = np.ones(v.size, bool)
+    if base is not None and v.size:
+        b = np.asarray(base, N.VIOLATION_DTYPE)
+        if b.size and (b["scenario"] != b["scenario"][0]).any():
+            raise ValueError("base violations must come from one state")
+        bkeys = _keys(b, n_table_branches)
+        blast, _ = _last_per_key(bkeys)
+        bk = bkeys[blast]
+        order = np.argsort(bk)
+        bk, brec = bk[order], b[blast][order]
+        pos = np.minimum(np.searchsorted(bk, keys), max(bk.size - 1, 0))
+        hit = bk[pos] == keys if bk.size else np.zeros(v.size, bool)
+        if hit.any():
+            idx = np.nonzero(hit)[0]
+            ref = brec[pos[idx]]
+            keep[idx] = ~weakened_or_equivalent(ref, v[idx], params, to_kv(ref), to_kv(v[idx]))
+    kept = np.nonzero(keep)[0]
+    if kept.size == 0:
+        return v[:0].copy()
+    # one map per scenario: the key's first kept place, its last kept record
+    skey = v["scenario"][kept].astype(np.int64) * (2 * np.int64(n_table_branches) + 2 ** 31) + keys[kept]
+    last, first = _last_per_key(skey)
+    place = np.lexsort((kept[first], v["scenario"][kept[first]]))
+    return v[kept[last[place]]].copy()

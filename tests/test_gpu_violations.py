@@ -94,3 +94,61 @@ def test_violation_lists_match_the_oracle_n1(grid, name, count, bridges):
     print(f"[viol] {name}: {total} violations over {len(ctg)} contingencies, {ambiguous} within 1e-6 of a limit",
           flush=True)
     assert total > 10
+
+
+def _records(rows, scenario):
+    return np.array([(scenario, q[0], q[1], q[2], q[3], q[4], q[5], q[6]) for q in rows], N.VIOLATION_DTYPE)
+
+
+def test_contingency_lists_against_the_base_case(grid):
+    """OLF's post-contingency lists (its LimitViolationManager against the base case's violations): the base case's
+    violations, detected on the device's base state, equal the oracle's scan of that state; solve_batch_n1 given them
+    returns violations.manage of its full scan (in the caller's order, also when it solves longest first); and each
+    contingency's list equals manage of the oracle's scan of the device's state against the oracle's base list, under
+    the thresholds recommended in docs/increased-violations-thresholds.md:60-71 (0.5 kV and 0.1 % on voltages, 0.1 %
+    on flows, 400 kV buses). Pre-existing violations that did not grow are dropped."""
+    from olfgpu.violations import IncreasedViolationsParameters, manage
+
+    net = grid("grid2000")
+    print("[viol] grid2000 post-contingency lists against the base case", flush=True)
+    st = N.Structure.from_network(net)
+    rl = net.reactive_limits()
+    x0, tg, md, _ = net.system()
+    solver = GpuNewtonRaphson(st, rl)
+    base = solver.run(x0, tg, md)
+    assert base.status == "CONVERGED"
+    table = dict(gridgen.limit_table(net, base.x), bus_nominal_kv=400.0)
+    lt = N.LimitTable(st, table)
+    o = po.OracleSystem(net)
+    raw_base = solver.detect_violations(lt, managed=False)
+    want_base = po.violations(o, base.x, table)
+    assert len(want_base) > 0
+    assert [g[:5] for g in _by_scenario(raw_base, 1)[0]] == [q[:5] for q in want_base]
+    np.testing.assert_allclose(raw_base["value"], [q[5] for q in want_base], rtol=1e-12)
+    bv = solver.detect_violations(lt)
+    np.testing.assert_array_equal(bv, manage(raw_base, lt.n_branches, bus_nominal_kv=400.0))
+    prm = IncreasedViolationsParameters(flowProportionalThreshold=0.001, lowVoltageAbsoluteThreshold=0.5,
+                                        lowVoltageProportionalThreshold=0.001, highVoltageAbsoluteThreshold=0.5,
+                                        highVoltageProportionalThreshold=0.001)
+    ctg = gridgen.sample_contingencies(net, 48, 0.25, 6)
+    sa = GpuBatchedSecurityAnalysisSolver(st, rl, GpuLoadFlowParameters(linearSolver="BATCHED_LU_FP32_REFINED"))
+    raw = sa.solve_batch_n1(base.x, base.target, base.row_mode, base.reactive_limit_types, ctg, limit_table=lt)
+    got = sa.solve_batch_n1(base.x, base.target, base.row_mode, base.reactive_limit_types, ctg, limit_table=lt,
+                            base_violations=bv, increased_violations=prm)
+    np.testing.assert_array_equal(got.violations, manage(raw.violations, lt.n_branches, bv, prm, 400.0))
+    assert len(got.violations) < len(raw.violations)
+    longest = sa.solve_batch_n1(base.x, base.target, base.row_mode, base.reactive_limit_types, ctg, limit_table=lt,
+                                base_violations=bv, increased_violations=prm, cost=np.arange(len(ctg)) % 5)
+    assert _by_scenario(longest.violations, len(ctg)) == _by_scenario(got.violations, len(ctg))
+    obase = manage(_records(want_base, 0), lt.n_branches, bus_nominal_kv=400.0)
+    per = _by_scenario(got.violations, len(ctg))
+    for j, e in enumerate(ctg):
+        if got.status[j] != 0:
+            assert per[j] == []
+            continue
+        scan = po.violations(o, got.x[j], table, outages=[e], islanded=o.islands([e]))
+        want = manage(_records(scan, j), lt.n_branches, obase, prm, 400.0)
+        assert [g[:5] for g in per[j]] == [tuple(int(w[f]) for f in ("kind", "element", "side", "quantity", "limit"))
+                                           for w in want], (j, e)
+    print(f"[viol] {len(raw.violations)} violations scanned, {len(got.violations)} kept against "
+          f"{len(bv)} base violations", flush=True)

@@ -1,0 +1,112 @@
+"""Host logic of OLF's LimitViolationManager over the scan's feed (CPU): one violation per key, the last one fed, and
+the increased-violations comparison with the base case (olfgpu.violations.manage).
+
+Known answers, each from the rule as OLF states it (GpuBaseViolations.java:100-170, the reference's restatement of
+LimitViolationManager.violationWeakenedOrEquivalent; docs/increased-violations-thresholds.md:20-48 for the voltage
+legs): a dropped epsilon, a max in place of the min, a key that includes the type, the first record kept in place of
+the last, or a base lookup by type would each fail one case."""
+import numpy as np
+import pytest
+
+from olfgpu import _native as N
+from olfgpu.violations import IncreasedViolationsParameters, manage
+
+NB = 10   # table branches
+
+
+def rec(*rows):
+    """rows of (scenario, kind, element, side, quantity, limit, value, threshold)"""
+    return np.array([tuple(r) for r in rows], N.VIOLATION_DTYPE)
+
+
+def cur(s, e, side, value, limit=0, threshold=1.0, q=0):
+    return (s, 0, e, side, q, limit, value, threshold)
+
+
+def high(s, bus, value, threshold=1.05):
+    return (s, 1, bus, 0, 0, -1, value, threshold)
+
+
+def low(s, bus, value, threshold=0.95):
+    return (s, 2, bus, 0, 0, -1, value, threshold)
+
+
+def kept(post, base=None, **kw):
+    return [tuple(r)[1:5] + (float(r["value"]),) for r in manage(rec(*post), NB, rec(*base) if base else None, **kw)]
+
+
+def test_no_base_keeps_the_last_violation_per_key_at_its_first_place():
+    post = [cur(0, 3, 1, 1.3), cur(0, 3, 2, 1.2), cur(0, 3, 1, 0.9, limit=5, threshold=0.8, q=1),
+            high(0, 4, 1.07), low(0, 4, 0.80), high(0, 5, 1.08), cur(1, 3, 1, 1.4)]
+    got = kept(post)
+    # side 1's active power overwrites its current (one key per side and limits group), in the current's place;
+    # bus 4's low overwrites its high; scenario 1 has a map of its own
+    assert got == [(0, 3, 1, 1, 0.9), (0, 3, 2, 0, 1.2), (2, 4, 0, 0, 0.80), (1, 5, 0, 0, 1.08), (0, 3, 1, 0, 1.4)]
+
+
+def test_flow_within_the_proportional_threshold_is_dropped():
+    base = [cur(0, 2, 1, 1.2)]
+    at = 1.2 * 1.1
+    assert kept([cur(3, 2, 1, at)], base) == []                       # |v2| = |v1| (1 + 0.1): equivalent
+    assert kept([cur(3, 2, 1, -at)], base) == []                      # magnitudes compared
+    assert len(kept([cur(3, 2, 1, at * (1 + 1e-9))], base)) == 1       # beyond: reported
+    assert len(kept([cur(3, 2, 1, at)], base,
+                    params=IncreasedViolationsParameters(flowProportionalThreshold=0.05))) == 1
+    assert len(kept([cur(3, 2, 2, 1.2)], base)) == 1                  # the other side: another key
+    # a value within eps of the band counts as equal (OLF's EQUALITY_EPSILON)
+    assert kept([cur(3, 2, 1, 1.2 * (1.1 + 0.5e-12))], base) == []
+
+
+def test_limit_order_decides_before_the_value():
+    base = [cur(0, 2, 1, 1.30, limit=1, threshold=1.25)]
+    # a smaller limit exceeded: weaker, dropped whatever the value
+    assert kept([cur(1, 2, 1, 5.0, limit=2, threshold=1.0)], base) == []
+    # a larger limit exceeded: increased, reported however close the value
+    assert len(kept([cur(1, 2, 1, 1.30, limit=0, threshold=1.5)], base)) == 1
+
+
+def test_base_key_keeps_its_last_violation_and_types_must_match():
+    # base: side 1 current then apparent power; the map keeps the apparent power
+    base = [cur(0, 6, 1, 1.2), cur(0, 6, 1, 2.0, limit=7, threshold=1.9, q=2)]
+    assert len(kept([cur(2, 6, 1, 1.2)], base)) == 1                           # current: no current in the map
+    assert kept([cur(2, 6, 1, 2.0, limit=7, threshold=1.9, q=2)], base) == []  # apparent power: equivalent
+
+
+def test_voltage_thresholds_default_to_a_zero_band():
+    base = [high(0, 1, 1.07), low(0, 2, 0.90)]
+    one_ulp_up = np.nextafter(1.07, 2.0)
+    assert kept([high(1, 1, 1.07)], base) == []
+    # a ulp is within |v1| eps: equivalent; 1e-9 pu is beyond the zero band
+    assert kept([high(1, 1, one_ulp_up)], base) == []
+    assert len(kept([high(1, 1, 1.07 + 1e-9)], base)) == 1
+    assert kept([low(1, 2, 0.90 + 1e-9)], base) == []                   # a low voltage that rose: weaker
+    assert len(kept([low(1, 2, 0.90 - 1e-9)], base)) == 1
+
+
+@pytest.mark.parametrize("kv", [400.0, np.full(8, 400.0)])
+def test_voltage_legs_combine_with_min_in_kv(kv):
+    base = [high(0, 1, 1.05 * 1.0)]                                     # 420 kV
+    v1 = 420.0
+    only_abs = IncreasedViolationsParameters(highVoltageAbsoluteThreshold=0.5)
+    # the absolute leg alone: min(0.5, v1 * 0) = 0, no band
+    assert len(kept([high(1, 1, (v1 + 0.1) / 400.0)], base, params=only_abs, bus_nominal_kv=kv)) == 1
+    both = IncreasedViolationsParameters(highVoltageAbsoluteThreshold=0.5, highVoltageProportionalThreshold=0.001)
+    # min(0.5 kV, 0.42 kV) = 0.42 kV
+    assert kept([high(1, 1, (v1 + 0.41) / 400.0)], base, params=both, bus_nominal_kv=kv) == []
+    assert len(kept([high(1, 1, (v1 + 0.43) / 400.0)], base, params=both, bus_nominal_kv=kv)) == 1
+    lowp = IncreasedViolationsParameters(lowVoltageAbsoluteThreshold=0.3, lowVoltageProportionalThreshold=0.01)
+    lbase = [low(0, 3, 0.9)]                                            # 360 kV: min(0.3, 3.6) = 0.3 kV
+    assert kept([low(1, 3, (360.0 - 0.29) / 400.0)], lbase, params=lowp, bus_nominal_kv=kv) == []
+    assert len(kept([low(1, 3, (360.0 - 0.31) / 400.0)], lbase, params=lowp, bus_nominal_kv=kv)) == 1
+
+
+def test_a_filtered_record_leaves_the_earlier_one_under_its_key():
+    # side 1: current (new) then active power (equivalent to the base's): the current stays
+    base = [cur(0, 4, 1, 0.9, limit=3, threshold=0.8, q=1)]
+    post = [cur(5, 4, 1, 1.3), cur(5, 4, 1, 0.9, limit=3, threshold=0.8, q=1)]
+    assert kept(post, base) == [(0, 4, 1, 0, 1.3)]
+
+
+def test_base_from_several_states_is_refused():
+    with pytest.raises(ValueError):
+        manage(rec(cur(1, 1, 1, 1.0)), NB, rec(cur(0, 1, 1, 1.0), cur(1, 2, 1, 1.0)))
