@@ -291,7 +291,8 @@ int olfhip_autotune(olfhip_structure *structure, int mode, const olfhip_reactive
  * power |S|, voltage V. Within a group the first exceeded threshold is reported and the group stops there. Outaged
  * branches, islanded buses and branches with an islanded end are out of service. */
 enum { OLFHIP_QUANTITY_CURRENT = 0, OLFHIP_QUANTITY_ACTIVE_POWER = 1, OLFHIP_QUANTITY_APPARENT_POWER = 2 };
-enum { OLFHIP_VIOLATION_BRANCH = 0, OLFHIP_VIOLATION_HIGH_VOLTAGE = 1, OLFHIP_VIOLATION_LOW_VOLTAGE = 2 };
+enum { OLFHIP_VIOLATION_BRANCH = 0, OLFHIP_VIOLATION_HIGH_VOLTAGE = 1, OLFHIP_VIOLATION_LOW_VOLTAGE = 2,
+       OLFHIP_VIOLATION_HIGH_VOLTAGE_ANGLE = 3, OLFHIP_VIOLATION_LOW_VOLTAGE_ANGLE = 4 };
 typedef struct olfhip_limit_table olfhip_limit_table;
 typedef struct olfhip_limit_table_desc {
     int n_branches;                 /* monitored branches (table order) */
@@ -304,15 +305,24 @@ typedef struct olfhip_limit_table_desc {
     int n_buses;                    /* buses 0 .. n_buses - 1 carry voltage limits */
     const double *bus_high;         /* [n_buses] pu, NaN: none */
     const double *bus_low;          /* [n_buses] pu, NaN: none */
+    /* Voltage-angle limits (OLF's LfVoltageAngleLimit, checked by GpuLimitScanner.detectVoltageAngleViolations,
+     * GpuLimitScanner.java:239-267): the angle difference to - from, in radians, between two buses both in service,
+     * against a high and a low value (NaN: none); reported after the buses, high before low, per limit. The screen
+     * does not cover them: every limit is decided on the host for every converged scenario. */
+    int n_angle_limits;
+    const int *angle_from;          /* [n_angle_limits] bus index */
+    const int *angle_to;            /* [n_angle_limits] bus index */
+    const double *angle_high;       /* [n_angle_limits] rad, NaN: none */
+    const double *angle_low;        /* [n_angle_limits] rad, NaN: none */
 } olfhip_limit_table_desc;
 typedef struct olfhip_violation {
     int scenario;                   /* index in the last solve's scenarios */
     int kind;                       /* OLFHIP_VIOLATION_* */
-    int element;                    /* table branch index, or bus index */
-    int side;                       /* 1 or 2 (branch), 0 (bus) */
+    int element;                    /* table branch index, bus index, or voltage-angle limit index */
+    int side;                       /* 1 or 2 (branch), 0 (bus, angle) */
     int quantity;                   /* OLFHIP_QUANTITY_* (branch) */
-    int limit;                      /* limit index (branch), -1 (bus) */
-    double value;                   /* the value (signed active power) */
+    int limit;                      /* limit index (branch), -1 (bus, angle) */
+    double value;                   /* the value (signed active power; angle difference in rad) */
     double threshold;
 } olfhip_violation;
 int olfhip_create_limit_table(olfhip_structure *structure, const olfhip_limit_table_desc *desc,
@@ -324,7 +334,8 @@ void olfhip_destroy_limit_table(olfhip_limit_table *table);
 int olfhip_screen_violations(olfhip_context *context, olfhip_limit_table *table, double band, int capacity,
                              int *counts, int *records);
 /* Screen plus the exact host decisions: the violations of every converged scenario, ordered by scenario, then
- * branches in table order (a group's first exceeded limit), then buses (high before low). Fails with
+ * branches in table order (a group's first exceeded limit), then buses (high before low), then voltage-angle limits
+ * (high before low). Fails with
  * OLFHIP_E_ARGUMENT, *n_violations set to the count, when max_violations is too small. */
 int olfhip_detect_violations(olfhip_context *context, olfhip_limit_table *table, double band, int capacity,
                              int max_violations, olfhip_violation *violations, int *n_violations);

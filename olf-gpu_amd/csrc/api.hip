@@ -756,6 +756,9 @@ struct LimitTableImpl {
     int nBranches = 0, nGroups = 0, nLimits = 0, nBuses = 0;
     std::vector<int> branch, groupStart, groupSide, groupQty, limitStart, groupBranch, limitGroup;
     std::vector<double> threshold, busHigh, busLow;
+    int nAngle = 0;
+    std::vector<int> angleFrom, angleTo;
+    std::vector<double> angleHigh, angleLow;
     DBuf dBranch, dGroupStart, dGroupSide, dGroupQty, dLimitStart, dThreshold, dBusHigh, dBusLow, dCoef, dNode;
     DBuf records, counts, screenLane, req, vals, isl;
     DevLimits dev() const {
@@ -774,6 +777,12 @@ static LimitTableImpl *createLimitTable(StructureImpl *st, const olfhip_limit_ta
                               !d->limit_start || !d->limit_threshold))
         throw ArgumentError("branch limit arrays are null");
     if (d->n_buses > 0 && (!d->bus_high || !d->bus_low)) throw ArgumentError("bus limit arrays are null");
+    if (d->n_angle_limits < 0) throw ArgumentError("n_angle_limits must not be negative");
+    if (d->n_angle_limits > 0 && (!d->angle_from || !d->angle_to || !d->angle_high || !d->angle_low))
+        throw ArgumentError("voltage-angle limit arrays are null");
+    for (int a = 0; a < d->n_angle_limits; a++)
+        if (d->angle_from[a] < 0 || d->angle_from[a] >= P.nb || d->angle_to[a] < 0 || d->angle_to[a] >= P.nb)
+            throw ArgumentError("voltage-angle limit bus out of range");
     std::unique_ptr<LimitTableImpl> t(new LimitTableImpl());
     t->nBranches = d->n_branches;
     t->nBuses = d->n_buses;
@@ -811,6 +820,13 @@ static LimitTableImpl *createLimitTable(StructureImpl *st, const olfhip_limit_ta
     if (d->n_buses > 0) {
         t->busHigh.assign(d->bus_high, d->bus_high + d->n_buses);
         t->busLow.assign(d->bus_low, d->bus_low + d->n_buses);
+    }
+    t->nAngle = d->n_angle_limits;
+    if (t->nAngle > 0) {
+        t->angleFrom.assign(d->angle_from, d->angle_from + t->nAngle);
+        t->angleTo.assign(d->angle_to, d->angle_to + t->nAngle);
+        t->angleHigh.assign(d->angle_high, d->angle_high + t->nAngle);
+        t->angleLow.assign(d->angle_low, d->angle_low + t->nAngle);
     }
     st->ensureDevice();
     t->dBranch.upload(t->branch);
@@ -883,7 +899,9 @@ static void branchFlows(const Plan &P, int e, double2 x1, double2 x2, double f[6
 // Screen, then the exact decisions: every candidate re-tested with the exact comparison on the host's FP64 value,
 // the first confirmed limit of a group reported and the rest of the group skipped (GpuLimitScanner.java:178-262,
 // 394-410); a scenario whose candidates overflowed the capacity is scanned in full (all its groups and buses in table
-// order, GpuLimitScanner.java:95-131). Output ordered by scenario, then branches in table order, then buses.
+// order, GpuLimitScanner.java:95-131). Output ordered by scenario, then branches in table order, then buses, then
+// the voltage-angle limits, which the screen does not cover: both buses of every limit are gathered for every
+// converged scenario and decided here (GpuLimitScanner.java:239-267).
 static void detectViolations(ContextImpl *c, LimitTableImpl *t, double band, int capacity,
                              std::vector<olfhip_violation> &out) {
     std::vector<int> counts, records;
@@ -912,7 +930,13 @@ static void detectViolations(ContextImpl *c, LimitTableImpl *t, double band, int
     // node states to gather: both ends of each candidate branch, the bus of each candidate voltage
     std::vector<int> req;
     std::vector<std::vector<int>> at(K);   // per scenario, per candidate: index of its first gathered node
+    std::vector<int> angleAt(K, -1);       // per converged scenario: index of its first angle-limit node pair
     for (int k = 0; k < K; k++) {
+        if (t->nAngle > 0 && c->lastStatus[k] == ST_CONVERGED) {
+            angleAt[k] = (int)(req.size() / 2);
+            for (int a = 0; a < t->nAngle; a++)
+                req.insert(req.end(), {k, P.pos[t->angleFrom[a]], k, P.pos[t->angleTo[a]]});
+        }
         at[k].resize(cand[k].size());
         for (size_t q = 0; q < cand[k].size(); q++) {
             const int r = cand[k][q];
@@ -985,6 +1009,19 @@ static void detectViolations(ContextImpl *c, LimitTableImpl *t, double band, int
                     out.push_back(olfhip_violation{k, OLFHIP_VIOLATION_LOW_VOLTAGE, i, 0, 0, -1, v, t->busLow[i]});
                 }
             }
+        }
+        if (angleAt[k] < 0) continue;
+        for (int a = 0; a < t->nAngle; a++) {
+            const int j = angleAt[k] + 2 * a;
+            if (iv[j] || iv[j + 1]) continue;      // a bus out of service in this scenario
+            const double diff = xv[j + 1].x - xv[j].x;
+            // NaN limits make the comparisons false
+            if (diff > t->angleHigh[a])
+                out.push_back(olfhip_violation{k, OLFHIP_VIOLATION_HIGH_VOLTAGE_ANGLE, a, 0, 0, -1, diff,
+                                               t->angleHigh[a]});
+            if (diff < t->angleLow[a])
+                out.push_back(olfhip_violation{k, OLFHIP_VIOLATION_LOW_VOLTAGE_ANGLE, a, 0, 0, -1, diff,
+                                               t->angleLow[a]});
         }
     }
 }

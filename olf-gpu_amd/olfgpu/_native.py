@@ -85,14 +85,15 @@ class LimitTableDesc(C.Structure):
     _fields_ = [("n_branches", C.c_int), ("branch", C.c_void_p), ("group_start", C.c_void_p),
                 ("group_side", C.c_void_p), ("group_quantity", C.c_void_p), ("limit_start", C.c_void_p),
                 ("limit_threshold", C.c_void_p), ("n_buses", C.c_int), ("bus_high", C.c_void_p),
-                ("bus_low", C.c_void_p)]
+                ("bus_low", C.c_void_p), ("n_angle_limits", C.c_int), ("angle_from", C.c_void_p),
+                ("angle_to", C.c_void_p), ("angle_high", C.c_void_p), ("angle_low", C.c_void_p)]
 
 
 # olfhip_violation as a numpy record
 VIOLATION_DTYPE = np.dtype([("scenario", np.int32), ("kind", np.int32), ("element", np.int32), ("side", np.int32),
                             ("quantity", np.int32), ("limit", np.int32), ("value", np.float64),
                             ("threshold", np.float64)])
-VIOLATION_KINDS = {0: "BRANCH", 1: "HIGH_VOLTAGE", 2: "LOW_VOLTAGE"}
+VIOLATION_KINDS = {0: "BRANCH", 1: "HIGH_VOLTAGE", 2: "LOW_VOLTAGE", 3: "HIGH_VOLTAGE_ANGLE", 4: "LOW_VOLTAGE_ANGLE"}
 
 
 def lib():
@@ -403,12 +404,20 @@ class LimitTable:
                       ("branch", "group_start", "group_side", "group_quantity", "limit_start")}
         self._keep.update({k: np.ascontiguousarray(table[k], np.float64) for k in ("threshold", "bus_high",
                                                                                    "bus_low")})
+        # optional voltage-angle limits: bus pairs (angle_from, angle_to), rad bounds (NaN: none)
+        self._keep.update({k: np.ascontiguousarray(table.get(k, ()), np.int32) for k in ("angle_from", "angle_to")})
+        self._keep.update({k: np.ascontiguousarray(table.get(k, ()), np.float64) for k in ("angle_high",
+                                                                                           "angle_low")})
         k = self._keep
         self.n_limits = len(k["threshold"])
         self.n_buses = len(k["bus_high"])
+        self.n_angle_limits = len(k["angle_from"])
+        if not (len(k["angle_to"]) == len(k["angle_high"]) == len(k["angle_low"]) == self.n_angle_limits):
+            raise ValueError("voltage-angle limit arrays must have one length")
         desc = LimitTableDesc(len(k["branch"]), ptr(k["branch"]), ptr(k["group_start"]), ptr(k["group_side"]),
                               ptr(k["group_quantity"]), ptr(k["limit_start"]), ptr(k["threshold"]),
-                              len(k["bus_high"]), ptr(k["bus_high"]), ptr(k["bus_low"]))
+                              len(k["bus_high"]), ptr(k["bus_high"]), ptr(k["bus_low"]), self.n_angle_limits,
+                              ptr(k["angle_from"]), ptr(k["angle_to"]), ptr(k["angle_high"]), ptr(k["angle_low"]))
         h = C.c_void_p()
         check(lib().olfhip_create_limit_table(structure.h, C.byref(desc), C.byref(h)))
         self.h = h

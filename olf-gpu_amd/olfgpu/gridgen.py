@@ -382,7 +382,7 @@ def branch_flows(net: Network, x: np.ndarray):
     return S1, S2, v[a], v[b]
 
 
-def limit_table(net: Network, x_base: np.ndarray, seed: int = 7) -> dict:
+def limit_table(net: Network, x_base: np.ndarray, seed: int = 7, angle_limits: int = 0) -> dict:
     """Synthetic operational limits for violation detection (the workload definition carries none), rated from the
     base case so that outages overload some neighbours:
 
@@ -390,7 +390,10 @@ def limit_table(net: Network, x_base: np.ndarray, seed: int = 7) -> dict:
       [1.5, 1.25, 1.0] x PATL, most severe first, PATL = max(1.1 - 1.5 x the larger base-case end current, 0.02) pu;
     * 5 % of the monitored branches also an active-power group on side 1 ([1.3, 1.0] x a rating of the same kind on
       |P1|), every monitored transformer an apparent-power group on side 1 ([1.0] x a rating on |S1|);
-    * every bus a low limit U(0.90, 0.95) and a high limit U(1.05, 1.10) pu, 10 % of each NaN (no limit).
+    * every bus a low limit U(0.90, 0.95) and a high limit U(1.05, 1.10) pu, 10 % of each NaN (no limit);
+    * angle_limits voltage-angle limits across as many distinct branches (bus1 -> bus2), the base-case difference
+      +- U(0.0002, 0.003) rad (N-1 moves them by that much), 10 % of each bound NaN (drawn apart, so the other
+      entries do not depend on it).
     Returns the table as arrays in the C ABI's layout (olfhip_limit_table_desc)."""
     rng = np.random.default_rng(seed)
     S1, S2, v1, v2 = branch_flows(net, x_base)
@@ -422,10 +425,21 @@ def limit_table(net: Network, x_base: np.ndarray, seed: int = 7) -> dict:
     high = rng.uniform(1.05, 1.10, net.n_bus)
     low[rng.random(net.n_bus) < 0.1] = np.nan
     high[rng.random(net.n_bus) < 0.1] = np.nan
-    return dict(branch=np.arange(net.n_branch, dtype=np.int32), group_start=np.array(group_start, np.int32),
-                group_side=np.array(group_side, np.int32), group_quantity=np.array(group_qty, np.int32),
-                limit_start=np.array(limit_start, np.int32), threshold=np.array(threshold, np.float64),
-                bus_high=high, bus_low=low)
+    table = dict(branch=np.arange(net.n_branch, dtype=np.int32), group_start=np.array(group_start, np.int32),
+                 group_side=np.array(group_side, np.int32), group_quantity=np.array(group_qty, np.int32),
+                 limit_start=np.array(limit_start, np.int32), threshold=np.array(threshold, np.float64),
+                 bus_high=high, bus_low=low)
+    if angle_limits:
+        ra = np.random.default_rng(seed + 1)
+        br = ra.choice(net.n_branch, angle_limits, replace=False)
+        fr, to = net.bus1[br].astype(np.int32), net.bus2[br].astype(np.int32)
+        d = x_base[2 * to] - x_base[2 * fr]
+        ahigh = d + ra.uniform(0.0002, 0.003, angle_limits)
+        alow = d - ra.uniform(0.0002, 0.003, angle_limits)
+        ahigh[ra.random(angle_limits) < 0.1] = np.nan
+        alow[ra.random(angle_limits) < 0.1] = np.nan
+        table.update(angle_from=fr, angle_to=to, angle_high=ahigh, angle_low=alow)
+    return table
 
 
 def distributed_slack(net: Network, target: np.ndarray | None = None, headroom: float = 1.25) -> dict:

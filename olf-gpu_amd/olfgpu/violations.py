@@ -7,7 +7,7 @@ which the reference leaves to it and restates for its device screen (GpuBaseViol
 
 * it keeps one violation per key, the last one fed: a branch's key is (branch, side, operational limits group), so
   the current, active-power and apparent-power groups of one side share a key (the synthetic tables carry one limits
-  group per side); a bus's key is the bus, high and low sharing it;
+  group per side); a bus's key is the bus, high and low sharing it; a voltage-angle limit's key is the limit;
 * after a contingency, it drops a violation that is weaker than or equivalent to the base case's violation under the
   same key (violationWeakenedOrEquivalent): same type and a smaller limit, or the same limit and a value within the
   increased-violations thresholds (powsybl-core's SecurityAnalysisParameters.IncreasedViolationsParameters):
@@ -20,7 +20,8 @@ which the reference leaves to it and restates for its device screen (GpuBaseViol
   nominal voltage) and eps = 1e-12, OLF's EQUALITY_EPSILON (GpuBaseViolations.java:129-140). The voltage
   thresholds default to zero, so with the defaults a pre-existing voltage violation comes back as soon as its value
   moves past the base value by more than that relative margin; the two legs are combined with min, so an absolute
-  threshold alone leaves a zero band (docs/increased-violations-thresholds.md:20-48).
+  threshold alone leaves a zero band (docs/increased-violations-thresholds.md:20-48). Voltage-angle violations are
+  kept (parity unpinned: the reference leaves their comparison to OLF's manager, GpuBaseViolations.java:52-54).
 """
 from __future__ import annotations
 
@@ -32,7 +33,7 @@ from . import _native as N
 
 EQUALITY_EPSILON = 1e-12
 
-KIND_BRANCH, KIND_HIGH_VOLTAGE, KIND_LOW_VOLTAGE = 0, 1, 2
+KIND_BRANCH, KIND_HIGH_VOLTAGE, KIND_LOW_VOLTAGE, KIND_HIGH_ANGLE, KIND_LOW_ANGLE = 0, 1, 2, 3, 4
 
 
 @dataclass
@@ -47,15 +48,20 @@ class IncreasedViolationsParameters:
     highVoltageAbsoluteThreshold: float = 0.0
 
 
-def _keys(v: np.ndarray, n_table_branches: int) -> np.ndarray:
-    """OLF's manager key of each record, as an integer: 2 element + side - 1 for a branch side, 2 n + bus for a bus."""
-    branch = v["kind"] == KIND_BRANCH
-    return np.where(branch, 2 * v["element"].astype(np.int64) + v["side"] - 1,
-                    2 * np.int64(n_table_branches) + v["element"])
+_CLASS = np.int64(1) << 32    # key classes: branch sides, buses, voltage-angle limits
+
+
+def _keys(v: np.ndarray) -> np.ndarray:
+    """OLF's manager key of each record, as an integer: 2 element + side - 1 for a branch side, then the bus, then
+    the voltage-angle limit (its id: high and low share it), each class in a range of its own."""
+    e = v["element"].astype(np.int64)
+    return np.where(v["kind"] == KIND_BRANCH, 2 * e + v["side"] - 1,
+                    np.where(v["kind"] <= KIND_LOW_VOLTAGE, _CLASS + e, 2 * _CLASS + e))
 
 
 def _type(v: np.ndarray) -> np.ndarray:
-    """OLF's LimitViolationType of each record as an integer: 0-2 the branch quantities, 3 high, 4 low voltage."""
+    """OLF's LimitViolationType of each record as an integer: 0-2 the branch quantities, 3 high, 4 low voltage,
+    5 high, 6 low voltage angle."""
     return np.where(v["kind"] == KIND_BRANCH, v["quantity"], 2 + v["kind"])
 
 
@@ -87,7 +93,9 @@ def weakened_or_equivalent(ref: np.ndarray, rep: np.ndarray, params: IncreasedVi
     low = v2 >= v1 - np.minimum(params.lowVoltageAbsoluteThreshold, v1 * params.lowVoltageProportionalThreshold) \
         - np.abs(v1) * eps
     within = np.where(rep["kind"] == KIND_BRANCH, flow, np.where(rep["kind"] == KIND_HIGH_VOLTAGE, high, low))
-    return same_type & (smaller | (same & within))
+    # voltage-angle violations are kept: the reference's restatement leaves their comparison to OLF's own manager
+    # (GpuBaseViolations.java:52-54), which is not restated here
+    return same_type & (smaller | (same & within)) & (rep["kind"] <= KIND_LOW_VOLTAGE)
 
 
 def manage(violations: np.ndarray, n_table_branches: int, base: np.ndarray | None = None,
@@ -105,16 +113,21 @@ def manage(violations: np.ndarray, n_table_branches: int, base: np.ndarray | Non
     def to_kv(r):
         if kv is None:
             return np.ones(r.size)
-        bus_kv = kv if kv.ndim == 0 else kv[np.where(r["kind"] == KIND_BRANCH, 0, r["element"])]
-        return np.where(r["kind"] == KIND_BRANCH, 1.0, bus_kv)
+        bus = (r["kind"] == KIND_HIGH_VOLTAGE) | (r["kind"] == KIND_LOW_VOLTAGE)
+        bus_kv = kv if kv.ndim == 0 else kv[np.where(bus, r["element"], 0)]
+        return np.where(bus, bus_kv, 1.0)
 
-    keys = _keys(v, n_table_branches)
+    for r in (v,) if base is None else (v, np.asarray(base, N.VIOLATION_DTYPE)):
+        br = r["kind"] == KIND_BRANCH
+        if br.any() and (r["element"][br] >= n_table_branches).any():
+            raise ValueError("a branch violation's element is beyond the table's branches")
+    keys = _keys(v)
     keep = np.ones(v.size, bool)
     if base is not None and v.size:
         b = np.asarray(base, N.VIOLATION_DTYPE)
         if b.size and (b["scenario"] != b["scenario"][0]).any():
             raise ValueError("base violations must come from one state")
-        bkeys = _keys(b, n_table_branches)
+        bkeys = _keys(b)
         blast, _ = _last_per_key(bkeys)
         bk = bkeys[blast]
         order = np.argsort(bk)
@@ -129,7 +142,7 @@ def manage(violations: np.ndarray, n_table_branches: int, base: np.ndarray | Non
     if kept.size == 0:
         return v[:0].copy()
     # one map per scenario: the key's first kept place, its last kept record
-    skey = v["scenario"][kept].astype(np.int64) * (2 * np.int64(n_table_branches) + 2 ** 31) + keys[kept]
+    skey = v["scenario"][kept].astype(np.int64) * (4 * _CLASS) + keys[kept]
     last, first = _last_per_key(skey)
     place = np.lexsort((kept[first], v["scenario"][kept[first]]))
     return v[kept[last[place]]].copy()

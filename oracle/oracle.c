@@ -1049,5 +1049,18 @@ int oracle_violations(const oracle_sys *s, const double *x, const int *out_br, i
             count++;
         }
     }
+    for (int a = 0; a < lim->n_angle; a++) {
+        const int f = lim->angle_from[a], t = lim->angle_to[a];
+        if (islanded && (islanded[f] || islanded[t])) continue;
+        const double d = x[2 * t] - x[2 * f];
+        if (!isnan(lim->angle_high[a]) && d > lim->angle_high[a]) {
+            if (count < max_out) { oracle_violation v0 = {3, a, 0, 0, -1, d, lim->angle_high[a]}; out[count] = v0; }
+            count++;
+        }
+        if (!isnan(lim->angle_low[a]) && d < lim->angle_low[a]) {
+            if (count < max_out) { oracle_violation v0 = {4, a, 0, 0, -1, d, lim->angle_low[a]}; out[count] = v0; }
+            count++;
+        }
+    }
     return count;
 }

@@ -150,11 +150,14 @@ typedef struct {
     const double *threshold;
     int n_buses;
     const double *bus_high, *bus_low;
+    int n_angle;                         /* voltage-angle limits: to - from (rad) against high / low (NaN: none) */
+    const int *angle_from, *angle_to;
+    const double *angle_high, *angle_low;
 } oracle_limits;
 
 typedef struct {
-    int kind;           /* 0 branch, 1 high voltage, 2 low voltage */
-    int element;        /* table branch or bus */
+    int kind;           /* 0 branch, 1 high voltage, 2 low voltage, 3 high / 4 low voltage angle */
+    int element;        /* table branch, bus or voltage-angle limit */
     int side, quantity, limit;
     double value, threshold;
 } oracle_violation;
@@ -162,7 +165,9 @@ typedef struct {
 /* OLF's full detection scan on one state (reference sa/gpu/GpuLimitScanner.java:95-131,394-410, which follows
  * LimitViolationManager.detectViolations): every monitored branch in service (not outaged, both ends not
  * islanded) in table order, each group's first exceeded threshold (active power compares |p|, reports p;
- * a NaN apparent power is skipped); then every bus in service, high then low (NaN limits never fire).
+ * a NaN apparent power is skipped); then every bus in service, high then low (NaN limits never fire); then every
+ * voltage-angle limit whose two buses are in service, the angle difference to - from against its high then its low
+ * value (GpuLimitScanner.java:239-267, OLF's detectVoltageAngleViolations).
  * Flows from oracle_closed_branch. Writes at most max_out violations, returns the count found.
  * Pinned by tests/test_oracle_violations.py (textbook flows, hand cases, brute-force rescan). */
 int oracle_violations(const oracle_sys *s, const double *x, const int *out_br, int n_out,

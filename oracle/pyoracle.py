@@ -37,7 +37,9 @@ class Rl(C.Structure):
 class Limits(C.Structure):
     _fields_ = [("n_branches", C.c_int), ("branch", C.c_void_p), ("group_start", C.c_void_p),
                 ("group_side", C.c_void_p), ("group_quantity", C.c_void_p), ("limit_start", C.c_void_p),
-                ("threshold", C.c_void_p), ("n_buses", C.c_int), ("bus_high", C.c_void_p), ("bus_low", C.c_void_p)]
+                ("threshold", C.c_void_p), ("n_buses", C.c_int), ("bus_high", C.c_void_p), ("bus_low", C.c_void_p),
+                ("n_angle", C.c_int), ("angle_from", C.c_void_p), ("angle_to", C.c_void_p),
+                ("angle_high", C.c_void_p), ("angle_low", C.c_void_p)]
 
 
 class Violation(C.Structure):
@@ -240,10 +242,13 @@ def limits_struct(table):
     keep = {k: np.ascontiguousarray(table[k], np.int32) for k in
             ("branch", "group_start", "group_side", "group_quantity", "limit_start")}
     keep.update({k: np.ascontiguousarray(table[k], np.float64) for k in ("threshold", "bus_high", "bus_low")})
+    keep.update({k: np.ascontiguousarray(table.get(k, ()), np.int32) for k in ("angle_from", "angle_to")})
+    keep.update({k: np.ascontiguousarray(table.get(k, ()), np.float64) for k in ("angle_high", "angle_low")})
     st = Limits(len(keep["branch"]), keep["branch"].ctypes.data, keep["group_start"].ctypes.data,
                 keep["group_side"].ctypes.data, keep["group_quantity"].ctypes.data, keep["limit_start"].ctypes.data,
                 keep["threshold"].ctypes.data, len(keep["bus_high"]), keep["bus_high"].ctypes.data,
-                keep["bus_low"].ctypes.data)
+                keep["bus_low"].ctypes.data, len(keep["angle_from"]), keep["angle_from"].ctypes.data,
+                keep["angle_to"].ctypes.data, keep["angle_high"].ctypes.data, keep["angle_low"].ctypes.data)
     return st, keep
 
 

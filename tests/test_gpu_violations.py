@@ -152,3 +152,30 @@ def test_contingency_lists_against_the_base_case(grid):
                                            for w in want], (j, e)
     print(f"[viol] {len(raw.violations)} violations scanned, {len(got.violations)} kept against "
           f"{len(bv)} base violations", flush=True)
+
+
+def test_voltage_angle_limits_match_the_oracle_scan(grid):
+    """Voltage-angle limits (300 across branches of grid2000, bounds within 0.003 rad of the base-case difference):
+    the detected lists, angle records last, equal the oracle's scan of the device's own N-1 states, values to 1e-12
+    relative; islanded buses and non-converged scenarios report none."""
+    net = grid("grid2000")
+    print("[viol] grid2000 voltage-angle limits", flush=True)
+    st, base, table, ctg, sa = _setup(net, "BATCHED_LU_FP32_REFINED", 48, 0.25, 6)
+    table = gridgen.limit_table(net, base.x, angle_limits=300)
+    lt = N.LimitTable(st, table)
+    assert lt.n_angle_limits == 300
+    r = sa.solve_batch_n1(base.x, base.target, base.row_mode, base.reactive_limit_types, ctg, limit_table=lt)
+    got = _by_scenario(r.violations, len(ctg))
+    o = po.OracleSystem(net)
+    n_angle = 0
+    for j, e in enumerate(ctg):
+        if r.status[j] != 0:
+            assert got[j] == []
+            continue
+        ref = po.violations(o, r.x[j], table, outages=[e], islanded=o.islands([e]))
+        assert [g[:5] for g in got[j]] == [q[:5] for q in ref], (j, e)
+        if ref:
+            np.testing.assert_allclose([g[5] for g in got[j]], [q[5] for q in ref], rtol=1e-12, atol=1e-15)
+        n_angle += sum(1 for q in ref if q[0] >= 3)
+    print(f"[viol] {n_angle} voltage-angle violations over {len(ctg)} contingencies", flush=True)
+    assert n_angle > 0

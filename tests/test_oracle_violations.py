@@ -109,3 +109,35 @@ def test_scan_matches_a_brute_force_rescan_on_n1_states(grid):
         np.testing.assert_allclose([g[5] for g in got], [q[5] for q in ref], rtol=1e-9)
         total += len(got)
     assert total > 20
+
+
+def test_voltage_angle_limits_by_hand(grid):
+    """The oracle's voltage-angle check (GpuLimitScanner.java:239-267): on a state with chosen angles, the difference
+    to - from against high then low, NaN bounds never firing, a limit with an islanded bus skipped, after every bus;
+    a swapped from / to or a strict comparison turned inclusive fails a case."""
+    net = grid("grid118")
+    x = net.system()[0].copy()
+    x[0::2] = 0.0
+    x[1::2] = 1.0                         # no bus limit fires (1.0 within every bus's band)
+    x[2 * 3] = 0.30
+    x[2 * 7] = -0.10
+    table = _zero_table(net, 0, 1)
+    table.update(branch=np.zeros(0, np.int32), group_start=np.zeros(1, np.int32), group_side=np.zeros(0, np.int32),
+                 group_quantity=np.zeros(0, np.int32), limit_start=np.zeros(1, np.int32), threshold=np.zeros(0),
+                 bus_high=np.full(net.n_bus, 1.1), bus_low=np.full(net.n_bus, 0.9),
+                 angle_from=np.array([7, 3, 7, 3, 7, 7], np.int32), angle_to=np.array([3, 7, 3, 7, 3, 5], np.int32),
+                 # to - from: 0.4, -0.4, 0.4, -0.4, 0.4 (exactly at the high bound), 0.1 (bus 5 islanded)
+                 angle_high=np.array([0.35, np.nan, np.nan, 0.5, 0.4, 0.0]),
+                 angle_low=np.array([np.nan, -0.3, 0.5, -0.45, 0.0, 0.2]))
+    o = po.OracleSystem(net)
+    isl = np.zeros(net.n_bus, np.uint8)
+    isl[5] = 1
+    got = po.violations(o, x, table, islanded=isl)
+    want = [(3, 0, 0, 0, -1, 0.4, 0.35), (4, 1, 0, 0, -1, -0.4, -0.3), (4, 2, 0, 0, -1, 0.4, 0.5)]
+    assert [g[:5] for g in got] == [w[:5] for w in want]
+    np.testing.assert_allclose([g[5] for g in got], [w[5] for w in want], rtol=0, atol=1e-15)
+    assert [g[6] for g in got] == [w[6] for w in want]
+    # the same limits after a bus violation: buses first
+    x[2 * 9 + 1] = 1.2
+    got = po.violations(o, x, table, islanded=isl)
+    assert got[0][:2] == (1, 9) and [g[0] for g in got[1:]] == [3, 4, 4]

@@ -110,3 +110,23 @@ def test_a_filtered_record_leaves_the_earlier_one_under_its_key():
 def test_base_from_several_states_is_refused():
     with pytest.raises(ValueError):
         manage(rec(cur(1, 1, 1, 1.0)), NB, rec(cur(0, 1, 1, 1.0), cur(1, 2, 1, 1.0)))
+
+
+def angle(s, a, kind, value, threshold):
+    return (s, kind, a, 0, 0, -1, value, threshold)
+
+
+def test_voltage_angle_violations_have_a_key_per_limit_and_are_kept():
+    # limit 2's high and low share its key (the last fed kept); limit 2 and bus 2 are different keys; a base
+    # violation of the same angle limit filters nothing
+    post = [high(0, 2, 1.07), angle(0, 2, 3, 0.4, 0.35), angle(0, 2, 4, 0.4, 0.5), angle(0, 3, 3, 0.6, 0.5)]
+    assert kept(post) == [(1, 2, 0, 0, 1.07), (4, 2, 0, 0, 0.4), (3, 3, 0, 0, 0.6)]
+    base = [angle(0, 3, 3, 0.6, 0.5), high(0, 2, 1.07)]
+    assert kept(post, base) == [(4, 2, 0, 0, 0.4), (3, 3, 0, 0, 0.6)]
+    # a per-bus nominal voltage table indexes buses only, never an angle limit's index
+    assert len(kept([angle(1, 9, 3, 0.6, 0.5)], base, bus_nominal_kv=np.full(3, 400.0))) == 1
+
+
+def test_branch_element_beyond_the_table_is_refused():
+    with pytest.raises(ValueError):
+        manage(rec(cur(0, NB, 1, 1.0)), NB)
