@@ -153,3 +153,20 @@ def test_limit_table_contract_violations_return_codes(grid):
     with pytest.raises(N.OlfHipError) as e:
         N.LimitTable(st, t)
     assert e.value.code == N.E_ARGUMENT
+
+
+def test_voltage_angle_limit_contract(grid):
+    """Voltage-angle limits: a bus out of range fails with a code before any device use; arrays of unequal lengths
+    are refused by the Python owner."""
+    net = grid("grid118")
+    st = N.Structure.from_network(net)
+    good = gridgen.limit_table(net, net.system()[0], angle_limits=4)
+    for key, bad in (("angle_from", net.n_bus), ("angle_to", -1)):
+        t = {k: np.array(v, copy=True) for k, v in good.items()}
+        t[key][2] = bad
+        with pytest.raises(N.OlfHipError) as e:
+            N.LimitTable(st, t)
+        assert e.value.code == N.E_ARGUMENT, key
+    t = dict(good, angle_low=good["angle_low"][:3])
+    with pytest.raises(ValueError):
+        N.LimitTable(st, t)
