@@ -70,11 +70,12 @@ def main():
     ap.add_argument("--robust-sample", type=int, default=64,
                     help="contingencies of the robust-retry measurement on the grid with series-resonant paths (0: off)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-verify", action="store_true", help="skip the per-contingency comparison with the oracle")
     ap.add_argument("--lanes-per-tile", type=int, default=0)
     ap.add_argument("--threads-per-tile", type=int, default=0)
     ap.add_argument("--autotune", action="store_true", help="time every tile shape first and use the fastest")
     ap.add_argument("--no-warm-start", action="store_true", help="base-case start only (no learned warm start)")
-    ap.add_argument("--train-samples", type=int, default=40000)
+    ap.add_argument("--train-samples", type=int, default=20000)
     ap.add_argument("--train-epochs", type=int, default=150)
     ap.add_argument("--dump-schedule", default=None, help="write the solve order's costs and the timed solve's "
                     "per-contingency iterations to this .npz")
@@ -235,9 +236,16 @@ def main():
     # violation detection on the last step's states (reported beside the value, not inside it: the metric is the
     # Newton solve, as for the plain FP64 arm)
     viol_info = None
+    table = gridgen.limit_table(net, xb)
+    ltab = N.LimitTable(structure, table)
+    # OLF's post-contingency lists: its manager against the base case's violations, under the increased-violation
+    # thresholds of docs/increased-violations-thresholds.md:60-71 (buses taken at 400 kV: the workload gives no
+    # nominal voltages)
+    bv = base_solver.detect_violations(ltab)
+    recommended = IncreasedViolationsParameters(
+        flowProportionalThreshold=0.001, lowVoltageAbsoluteThreshold=0.5, lowVoltageProportionalThreshold=0.001,
+        highVoltageAbsoluteThreshold=0.5, highVoltageProportionalThreshold=0.001)
     if not args.no_violations:
-        table = gridgen.limit_table(net, xb)
-        ltab = N.LimitTable(structure, table)
         if mlp is not None:
             inputs.x0_per_scenario_device = mlp.predict_warm_start(xb, ctg, feats)
         ctx.solve(inputs, want_states=False)
@@ -247,24 +255,17 @@ def main():
         for _ in range(3):
             v = ltab.detect(ctx)
         dv = (time.perf_counter() - tv) / 3
-        # OLF's post-contingency lists: its manager against the base case's violations, under the increased-violation
-        # thresholds of docs/increased-violations-thresholds.md:60-71 (buses taken at 400 kV: the workload gives no
-        # nominal voltages) and under powsybl-core's defaults
-        bv = base_solver.detect_violations(ltab)
-        recommended = IncreasedViolationsParameters(
-            flowProportionalThreshold=0.001, lowVoltageAbsoluteThreshold=0.5, lowVoltageProportionalThreshold=0.001,
-            highVoltageAbsoluteThreshold=0.5, highVoltageProportionalThreshold=0.001)
         tm = time.perf_counter()
         kept = manage(v, ltab.n_branches, bv, recommended, 400.0)
         dm = time.perf_counter() - tm
         kept_default = manage(v, ltab.n_branches, bv)
         viol_info = {"violations": int(len(v)), "contingencies_with_violations": int(len(np.unique(v["scenario"]))),
-                     "by_kind": {N.VIOLATION_KINDS[k]: int((v["kind"] == k).sum()) for k in range(3)},
+                     "by_kind": {N.VIOLATION_KINDS[k]: int((v["kind"] == k).sum()) for k in N.VIOLATION_KINDS},
                      "ms_per_step": round(dv * 1e3, 2), "monitored_groups": int(len(table["group_side"])),
                      "screen_band": 1e-7,
                      "against_base_case": {"base_violations": int(len(bv)), "kept": int(len(kept)),
                                            "by_kind": {N.VIOLATION_KINDS[k]: int((kept["kind"] == k).sum())
-                                                       for k in range(3)},
+                                                       for k in N.VIOLATION_KINDS},
                                            "host_ms": round(dm * 1e3, 2),
                                            "thresholds": "flow 0.1 %, voltage min(0.5 kV, 0.1 %), buses at 400 kV",
                                            "kept_with_core_defaults": int(len(kept_default))}}
@@ -292,9 +293,29 @@ def main():
         roofline["traffic"] = traffic["bytes_per_launch"]
         roofline["traffic_source"] = traffic["source"]
 
+    # the product's answer for every contingency, outside the timed region: the same batch through
+    # solve_batch_n1 (same order, tiles, warm starts and fallbacks as the timed steps), states and post-contingency
+    # violation lists to the host, for the per-contingency comparison with the oracle
+    verify = None
+    if not args.no_verify:
+        cost_kw = {}
+        if data is not None and args.order == "cost":
+            cost_kw = dict(cost=W.iteration_costs(data, net.n_branch)[ctg_given],
+                           cost_tie=outage_flow_magnitude(net, xb, ctg_given))
+        feats_given = None if mlp is None else W.endpoint_features(net, xb, ctg_given, 1.0, gridgen.bridges(net))
+        ver = sa.solve_batch_n1(xb, tb, mb, lt, ctg_given, states_to_host=True, warm_start=mlp,
+                                warm_start_features=feats_given, limit_table=ltab, base_violations=bv,
+                                increased_violations=recommended, **cost_kw)
+        verify = {"result": ver, "table": table, "recommended": recommended,
+                  "x0": None if mlp is None else mlp.warm_starts_host(xb, ctg_given, feats_given),
+                  "matches_timed_step": bool(np.array_equal(ver.status, status_given)
+                                             and np.array_equal(ver.total_iterations[order],
+                                                                last["total_iterations"]))}
+
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(net, xb, mb, tb, lt, ctg_given[: args.cpu_sample], ctg_given[: args.cpu_sample_1t])
+        cpu = cpu_baseline(net, xb, mb, tb, lt, ctg_given[: args.cpu_sample], ctg_given[: args.cpu_sample_1t],
+                           ctg_given, verify)
 
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "contingencies/s", "n_gpus": 1, "steps": args.steps,
@@ -318,6 +339,7 @@ def main():
         "robust_retry": robust_info,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "oracle_error": None if cpu is None else cpu.pop("oracle_error", None),
         "setup_s": round(setup_s, 2),
     }
     print(json.dumps(out), flush=True)
@@ -372,9 +394,10 @@ def ctx_lanes(ctx):
             ctx._tuning.threads_per_tile or "default"}
 
 
-def cpu_baseline(net, xb, mb, tb, lt, sample, sample_1t):
+def cpu_baseline(net, xb, mb, tb, lt, sample, sample_1t, ctg_all, verify):
     """The FP64 CPU oracle (own sparse LU, OpenMP over contingencies) on a bounded sample, all host cores, and on a
-    smaller sample with one thread."""
+    smaller sample with one thread. With the device's verification solve (verify), the oracle is also the checker of
+    every contingency's answer (oracle_error)."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import pyoracle as po
 
@@ -385,11 +408,54 @@ def cpu_baseline(net, xb, mb, tb, lt, sample, sample_1t):
     t = time.perf_counter()
     osys.n1(xb, mb, tb, lt, sample_1t, threads=1)
     d1 = time.perf_counter() - t
-    return {"value": round(len(sample) / dt, 2), "unit": "contingencies/s", "cores": int(r["threads"]),
-            "cpu_model": host_cpu_model(), "kind": "port", "sample": f"first {len(sample)} of the 2000 contingencies, same base state and rules, "
-                                       f"{dt:.1f} s wall on {int(r['threads'])} threads",
-            "one_thread_value": round(len(sample_1t) / d1, 2),
-            "one_thread_sample": f"first {len(sample_1t)} contingencies, {d1:.1f} s on 1 thread"}
+    out = {"value": round(len(sample) / dt, 2), "unit": "contingencies/s", "cores": int(r["threads"]),
+           "cpu_model": host_cpu_model(), "kind": "port", "label": "CPU restatement of OLF NR (not OLF)",
+           "sample": f"first {len(sample)} of the 2000 contingencies, same base state and rules, "
+                     f"{dt:.1f} s wall on {int(r['threads'])} threads",
+           "one_thread_value": round(len(sample_1t) / d1, 2),
+           "one_thread_sample": f"first {len(sample_1t)} contingencies, {d1:.1f} s on 1 thread"}
+    if verify is not None:
+        out["oracle_error"] = oracle_error(po, osys, net, xb, mb, tb, lt, ctg_all,
+                                           r if len(sample) == len(ctg_all) else None, verify)
+    return out
+
+
+def oracle_error(po, osys, net, xb, mb, tb, lt, ctg, ref_base, verify):
+    """Every contingency's device answer against the FP64 oracle: (1) the oracle from the same start (the warm start
+    the device used, or the base state where the answer is a base-start solve): statuses, |V| and angle;
+    (2) the oracle from the base-case start, the reference semantics: statuses, states, and the post-contingency
+    violation lists (the oracle's own scan of its own states through OLF's manager against the oracle's scan of the
+    base state), entry for entry."""
+    import parity
+    from olfgpu.violations import manage
+
+    ver = verify["result"]
+    k = len(ctg)
+    if ref_base is None:
+        ref_base = osys.n1(xb, mb, tb, lt, ctg)
+    out = {"contingencies": k, "verification_solve_matches_timed_step": verify["matches_timed_step"]}
+    if verify["x0"] is not None:
+        t = time.perf_counter()
+        ref_same = osys.n1(xb, mb, tb, lt, ctg, x0=verify["x0"])
+        warm = np.asarray(ver.warm_started)
+        same = parity.compare_states(ver.x, ver.status, ref_same, np.nonzero(warm)[0])
+        same["oracle_s"] = round(time.perf_counter() - t, 2)
+        out["same_start"] = same
+        out["same_start"]["warm_started_answers"] = int(warm.sum())
+        base_rows = np.nonzero(~warm)[0]
+        if base_rows.size:
+            out["base_start_fallbacks"] = parity.compare_states(ver.x, ver.status, ref_base, base_rows)
+    out["vs_base_start_oracle"] = parity.compare_states(ver.x, ver.status, ref_base)
+    table = dict(verify["table"])
+    obase = manage(parity.scan_records(osys, xb, table, [], 0, N.VIOLATION_DTYPE), len(table["branch"]),
+                   bus_nominal_kv=400.0)
+    scans = parity.n1_scans(osys, ref_base["x"], ref_base["status"], ctg, table, N.VIOLATION_DTYPE)
+    want = manage(scans, len(table["branch"]), obase, verify["recommended"], 400.0)
+    viol = parity.compare_violations(ver.violations, want, k)
+    viol["oracle_scan_entries"] = int(len(scans))
+    viol["oracle_scan_entries_within_1e-6_of_limit"] = parity.near_limit(scans, 1e-6)
+    out["violation_lists_vs_base_start_oracle"] = viol
+    return out
 
 
 def host_cpu_model():

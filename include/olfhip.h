@@ -273,7 +273,8 @@ int olfhip_structure_pattern(const olfhip_structure *structure, int *nnz, int *r
  * contribution begin, end, node or -1), 1 item buckets (first, end, thread split, diagonal flag), 2 bucket range of
  * each level, 3 backward records (diagonal slot, S begin, S end, node), 4 backward buckets, 5 backward bucket range of
  * each level, 6 item range of each level, 7 node range of each level, 8 diagonal item of each node, 9 contribution
- * records (4 ints), 10 nodes in level order. values NULL: *count only. */
+ * records (4 ints), 10 nodes in level order, 11 the block-column structure S(k) of every node (the members of node
+ * k's S set at [S begin, S end) of its backward record). values NULL: *count only. */
 int olfhip_structure_schedule(const olfhip_structure *structure, int which, int *count, int *values);
 
 /* Picks lanes_per_tile and threads_per_tile for this workload by timing full solves of `inputs` on the device for

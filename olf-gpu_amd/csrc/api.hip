@@ -988,13 +988,10 @@ static void detectViolations(ContextImpl *c, LimitTableImpl *t, double band, int
                 branchFlows(P, e, xv[a], xv[a + 1], f);
                 const int side = t->groupSide[g], qty = t->groupQty[g];
                 const double p = side == 1 ? f[0] : f[3], qq = side == 1 ? f[1] : f[4], v = side == 1 ? f[2] : f[5];
-                double value;
-                if (qty == OLFHIP_QUANTITY_CURRENT) value = std::hypot(p, qq) / v;
-                else if (qty == OLFHIP_QUANTITY_ACTIVE_POWER) value = p;
-                else value = std::sqrt(p * p + qq * qq);
-                if (qty == OLFHIP_QUANTITY_APPARENT_POWER && std::isnan(value)) continue;
-                const double compared = qty == OLFHIP_QUANTITY_ACTIVE_POWER ? std::fabs(value) : value;
-                if (!(compared > t->threshold[r])) continue;
+                // the compared value is the screen's (comparedQuantity); active power reports its signed value
+                const double compared = comparedQuantity(qty, p, qq, v);
+                if (!(compared > t->threshold[r])) continue;   // NaN never fires
+                const double value = qty == OLFHIP_QUANTITY_ACTIVE_POWER ? p : compared;
                 out.push_back(olfhip_violation{k, OLFHIP_VIOLATION_BRANCH, t->groupBranch[g], side, qty, r, value,
                                                t->threshold[r]});
                 reportedGroup = g;
@@ -1457,9 +1454,10 @@ int olfhip_structure_schedule(const olfhip_structure *structure, int which, int 
         const Plan &P = reinterpret_cast<const StructureImpl *>(structure)->plan;
         const std::vector<int> *arrays[] = {&P.itemRec,     &P.bucketRec,    &P.levelBucketPtr, &P.bwdRec,
                                             &P.bwdBucketRec, &P.levelBwdBucketPtr, &P.levelItemPtr,
-                                            &P.levelNodePtr, &P.diagItemOf,   &P.contribRec,   &P.levelNodes};
+                                            &P.levelNodePtr, &P.diagItemOf,   &P.contribRec,   &P.levelNodes,
+                                            &P.sIdx};
         if (which < 0 || which >= (int)(sizeof(arrays) / sizeof(arrays[0])))
-            throw ArgumentError("which selects one of the 11 schedule arrays");
+            throw ArgumentError("which selects one of the 12 schedule arrays");
         const std::vector<int> &v = *arrays[which];
         *count = (int)v.size();
         if (values && !v.empty()) std::memcpy(values, v.data(), v.size() * 4);

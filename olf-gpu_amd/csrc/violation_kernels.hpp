@@ -18,6 +18,15 @@ namespace olfhip {
 
 constexpr int QTY_I = 0, QTY_P = 1, QTY_S = 2;
 
+// What a limit group compares with its thresholds, from one side's flow p + jq at voltage v (per unit of the state):
+// the current |S| / v, the active power's magnitude |p|, or the apparent power |S|. One definition for the device
+// screen and the host's exact decision, so the two cannot drift apart.
+__host__ __device__ inline double comparedQuantity(int quantity, double p, double q, double v) {
+    if (quantity == QTY_P) return fabs(p);
+    const double apparent = sqrt(p * p + q * q);
+    return quantity == QTY_I ? apparent / v : apparent;
+}
+
 struct DevLimits {
     int nBranches, nGroups, nLimits, nBuses;
     const int *branch;        // table branch -> closed branch
@@ -73,21 +82,18 @@ __global__ void screenBranchesKernel(DevPlan P, DevState S, DevLimits L, const i
         const double p1 = c[0] * v1 * v1 - bvv * s1, q1 = c[2] * v1 * v1 - bvv * c1;
         const double p2 = c[4] * v2 * v2 - bvv * s2, q2 = c[5] * v2 * v2 - bvv * c2;
         for (int g = g0; g < g1; g++) {
-            const bool one = L.groupSide[g] == 1;
-            const double p = one ? p1 : p2, q = one ? q1 : q2, v = one ? v1 : v2;
-            double value;
-            switch (L.groupQty[g]) {
-                case QTY_I: value = hypot(p, q) / v; break;
-                case QTY_P: value = fabs(p); break;
-                default: value = sqrt(p * p + q * q); break;
-            }
-            if (isnan(value)) continue;
-            for (int l = L.limitStart[g]; l < L.limitStart[g + 1]; l++) {
-                const double th = L.threshold[l];
-                if (value > th - fabs(th) * band) {
-                    emitRecord(records, counts, s, capacity, l);
-                    if (value > th + fabs(th) * band) break;   // certain to confirm: the group reports this one
-                }
+            const double value = L.groupSide[g] == 1 ? comparedQuantity(L.groupQty[g], p1, q1, v1)
+                                                     : comparedQuantity(L.groupQty[g], p2, q2, v2);
+            // thresholds run most severe first and the host reports the first one exceeded: hand it every limit the
+            // value may exceed (threshold lowered by the band); past one it exceeds for certain (threshold raised by
+            // the band) the rest of the group cannot be reported. A NaN value fails every comparison.
+            const int lEnd = L.limitStart[g + 1];
+            bool settled = false;
+            for (int l = L.limitStart[g]; l < lEnd && !settled; l++) {
+                const double margin = fabs(L.threshold[l]) * band;
+                if (!(value > L.threshold[l] - margin)) continue;
+                emitRecord(records, counts, s, capacity, l);
+                settled = value > L.threshold[l] + margin;
             }
         }
     }

@@ -146,6 +146,17 @@ def ptr(a):
     return None if a is None else a.ctypes.data
 
 
+def device_to_host(device_ptr: int, shape, dtype=np.float64) -> np.ndarray:
+    """A synchronous copy of device memory (e.g. olfhip_mlp_predict_warm_start's starts) into a new host array."""
+    out = np.zeros(shape, dtype)
+    hip = C.CDLL("libamdhip64.so")
+    rc = hip.hipMemcpy(C.c_void_p(out.ctypes.data), C.c_void_p(device_ptr), C.c_size_t(out.nbytes),
+                       C.c_int(2))   # hipMemcpyDeviceToHost
+    if rc != 0:
+        raise OlfHipError(E_DEVICE, f"hipMemcpy device to host failed: {rc}")
+    return out
+
+
 def device_count() -> int:
     return lib().olfhip_device_count()
 
@@ -201,7 +212,7 @@ class Structure:
         return rp, ci
 
     SCHEDULE = ("item", "bucket", "level_bucket_ptr", "bwd", "bwd_bucket", "level_bwd_bucket_ptr", "level_item_ptr",
-                "level_node_ptr", "diag_item_of", "contrib", "level_nodes")
+                "level_node_ptr", "diag_item_of", "contrib", "level_nodes", "s_members")
 
     def schedule(self):
         """The LU level schedule (olfhip_structure_schedule), records reshaped to [.., 4]."""
@@ -365,6 +376,7 @@ class BatchContext:
         nm = np.zeros(k)
         out = SolveOutputs(ptr(res), ptr(st), ptr(it), ptr(tot), ptr(nm))
         check(lib().olfhip_solve_batch_context(self.h, C.byref(inputs), C.byref(out)))
+        self.last_n_scenarios = k
         return dict(x=None if res is None else res[:, :n], results=res, status=st, iterations=it,
                     total_iterations=tot, norm=nm)
 
@@ -434,8 +446,11 @@ class LimitTable:
             pass
 
     def screen(self, context: "BatchContext", k: int, band=1e-7, capacity=1024):
-        """The device screen's candidates for the last solve of `context` (k scenarios): (counts [k],
-        records [k][capacity])."""
+        """The device screen's candidates for the last solve of `context` (k scenarios, its solve's count): (counts
+        [k], records [k][capacity])."""
+        if k != getattr(context, "last_n_scenarios", None):
+            raise ValueError(f"k = {k}: the context's last solve had {getattr(context, 'last_n_scenarios', None)} "
+                             "scenarios (the screen fills one row per scenario of that solve)")
         counts = np.zeros(k, np.int32)
         records = np.zeros((k, capacity), np.int32)
         check(lib().olfhip_screen_violations(context.h, self.h, band, capacity, ptr(counts), ptr(records)))

@@ -300,6 +300,7 @@ class BatchScenarioResult:
     robust_attempted: int = 0              # scenarios the robust retry pass solved again
     robust_recovered: int = 0              # of which it kept the answer (CONVERGED or MAX_ITERATION_REACHED)
     robust_errors: list = field(default_factory=list)   # failures of the retry pass (the first pass's answers kept)
+    warm_started: np.ndarray | None = None  # [K] bool: the answer is the warm-started solve's (not a base-start redo)
 
 
 class GpuBatchedSecurityAnalysisSolver:
@@ -380,7 +381,7 @@ class GpuBatchedSecurityAnalysisSolver:
                        x0_per_scenario=None, states_to_host=True, warm_start=None,
                        warm_start_features=None, limit_table: N.LimitTable | None = None,
                        violation_band=1e-7, cost=None, cost_tie=None, base_violations=None,
-                       increased_violations=None) -> BatchScenarioResult:
+                       increased_violations=None, keep_voltage_angle_violations=False) -> BatchScenarioResult:
         """solveBatchN1: contingencies is [K] or [K, m] closed-branch ids (-1 padded). With a warm_start network
         (olfgpu.warmstart.WarmStartMLP, single outages) every scenario starts from its prediction, and any that does
         not converge is solved again from the base-case start. With a limit_table the violations of every converged
@@ -391,7 +392,8 @@ class GpuBatchedSecurityAnalysisSolver:
         reference's post-contingency detection, GpuBaseViolations.java:27-41). With a cost per
         contingency (e.g. warmstart.iteration_costs of its branch) the batch is solved longest first
         (longest_first, ties broken by cost_tie when given); the results are in the contingencies' order either
-        way."""
+        way. Tables with voltage-angle limits need keep_voltage_angle_violations to be filtered against the base case
+        (violations.manage)."""
         import time
 
         if cost is not None:
@@ -404,10 +406,13 @@ class GpuBatchedSecurityAnalysisSolver:
             r = self.solve_batch_n1(x_base, target_base, row_mode_base, limit_types_base,
                                     np.asarray(contingencies)[order], part(x0_per_scenario), states_to_host,
                                     warm_start, part(warm_start_features), limit_table, violation_band,
-                                    base_violations=base_violations, increased_violations=increased_violations)
+                                    base_violations=base_violations, increased_violations=increased_violations,
+                                    keep_voltage_angle_violations=keep_voltage_angle_violations)
             r.status, r.iterations, r.total_iterations, r.norm = (r.status[inv], r.iterations[inv],
                                                                   r.total_iterations[inv], r.norm[inv])
             r.islanded_count = r.islanded_count[inv]
+            if r.warm_started is not None:
+                r.warm_started = r.warm_started[inv]
             if r.x is not None:
                 r.x = r.x[inv]
             if r.violations is not None:
@@ -436,6 +441,7 @@ class GpuBatchedSecurityAnalysisSolver:
         robust_attempted = 0
         robust_recovered = 0
         robust_errors = []
+        from_warm = np.zeros(k, bool)
 
         def solve_part(part, x0_host=None, x0_dev=None, robust=False):
             nonlocal device_s, chunks, seed
@@ -487,8 +493,10 @@ class GpuBatchedSecurityAnalysisSolver:
                 xs[c0:c1] = r["x"]
             detect(np.arange(c0, c1))
             if warm_start is not None:
+                from_warm[c0:c1] = True
                 # a warm start that did not converge: solve again from the base-case start, whose answer stands
                 redo = np.nonzero(r["status"] != 0)[0]
+                from_warm[c0 + redo] = False
                 if redo.size:
                     fallback += redo.size
                     r2, _ = solve_part(part[redo])
@@ -513,6 +521,7 @@ class GpuBatchedSecurityAnalysisSolver:
                         continue
                     keep = np.isin(r3["status"], (N.CONVERGED, N.MAX_ITERATION_REACHED))
                     rows = c0 + redo[keep]
+                    from_warm[rows] = False
                     robust_recovered += rows.size
                     detect(c0 + redo, self._robust_ctx)   # converged lanes only, all of them kept
                     st[rows] = r3["status"][keep]
@@ -528,6 +537,7 @@ class GpuBatchedSecurityAnalysisSolver:
             if base_violations is not None:
                 from .violations import manage
                 violations = manage(violations, limit_table.n_branches, base_violations, increased_violations,
-                                    limit_table.bus_nominal_kv)
+                                    limit_table.bus_nominal_kv, keep_voltage_angle_violations)
         return BatchScenarioResult(st, it, tot, nm, xs, islanded, chunks, device_s, fallback, violations,
-                                   robust_attempted, robust_recovered, robust_errors)
+                                   robust_attempted, robust_recovered, robust_errors,
+                                   from_warm if warm_start is not None else None)

@@ -99,14 +99,24 @@ def weakened_or_equivalent(ref: np.ndarray, rep: np.ndarray, params: IncreasedVi
 
 
 def manage(violations: np.ndarray, n_table_branches: int, base: np.ndarray | None = None,
-           params: IncreasedViolationsParameters | None = None, bus_nominal_kv=None) -> np.ndarray:
+           params: IncreasedViolationsParameters | None = None, bus_nominal_kv=None,
+           keep_voltage_angle_violations: bool = False) -> np.ndarray:
     """OLF's LimitViolationManager fed `violations` (N.VIOLATION_DTYPE in the scan's feed order, any number of
     scenarios): per scenario, the records it keeps, in its map's order (a key's first place, its last kept record).
 
     base: the base case's violations (one scenario, the scan's records of the base state), or None for the base case
     itself (no filtering); params: the increased-violations thresholds (default: powsybl-core's); bus_nominal_kv:
-    per-bus nominal voltage in kV (scalar or [n_buses]; None: 1, the voltage thresholds then in per unit)."""
+    per-bus nominal voltage in kV (scalar or [n_buses]; None: 1, the voltage thresholds then in per unit).
+
+    OLF's comparison of a voltage-angle violation with the base case's is not restated here (the reference leaves
+    it to OLF's own manager), so filtering against a base case refuses voltage-angle records (ValueError) unless
+    keep_voltage_angle_violations is set: they are then all kept, which can list more of them than OLF."""
     v = np.asarray(violations, N.VIOLATION_DTYPE)
+    if base is not None and not keep_voltage_angle_violations:
+        b = np.asarray(base, N.VIOLATION_DTYPE)
+        if (v["kind"] >= KIND_HIGH_ANGLE).any() or (b["kind"] >= KIND_HIGH_ANGLE).any():
+            raise ValueError("filtering voltage-angle violations against the base case is not modelled; pass "
+                             "keep_voltage_angle_violations=True to keep them all")
     params = params or IncreasedViolationsParameters()
     kv = None if bus_nominal_kv is None else np.asarray(bus_nominal_kv, np.float64)
 

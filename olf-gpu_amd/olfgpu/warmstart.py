@@ -199,6 +199,11 @@ class WarmStartMLP:
             self.set_base_states(x)
         return self.predict_raw(branch, feat, flow_scale(feat))
 
+    def warm_starts_host(self, x_base, branch, feat) -> np.ndarray:
+        """The start states predict_warm_start hands the solver, copied to the host ([B][n])."""
+        ptr = self.predict_warm_start(x_base, branch, feat)
+        return N.device_to_host(ptr, (len(np.atleast_1d(branch)), 2 * self.n_bus))
+
 
 def endpoint_features(net, x_base, branches, load_scale, bridge_mask=None):
     """16 features of each outaged branch at the base state: endpoint V and phi, their difference, the base-case

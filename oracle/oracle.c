@@ -935,7 +935,7 @@ int oracle_n1(const oracle_sys *s, const oracle_nr_params *prm, const oracle_rl 
               int slack_bus, const double *x_base, const int *mode_base, const double *target_base,
               const signed char *limit_type_base, int n_ctg, int max_out, const int *ctg,
               int n_threads, double *x_out, int *status, int *iterations, int *total_iterations, double *norm,
-              double *target_out)
+              double *target_out, const double *x0_per)
 {
     const int n = s->n;
     const int nrl = rl ? rl->n : 0;
@@ -963,7 +963,7 @@ int oracle_n1(const oracle_sys *s, const oracle_nr_params *prm, const oracle_rl 
             for (int k = 0; k < max_out; k++)
                 if (ctg[(size_t)c * max_out + k] >= 0) outs[n_out++] = ctg[(size_t)c * max_out + k];
             double *x = x_out + (size_t)c * n;
-            memcpy(x, x_base, sizeof(double) * (size_t)n);
+            memcpy(x, x0_per ? x0_per + (size_t)c * n : x_base, sizeof(double) * (size_t)n);
             memcpy(mode, mode_base, sizeof(int) * (size_t)n);
             memcpy(target, target_base, sizeof(double) * (size_t)n);
             if (nrl) {

@@ -130,7 +130,8 @@ double oracle_bus_p(const oracle_sys *s, const double *x, const int *out_br, int
  * tests/test_oracle_nr.py (bridge outages vs the generator's bridge list). */
 int oracle_islands(const oracle_sys *s, int slack_bus, const int *out_br, int n_out, unsigned char *islanded);
 
-/* Batched N-1 on the CPU: every contingency starts from the base state/modes/targets/limit types,
+/* Batched N-1 on the CPU: every contingency starts from the base state (or from its own row of x0_per [n_ctg][n] when
+ * x0_per is not NULL, the per-scenario start of BatchSolveNative.java:119) and the base modes/targets/limit types,
  * islanded buses are frozen at 1<0 (identity rows), OpenMP over contingencies with n_threads threads.
  * ctg is n_ctg x max_out branch ids, -1 padded. Outputs per contingency (target_out, may be NULL: the final
  * targets, n per contingency). A case whose distribution failed under ds->reject_failure ends REJECTED.
@@ -139,7 +140,7 @@ int oracle_n1(const oracle_sys *s, const oracle_nr_params *prm, const oracle_rl 
               int slack_bus, const double *x_base, const int *mode_base, const double *target_base,
               const signed char *limit_type_base, int n_ctg, int max_out, const int *ctg,
               int n_threads, double *x_out, int *status, int *iterations, int *total_iterations,
-              double *norm, double *target_out);
+              double *norm, double *target_out, const double *x0_per);
 
 /* Limit table (the reference's GpuLimitTable, flattened): per monitored closed branch its groups (side 1|2,
  * quantity 0 current |S|/V, 1 active power, 2 apparent power) with thresholds most severe first; per bus a high

@@ -94,7 +94,8 @@ def lib():
                                         C.POINTER(Violation), C.c_int]
         L.oracle_n1.restype = C.c_int
         L.oracle_n1.argtypes = [C.c_void_p, C.POINTER(NrParams), C.POINTER(Rl), C.c_void_p, C.c_int, _dp, _ip, _dp,
-                                _bp, C.c_int, C.c_int, _ip, C.c_int, _dp, _ip, _ip, _ip, _dp, C.c_void_p]
+                                _bp, C.c_int, C.c_int, _ip, C.c_int, _dp, _ip, _ip, _ip, _dp, C.c_void_p,
+                                C.c_void_p]
         _LIB = L
     return _LIB
 
@@ -181,9 +182,10 @@ class OracleSystem:
         return lib().oracle_bus_p(self.h, np.ascontiguousarray(x, np.float64), out, len(out), int(bus))
 
     def solve(self, x0, mode, target, limit_type=None, prm: NrParams | None = None, outages=(), islanded=None,
-              ds=None, ds_tol=0.0, ds_reject=False):
+              ds=None, ds_tol=0.0, ds_reject=False, switch_counts=False):
         """One load flow (distributed slack when ds is a gridgen.distributed_slack dict); returns
-        (x, mode, target, limit_type, Result)."""
+        (x, mode, target, limit_type, Result), and the reactive-limit entries' PV -> PQ switch counts as a sixth
+        item when switch_counts is set."""
         prm = prm or params()
         x = np.ascontiguousarray(x0, np.float64).copy()
         mode = np.ascontiguousarray(mode, np.int32).copy()
@@ -198,6 +200,8 @@ class OracleSystem:
         lib().oracle_nr_solve(self.h, C.byref(prm), C.byref(self.rl), C.byref(dst) if dst is not None else None,
                               out, len(out), isl.ctypes.data if isl is not None else None, x, mode, target, lt, sc,
                               C.byref(res))
+        if switch_counts:
+            return x, mode, target, lt[:nrl], res, sc[:nrl].copy()
         return x, mode, target, lt[:nrl], res
 
     def islands(self, outages, slack=0):
@@ -207,10 +211,10 @@ class OracleSystem:
         return isl.astype(bool)
 
     def n1(self, x_base, mode_base, target_base, limit_type_base, contingencies, prm: NrParams | None = None,
-           threads=0, slack=0, ds=None, ds_tol=0.0, ds_reject=False, want_targets=False):
+           threads=0, slack=0, ds=None, ds_tol=0.0, ds_reject=False, want_targets=False, x0=None):
         """Batched N-1 on the CPU. contingencies: [K] or [K, max_out] branch ids (-1 pad). With ds (a
         gridgen.distributed_slack dict) the distributed-slack loop runs first; want_targets returns each
-        contingency's final targets."""
+        contingency's final targets; x0 [K][n] starts each contingency from its own state instead of x_base."""
         prm = prm or params(batch_of_one=False)
         ctg = np.asarray(contingencies, dtype=np.int32)
         if ctg.ndim == 1:
@@ -227,10 +231,16 @@ class OracleSystem:
             lt = np.zeros(1, np.int8)
         tg = np.zeros((k, self.n)) if want_targets else None
         dst, dkeep = ds_struct(ds, ds_tol, ds_reject) if ds is not None else (None, None)
+        x0p = None
+        if x0 is not None:
+            x0p = np.ascontiguousarray(x0, np.float64)
+            if x0p.shape != (k, self.n):
+                raise ValueError(f"x0 must be [{k}][{self.n}]")
         used = lib().oracle_n1(self.h, C.byref(prm), C.byref(self.rl), C.byref(dst) if dst is not None else None,
                                slack, np.ascontiguousarray(x_base, np.float64),
                                np.ascontiguousarray(mode_base, np.int32), np.ascontiguousarray(target_base, np.float64),
-                               lt, k, mo, ctg, threads, x, st, it, tot, nm, tg.ctypes.data if tg is not None else None)
+                               lt, k, mo, ctg, threads, x, st, it, tot, nm, tg.ctypes.data if tg is not None else None,
+                               x0p.ctypes.data if x0p is not None else None)
         out = dict(x=x, status=st, iterations=it, total_iterations=tot, norm=nm, threads=used)
         if want_targets:
             out["target"] = tg
