@@ -214,7 +214,15 @@ typedef struct {
     /* 1: reactive_limits_seed_state is ONE row [n_entries] that every scenario starts from (the base case's types,
      * the usual N-1 seed), uploaded once instead of n_scenarios times */
     int reactive_limits_seed_shared;
+    /* > 0: the linear-solve guard's bound on ||J dx + F||inf / ||F||inf (checked at the 2nd iteration of a scenario's
+     * first solve, and on the solve before its budget); <= 0: the reference's LIN_SOLVE_FAIL_RATIO 1e-2
+     * (native/cuda/solver/iterative_refinement.cuh:23, newton_iteration.cu:612-652) */
+    double linear_solve_fail_ratio;
 } olfhip_solve_inputs;
+/* tol <= 0 with conv_eps_per_eq <= 0 and no per-row criterion: the fixed-iteration mode (the reference's tol <= 0,
+ * native/cuda/solver/outer_loop_driver.cu:165-175, newton_iteration.cu:141-151,336-343): every scenario runs exactly
+ * max_iter Newton steps with no convergence test, no linear-solve guard and no outer loop, and ends
+ * MAX_ITERATION_REACHED, or DIVERGED when its final mismatch is not finite. */
 
 /* Outputs (any may be NULL): results [n_scenarios][n+3] = (x, iterations of the last solve, ||F||inf, status) in
  * install order, the reference's layout (jni/batch_solve_jni.cu:308-337); status, iterations (last solve),
@@ -239,6 +247,13 @@ int olfhip_solve_batch_context(olfhip_context *context, const olfhip_solve_input
  * No reference counterpart: the reference's phase timers are print-only (OLF_GPU_PHASE_TIME,
  * native/cuda/jni/batch_solve_jni.cu:12-26). */
 int olfhip_context_timing(const olfhip_context *context, double *step_ms, int *steps);
+
+/* How the context's solves ran their graphs: a solve captures its install kernels and its Newton-step launch (with
+ * the running-tile count and its copy) as two graphs and brings the context's instantiated graphs up to them, in
+ * place when only kernel arguments changed (updates) or instantiating again when the sequence changed (the first
+ * solve, or another set of install kernels: instantiations). OLFHIP_NO_GRAPH=1 launches eagerly. No reference
+ * counterpart (the reference launches per level and per step). */
+int olfhip_context_graph_statistics(const olfhip_context *context, int *instantiations, int *updates);
 
 /* Row modes and targets ([n_scenarios][n], stable rows) and reactive-limit types ([n_scenarios][n_entries], may be
  * NULL) the outer loop left in each scenario after the last solve: what the reference writes back into OLF's buses
@@ -334,6 +349,15 @@ void olfhip_destroy_limit_table(olfhip_limit_table *table);
  * count above capacity means the scenario's candidates overflowed. band: relative screen band (e.g. 1e-7). */
 int olfhip_screen_violations(olfhip_context *context, olfhip_limit_table *table, double band, int capacity,
                              int *counts, int *records);
+/* The scenarios of the context's last solve (n_scenarios must be its count) whose violation decisions a state
+ * difference of relative size `band` could change: flags[k] = 1 when scenario k converged and a monitored quantity lies
+ * within band x |threshold| of one of its group's thresholds (up to the first it exceeds beyond the band), or a bus
+ * voltage within band x |limit| of its high or low limit unless the bus holds its voltage (PV, slack, islanded). A
+ * warm-started answer is another Newton iterate inside the same stopping tolerance as the base-start answer (the
+ * reference's semantics): flagged scenarios are the ones to solve again from the base start for lists equal to the
+ * base start's. No reference counterpart (the reference has no warm start). */
+int olfhip_ambiguous_scenarios(olfhip_context *context, olfhip_limit_table *table, double band, int n_scenarios,
+                               int *flags);
 /* Screen plus the exact host decisions: the violations of every converged scenario, ordered by scenario, then
  * branches in table order (a group's first exceeded limit), then buses (high before low), then voltage-angle limits
  * (high before low). Fails with

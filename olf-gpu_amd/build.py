@@ -1,4 +1,6 @@
-"""Builds libolfhip.so (gfx950) in-tree with hipcc: `python olf-gpu_amd/build.py`."""
+"""Builds libolfhip.so (gfx950) in-tree with hipcc: `python olf-gpu_amd/build.py`; and, where <jni.h> is found
+($JAVA_HOME/include or /usr/lib/jvm/*/include), the JNI shim libolfgpu.so over it."""
+import glob
 import os
 import subprocess
 import sys
@@ -15,7 +17,16 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
          "-Wall", "-Wno-unused-function", "-Wno-unused-variable"]
 
 
+def check_generated():
+    """The generated term headers must be the java2hip rendering of the committed manifests (no hand edits, no
+    stale rendering)."""
+    gen = os.path.join(HERE, "codegen", "java2hip.py")
+    if subprocess.run([sys.executable, gen, "--check"]).returncode != 0:
+        raise RuntimeError("csrc/generated is out of date: run python olf-gpu_amd/codegen/java2hip.py")
+
+
 def build(verbose=True):
+    check_generated()
     cmd = [HIPCC] + FLAGS + [os.path.join(CSRC, s) for s in SOURCES] + ["-o", LIB + ".tmp"]
     if verbose:
         print(" ".join(cmd), flush=True)
@@ -26,7 +37,39 @@ def build(verbose=True):
     subprocess.run([HIPCC, "--offload-arch=gfx950", "-O3", "-Xclang", "-target-feature", "-Xclang",
                     "-packed-fp32-ops", os.path.join(CSRC, "hbm_copy.hip"), "-o", probe + ".tmp"], check=True)
     os.replace(probe + ".tmp", probe)
+    build_jni(verbose)
     return LIB
+
+
+def jni_include():
+    """The directory holding jni.h, or None."""
+    cands = []
+    if os.environ.get("JAVA_HOME"):
+        cands.append(os.path.join(os.environ["JAVA_HOME"], "include"))
+    cands += sorted(glob.glob("/usr/lib/jvm/*/include"))
+    for c in cands:
+        if os.path.exists(os.path.join(c, "jni.h")):
+            return c
+    return None
+
+
+def build_jni(verbose=True):
+    """libolfgpu.so: the JNI binding of StructureNative / BatchSolveNative over libolfhip.so (jni/olfhip_jni.c),
+    found beside it through its rpath. Skipped (None) without jni.h."""
+    inc = jni_include()
+    if inc is None:
+        if verbose:
+            print("jni.h not found: the JNI shim libolfgpu.so is not built", flush=True)
+        return None
+    out = os.path.join(HERE, "libolfgpu.so")
+    cmd = ["gcc", "-std=c99", "-O2", "-fPIC", "-shared", "-Wall", "-I" + inc, "-I" + os.path.join(inc, "linux"),
+           "-I" + os.path.join(os.path.dirname(HERE), "include"), os.path.join(HERE, "jni", "olfhip_jni.c"),
+           "-L" + HERE, "-lolfhip", "-Wl,-rpath,$ORIGIN", "-o", out + ".tmp"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(out + ".tmp", out)
+    return out
 
 
 if __name__ == "__main__":

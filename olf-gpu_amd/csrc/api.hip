@@ -316,6 +316,11 @@ struct ContextImpl {
     std::vector<int> lastStatus;             // statuses of the last solve's scenarios
     DBuf kryV, kryH;                         // robust mode: the GMRES basis and per-tile dense state
     std::vector<int> lastOutages;            // its outage lists, MAX_OUTAGES per lane (-1 padded)
+    // A solve as two graphs, each instantiated once and updated in place by later solves (their kernel arguments
+    // change, their sequence does not): the install kernels, and the Newton-step launch with the running-tile count
+    // and its copy to the host (replayed when a launch is capped at k steps, OLFHIP_STEPS_PER_LAUNCH).
+    hipGraphExec_t installGraph = nullptr, stepGraph = nullptr;
+    int graphInstantiations = 0, graphUpdates = 0;
 
     DevState devState() const {
         DevState S{};
@@ -375,6 +380,8 @@ struct ContextImpl {
                      slackNode};
     }
     ~ContextImpl() {
+        if (installGraph) (void)hipGraphExecDestroy(installGraph);
+        if (stepGraph) (void)hipGraphExecDestroy(stepGraph);
         if (pinnedCount) (void)hipHostFree(pinnedCount);
         if (evStart) (void)hipEventDestroy(evStart);
         if (evStop) (void)hipEventDestroy(evStop);
@@ -563,6 +570,11 @@ static void solve(ContextImpl *c, const olfhip_solve_inputs *in, olfhip_solve_ou
     prm.kryDim = OLFHIP_ROBUST_KRYLOV_DIM;
     prm.kryRestarts = OLFHIP_ROBUST_RESTARTS;
     prm.kryTol = OLFHIP_ROBUST_TOL;
+    if (std::isnan(in->linear_solve_fail_ratio)) throw ArgumentError("linear_solve_fail_ratio is NaN");
+    prm.linFailRatio = in->linear_solve_fail_ratio > 0.0 ? in->linear_solve_fail_ratio : LIN_SOLVE_FAIL_RATIO;
+    // tol <= 0 with no other criterion: exactly max_iter steps per scenario (the reference's fixed-iteration mode,
+    // native/cuda/solver/outer_loop_driver.cu:165-175)
+    prm.fixedIter = !(in->tol > 0.0) && !(in->conv_eps_per_eq > 0.0) && !c->rowCriteria;
     if (in->distributed_slack && c->slackNode < 0)
         throw ArgumentError("distributed_slack is set but the context has no distributed slack (olfhip_set_distributed_slack)");
     if (in->distributed_slack && !(in->slack_dist_tol >= 0.0)) throw ArgumentError("slack_dist_tol must be >= 0");
@@ -609,18 +621,17 @@ static void solve(ContextImpl *c, const olfhip_solve_inputs *in, olfhip_solve_ou
     withT(c->T, [&](auto TC) {
         constexpr int T = decltype(TC)::value;
         const size_t work = (size_t)c->nTiles * P.nb * T;
-        installKernel<T><<<gridFor(work, 256), 256, 0, s>>>(dp, S, c->x0Base.as<double>(),
-                                                            x0PerDev,
-                                                            c->targetBase.as<double>(), c->modeBase.as<int>());
-        if (in->n_row_mode_entries + in->n_target_overrides > 0)
-            overrideKernel<T><<<gridFor(in->n_row_mode_entries + in->n_target_overrides, 256), 256, 0, s>>>(
-                dp, S, in->n_row_mode_entries, c->modeEntries.as<int>(), in->n_target_overrides,
-                c->targetIdx.as<long long>(), c->targetVal.as<double>());
-        laneInitKernel<T><<<gridFor(lanes * (1 + (size_t)c->nRL), 256), 256, 0, s>>>(
-            dp, S, R, (in->reactive_limits_seed_state && c->nRL > 0) ? c->seed.as<signed char>() : nullptr,
-            seedRows == 1 ? 0 : c->nRL,
-            c->modeBase.as<int>());
-        HIP_OK(hipGetLastError());
+        auto enqueueInstall = [&]() {
+            installKernel<T><<<gridFor(work, 256), 256, 0, s>>>(dp, S, c->x0Base.as<double>(), x0PerDev,
+                                                                c->targetBase.as<double>(), c->modeBase.as<int>());
+            if (in->n_row_mode_entries + in->n_target_overrides > 0)
+                overrideKernel<T><<<gridFor(in->n_row_mode_entries + in->n_target_overrides, 256), 256, 0, s>>>(
+                    dp, S, in->n_row_mode_entries, c->modeEntries.as<int>(), in->n_target_overrides,
+                    c->targetIdx.as<long long>(), c->targetVal.as<double>());
+            laneInitKernel<T><<<gridFor(lanes * (1 + (size_t)c->nRL), 256), 256, 0, s>>>(
+                dp, S, R, (in->reactive_limits_seed_state && c->nRL > 0) ? c->seed.as<signed char>() : nullptr,
+                seedRows == 1 ? 0 : c->nRL, c->modeBase.as<int>());
+        };
         // the per-row criterion runs in the kernels' ROWCRIT variants (the same code plus its test)
         auto launchStep = [&](auto rowCrit) {
             constexpr bool RC = decltype(rowCrit)::value;
@@ -640,19 +651,55 @@ static void solve(ContextImpl *c, const olfhip_solve_inputs *in, olfhip_solve_ou
             countRunningKernel<<<1, 256, 0, s>>>(S.tileRunning, c->nTiles, c->count.as<int>());
             HIP_OK(hipMemcpyAsync(c->pinnedCount, c->count.p, sizeof(int), hipMemcpyDeviceToHost, s));
         };
-        hipGraphExec_t exec = nullptr;
-        hipGraph_t graph = nullptr;
-        if (useGraph) {
+        // Captures `enqueue` and brings `exec` to it: updated in place when its sequence is the cached graph's
+        // (only kernel arguments changed), instantiated again otherwise.
+        auto graphOf = [&](hipGraphExec_t &exec, auto &&enqueue) {
+            hipGraph_t graph = nullptr;
             HIP_OK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            enqueueStep();
+            try {
+                enqueue();
+            } catch (...) {
+                if (hipStreamEndCapture(s, &graph) == hipSuccess && graph) (void)hipGraphDestroy(graph);
+                throw;
+            }
             HIP_OK(hipStreamEndCapture(s, &graph));
-            HIP_OK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-        }
+            bool updated = false;
+            if (exec) {
+                hipGraphExecUpdateResult result;
+                hipGraphNode_t failed = nullptr;
+                updated = hipGraphExecUpdate(exec, graph, &failed, &result) == hipSuccess &&
+                          result == hipGraphExecUpdateSuccess;
+                if (!updated) {
+                    (void)hipGetLastError();
+                    (void)hipGraphExecDestroy(exec);
+                    exec = nullptr;
+                }
+            }
+            if (!updated) {
+                const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+                if (e != hipSuccess) {
+                    (void)hipGraphDestroy(graph);
+                    HIP_OK(e);
+                }
+                c->graphInstantiations++;
+            } else {
+                c->graphUpdates++;
+            }
+            (void)hipGraphDestroy(graph);
+        };
         c->lastStepMs = 0.0;
         const int maxLaunches = (maxSteps + prm.stepsPerLaunch - 1) / prm.stepsPerLaunch;
+        if (useGraph) {
+            graphOf(c->installGraph, enqueueInstall);
+            graphOf(c->stepGraph, enqueueStep);
+            HIP_OK(hipGraphLaunch(c->installGraph, s));
+        } else {
+            enqueueInstall();
+        }
+        HIP_OK(hipGetLastError());
         for (steps = 0; steps < maxLaunches; steps++) {
             HIP_OK(hipEventRecord(c->evStart, s));
-            if (useGraph) HIP_OK(hipGraphLaunch(exec, s));
+            if (useGraph) HIP_OK(hipGraphLaunch(c->stepGraph, s));
             else enqueueStep();
             HIP_OK(hipEventRecord(c->evStop, s));
             HIP_OK(hipStreamSynchronize(s));
@@ -666,8 +713,6 @@ static void solve(ContextImpl *c, const olfhip_solve_inputs *in, olfhip_solve_ou
                 break;
             }
         }
-        if (exec) (void)hipGraphExecDestroy(exec);
-        if (graph) (void)hipGraphDestroy(graph);
     });
     HIP_OK(hipGetLastError());
     if (profile) {
@@ -760,7 +805,7 @@ struct LimitTableImpl {
     std::vector<int> angleFrom, angleTo;
     std::vector<double> angleHigh, angleLow;
     DBuf dBranch, dGroupStart, dGroupSide, dGroupQty, dLimitStart, dThreshold, dBusHigh, dBusLow, dCoef, dNode;
-    DBuf records, counts, screenLane, req, vals, isl;
+    DBuf records, counts, screenLane, req, vals, isl, near;
     DevLimits dev() const {
         return DevLimits{nBranches, nGroups, nLimits, nBuses, dBranch.as<int>(), dGroupStart.as<int>(),
                          dGroupSide.as<int>(), dGroupQty.as<int>(), dLimitStart.as<int>(), dThreshold.as<double>(),
@@ -879,6 +924,31 @@ static void screenViolations(ContextImpl *c, LimitTableImpl *t, double band, int
     records.resize((size_t)K * capacity);
     HIP_OK(hipMemcpyAsync(counts.data(), t->counts.p, (size_t)K * 4, hipMemcpyDeviceToHost, s));
     HIP_OK(hipMemcpyAsync(records.data(), t->records.p, records.size() * 4, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+}
+
+// Which scenarios of the last solve a state difference of relative size `band` could change the violations of
+// (nearLimitsKernel): flags [K] on the host.
+static void ambiguousScenarios(ContextImpl *c, LimitTableImpl *t, double band, int nScenarios, int *flags) {
+    if (c->st != t->st) throw ArgumentError("the limit table belongs to another structure");
+    const int K = c->lastInstalled;
+    if (K <= 0 || (int)c->lastStatus.size() != K) throw ArgumentError("no solve to check");
+    if (nScenarios != K) throw ArgumentError("n_scenarios must be the scenario count of the context's last solve");
+    if (!(band >= 0.0)) throw ArgumentError("band must be non-negative");
+    hipStream_t s = c->stream;
+    t->near.alloc((size_t)K * 4);
+    zeroIntKernel<<<gridFor(K, 256), 256, 0, s>>>(t->near.as<int>(), K);
+    const DevPlan dp = c->st->dp;
+    const DevState S = c->devState();
+    const DevLimits L = t->dev();
+    if (t->nBranches + t->nBuses > 0)
+        withT(c->T, [&](auto TC) {
+            constexpr int T = decltype(TC)::value;
+            nearLimitsKernel<T><<<gridFor((size_t)K * (t->nBranches + t->nBuses), 256), 256, 0, s>>>(
+                dp, S, L, c->modeBase.as<int>(), K, band, t->near.as<int>());
+        });
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(flags, t->near.p, (size_t)K * 4, hipMemcpyDeviceToHost, s));
     HIP_OK(hipStreamSynchronize(s));
 }
 
@@ -1340,6 +1410,15 @@ int olfhip_context_timing(const olfhip_context *context, double *step_ms, int *s
     });
 }
 
+int olfhip_context_graph_statistics(const olfhip_context *context, int *instantiations, int *updates) {
+    return guarded([&] {
+        if (!context || !instantiations || !updates) throw ArgumentError("null argument");
+        const ContextImpl *c = reinterpret_cast<const ContextImpl *>(context);
+        *instantiations = c->graphInstantiations;
+        *updates = c->graphUpdates;
+    });
+}
+
 int olfhip_read_outer_loop_state(olfhip_context *context, int n_scenarios, int *row_mode, double *target,
                                  signed char *reactive_limit_types) {
     return guarded([&] {
@@ -1475,6 +1554,15 @@ int olfhip_create_limit_table(olfhip_structure *structure, const olfhip_limit_ta
 }
 
 void olfhip_destroy_limit_table(olfhip_limit_table *table) { delete reinterpret_cast<LimitTableImpl *>(table); }
+
+int olfhip_ambiguous_scenarios(olfhip_context *context, olfhip_limit_table *table, double band, int n_scenarios,
+                               int *flags) {
+    return guarded([&] {
+        if (!context || !table || !flags) throw ArgumentError("null argument");
+        ambiguousScenarios(reinterpret_cast<ContextImpl *>(context), reinterpret_cast<LimitTableImpl *>(table), band,
+                           n_scenarios, flags);
+    });
+}
 
 int olfhip_screen_violations(olfhip_context *context, olfhip_limit_table *table, double band, int capacity,
                              int *counts, int *records) {

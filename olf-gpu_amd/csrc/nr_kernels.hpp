@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "generated/shunt_terms.hpp"   // OLF's shunt terms, rendered by codegen/java2hip.py
 #include "plan.hpp"   // ITEM_FILL_PAIR
 #include "trig.hpp"
 
@@ -151,6 +152,8 @@ struct NrParams {
     int kryDim;           // robust solve: GMRES basis size, restarts after the first cycle, relative 2-norm target
     int kryRestarts;
     double kryTol;
+    double linFailRatio;  // the linear-solve guard's bound on ||J dx + F||inf / ||F||inf
+    int fixedIter;        // exactly maxIter steps: no convergence test, no guard, no outer loop
 };
 
 template <typename VT>
@@ -331,12 +334,12 @@ __device__ __forceinline__ void nodeTerms(const DevPlan &P, int tile, int k, int
             jQ += oQth * vf2.x + oQv * vf2.y;
         }
     }
-    // bus shunt: p = g v^2, q = -b v^2
+    // bus shunt: OLF's ShuntCompensator terms (generated), p = g v^2, q = -b v^2
     const double gs = P.shuntG[k], bs = P.shuntB[k];
-    sP += gs * vo * vo;
-    sQ -= bs * vo * vo;
-    dPdv += 2.0 * gs * vo;
-    dQdv -= 2.0 * bs * vo;
+    sP += gen::shP(vo, gs);
+    sQ += gen::shQ(vo, bs);
+    dPdv += gen::shDpdv(vo, gs);
+    dQdv += gen::shDqdv(vo, bs);
     if (modeP == MODE_FIXED) {
         dPdth = 1;
         dPdv = 0;
@@ -1302,8 +1305,14 @@ __device__ void dsCheck(const DevPlan &P, const DevState &S, const DevRL &R, con
 // (native/cuda/common/check.cuh:38-40).
 // ROWCRIT: the per-row stopping criterion (prm.rowCriteria), a variant of its own so that the default kernel carries
 // no code for it.
+// Workgroups of MAX_THREADS threads the step kernel's register budget leaves room for on one CU (a build-time
+// tuning knob: 2 caps the kernel at 64 VGPRs, spilling the rest). HIP's second launch bound is the minimum waves per
+// SIMD: blocks x MAX_THREADS / 64 waves over the CU's 4 SIMDs.
+#ifndef OLFHIP_STEP_MIN_BLOCKS
+#define OLFHIP_STEP_MIN_BLOCKS 1
+#endif
 template <int T, typename VT, bool ROBUST = false, bool ROWCRIT = false>
-__global__ __launch_bounds__(MAX_THREADS) void nrStepKernel(DevPlan P, DevState S, DevRL R, NrParams prm, DevTop top) {
+__global__ __launch_bounds__(MAX_THREADS, OLFHIP_STEP_MIN_BLOCKS *MAX_THREADS / 256) void nrStepKernel(DevPlan P, DevState S, DevRL R, NrParams prm, DevTop top) {
     const int tile = blockIdx.x;
     const int tid = threadIdx.x, nt = blockDim.x;
     const int myLane = tid % T;
@@ -1461,12 +1470,17 @@ __global__ __launch_bounds__(MAX_THREADS) void nrStepKernel(DevPlan P, DevState 
                            : prm.convEps > 0.0
                                ? sqrt(sSsq[tid]) < sqrt(prm.convEps * prm.convEps * (double)S.nActive[tile * T + tid])
                                : norm < prm.tol;
-        if (!fin) {
+        if (prm.fixedIter) {
+            // the fixed-iteration mode (native/cuda/solver/newton_iteration.cu:141-151,336-343): no test until the
+            // budget, then the status of the final mismatch
+            if (it >= prm.maxIter) sStatus[tid] = fin ? ST_MAX_ITER : ST_DIVERGED;
+            else sSolve[tid] = 1;
+        } else if (!fin) {
             sStatus[tid] = ST_DIVERGED;
         } else if (it > 0 && below && it < prm.maxIter) {
             sConv[tid] = 1;
         } else if (it > 0 && it >= prm.maxIter) {
-            const bool bad = !(sRatio[tid] <= LIN_SOLVE_FAIL_RATIO);
+            const bool bad = !(sRatio[tid] <= prm.linFailRatio);
             sStatus[tid] = bad ? (prm.batchOfOne ? ST_LIN_FAILED : ST_REJECTED) : ST_MAX_ITER;
         } else {
             sSolve[tid] = 1;
@@ -1700,7 +1714,7 @@ __global__ __launch_bounds__(MAX_THREADS) void nrStepKernel(DevPlan P, DevState 
             int due = 0;
             for (int t = 0; t < T; t++)
                 due |= sSolve[t] && ((sSolves[t] == 0 && sIt[t] == 1) || sIt[t] == prm.maxIter - 1);
-            sFlag[3] = due;
+            sFlag[3] = due && !prm.fixedIter;
         }
         __syncthreads();
         if (sFlag[3]) {
@@ -1710,7 +1724,7 @@ __global__ __launch_bounds__(MAX_THREADS) void nrStepKernel(DevPlan P, DevState 
             laneReduce<T, true>(red, tid, nt);
             if (tid < T && sSolve[tid]) {
                 const double ratio = sNorm[tid] > 0.0 ? red[tid] / sNorm[tid] : 0.0;
-                if (sSolves[tid] == 0 && sIt[tid] == 1 && !(ratio <= LIN_SOLVE_FAIL_RATIO)) {
+                if (sSolves[tid] == 0 && sIt[tid] == 1 && !(ratio <= prm.linFailRatio)) {
                     sStatus[tid] = ST_LIN_FAILED;
                     sSolve[tid] = 0;
                 }

@@ -1,4 +1,7 @@
 // Host-side structure and symbolic plan of the batched block LU. See plan.hpp.
+// host-only use of the generated term functions
+#define OLFHIP_GEN_FN inline
+#include "generated/closed_branch_terms.hpp"
 #include "plan.hpp"
 
 #include <algorithm>
@@ -315,21 +318,21 @@ Plan buildPlan(int n, int nBranch, const double *p8, const int *idx12, int nShun
         P.levelBwdBucketPtr[l + 1] = (int)(P.bwdBucketRec.size() / 4);
     }
 
-    // --- per-branch flow coefficients (the same half-branch forms as the fill)
+    // --- per-branch flow coefficients: both ends' own/far forms, read off OLF's closed-branch expressions by the
+    // generated closedBranchHalfForms (the same forms the fill gathers)
     P.branchCoef.assign(8 * (size_t)nBranch, 0.0);
     P.branchNode.assign(2 * (size_t)nBranch, 0);
+    std::vector<gen::HalfBranchForm> side1(nBranch), side2(nBranch);
     for (int e = 0; e < nBranch; e++) {
-        const double *p = p8 + 8 * e;
-        const double y = p[0], g1 = p[1], g2 = p[2], b1 = p[3], b2 = p[4], ksi = p[5], r1 = p[6], a1 = p[7];
-        const double sk = std::sin(ksi), ck = std::cos(ksi);
+        gen::closedBranchHalfForms(p8 + 8 * e, &side1[e], &side2[e]);
         double *c = &P.branchCoef[8 * (size_t)e];
-        c[0] = r1 * r1 * (g1 + y * sk);
-        c[1] = y * r1;
-        c[2] = r1 * r1 * (-b1 + y * ck);
-        c[3] = ksi - a1;
-        c[4] = g2 + y * sk;
-        c[5] = -b2 + y * ck;
-        c[6] = ksi + a1;
+        c[0] = side1[e].A;
+        c[1] = side1[e].B;
+        c[2] = side1[e].C;
+        c[3] = side1[e].psi;
+        c[4] = side2[e].A;
+        c[5] = side2[e].C;
+        c[6] = side2[e].psi;
         P.branchNode[2 * e] = P.pos[bus1[e]];
         P.branchNode[2 * e + 1] = P.pos[bus2[e]];
     }
@@ -349,16 +352,9 @@ Plan buildPlan(int n, int nBranch, const double *p8, const int *idx12, int nShun
         const int b = P.perm[k];
         hs.clear();
         for (int e : inc[b]) {
-            const double *p = p8 + 8 * e;
-            const double y = p[0], g1 = p[1], g2 = p[2], b1 = p[3], b2 = p[4], ksi = p[5], r1 = p[6], a1 = p[7];
-            const double sk = std::sin(ksi), ck = std::cos(ksi);
-            if (bus1[e] == b) {
-                // side 1: p1 = r1^2 (g1 + y sin ksi) v1^2 - y r1 v1 v2 sin(ksi - a1 - ph1 + ph2)
-                hs.push_back({P.pos[bus2[e]], e, r1 * r1 * (g1 + y * sk), y * r1, r1 * r1 * (-b1 + y * ck), ksi - a1});
-            } else {
-                // side 2: p2 = (g2 + y sin ksi) v2^2 - y r1 v1 v2 sin(ksi + a1 - ph2 + ph1)
-                hs.push_back({P.pos[bus1[e]], e, g2 + y * sk, y * r1, -b2 + y * ck, ksi + a1});
-            }
+            // the branch seen from bus b: side 1 (far end bus2) or side 2 (far end bus1)
+            const gen::HalfBranchForm &f = bus1[e] == b ? side1[e] : side2[e];
+            hs.push_back({P.pos[bus1[e] == b ? bus2[e] : bus1[e]], e, f.A, f.B, f.C, f.psi});
         }
         std::sort(hs.begin(), hs.end(), [](const Half &u, const Half &v) {
             return u.far != v.far ? u.far < v.far : u.branch < v.branch;

@@ -53,6 +53,38 @@ __device__ __forceinline__ bool islandedNode(const DevPlan &P, const DevState &S
     return S.mode[vIdx<T>(tile, P.nb, k, t)] == MODE_FIXED && modeBase[2 * P.perm[k]] != MODE_FIXED;
 }
 
+// Flows at both ends of closed branch e in lane t of a tile, from the own/far coefficients of the fill: f = p1 q1 v1
+// p2 q2 v2.
+template <int T>
+__device__ __forceinline__ void branchEndFlows(const DevPlan &P, const DevState &S, const DevLimits &L, int tile, int t,
+                                               int e, double f[6]) {
+    const int k1 = L.branchNode[2 * e], k2 = L.branchNode[2 * e + 1];
+    const double2 x1 = *reinterpret_cast<const double2 *>(S.x + vIdx<T>(tile, P.nb, k1, t));
+    const double2 x2 = *reinterpret_cast<const double2 *>(S.x + vIdx<T>(tile, P.nb, k2, t));
+    const double *c = L.coef + 8 * (size_t)e;
+    double s1, c1, s2, c2;
+    sincosReduced(c[3] - x1.x + x2.x, &s1, &c1);
+    sincosReduced(c[6] - x2.x + x1.x, &s2, &c2);
+    const double bvv = c[1] * x1.y * x2.y;
+    f[0] = c[0] * x1.y * x1.y - bvv * s1;
+    f[1] = c[2] * x1.y * x1.y - bvv * c1;
+    f[2] = x1.y;
+    f[3] = c[4] * x2.y * x2.y - bvv * s2;
+    f[4] = c[5] * x2.y * x2.y - bvv * c2;
+    f[5] = x2.y;
+}
+
+// Whether closed branch e is out of service in scenario s: outaged, or an end cut off from the slack.
+template <int T>
+__device__ __forceinline__ bool branchOutOfService(const DevPlan &P, const DevState &S, const DevLimits &L,
+                                                   const int *modeBase, int s, int e) {
+    const int tile = s / T, t = s % T;
+    bool out = false;
+    for (int q = 0; q < S.maxOut; q++) out |= S.outages[(size_t)s * S.maxOut + q] == e;
+    return out || islandedNode<T>(P, S, modeBase, tile, L.branchNode[2 * e], t) ||
+           islandedNode<T>(P, S, modeBase, tile, L.branchNode[2 * e + 1], t);
+}
+
 // One thread per (scenario, table branch). Scenarios whose flag is 0 (not converged) emit nothing.
 template <int T>
 __global__ void screenBranchesKernel(DevPlan P, DevState S, DevLimits L, const int *modeBase, const int *screenLane,
@@ -65,25 +97,12 @@ __global__ void screenBranchesKernel(DevPlan P, DevState S, DevLimits L, const i
         const int g0 = L.groupStart[b], g1 = L.groupStart[b + 1];
         if (g0 == g1) continue;
         const int e = L.branch[b];
-        const int tile = s / T, t = s % T;
-        bool out = false;
-        for (int q = 0; q < S.maxOut; q++) out |= S.outages[(size_t)s * S.maxOut + q] == e;
-        if (out) continue;
-        const int k1 = L.branchNode[2 * e], k2 = L.branchNode[2 * e + 1];
-        if (islandedNode<T>(P, S, modeBase, tile, k1, t) || islandedNode<T>(P, S, modeBase, tile, k2, t)) continue;
-        const double2 x1 = *reinterpret_cast<const double2 *>(S.x + vIdx<T>(tile, P.nb, k1, t));
-        const double2 x2 = *reinterpret_cast<const double2 *>(S.x + vIdx<T>(tile, P.nb, k2, t));
-        const double *c = L.coef + 8 * (size_t)e;
-        const double v1 = x1.y, v2 = x2.y;
-        double s1, c1, s2, c2;
-        sincosReduced(c[3] - x1.x + x2.x, &s1, &c1);
-        sincosReduced(c[6] - x2.x + x1.x, &s2, &c2);
-        const double bvv = c[1] * v1 * v2;
-        const double p1 = c[0] * v1 * v1 - bvv * s1, q1 = c[2] * v1 * v1 - bvv * c1;
-        const double p2 = c[4] * v2 * v2 - bvv * s2, q2 = c[5] * v2 * v2 - bvv * c2;
+        if (branchOutOfService<T>(P, S, L, modeBase, s, e)) continue;
+        double f[6];
+        branchEndFlows<T>(P, S, L, s / T, s % T, e, f);
         for (int g = g0; g < g1; g++) {
-            const double value = L.groupSide[g] == 1 ? comparedQuantity(L.groupQty[g], p1, q1, v1)
-                                                     : comparedQuantity(L.groupQty[g], p2, q2, v2);
+            const double *end = L.groupSide[g] == 1 ? f : f + 3;
+            const double value = comparedQuantity(L.groupQty[g], end[0], end[1], end[2]);
             // thresholds run most severe first and the host reports the first one exceeded: hand it every limit the
             // value may exceed (threshold lowered by the band); past one it exceeds for certain (threshold raised by
             // the band) the rest of the group cannot be reported. A NaN value fails every comparison.
@@ -115,6 +134,50 @@ __global__ void screenBusesKernel(DevPlan P, DevState S, DevLimits L, const int 
         const double hi = L.busHigh[i], lo = L.busLow[i];
         if (v > hi - fabs(hi) * band) emitRecord(records, counts, s, capacity, L.nLimits + 2 * i);
         if (v < lo + fabs(lo) * band) emitRecord(records, counts, s, capacity, L.nLimits + 2 * i + 1);
+    }
+}
+
+// Scenarios of the last solve whose decisions a state difference of relative size `band` could change: a converged
+// scenario gets flag 1 when a monitored quantity lies within band x |threshold| of one of its group's thresholds up to
+// the first one it exceeds beyond the band (a later threshold cannot be reported), or a bus voltage within band x
+// |limit| of its high or low limit, unless the bus holds its voltage (its V row an identity: PV, slack or islanded
+// buses, whose voltage is the setpoint whatever the start). One thread per (scenario, table branch or bus).
+template <int T>
+__global__ void nearLimitsKernel(DevPlan P, DevState S, DevLimits L, const int *modeBase, int nScenarios, double band,
+                                 int *flag) {
+    const int items = L.nBranches + L.nBuses;
+    const size_t total = (size_t)nScenarios * items;
+    for (size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x; w < total; w += (size_t)gridDim.x * blockDim.x) {
+        const int item = (int)(w % items);
+        const int s = (int)(w / items);
+        if (S.status[s] != ST_CONVERGED) continue;
+        const int tile = s / T, t = s % T;
+        bool near = false;
+        if (item < L.nBranches) {
+            const int g0 = L.groupStart[item], g1 = L.groupStart[item + 1];
+            const int e = L.branch[item];
+            if (g0 == g1 || branchOutOfService<T>(P, S, L, modeBase, s, e)) continue;
+            double f[6];
+            branchEndFlows<T>(P, S, L, tile, t, e, f);
+            for (int g = g0; g < g1 && !near; g++) {
+                const double *end = L.groupSide[g] == 1 ? f : f + 3;
+                const double value = comparedQuantity(L.groupQty[g], end[0], end[1], end[2]);
+                for (int l = L.limitStart[g]; l < L.limitStart[g + 1]; l++) {
+                    const double margin = fabs(L.threshold[l]) * band;
+                    near |= fabs(value - L.threshold[l]) <= margin;
+                    if (near || value > L.threshold[l] + margin) break;
+                }
+            }
+        } else {
+            const int i = item - L.nBranches;
+            const int k = P.pos[i];
+            const size_t vi = vIdx<T>(tile, P.nb, k, t);
+            if (S.mode[vi + 1] == MODE_FIXED) continue;   // voltage held at its setpoint (or frozen)
+            const double v = S.x[vi + 1];
+            near = fabs(v - L.busHigh[i]) <= fabs(L.busHigh[i]) * band ||
+                   fabs(v - L.busLow[i]) <= fabs(L.busLow[i]) * band;   // NaN limits compare false
+        }
+        if (near) flag[s] = 1;
     }
 }
 

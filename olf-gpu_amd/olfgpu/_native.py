@@ -30,7 +30,8 @@ EXPORTS = [
     "olfhip_mlp_parameters", "olfhip_mlp_predict_warm_start", "olfhip_mlp_gemm_test",
     "olfhip_create_limit_table", "olfhip_destroy_limit_table", "olfhip_screen_violations", "olfhip_detect_violations",
     "olfhip_set_distributed_slack",
-    "olfhip_set_row_criteria",
+    "olfhip_set_row_criteria", "olfhip_ambiguous_scenarios",
+    "olfhip_context_graph_statistics",
 ]
 
 
@@ -64,7 +65,8 @@ class SolveInputs(C.Structure):
                 ("distributed_slack", C.c_int), ("slack_dist_tol", C.c_double),
                 ("reject_slack_distribution_failure", C.c_int), ("line_search_max_iteration", C.c_int),
                 ("line_search_step_fold", C.c_double), ("conv_eps_per_eq", C.c_double),
-                ("active_row_counts", C.c_void_p), ("reactive_limits_seed_shared", C.c_int)]
+                ("active_row_counts", C.c_void_p), ("reactive_limits_seed_shared", C.c_int),
+                ("linear_solve_fail_ratio", C.c_double)]
 
 
 class DistributedSlackDesc(C.Structure):
@@ -121,6 +123,7 @@ def lib():
         L.olfhip_solve_batch_context.argtypes = [vp, C.POINTER(SolveInputs), C.POINTER(SolveOutputs)]
         L.olfhip_read_states.argtypes = [vp, i, vp]
         L.olfhip_context_timing.argtypes = [vp, C.POINTER(d), C.POINTER(i)]
+        L.olfhip_context_graph_statistics.argtypes = [vp, C.POINTER(i), C.POINTER(i)]
         L.olfhip_set_distributed_slack.argtypes = [vp, C.c_void_p]
         L.olfhip_set_row_criteria.argtypes = [vp, C.c_void_p, C.c_void_p]
         L.olfhip_read_outer_loop_state.argtypes = [vp, i, vp, vp, vp]
@@ -132,6 +135,7 @@ def lib():
         L.olfhip_destroy_limit_table.argtypes = [vp]
         L.olfhip_destroy_limit_table.restype = None
         L.olfhip_screen_violations.argtypes = [vp, vp, d, i, vp, vp]
+        L.olfhip_ambiguous_scenarios.argtypes = [vp, vp, d, i, vp]
         L.olfhip_detect_violations.argtypes = [vp, vp, d, i, i, vp, C.POINTER(i)]
         _LIB = L
     return _LIB
@@ -292,7 +296,8 @@ class BatchContext:
                     max_dphi=0.174533, max_outer_iterations=20, max_pq_pv_switch=3, skip_reactive_limits=False,
                     states_to_host=True, x0_per_scenario_device=None, distributed_slack=False, slack_dist_tol=0.0,
                     reject_slack_distribution_failure=False, line_search_max_iteration=10,
-                    line_search_step_fold=4.0 / 3.0, conv_eps_per_eq=0.0, active_row_counts=None):
+                    line_search_step_fold=4.0 / 3.0, conv_eps_per_eq=0.0, active_row_counts=None,
+                    linear_solve_fail_ratio=0.0):
         keep = []
 
         def arr(a, dt):
@@ -336,7 +341,7 @@ class BatchContext:
                          max_pq_pv_switch, int(skip_reactive_limits), int(states_to_host), x0_per_scenario_device,
                          int(distributed_slack), float(slack_dist_tol), int(reject_slack_distribution_failure),
                          int(line_search_max_iteration), float(line_search_step_fold), float(conv_eps_per_eq), ptr(ac),
-                         shared)
+                         shared, float(linear_solve_fail_ratio))
         return si, keep
 
     def set_row_criteria(self, thresholds):
@@ -386,6 +391,12 @@ class BatchContext:
         steps = C.c_int()
         check(lib().olfhip_context_timing(self.h, C.byref(ms), C.byref(steps)))
         return ms.value, steps.value
+
+    def graph_statistics(self):
+        """(instantiations, in-place updates) of the context's solve graphs so far."""
+        a, b = C.c_int(), C.c_int()
+        check(lib().olfhip_context_graph_statistics(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     def read_outer_loop_state(self, k, n_entries):
         """(row modes [k, n], targets [k, n], reactive-limit types [k, n_entries]) left by the last solve."""
@@ -455,6 +466,14 @@ class LimitTable:
         records = np.zeros((k, capacity), np.int32)
         check(lib().olfhip_screen_violations(context.h, self.h, band, capacity, ptr(counts), ptr(records)))
         return counts, records
+
+    def ambiguous(self, context: "BatchContext", band=1e-4) -> np.ndarray:
+        """olfhip_ambiguous_scenarios: [k] bool, the converged scenarios of the context's last solve with a monitored
+        quantity within relative `band` of a threshold (voltages held at a setpoint excluded)."""
+        k = context.last_n_scenarios
+        flags = np.zeros(k, np.int32)
+        check(lib().olfhip_ambiguous_scenarios(context.h, self.h, band, k, ptr(flags)))
+        return flags.astype(bool)
 
     def detect(self, context: "BatchContext", band=1e-7, capacity=1024, max_violations=1 << 20):
         """Screen + exact host decisions: a VIOLATION_DTYPE array ordered by scenario, branches, buses."""

@@ -120,7 +120,8 @@ class NewtonRaphsonParameters:
     """The OLF Newton-Raphson parameters the AcSolver copies (GpuNewtonRaphsonFactory.java:62-79)."""
 
     maxIterations: int = 15
-    maxMismatch: float = 1e-4                 # ||F||inf stopping threshold, pu
+    maxMismatch: float = 1e-4                 # ||F||inf stopping threshold, pu; <= 0 (with convEpsPerEq <= 0 and no
+                                              # per-equation-type criterion): exactly maxIterations steps, no test
     convEpsPerEq: float = 0.0                 # > 0: OLF's ||F||_2 < sqrt(convEpsPerEq^2 * equations) instead
     stateVectorScalingMode: str = "NONE"      # NONE, LINE_SEARCH or MAX_VOLTAGE_CHANGE
     maxVoltageChangeV: float = 0.1
@@ -140,6 +141,8 @@ class NewtonRaphsonParameters:
     # OLF's stopping criterion object: None for the uniform ones (maxMismatch, or convEpsPerEq), or a
     # PerEquationTypeStoppingCriteria (per-row thresholds installed on the device context)
     stoppingCriteria: PerEquationTypeStoppingCriteria | None = None
+    # the linear-solve guard's bound on ||J dx + F||inf / ||F||inf; 0: the reference's 1e-2
+    linearSolveFailRatio: float = 0.0
 
     def reactive_limit_band(self) -> float:
         """The reactive-limit loop's switch band (pu), derived from the stopping criterion as OLF's
@@ -237,7 +240,8 @@ class GpuNewtonRaphson:
             max_outer_iterations=nr.maxOuterLoopIterations, max_pq_pv_switch=nr.maxPqPvSwitch,
             skip_reactive_limits=not nr.reactiveLimits, **nr.slack_inputs(self.distributed_slack),
             line_search_max_iteration=nr.lineSearchStateVectorScalingMaxIteration,
-            line_search_step_fold=nr.lineSearchStateVectorScalingStepFold, conv_eps_per_eq=nr.convEpsPerEq)
+            line_search_step_fold=nr.lineSearchStateVectorScalingStepFold, conv_eps_per_eq=nr.convEpsPerEq,
+            linear_solve_fail_ratio=nr.linearSolveFailRatio)
         r = self.context.solve(inp)
         n_rl = 0 if self.reactive_limits is None else len(self.reactive_limits["voltage_row"])
         m, t, lt = self.context.read_outer_loop_state(1, n_rl)
@@ -301,6 +305,7 @@ class BatchScenarioResult:
     robust_recovered: int = 0              # of which it kept the answer (CONVERGED or MAX_ITERATION_REACHED)
     robust_errors: list = field(default_factory=list)   # failures of the retry pass (the first pass's answers kept)
     warm_started: np.ndarray | None = None  # [K] bool: the answer is the warm-started solve's (not a base-start redo)
+    confirmed: int = 0                      # converged warm-started answers solved again for their limit decisions
 
 
 class GpuBatchedSecurityAnalysisSolver:
@@ -330,7 +335,7 @@ class GpuBatchedSecurityAnalysisSolver:
         self.lanes_per_tile = lanes_per_tile
         self.threads_per_tile = threads_per_tile
         self._ctx = None
-        self._bridges = None
+        self._islands = {}    # outage set -> islanded buses (they depend on the structure only)
 
     def plan_islands(self, contingencies: np.ndarray):
         """(row_mode_entries, target_override_index, target_override_value, islanded_count) for the chunk."""
@@ -338,9 +343,11 @@ class GpuBatchedSecurityAnalysisSolver:
         rme, tix, tvl = [], [], []
         counts = np.zeros(len(contingencies), np.int32)
         for s, outs in enumerate(contingencies):
-            outs = [int(o) for o in np.atleast_1d(outs) if o >= 0]
-            isl = self.structure.islanded_buses(outs, self.slack_bus)
-            buses = np.nonzero(isl)[0]
+            key = tuple(sorted(int(o) for o in np.atleast_1d(outs) if o >= 0))
+            buses = self._islands.get(key)
+            if buses is None:
+                buses = np.nonzero(self.structure.islanded_buses(list(key), self.slack_bus))[0]
+                self._islands[key] = buses
             counts[s] = buses.size
             for b in buses:
                 for r, val in ((2 * b, 0.0), (2 * b + 1, 1.0)):
@@ -381,7 +388,8 @@ class GpuBatchedSecurityAnalysisSolver:
                        x0_per_scenario=None, states_to_host=True, warm_start=None,
                        warm_start_features=None, limit_table: N.LimitTable | None = None,
                        violation_band=1e-7, cost=None, cost_tie=None, base_violations=None,
-                       increased_violations=None, keep_voltage_angle_violations=False) -> BatchScenarioResult:
+                       increased_violations=None, keep_voltage_angle_violations=False,
+                       confirm_band=1e-4) -> BatchScenarioResult:
         """solveBatchN1: contingencies is [K] or [K, m] closed-branch ids (-1 padded). With a warm_start network
         (olfgpu.warmstart.WarmStartMLP, single outages) every scenario starts from its prediction, and any that does
         not converge is solved again from the base-case start. With a limit_table the violations of every converged
@@ -393,7 +401,14 @@ class GpuBatchedSecurityAnalysisSolver:
         contingency (e.g. warmstart.iteration_costs of its branch) the batch is solved longest first
         (longest_first, ties broken by cost_tie when given); the results are in the contingencies' order either
         way. Tables with voltage-angle limits need keep_voltage_angle_violations to be filtered against the base case
-        (violations.manage)."""
+        (violations.manage).
+
+        With a warm start and a limit table, a converged warm-started answer is another Newton iterate inside the same
+        stopping tolerance as the base-start one (the reference's semantics), so a limit decision on a quantity that
+        close to its threshold can differ: every contingency whose warm-started answer has a monitored quantity within
+        relative confirm_band of a threshold (voltages held at a setpoint excluded; LimitTable.ambiguous) is solved
+        again from the base-case start, whose answer and violations stand (result.confirmed counts them; 0 turns the
+        rule off)."""
         import time
 
         if cost is not None:
@@ -407,7 +422,8 @@ class GpuBatchedSecurityAnalysisSolver:
                                     np.asarray(contingencies)[order], part(x0_per_scenario), states_to_host,
                                     warm_start, part(warm_start_features), limit_table, violation_band,
                                     base_violations=base_violations, increased_violations=increased_violations,
-                                    keep_voltage_angle_violations=keep_voltage_angle_violations)
+                                    keep_voltage_angle_violations=keep_voltage_angle_violations,
+                                    confirm_band=confirm_band)
             r.status, r.iterations, r.total_iterations, r.norm = (r.status[inv], r.iterations[inv],
                                                                   r.total_iterations[inv], r.norm[inv])
             r.islanded_count = r.islanded_count[inv]
@@ -459,7 +475,7 @@ class GpuBatchedSecurityAnalysisSolver:
                 **nr.slack_inputs(self.distributed_slack),
                 line_search_max_iteration=nr.lineSearchStateVectorScalingMaxIteration,
                 line_search_step_fold=nr.lineSearchStateVectorScalingStepFold, conv_eps_per_eq=nr.convEpsPerEq,
-                active_row_counts=(n - 2 * cnt).astype(np.int32))
+                active_row_counts=(n - 2 * cnt).astype(np.int32), linear_solve_fail_ratio=nr.linearSolveFailRatio)
             ctx = self.robust_context(len(part)) if robust else self.context(len(part))
             t0 = time.perf_counter()
             r = ctx.solve(inp, want_states=states_to_host)
@@ -468,12 +484,16 @@ class GpuBatchedSecurityAnalysisSolver:
             return r, cnt
 
         viol = []
+        confirmed = 0
 
-        def detect(rows, ctx=None):
-            """violations of the last solve, scenario indices mapped through rows"""
+        def detect(rows, ctx=None, keep=None):
+            """violations of the last solve, scenario indices mapped through rows (those of scenarios not in keep
+            dropped)"""
             if limit_table is None:
                 return
             v = limit_table.detect(ctx or self._ctx, band=violation_band)
+            if keep is not None:
+                v = v[keep[v["scenario"]]]
             v["scenario"] = rows[v["scenario"]]
             viol.append(v)
 
@@ -491,14 +511,22 @@ class GpuBatchedSecurityAnalysisSolver:
             nm[c0:c1] = r["norm"]
             if states_to_host:
                 xs[c0:c1] = r["x"]
-            detect(np.arange(c0, c1))
-            if warm_start is not None:
+            if warm_start is None:
+                detect(np.arange(c0, c1))
+            else:
                 from_warm[c0:c1] = True
-                # a warm start that did not converge: solve again from the base-case start, whose answer stands
-                redo = np.nonzero(r["status"] != 0)[0]
+                # a warm start that did not converge, or whose answer has a limit decision within confirm_band of
+                # its threshold: solve again from the base-case start, whose answer stands
+                again = r["status"] != 0
+                if limit_table is not None and confirm_band > 0:
+                    near = limit_table.ambiguous(self._ctx, confirm_band) & (r["status"] == 0)
+                    confirmed += int(near.sum())
+                    again |= near
+                detect(np.arange(c0, c1), keep=~again)
+                redo = np.nonzero(again)[0]
                 from_warm[c0 + redo] = False
                 if redo.size:
-                    fallback += redo.size
+                    fallback += int((r["status"] != 0).sum())
                     r2, _ = solve_part(part[redo])
                     detect(c0 + redo)
                     st[c0 + redo] = r2["status"]
@@ -540,4 +568,4 @@ class GpuBatchedSecurityAnalysisSolver:
                                     limit_table.bus_nominal_kv, keep_voltage_angle_violations)
         return BatchScenarioResult(st, it, tot, nm, xs, islanded, chunks, device_s, fallback, violations,
                                    robust_attempted, robust_recovered, robust_errors,
-                                   from_warm if warm_start is not None else None)
+                                   from_warm if warm_start is not None else None, confirmed)

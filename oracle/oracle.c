@@ -770,6 +770,8 @@ void oracle_nr_solve(const oracle_sys *s, const oracle_nr_params *prm, const ora
     const int ds_on = ds != NULL;
     const int rl_on = rl && rl->n > 0 && prm->reactive_limits;
     double retain_applied = NAN;
+    const double fail_ratio = prm->lin_fail_ratio > 0.0 ? prm->lin_fail_ratio : LIN_SOLVE_FAIL_RATIO;
+    const int fixed_iter = !(prm->tol > 0.0) && !(prm->conv_eps_per_eq > 0.0) && !(prm->row_thr_fixed && prm->row_thr_eq);
     int outer = 0;
     int loop = 0;             /* 0 distributed slack, 1 reactive limits, 2 end of the sweep */
     int sweep_changed = 0;
@@ -805,8 +807,15 @@ void oracle_nr_solve(const oracle_sys *s, const oracle_nr_params *prm, const ora
             }
             res->norm = norm;
             res->iterations = it;
-            if (!finite) { res->status = ORC_DIVERGED; ended = 1; break; }
-            if (it > 0) {
+            if (fixed_iter) {
+                /* exactly max_iter steps, no test; the status from the final mismatch */
+                if (it == prm->max_iter) {
+                    res->status = finite ? ORC_MAX_ITERATION_REACHED : ORC_DIVERGED;
+                    ended = 1;
+                    break;
+                }
+            } else if (!finite) { res->status = ORC_DIVERGED; ended = 1; break; }
+            if (!fixed_iter && it > 0) {
                 int below = norm < prm->tol;
                 if (prm->row_thr_fixed && prm->row_thr_eq) {
                     below = 1;
@@ -820,7 +829,7 @@ void oracle_nr_solve(const oracle_sys *s, const oracle_nr_params *prm, const ora
                 }
                 if (below && it < prm->max_iter) { converged = 1; break; }
                 if (it == prm->max_iter) {
-                    int bad = !(last_ratio <= LIN_SOLVE_FAIL_RATIO);
+                    int bad = !(last_ratio <= fail_ratio);
                     res->status = bad ? (prm->batch_of_one ? ORC_LINEAR_SOLVE_FAILED : ORC_REJECTED)
                                       : ORC_MAX_ITERATION_REACHED;
                     ended = 1;
@@ -828,7 +837,10 @@ void oracle_nr_solve(const oracle_sys *s, const oracle_nr_params *prm, const ora
                 }
             }
             for (int i = 0; i < n; i++) b[i] = -F[i];
-            if (lu_solve_ordered(n, s->rowptr, s->colidx, J, s->colorder, b, dx) != 0) {
+            if (fixed_iter && !finite) {
+                /* the fixed-iteration mode does not isolate a non-finite scenario: its steps stay non-finite */
+                for (int i = 0; i < n; i++) dx[i] = NAN;
+            } else if (lu_solve_ordered(n, s->rowptr, s->colidx, J, s->colorder, b, dx) != 0) {
                 res->status = ORC_LINEAR_SOLVE_FAILED;
                 ended = 1;
                 break;
@@ -840,7 +852,7 @@ void oracle_nr_solve(const oracle_sys *s, const oracle_nr_params *prm, const ora
                 r[i] = acc;
             }
             double ratio = norm > 0.0 ? inf_norm(r, n, NULL) / norm : 0.0;
-            if (res->solves == 0 && it == 1 && !(ratio <= LIN_SOLVE_FAIL_RATIO)) {
+            if (!fixed_iter && res->solves == 0 && it == 1 && !(ratio <= fail_ratio)) {
                 res->status = ORC_LINEAR_SOLVE_FAILED;
                 ended = 1;
                 break;

@@ -80,7 +80,13 @@ typedef struct {
        every row's |F| is below its mode's threshold (instead of the two criteria above) */
     const double *row_thr_fixed;
     const double *row_thr_eq;
+    double lin_fail_ratio; /* the linear-solve guard's ||J dx + F||inf / ||F||inf bound (<= 0: the reference's 1e-2,
+                              solver/iterative_refinement.cuh:23) */
 } oracle_nr_params;
+/* tol <= 0 with neither conv_eps_per_eq nor row thresholds: the fixed-iteration mode (reference
+ * solver/outer_loop_driver.cu:165-175, newton_iteration.cu:141-151,336-343): exactly max_iter Newton steps with no
+ * convergence test, no linear-solve guard and no outer loop; MAX_ITERATION_REACHED, or DIVERGED when the final
+ * mismatch is not finite. Pinned by tests/test_oracle_fixed_iterations.py. */
 
 typedef struct {
     int n;                 /* entries */

@@ -26,7 +26,8 @@ class NrParams(C.Structure):
                 ("max_outer", C.c_int), ("max_switch", C.c_int), ("scaling_mode", C.c_int),
                 ("max_dv", C.c_double), ("max_dphi", C.c_double), ("batch_of_one", C.c_int),
                 ("ls_max_iteration", C.c_int), ("ls_step_fold", C.c_double), ("conv_eps_per_eq", C.c_double),
-                ("n_active", C.c_int), ("row_thr_fixed", C.c_void_p), ("row_thr_eq", C.c_void_p)]
+                ("n_active", C.c_int), ("row_thr_fixed", C.c_void_p), ("row_thr_eq", C.c_void_p),
+                ("lin_fail_ratio", C.c_double)]
 
 
 class Rl(C.Structure):
@@ -125,11 +126,14 @@ def min_degree(adjptr, adj):
 
 def params(max_iter=15, tol=1e-4, reactive_limits=True, rl_tol=1e-4, max_outer=20, max_switch=3,
            scaling_mode=0, max_dv=0.1, max_dphi=0.174533, batch_of_one=True, ls_max_iteration=10,
-           ls_step_fold=4.0 / 3.0, conv_eps_per_eq=0.0, n_active=0, row_thresholds=None) -> NrParams:
+           ls_step_fold=4.0 / 3.0, conv_eps_per_eq=0.0, n_active=0, row_thresholds=None,
+           lin_fail_ratio=0.0) -> NrParams:
     """scaling_mode: 0 NONE, 1 LINE_SEARCH, 2 MAX_VOLTAGE_CHANGE; conv_eps_per_eq > 0: the 2-norm criterion;
-    row_thresholds = (fixed [n], equation [n]): the per-row criterion (every |F_i| below its mode's threshold)"""
+    row_thresholds = (fixed [n], equation [n]): the per-row criterion (every |F_i| below its mode's threshold);
+    tol <= 0 without either: the fixed-iteration mode; lin_fail_ratio > 0: the linear-solve guard's bound"""
     p = NrParams(max_iter, tol, int(reactive_limits), rl_tol, max_outer, max_switch, scaling_mode, max_dv,
-                 max_dphi, int(batch_of_one), ls_max_iteration, ls_step_fold, conv_eps_per_eq, n_active, None, None)
+                 max_dphi, int(batch_of_one), ls_max_iteration, ls_step_fold, conv_eps_per_eq, n_active, None, None,
+                 lin_fail_ratio)
     if row_thresholds is not None:
         p._rows = tuple(np.ascontiguousarray(a, np.float64) for a in row_thresholds)
         p.row_thr_fixed, p.row_thr_eq = p._rows[0].ctypes.data, p._rows[1].ctypes.data

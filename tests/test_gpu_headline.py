@@ -151,15 +151,23 @@ def test_2000_outages_learned_warm_start_match_oracle_from_the_same_start(h, hea
     print(f"[headline] against the base-start oracle: states {st}", flush=True)
     assert st["status_mismatches"] == 0, st
     cold_total = ref["total_iterations"].sum()
-    print(f"[headline] Newton iterations: warm {r.total_iterations.sum()} vs base start {cold_total}", flush=True)
-    assert r.total_iterations.sum() < 0.8 * cold_total
+    print(f"[headline] Newton iterations: warm {r.total_iterations.sum()} (base-start confirmations of {r.confirmed} "
+          f"included) vs base start {cold_total}", flush=True)
+    assert r.total_iterations.sum() < 0.9 * cold_total
 
 
-@pytest.mark.xfail(reason="a warm-started answer is another iterate within the 1e-4 pu stopping tolerance: one entry "
-                          "in one of 2,000 lists differs from the base-start oracle's", strict=False)
 def test_2000_outages_learned_warm_start_violation_lists_match_oracle(h, headline):
+    """The post-contingency lists equal the base-start oracle's entry for entry. A warm-started answer is another
+    iterate inside the 1e-4 pu tolerance, so the product solves again from the base start every contingency whose
+    warm-started answer has a monitored quantity within 1e-4 (relative) of a threshold (solve_batch_n1's
+    confirm_band); those answers then equal the oracle's base-start ones to 1e-6."""
     ctg, r, x0, ref = headline
     want, scans = oracle_lists(h, ref, ctg)
     viol = parity.compare_violations(r.violations, want, len(ctg))
-    print(f"[headline] violations against the base-start oracle: {viol}", flush=True)
+    print(f"[headline] {r.confirmed} answers confirmed from the base start; violations against the base-start "
+          f"oracle: {viol}", flush=True)
+    assert 0 < r.confirmed < 0.2 * len(ctg)
     assert viol["contingencies_differing"] == 0, viol
+    confirmed = np.nonzero(~r.warm_started & (r.status == 0))[0]
+    st = parity.compare_states(r.x, r.status, ref, confirmed)
+    assert st["within_1e-6"] == st["compared"] == confirmed.size, st

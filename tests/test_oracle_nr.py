@@ -125,3 +125,23 @@ def test_n1_results_solve_the_post_contingency_grid(grid):
         main = ~isl
         main[0] = False
         assert np.abs(F[0::2][main] - tb[0::2][main]).max() < 1e-4
+
+
+def test_n1_per_scenario_starts(grid):
+    """oracle_n1's per-scenario start (x0): the base state repeated gives bitwise the base-start run; each
+    contingency restarted from its own solved state converges after one Newton step to the same answer (within the
+    tolerance's reach)."""
+    net = grid("grid2000")
+    osys = po.OracleSystem(net)
+    x0, tgt, mode, _ = net.system()
+    xb, mb, tb, ltb, res = osys.solve(x0, mode, tgt)
+    ctg = gridgen.sample_contingencies(net, 12, 0.25, 19)
+    a = osys.n1(xb, mb, tb, ltb, ctg)
+    b = osys.n1(xb, mb, tb, ltb, ctg, x0=np.tile(xb, (len(ctg), 1)))
+    assert np.array_equal(a["x"], b["x"]) and np.array_equal(a["total_iterations"], b["total_iterations"])
+    c = osys.n1(xb, mb, tb, ltb, ctg, x0=a["x"])
+    ok = a["status"] == 0
+    assert (c["status"][ok] == 0).all()
+    assert np.abs(c["x"][ok] - a["x"][ok]).max() < 1e-3
+    with pytest.raises(ValueError):
+        osys.n1(xb, mb, tb, ltb, ctg, x0=np.tile(xb, (len(ctg) + 1, 1)))
