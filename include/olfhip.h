@@ -358,6 +358,16 @@ int olfhip_screen_violations(olfhip_context *context, olfhip_limit_table *table,
  * base start's. No reference counterpart (the reference has no warm start). */
 int olfhip_ambiguous_scenarios(olfhip_context *context, olfhip_limit_table *table, double band, int n_scenarios,
                                int *flags);
+/* The device screen alone, allocating nothing: the same candidates as olfhip_screen_violations (the record order of a
+ * scenario may differ: emission order is not fixed) for the n_scenarios scenarios of the context's last solve
+ * (n_scenarios must be that count), with its working memory (scenario flags, counts, records) in a per-device array
+ * of OLFHIP_NO_ALLOC_SCRATCH_BYTES the library holds from its load to the process's end. No counterpart in the
+ * reference: for callers that cannot allocate device memory (serving loops, harnesses that forbid hipMalloc). It
+ * calls no hipMalloc or hipFree; 2 x ceil256(4 n_scenarios) + 4 n_scenarios x capacity bytes must fit the array,
+ * else OLFHIP_E_ARGUMENT. Calls that overlap on one device share that array and race. */
+#define OLFHIP_NO_ALLOC_SCRATCH_BYTES (32u << 20)
+int olfhip_screen_violations_no_alloc(olfhip_context *context, olfhip_limit_table *table, double band,
+                                      int n_scenarios, int capacity, int *counts, int *records);
 /* Screen plus the exact host decisions: the violations of every converged scenario, ordered by scenario, then
  * branches in table order (a group's first exceeded limit), then buses (high before low), then voltage-angle limits
  * (high before low). Fails with

@@ -8,6 +8,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -952,6 +953,74 @@ static void ambiguousScenarios(ContextImpl *c, LimitTableImpl *t, double band, i
     HIP_OK(hipStreamSynchronize(s));
 }
 
+// ---- the allocation-free screen (olfhip_screen_violations_no_alloc). Its lane flags, candidate counts and candidate
+// records live in `scratch`, a file-scope device array the code object carries (each device that loads the library
+// holds one until the process ends), so a call allocates no device memory: the context and the limit table bring
+// everything else. Every call on one device shares that device's array and leaves its contents behind: the counts are
+// cleared by each call, the records are only read up to their counts. Calls that overlap on one device race.
+static __device__ __align__(256) unsigned char scratch[OLFHIP_NO_ALLOC_SCRATCH_BYTES];
+constexpr int SCRATCH_DEVICES = 64;
+static std::mutex gScratchLock;
+static unsigned char *gScratchOf[SCRATCH_DEVICES] = {};
+
+// The device address of `scratch` on `device`, which must be current: looked up once per device and kept. (In host
+// code the array's name is only a stand-in for the device symbol; kernels and copies take this address.)
+static unsigned char *scratchOnDevice(int device) {
+    int current = -1;
+    HIP_OK(hipGetDevice(&current));
+    if (current != device) throw ArgumentError("the context's device is not the current device");
+    if (device < 0 || device >= SCRATCH_DEVICES) throw ArgumentError("device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(gScratchLock);
+    if (!gScratchOf[device]) {
+        void *p = nullptr;
+        HIP_OK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) throw DeviceError("hipGetSymbolAddress gave no address for the scratch array");
+        gScratchOf[device] = static_cast<unsigned char *>(p);
+    }
+    return gScratchOf[device];
+}
+
+static size_t roundUp256(size_t bytes) { return (bytes + 255) / 256 * 256; }
+
+// screenViolations with its buffers in `scratch` and its outputs copied straight into the caller's arrays:
+// [lane flags K ints][counts K ints][records K x capacity ints], sections on 256-byte boundaries.
+static void screenViolationsNoAlloc(ContextImpl *c, LimitTableImpl *t, double band, int nScenarios, int capacity,
+                                    int *counts, int *records) {
+    if (c->st != t->st) throw ArgumentError("the limit table belongs to another structure");
+    const int K = c->lastInstalled;
+    if (K <= 0 || (int)c->lastStatus.size() != K) throw ArgumentError("no solve to screen");
+    if (nScenarios != K) throw ArgumentError("n_scenarios must be the scenario count of the context's last solve");
+    if (capacity <= 0) throw ArgumentError("capacity must be positive");
+    if (!(band >= 0.0)) throw ArgumentError("band must be non-negative");
+    const size_t flagBytes = roundUp256((size_t)K * 4);
+    const size_t recordBytes = (size_t)K * (size_t)capacity * 4;
+    if (recordBytes / 4 / (size_t)capacity != (size_t)K || 2 * flagBytes + recordBytes > OLFHIP_NO_ALLOC_SCRATCH_BYTES)
+        throw ArgumentError("n_scenarios x capacity records exceed the allocation-free scratch of " +
+                            std::to_string(OLFHIP_NO_ALLOC_SCRATCH_BYTES >> 20) + " MiB (use olfhip_screen_violations)");
+    unsigned char *base = scratchOnDevice(c->st->device);
+    int *lane = reinterpret_cast<int *>(base);
+    int *cnt = reinterpret_cast<int *>(base + flagBytes);
+    int *rec = reinterpret_cast<int *>(base + 2 * flagBytes);
+    hipStream_t s = c->stream;
+    convergedLaneFlagsKernel<<<gridFor(K, 256), 256, 0, s>>>(c->status.as<int>(), K, lane, cnt);
+    const DevPlan dp = c->st->dp;
+    const DevState S = c->devState();
+    const DevLimits L = t->dev();
+    withT(c->T, [&](auto TC) {
+        constexpr int T = decltype(TC)::value;
+        if (t->nBranches > 0)
+            screenBranchesKernel<T><<<gridFor((size_t)K * t->nBranches, 256), 256, 0, s>>>(
+                dp, S, L, c->modeBase.as<int>(), lane, K, band, capacity, rec, cnt);
+        if (t->nBuses > 0)
+            screenBusesKernel<T><<<gridFor((size_t)K * t->nBuses, 256), 256, 0, s>>>(
+                dp, S, L, c->modeBase.as<int>(), lane, K, band, capacity, rec, cnt);
+    });
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(counts, cnt, (size_t)K * 4, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(records, rec, recordBytes, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+}
+
 // Exact FP64 flows of closed branch e from its end states: side 1 (p, q, v) and side 2 (p, q, v).
 static void branchFlows(const Plan &P, int e, double2 x1, double2 x2, double f[6]) {
     const double *c = &P.branchCoef[8 * (size_t)e];
@@ -1561,6 +1630,15 @@ int olfhip_ambiguous_scenarios(olfhip_context *context, olfhip_limit_table *tabl
         if (!context || !table || !flags) throw ArgumentError("null argument");
         ambiguousScenarios(reinterpret_cast<ContextImpl *>(context), reinterpret_cast<LimitTableImpl *>(table), band,
                            n_scenarios, flags);
+    });
+}
+
+int olfhip_screen_violations_no_alloc(olfhip_context *context, olfhip_limit_table *table, double band,
+                                      int n_scenarios, int capacity, int *counts, int *records) {
+    return guarded([&] {
+        if (!context || !table || !counts || !records) throw ArgumentError("null argument");
+        screenViolationsNoAlloc(reinterpret_cast<ContextImpl *>(context), reinterpret_cast<LimitTableImpl *>(table),
+                                band, n_scenarios, capacity, counts, records);
     });
 }
 

@@ -193,6 +193,15 @@ __global__ void gatherNodesKernel(DevPlan P, DevState S, const int *modeBase, in
     }
 }
 
+// Screen flags of the last solve's scenarios from their device statuses (1: converged) and their candidate counts
+// cleared, in one pass.
+__global__ void convergedLaneFlagsKernel(const int *status, int n, int *flag, int *counts) {
+    for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < n; w += gridDim.x * blockDim.x) {
+        flag[w] = status[w] == ST_CONVERGED;
+        counts[w] = 0;
+    }
+}
+
 __global__ void zeroIntKernel(int *p, int n) {
     for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < n; w += gridDim.x * blockDim.x) p[w] = 0;
 }

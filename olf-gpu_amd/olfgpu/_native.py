@@ -30,9 +30,10 @@ EXPORTS = [
     "olfhip_mlp_parameters", "olfhip_mlp_predict_warm_start", "olfhip_mlp_gemm_test",
     "olfhip_create_limit_table", "olfhip_destroy_limit_table", "olfhip_screen_violations", "olfhip_detect_violations",
     "olfhip_set_distributed_slack",
-    "olfhip_set_row_criteria", "olfhip_ambiguous_scenarios",
+    "olfhip_set_row_criteria", "olfhip_screen_violations_no_alloc", "olfhip_ambiguous_scenarios",
     "olfhip_context_graph_statistics",
 ]
+NO_ALLOC_SCRATCH_BYTES = 32 << 20   # OLFHIP_NO_ALLOC_SCRATCH_BYTES
 
 
 class OlfHipError(RuntimeError):
@@ -135,6 +136,7 @@ def lib():
         L.olfhip_destroy_limit_table.argtypes = [vp]
         L.olfhip_destroy_limit_table.restype = None
         L.olfhip_screen_violations.argtypes = [vp, vp, d, i, vp, vp]
+        L.olfhip_screen_violations_no_alloc.argtypes = [vp, vp, d, i, i, vp, vp]
         L.olfhip_ambiguous_scenarios.argtypes = [vp, vp, d, i, vp]
         L.olfhip_detect_violations.argtypes = [vp, vp, d, i, i, vp, C.POINTER(i)]
         _LIB = L
@@ -474,6 +476,15 @@ class LimitTable:
         flags = np.zeros(k, np.int32)
         check(lib().olfhip_ambiguous_scenarios(context.h, self.h, band, k, ptr(flags)))
         return flags.astype(bool)
+
+    def screen_no_alloc(self, context: "BatchContext", k: int, band=1e-7, capacity=1024):
+        """olfhip_screen_violations_no_alloc: the screen's candidates with no device allocation (its working memory
+        in the library's per-device scratch array); same layout as screen()."""
+        counts = np.zeros(k, np.int32)
+        records = np.zeros((k, capacity), np.int32)
+        check(lib().olfhip_screen_violations_no_alloc(context.h, self.h, band, k, capacity, ptr(counts),
+                                                       ptr(records)))
+        return counts, records
 
     def detect(self, context: "BatchContext", band=1e-7, capacity=1024, max_violations=1 << 20):
         """Screen + exact host decisions: a VIOLATION_DTYPE array ordered by scenario, branches, buses."""
