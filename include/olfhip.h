@@ -154,6 +154,12 @@ int olfhip_set_distributed_slack(olfhip_context *context, const olfhip_distribut
 int olfhip_set_row_criteria(olfhip_context *context, const double *fixed_row_max_mismatch,
                             const double *equation_row_max_mismatch);
 
+/* Sparse start-state seeds of the context's next solves (NewtonSettingsNative.setBatchX0Seeds,
+ * native/cuda/jni/newton_settings_jni.cu:43-63): each flat_index = scenario * n + row sets that scenario's start value
+ * of that row over the tiled base start (or its per-scenario start), until the next call; n_seeds 0 clears them.
+ * Indices must be below capacity x n and values finite; a solve refuses seeds of scenarios beyond its n_scenarios. */
+int olfhip_set_x0_seeds(olfhip_context *context, int n_seeds, const long long *flat_index, const double *value);
+
 /* Inputs of one batched solve: the hot-path subset of BatchSolveNative.solveBatchContext
  * (BatchSolveNative.java:106-125), same meaning and order conventions:
  *   x0_base, target_base, base_row_mode: [n] shared by every scenario (tiled on the device);
@@ -218,6 +224,11 @@ typedef struct {
      * first solve, and on the solve before its budget); <= 0: the reference's LIN_SOLVE_FAIL_RATIO 1e-2
      * (native/cuda/solver/iterative_refinement.cuh:23, newton_iteration.cu:612-652) */
     double linear_solve_fail_ratio;
+    /* 0: the solve returns when every scenario of the bank has ended; k > 0: it returns as soon as k scenarios have
+     * ended during the call, the others left solving between two Newton steps, for the caller to install new
+     * scenarios into the slots handed back (olfhip_begin_batch_install) and call again, which goes on with those still
+     * solving (solveBatchContext's pauseAtEnded, BatchSolveNative.java:106-125, newton_iteration.cu:488-493) */
+    int pause_at_ended;
 } olfhip_solve_inputs;
 /* tol <= 0 with conv_eps_per_eq <= 0 and no per-row criterion: the fixed-iteration mode (the reference's tol <= 0,
  * native/cuda/solver/outer_loop_driver.cu:165-175, newton_iteration.cu:141-151,336-343): every scenario runs exactly
@@ -233,6 +244,11 @@ typedef struct {
     int *iterations;
     int *total_iterations;
     double *norm;
+    /* the count of rows the solve wrote: the scenarios it handed back. Needed by a bank solve (after
+     * olfhip_begin_batch_install) and a paused one (pause_at_ended > 0), whose rows are the scenarios that ended and
+     * were not handed back before, in the order they were installed (their slots: olfhip_read_handed_back_slots);
+     * otherwise every installed scenario, n_scenarios rows */
+    int *n_handed_back;
 } olfhip_solve_outputs;
 
 /* Runs the batched Newton-Raphson with per-scenario retirement and the reactive-limit outer loop.
@@ -240,6 +256,22 @@ typedef struct {
  * native/cuda/solver/outer_loop_driver.cu:80-410, native/cuda/solver/newton_iteration.cu:93-203). */
 int olfhip_solve_batch_context(olfhip_context *context, const olfhip_solve_inputs *inputs,
                                olfhip_solve_outputs *outputs);
+
+/* The bank: local scenario j of the next solve (its per-scenario inputs, row_mode_entries, target overrides, start
+ * seeds, outages, seeds, active row counts) goes into bank slot bank_slots[j]; every other slot keeps what it holds,
+ * a scenario still solving after a paused solve going on from where it stopped. Each slot appears at most once and is
+ * below the context's capacity; n_slots 0 (or bank_slots NULL) chooses the whole bank, scenario j into slot j, with
+ * every slot emptied first (the default install). The choice holds for the next solve, whose n_scenarios must be
+ * n_slots. After a bank or paused solve the readbacks (olfhip_read_states, olfhip_read_outer_loop_state, the violation
+ * screens) index scenarios by bank slot, n_scenarios being the capacity; a slot the solve did not hand back reads as
+ * not converged. Reference: BatchSolveNative.beginBatchInstall (BatchSolveNative.java:53-59,
+ * native/cuda/jni/batch_solve_jni.cu:444-463). */
+int olfhip_begin_batch_install(olfhip_context *context, int n_slots, const int *bank_slots);
+
+/* The bank slots of the scenarios the last solve handed back, in the order of its result rows: *n_slots of them
+ * (capacity entries are enough). Reference: BatchSolveNative.readBatchHandedBackSlots (BatchSolveNative.java:98-100,
+ * native/cuda/jni/batch_solve_jni.cu:428-442). */
+int olfhip_read_handed_back_slots(const olfhip_context *context, int *n_slots, int *slots);
 
 /* Device time of the last solve's Newton-step launches, measured with HIP events on the context's stream around each
  * launch (the step kernel, plus the few-microsecond running-tile count and its 4-byte copy), and the launch count.

@@ -282,6 +282,11 @@ static void withT(int T, Fn &&fn) {
 // picks per workload).
 static int defaultLanesPerTile(int nScenarios) { return nScenarios <= 1 ? 1 : 2; }
 
+static int gridFor(size_t work, int block) {
+    size_t g = (work + block - 1) / block;
+    return (int)std::max<size_t>(1, std::min<size_t>(g, 65536));
+}
+
 struct ContextImpl {
     StructureImpl *st = nullptr;
     int capacity = 0, mode = 0, T = 1, NT = 512, nTiles = 0;
@@ -305,6 +310,9 @@ struct ContextImpl {
     DBuf rowThrFixed, rowThrEq;
     // install inputs
     DBuf x0Base, x0Per, targetBase, modeBase, modeEntries, targetIdx, targetVal, seed, states;
+    std::vector<long long> x0SeedIndex;      // olfhip_set_x0_seeds: start-state seeds of the next solves
+    std::vector<double> x0SeedValue;
+    DBuf x0SeedIdx, x0SeedVal;
     int lastInstalled = 0;
     double lastSolveMs = 0.0;
     int lastSteps = 0;
@@ -322,6 +330,18 @@ struct ContextImpl {
     // and its copy to the host (replayed when a launch is capped at k steps, OLFHIP_STEPS_PER_LAUNCH).
     hipGraphExec_t installGraph = nullptr, stepGraph = nullptr;
     int graphInstantiations = 0, graphUpdates = 0;
+    // The bank (BatchSolveNative.beginBatchInstall / pauseAtEnded / readBatchHandedBackSlots): the slots the next
+    // solve installs its scenarios into (bankInstall), which slots hold a scenario not handed back yet (pending), the
+    // slots the last solve handed back in the order of its rows, and per slot its outage list and equation count.
+    bool bankInstall = false;
+    std::vector<int> bankSlots;
+    std::vector<char> pending;
+    std::vector<long long> installSeq;       // per slot, when its scenario was installed (rows go in install order)
+    long long installCounter = 0;
+    std::vector<int> handedBack;
+    std::vector<int> laneOutages, laneActive;
+    DBuf ended, slotOf, handedNow;
+    bool bankReadback = false;               // the last solve's readbacks are by slot (a bank or paused solve)
 
     DevState devState() const {
         DevState S{};
@@ -354,6 +374,7 @@ struct ContextImpl {
         S.rlCount = rlCount.as<signed char>();
         S.rlPass = rlPass.as<signed char>();
         S.tileRunning = tileRunning.as<int>();
+        S.ended = ended.as<int>();
         S.kryV = kryV.as<double>();
         S.kryH = kryH.as<double>();
         S.loopAt = loopAt.as<signed char>();
@@ -419,7 +440,7 @@ static ContextImpl *createContext(StructureImpl *st, int nScenarios, int mode, c
     if (c->NT != 256 && c->NT != 512 && c->NT != 1024) throw ArgumentError("threads_per_tile must be 256, 512 or 1024");
     c->nTiles = (nScenarios + c->T - 1) / c->T;
     HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-    HIP_OK(hipHostMalloc((void **)&c->pinnedCount, sizeof(int), hipHostMallocDefault));
+    HIP_OK(hipHostMalloc((void **)&c->pinnedCount, 2 * sizeof(int), hipHostMallocDefault));
     HIP_OK(hipEventCreate(&c->evStart));
     HIP_OK(hipEventCreate(&c->evStop));
     const Plan &P = st->plan;
@@ -441,6 +462,17 @@ static ContextImpl *createContext(StructureImpl *st, int nScenarios, int mode, c
     c->outages.alloc(lanes * MAX_OUTAGES * 4);
     c->tileRunning.alloc((size_t)c->nTiles * 4);
     c->count.alloc(16);
+    c->ended.alloc(16);
+    c->slotOf.alloc(lanes * 4);
+    c->handedNow.alloc(lanes * 4);
+    c->pending.assign(lanes, 0);
+    c->installSeq.assign(lanes, 0);
+    c->laneOutages.assign(lanes * MAX_OUTAGES, -1);
+    c->laneActive.assign(lanes, P.n);
+    slotsEmptyKernel<<<gridFor(lanes, 256), 256, 0, c->stream>>>(c->status.as<int>(), (int)lanes,
+                                                                 c->tileRunning.as<int>(), c->nTiles);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(c->stream));
     for (DBuf *b : {&c->loopAt, &c->sweepChanged, &c->dsFailed}) b->alloc(lanes);
     c->slackP.alloc(lanes * 8);
     c->dsRetain.alloc(lanes * 8);
@@ -487,17 +519,22 @@ static ContextImpl *createContext(StructureImpl *st, int nScenarios, int mode, c
     return c.release();
 }
 
-static int gridFor(size_t work, int block) {
-    size_t g = (work + block - 1) / block;
-    return (int)std::max<size_t>(1, std::min<size_t>(g, 65536));
-}
-
 static void solve(ContextImpl *c, const olfhip_solve_inputs *in, olfhip_solve_outputs *out) {
     if (!in) throw ArgumentError("inputs are null");
     const Plan &P = c->st->plan;
     const int n = P.n;
     const int K = in->n_scenarios;
-    if (K <= 0 || K > c->capacity) throw ArgumentError("n_scenarios must be in [1, context capacity]");
+    // the slots olfhip_begin_batch_install chose hold for this solve only, refused or not
+    const bool bank = c->bankInstall;
+    std::vector<int> bankSlots;
+    bankSlots.swap(c->bankSlots);
+    c->bankInstall = false;
+    if (bank ? (K != (int)bankSlots.size()) : (K <= 0 || K > c->capacity))
+        throw ArgumentError(bank ? "n_scenarios must be the count of slots olfhip_begin_batch_install chose"
+                                 : "n_scenarios must be in [1, context capacity]");
+    if (in->pause_at_ended < 0) throw ArgumentError("pause_at_ended must be >= 0");
+    if ((bank || in->pause_at_ended > 0) && out && !out->n_handed_back)
+        throw ArgumentError("a bank or paused solve hands back its own count of rows: n_handed_back is needed");
     if (!in->x0_base || !in->target_base || !in->base_row_mode) throw ArgumentError("base state arrays are null");
     if (in->max_iter <= 0) throw ArgumentError("max_iter must be positive");
     if (in->max_outages < 0 || in->max_outages > MAX_OUTAGES)
@@ -522,17 +559,46 @@ static void solve(ContextImpl *c, const olfhip_solve_inputs *in, olfhip_solve_ou
             throw ArgumentError("target_override_index[" + std::to_string(q) + "] out of range");
     hipStream_t s = c->stream;
     auto t0 = std::chrono::steady_clock::now();
-    c->lastInstalled = K;
     const size_t lanes = (size_t)c->nTiles * c->T;
+    // the slots this solve installs into: its bank slots, or scenario k into slot k with every other slot emptied
+    std::vector<int> slots(K);
+    for (int k = 0; k < K; k++) slots[k] = bank ? bankSlots[k] : k;
+    for (size_t q = 0; q < (size_t)K * in->max_outages; q++)
+        if (in->outaged_branches[q] >= P.nBranch) throw ArgumentError("outaged branch id out of range");
+    // a solve refused after this point leaves the bank's host-side slot state as it was
+    struct SlotStateRestore {
+        ContextImpl *c;
+        std::vector<char> pending;
+        std::vector<int> outages, active;
+        std::vector<long long> seq;
+        bool committed = false;
+        ~SlotStateRestore() {
+            if (committed) return;
+            c->pending.swap(pending);
+            c->laneOutages.swap(outages);
+            c->laneActive.swap(active);
+            c->installSeq.swap(seq);
+        }
+    } restore{c, c->pending, c->laneOutages, c->laneActive, c->installSeq};
+    if (!bank) {
+        std::fill(c->pending.begin(), c->pending.end(), 0);
+        std::fill(c->laneOutages.begin(), c->laneOutages.end(), -1);
+        std::fill(c->laneActive.begin(), c->laneActive.end(), n);
+    }
     // outage lists, padded to MAX_OUTAGES
-    std::vector<int> outs(lanes * MAX_OUTAGES, -1);
-    for (int k = 0; k < K; k++)
+    for (int k = 0; k < K; k++) {
+        int *o = &c->laneOutages[(size_t)slots[k] * MAX_OUTAGES];
+        std::fill(o, o + MAX_OUTAGES, -1);
         for (int q = 0; q < in->max_outages; q++) {
             const int b = in->outaged_branches[(size_t)k * in->max_outages + q];
-            if (b >= P.nBranch) throw ArgumentError("outaged branch id out of range");
-            outs[(size_t)k * MAX_OUTAGES + q] = b < 0 ? -1 : b;
+            o[q] = b < 0 ? -1 : b;
         }
+        c->pending[slots[k]] = 1;
+        c->installSeq[slots[k]] = c->installCounter++;
+    }
+    const std::vector<int> &outs = c->laneOutages;
     c->outages.upload(outs, s);
+    if (bank && K > 0) c->slotOf.upload(slots, s);
     c->x0Base.upload(std::vector<double>(in->x0_base, in->x0_base + n), s);
     c->targetBase.upload(std::vector<double>(in->target_base, in->target_base + n), s);
     c->modeBase.upload(std::vector<int>(in->base_row_mode, in->base_row_mode + n), s);
@@ -543,6 +609,14 @@ static void solve(ContextImpl *c, const olfhip_solve_inputs *in, olfhip_solve_ou
         c->x0Per.alloc((size_t)K * n * 8);
         HIP_OK(hipMemcpyAsync(c->x0Per.p, in->x0_per_scenario, (size_t)K * n * 8, hipMemcpyHostToDevice, s));
         x0PerDev = c->x0Per.as<double>();
+    }
+    const int nSeeds = (int)c->x0SeedIndex.size();
+    for (int q = 0; q < nSeeds; q++)
+        if (c->x0SeedIndex[q] >= (long long)K * n)
+            throw ArgumentError("x0 seed " + std::to_string(q) + " names a scenario beyond n_scenarios");
+    if (nSeeds > 0) {
+        c->x0SeedIdx.upload(c->x0SeedIndex, s);
+        c->x0SeedVal.upload(c->x0SeedValue, s);
     }
     if (in->n_row_mode_entries > 0)
         c->modeEntries.upload(std::vector<int>(in->row_mode_entries, in->row_mode_entries + 3 * in->n_row_mode_entries), s);
@@ -566,7 +640,9 @@ static void solve(ContextImpl *c, const olfhip_solve_inputs *in, olfhip_solve_ou
     prm.maxDv = in->max_dv;
     prm.maxDphi = in->max_dphi;
     prm.refine = c->mode == OLFHIP_LU_FP32_REFINED;
-    prm.batchOfOne = K == 1;
+    // a batch of one: the bank holds one scenario (a bank solve), or the solve installs one (newton_iteration.cu:388,
+    // whose nS is the bank size)
+    prm.batchOfOne = bank ? c->capacity == 1 : K == 1;
     prm.rowCriteria = c->rowCriteria ? 1 : 0;
     prm.kryDim = OLFHIP_ROBUST_KRYLOV_DIM;
     prm.kryRestarts = OLFHIP_ROBUST_RESTARTS;
@@ -581,17 +657,16 @@ static void solve(ContextImpl *c, const olfhip_solve_inputs *in, olfhip_solve_ou
     if (in->distributed_slack && !(in->slack_dist_tol >= 0.0)) throw ArgumentError("slack_dist_tol must be >= 0");
     if (!(in->conv_eps_per_eq >= 0.0)) throw ArgumentError("conv_eps_per_eq must be >= 0");
     prm.convEps = in->conv_eps_per_eq;
+    for (int k = 0; k < K; k++) {
+        const int a = in->active_row_counts ? in->active_row_counts[k] : n;
+        if (a <= 0 || a > n) throw ArgumentError("active_row_counts[" + std::to_string(k) + "] out of range");
+        c->laneActive[slots[k]] = a;
+    }
     if (prm.convEps > 0.0) {
-        std::vector<int> na(lanes, n);
-        for (int k = 0; k < K; k++) {
-            if (!in->active_row_counts) break;
-            const int a = in->active_row_counts[k];
-            if (a <= 0 || a > n) throw ArgumentError("active_row_counts[" + std::to_string(k) + "] out of range");
-            na[k] = a;
-        }
-        HIP_OK(hipMemcpyAsync(c->nActive.p, na.data(), lanes * 4, hipMemcpyHostToDevice, s));
+        HIP_OK(hipMemcpyAsync(c->nActive.p, c->laneActive.data(), lanes * 4, hipMemcpyHostToDevice, s));
         HIP_OK(hipStreamSynchronize(s));
     }
+    prm.pauseAt = in->pause_at_ended;
     prm.lsMax = in->line_search_max_iteration > 0 ? in->line_search_max_iteration : 10;
     prm.lsFold = in->line_search_step_fold > 1.0 ? in->line_search_step_fold : 4.0 / 3.0;
     if (prm.scaling == OLFHIP_SCALING_LINE_SEARCH) c->xPrev.alloc((size_t)c->nTiles * c->T * P.nb * 2 * 8);
@@ -614,24 +689,46 @@ static void solve(ContextImpl *c, const olfhip_solve_inputs *in, olfhip_solve_ou
         c->profCap = 0;
         c->prof.release();
     }
+    restore.committed = true;
     const DevPlan dp = c->st->dp;
-    const DevState S = c->devState();
+    DevState S = c->devState();
+    S.nInstalled = bank ? 0 : K;   // the default install's scenario count (slots 0 .. K-1)
     const DevRL R = c->devRL();
     const bool useGraph = !std::getenv("OLFHIP_NO_GRAPH");
     int steps = 0;
     withT(c->T, [&](auto TC) {
         constexpr int T = decltype(TC)::value;
         const size_t work = (size_t)c->nTiles * P.nb * T;
+        const int *slotMap = bank ? c->slotOf.as<int>() : nullptr;
+        const signed char *seedRow =
+            (in->reactive_limits_seed_state && c->nRL > 0) ? c->seed.as<signed char>() : nullptr;
         auto enqueueInstall = [&]() {
-            installKernel<T><<<gridFor(work, 256), 256, 0, s>>>(dp, S, c->x0Base.as<double>(), x0PerDev,
-                                                                c->targetBase.as<double>(), c->modeBase.as<int>());
-            if (in->n_row_mode_entries + in->n_target_overrides > 0)
-                overrideKernel<T><<<gridFor(in->n_row_mode_entries + in->n_target_overrides, 256), 256, 0, s>>>(
-                    dp, S, in->n_row_mode_entries, c->modeEntries.as<int>(), in->n_target_overrides,
-                    c->targetIdx.as<long long>(), c->targetVal.as<double>());
-            laneInitKernel<T><<<gridFor(lanes * (1 + (size_t)c->nRL), 256), 256, 0, s>>>(
-                dp, S, R, (in->reactive_limits_seed_state && c->nRL > 0) ? c->seed.as<signed char>() : nullptr,
-                seedRows == 1 ? 0 : c->nRL, c->modeBase.as<int>());
+            zeroIntKernel<<<1, 64, 0, s>>>(S.ended, 1);
+            if (bank) {
+                if (K > 0)
+                    bankInstallKernel<T><<<gridFor((size_t)K * P.nb, 256), 256, 0, s>>>(
+                        dp, S, K, slotMap, c->x0Base.as<double>(), x0PerDev, c->targetBase.as<double>(),
+                        c->modeBase.as<int>());
+            } else {
+                installKernel<T><<<gridFor(work, 256), 256, 0, s>>>(dp, S, c->x0Base.as<double>(), x0PerDev,
+                                                                    c->targetBase.as<double>(),
+                                                                    c->modeBase.as<int>());
+            }
+            if (in->n_row_mode_entries + in->n_target_overrides + nSeeds > 0)
+                overrideKernel<T><<<gridFor(in->n_row_mode_entries + in->n_target_overrides + nSeeds, 256), 256, 0,
+                                    s>>>(dp, S, in->n_row_mode_entries, c->modeEntries.as<int>(),
+                                         in->n_target_overrides, c->targetIdx.as<long long>(),
+                                         c->targetVal.as<double>(), nSeeds, c->x0SeedIdx.as<long long>(),
+                                         c->x0SeedVal.as<double>(), slotMap);
+            if (bank) {
+                if (K > 0)
+                    bankLaneInitKernel<T><<<gridFor((size_t)K * (1 + (size_t)c->nRL), 256), 256, 0, s>>>(
+                        dp, S, R, K, slotMap, seedRow, seedRows == 1 ? 0 : c->nRL, c->modeBase.as<int>());
+                tileRunningKernel<T><<<gridFor(c->nTiles, 256), 256, 0, s>>>(S);
+            } else {
+                laneInitKernel<T><<<gridFor(lanes * (1 + (size_t)c->nRL), 256), 256, 0, s>>>(
+                    dp, S, R, seedRow, seedRows == 1 ? 0 : c->nRL, c->modeBase.as<int>());
+            }
         };
         // the per-row criterion runs in the kernels' ROWCRIT variants (the same code plus its test)
         auto launchStep = [&](auto rowCrit) {
@@ -649,8 +746,8 @@ static void solve(ContextImpl *c, const olfhip_solve_inputs *in, olfhip_solve_ou
         auto enqueueStep = [&]() {
             if (prm.rowCriteria) launchStep(std::true_type{});
             else launchStep(std::false_type{});
-            countRunningKernel<<<1, 256, 0, s>>>(S.tileRunning, c->nTiles, c->count.as<int>());
-            HIP_OK(hipMemcpyAsync(c->pinnedCount, c->count.p, sizeof(int), hipMemcpyDeviceToHost, s));
+            countRunningKernel<<<1, 256, 0, s>>>(S.tileRunning, c->nTiles, S.ended, c->count.as<int>());
+            HIP_OK(hipMemcpyAsync(c->pinnedCount, c->count.p, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
         };
         // Captures `enqueue` and brings `exec` to it: updated in place when its sequence is the cached graph's
         // (only kernel arguments changed), instantiated again otherwise.
@@ -709,7 +806,7 @@ static void solve(ContextImpl *c, const olfhip_solve_inputs *in, olfhip_solve_ou
             c->lastStepMs += ms;
             if (traceOn())
                 std::fprintf(stderr, "OLFHIP step %d %.3f ms, tiles still running %d\n", steps, ms, *c->pinnedCount);
-            if (*c->pinnedCount == 0) {
+            if (c->pinnedCount[0] == 0 || (prm.pauseAt > 0 && c->pinnedCount[1] >= prm.pauseAt)) {
                 steps++;
                 break;
             }
@@ -737,43 +834,74 @@ static void solve(ContextImpl *c, const olfhip_solve_inputs *in, olfhip_solve_ou
             std::fprintf(stderr, " total=%.1fus\n", (prev - r[0]) * usPerTick);
         }
     }
-    // results
-    std::vector<int> st(K), itv(K), tot(K);
-    std::vector<double> nm(K);
-    HIP_OK(hipMemcpyAsync(st.data(), c->status.p, K * 4, hipMemcpyDeviceToHost, s));
-    HIP_OK(hipMemcpyAsync(itv.data(), c->it.p, K * 4, hipMemcpyDeviceToHost, s));
-    HIP_OK(hipMemcpyAsync(tot.data(), c->total.p, K * 4, hipMemcpyDeviceToHost, s));
-    HIP_OK(hipMemcpyAsync(nm.data(), c->norm.p, K * 8, hipMemcpyDeviceToHost, s));
+    // results: every slot whose scenario has ended and was not handed back yet, in the order the scenarios were
+    // installed (a solve that runs to its end in the default install: scenarios 0 .. K-1 in order)
+    const int L = (int)lanes;
+    std::vector<int> stAll(L), itAll(L), totAll(L);
+    std::vector<double> nmAll(L);
+    HIP_OK(hipMemcpyAsync(stAll.data(), c->status.p, (size_t)L * 4, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(itAll.data(), c->it.p, (size_t)L * 4, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(totAll.data(), c->total.p, (size_t)L * 4, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(nmAll.data(), c->norm.p, (size_t)L * 8, hipMemcpyDeviceToHost, s));
     HIP_OK(hipStreamSynchronize(s));
-    for (int k = 0; k < K; k++)
-        if (st[k] == LANE_RUNNING) st[k] = ST_REJECTED;   // batched-iteration cap (newton_iteration.cu:410-429)
+    const bool paused = prm.pauseAt > 0 && c->pinnedCount[0] > 0 && c->pinnedCount[1] >= prm.pauseAt;
+    c->handedBack.clear();
+    for (int l = 0; l < L; l++) {
+        if (!c->pending[l]) continue;
+        if (stAll[l] == LANE_RUNNING) {
+            if (paused) continue;                 // goes on at the next call
+            stAll[l] = ST_REJECTED;               // batched-iteration cap (newton_iteration.cu:410-429)
+        }
+        c->handedBack.push_back(l);
+        c->pending[l] = 0;
+    }
+    std::sort(c->handedBack.begin(), c->handedBack.end(),
+              [&](int a, int b) { return c->installSeq[a] < c->installSeq[b]; });
+    const int H = (int)c->handedBack.size();
     c->lastSteps = steps;
-    c->lastStatus = st;
+    // the readbacks after the solve (states, outer-loop state, violations): by scenario in the default install,
+    // by slot after a bank or paused solve (a slot not handed back by this solve reads as not converged)
+    c->bankReadback = bank || prm.pauseAt > 0;
+    if (c->bankReadback) {
+        c->lastInstalled = c->capacity;
+        c->lastStatus.assign(c->capacity, LANE_RUNNING);
+        for (int l : c->handedBack) c->lastStatus[l] = stAll[l];
+        std::vector<int> now(lanes, 0);
+        for (int l : c->handedBack) now[l] = 1;
+        c->handedNow.upload(now, s);
+    } else {
+        c->lastInstalled = K;
+        c->lastStatus.assign(stAll.begin(), stAll.begin() + K);
+    }
     c->lastOutages = outs;
-    const bool wantStates = out && out->results && in->states_to_host;
+    const bool wantStates = out && out->results && in->states_to_host && H > 0;
+    const int gathered = H > 0 ? *std::max_element(c->handedBack.begin(), c->handedBack.end()) + 1 : 0;
     std::vector<double> xs;
     if (wantStates) {
-        c->states.alloc((size_t)K * n * 8);
+        c->states.alloc((size_t)gathered * n * 8);
         withT(c->T, [&](auto TC) {
             constexpr int T = decltype(TC)::value;
-            gatherStatesKernel<T><<<gridFor((size_t)K * P.nb, 256), 256, 0, s>>>(dp, S, K, c->states.as<double>());
+            gatherStatesKernel<T><<<gridFor((size_t)gathered * P.nb, 256), 256, 0, s>>>(dp, S, gathered,
+                                                                                       c->states.as<double>());
         });
-        xs.resize((size_t)K * n);
-        HIP_OK(hipMemcpyAsync(xs.data(), c->states.p, (size_t)K * n * 8, hipMemcpyDeviceToHost, s));
+        xs.resize((size_t)gathered * n);
+        HIP_OK(hipMemcpyAsync(xs.data(), c->states.p, (size_t)gathered * n * 8, hipMemcpyDeviceToHost, s));
         HIP_OK(hipStreamSynchronize(s));
     }
     if (out) {
-        for (int k = 0; k < K; k++) {
-            if (out->status) out->status[k] = st[k];
-            if (out->iterations) out->iterations[k] = itv[k];
-            if (out->total_iterations) out->total_iterations[k] = tot[k];
-            if (out->norm) out->norm[k] = nm[k];
+        if (out->n_handed_back) *out->n_handed_back = H;
+        for (int r = 0; r < H; r++) {
+            const int l = c->handedBack[r];
+            if (out->status) out->status[r] = stAll[l];
+            if (out->iterations) out->iterations[r] = itAll[l];
+            if (out->total_iterations) out->total_iterations[r] = totAll[l];
+            if (out->norm) out->norm[r] = nmAll[l];
             if (out->results) {
-                double *row = out->results + (size_t)k * (n + 3);
-                if (wantStates) std::memcpy(row, xs.data() + (size_t)k * n, (size_t)n * 8);
-                row[n] = itv[k];
-                row[n + 1] = nm[k];
-                row[n + 2] = st[k];
+                double *row = out->results + (size_t)r * (n + 3);
+                if (wantStates) std::memcpy(row, xs.data() + (size_t)l * n, (size_t)n * 8);
+                row[n] = itAll[l];
+                row[n + 1] = nmAll[l];
+                row[n + 2] = stAll[l];
             }
         }
     }
@@ -946,7 +1074,8 @@ static void ambiguousScenarios(ContextImpl *c, LimitTableImpl *t, double band, i
         withT(c->T, [&](auto TC) {
             constexpr int T = decltype(TC)::value;
             nearLimitsKernel<T><<<gridFor((size_t)K * (t->nBranches + t->nBuses), 256), 256, 0, s>>>(
-                dp, S, L, c->modeBase.as<int>(), K, band, t->near.as<int>());
+                dp, S, L, c->modeBase.as<int>(), K, band, c->bankReadback ? c->handedNow.as<int>() : nullptr,
+                t->near.as<int>());
         });
     HIP_OK(hipGetLastError());
     HIP_OK(hipMemcpyAsync(flags, t->near.p, (size_t)K * 4, hipMemcpyDeviceToHost, s));
@@ -1002,7 +1131,8 @@ static void screenViolationsNoAlloc(ContextImpl *c, LimitTableImpl *t, double ba
     int *cnt = reinterpret_cast<int *>(base + flagBytes);
     int *rec = reinterpret_cast<int *>(base + 2 * flagBytes);
     hipStream_t s = c->stream;
-    convergedLaneFlagsKernel<<<gridFor(K, 256), 256, 0, s>>>(c->status.as<int>(), K, lane, cnt);
+    convergedLaneFlagsKernel<<<gridFor(K, 256), 256, 0, s>>>(
+        c->status.as<int>(), c->bankReadback ? c->handedNow.as<int>() : nullptr, K, lane, cnt);
     const DevPlan dp = c->st->dp;
     const DevState S = c->devState();
     const DevLimits L = t->dev();
@@ -1479,6 +1609,55 @@ int olfhip_context_timing(const olfhip_context *context, double *step_ms, int *s
     });
 }
 
+int olfhip_set_x0_seeds(olfhip_context *context, int n_seeds, const long long *flat_index, const double *value) {
+    return guarded([&] {
+        if (!context) throw ArgumentError("context is null");
+        ContextImpl *c = reinterpret_cast<ContextImpl *>(context);
+        if (n_seeds < 0 || (n_seeds > 0 && (!flat_index || !value))) throw ArgumentError("bad seed arrays");
+        const long long limit = (long long)c->capacity * c->st->plan.n;
+        for (int q = 0; q < n_seeds; q++) {
+            if (flat_index[q] < 0 || flat_index[q] >= limit) throw ArgumentError("x0 seed index out of range");
+            if (!std::isfinite(value[q])) throw ArgumentError("x0 seed value is not finite");
+        }
+        c->x0SeedIndex.assign(flat_index, flat_index + n_seeds);
+        c->x0SeedValue.assign(value, value + n_seeds);
+    });
+}
+
+int olfhip_begin_batch_install(olfhip_context *context, int n_slots, const int *bank_slots) {
+    return guarded([&] {
+        if (!context) throw ArgumentError("context is null");
+        ContextImpl *c = reinterpret_cast<ContextImpl *>(context);
+        if (n_slots < 0) throw ArgumentError("n_slots must be >= 0");
+        if (n_slots == 0 || !bank_slots) {
+            c->bankInstall = false;   // the whole bank, scenario j into slot j
+            c->bankSlots.clear();
+            return;
+        }
+        std::vector<char> taken(c->capacity, 0);
+        for (int j = 0; j < n_slots; j++) {
+            const int b = bank_slots[j];
+            if (b < 0 || b >= c->capacity || taken[b])
+                throw ArgumentError("bank slot " + std::to_string(b) + " outside the bank of " +
+                                    std::to_string(c->capacity) + " or installed twice");
+            if (c->pending[b])
+                throw ArgumentError("bank slot " + std::to_string(b) + " holds a scenario not handed back yet");
+            taken[b] = 1;
+        }
+        c->bankSlots.assign(bank_slots, bank_slots + n_slots);
+        c->bankInstall = true;
+    });
+}
+
+int olfhip_read_handed_back_slots(const olfhip_context *context, int *n_slots, int *slots) {
+    return guarded([&] {
+        if (!context || !n_slots) throw ArgumentError("null argument");
+        const ContextImpl *c = reinterpret_cast<const ContextImpl *>(context);
+        *n_slots = (int)c->handedBack.size();
+        if (slots) std::copy(c->handedBack.begin(), c->handedBack.end(), slots);
+    });
+}
+
 int olfhip_context_graph_statistics(const olfhip_context *context, int *instantiations, int *updates) {
     return guarded([&] {
         if (!context || !instantiations || !updates) throw ArgumentError("null argument");
@@ -1580,6 +1759,7 @@ int olfhip_autotune(olfhip_structure *structure, int mode, const olfhip_reactive
                 std::unique_ptr<ContextImpl> c(createContext(st, K, mode, reactive_limits, &tu));
                 olfhip_solve_inputs in = *inputs;
                 in.states_to_host = 0;
+                in.pause_at_ended = 0;   // timed to the end
                 solve(c.get(), &in, nullptr);   // warm-up (code objects, caches)
                 auto t0 = std::chrono::steady_clock::now();
                 solve(c.get(), &in, nullptr);

@@ -87,6 +87,7 @@ struct DevState {
     int maxOut;
     signed char *rlType, *rlCount, *rlPass;
     int *tileRunning;
+    int *ended;   // lanes that ended during the current solve call (the pause test)
     // outer-loop sequence per lane: the loop it is at (0 distributed slack, 1 reactive limits) and whether the current
     // sweep changed something; the slack bus's computed P at the lane's last fill; the slack's applied share (NaN
     // before any distribution), whether the last distribution failed, and each participant's target before the
@@ -154,6 +155,7 @@ struct NrParams {
     double kryTol;
     double linFailRatio;  // the linear-solve guard's bound on ||J dx + F||inf / ||F||inf
     int fixedIter;        // exactly maxIter steps: no convergence test, no guard, no outer loop
+    int pauseAt;          // > 0: a tile stops before a step once this many lanes have ended in the call
 };
 
 template <typename VT>
@@ -1338,8 +1340,16 @@ __global__ __launch_bounds__(MAX_THREADS, OLFHIP_STEP_MIN_BLOCKS *MAX_THREADS / 
     const LevelMeta LM = loadLevelMeta(P, top, reinterpret_cast<unsigned char *>(olfhipTopLds), tid, nt);
     const bool dsOn = prm.distributedSlack && R.slackNode >= 0;
     const int slackNode = dsOn ? R.slackNode : -1;
-    __shared__ int sRunning;
+    __shared__ int sRunning, sPause;
+    __shared__ int sWasRunning[T];
     for (int step = 0; step < prm.stepsPerLaunch; step++) {
+    if (prm.pauseAt > 0) {
+        // the bank's pause (BatchSolveNative.solveBatchContext's pauseAtEnded): once enough lanes have ended in this
+        // call, the tile leaves its running lanes as they are, between two steps, for the next call to go on with
+        if (tid == 0) sPause = atomicAdd(S.ended, 0) >= prm.pauseAt;
+        __syncthreads();
+        if (sPause) break;
+    }
     if (tid == 0) {
         sProf = nullptr;
         if (S.prof && tile == S.profTile) {
@@ -1362,6 +1372,7 @@ __global__ __launch_bounds__(MAX_THREADS, OLFHIP_STEP_MIN_BLOCKS *MAX_THREADS / 
         sRatio[tid] = S.ratio[lane];
         sClamp[tid] = 0;
         sFill[tid] = sStatus[tid] == LANE_RUNNING;
+        sWasRunning[tid] = sFill[tid];
         sSolve[tid] = 0;
         sConv[tid] = 0;
         sRl[tid] = 0;
@@ -1783,10 +1794,14 @@ __global__ __launch_bounds__(MAX_THREADS, OLFHIP_STEP_MIN_BLOCKS *MAX_THREADS / 
         S.clamps[lane] += sClamp[tid];
     }
     if (tid == 0) {
-        int any = 0;
-        for (int t = 0; t < T; t++) any |= sStatus[t] == LANE_RUNNING;
+        int any = 0, newlyEnded = 0;
+        for (int t = 0; t < T; t++) {
+            any |= sStatus[t] == LANE_RUNNING;
+            newlyEnded += sWasRunning[t] && sStatus[t] != LANE_RUNNING;
+        }
         S.tileRunning[tile] = any;
         sRunning = any;
+        if (newlyEnded) atomicAdd(S.ended, newlyEnded);
     }
     __syncthreads();
     if (!sRunning) break;
@@ -1819,22 +1834,29 @@ __global__ void installKernel(DevPlan P, DevState S, const double *x0Base, const
     }
 }
 
-// per-scenario (scenario, row, mode) entries and target overrides (scenario * n + row -> value)
+// per-scenario (scenario, row, mode) entries, target overrides (scenario * n + row -> value) and start-state seeds
+// (scenario * n + row -> value, over the tiled or per-scenario start)
 template <int T>
 __global__ void overrideKernel(DevPlan P, DevState S, int nModes, const int *modeEntries, int nTargets,
-                               const long long *targetIndex, const double *targetValue) {
+                               const long long *targetIndex, const double *targetValue, int nSeeds,
+                               const long long *seedIndex, const double *seedValue, const int *slotOf) {
     const int nb = P.nb;
     const size_t n = 2 * (size_t)nb;
-    for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < nModes + nTargets; w += gridDim.x * blockDim.x) {
+    for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < nModes + nTargets + nSeeds;
+         w += gridDim.x * blockDim.x) {
         if (w < nModes) {
-            const int lane = modeEntries[3 * w], row = modeEntries[3 * w + 1];
+            const int lane = slotOf ? slotOf[modeEntries[3 * w]] : modeEntries[3 * w], row = modeEntries[3 * w + 1];
             const int tile = lane / T, t = lane % T;
             S.mode[vIdx<T>(tile, nb, P.pos[row >> 1], t) + (row & 1)] = (signed char)modeEntries[3 * w + 2];
         } else {
-            const long long g = targetIndex[w - nModes];
-            const int lane = (int)(g / (long long)n), row = (int)(g % (long long)n);
+            const bool target = w < nModes + nTargets;
+            const int q = target ? w - nModes : w - nModes - nTargets;
+            const long long g = target ? targetIndex[q] : seedIndex[q];
+            const int local = (int)(g / (long long)n), row = (int)(g % (long long)n);
+            const int lane = slotOf ? slotOf[local] : local;
             const int tile = lane / T, t = lane % T;
-            S.target[vIdx<T>(tile, nb, P.pos[row >> 1], t) + (row & 1)] = targetValue[w - nModes];
+            double *dst = target ? S.target : S.x;
+            dst[vIdx<T>(tile, nb, P.pos[row >> 1], t) + (row & 1)] = target ? targetValue[q] : seedValue[q];
         }
     }
 }
@@ -1886,6 +1908,87 @@ __global__ void laneInitKernel(DevPlan P, DevState S, DevRL R, const signed char
     }
 }
 
+// ---- the bank (BatchSolveNative.beginBatchInstall, BatchSolveNative.java:53-59): a solve installs its scenarios
+// into the slots slotOf[j] only; every other slot keeps what it holds (a scenario still solving goes on).
+template <int T>
+__global__ void bankInstallKernel(DevPlan P, DevState S, int nLocal, const int *slotOf, const double *x0Base,
+                                  const double *x0Per, const double *targetBase, const int *modeBase) {
+    const int nb = P.nb;
+    const size_t n = 2 * (size_t)nb;
+    for (size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x; w < (size_t)nLocal * nb;
+         w += (size_t)gridDim.x * blockDim.x) {
+        const int j = (int)(w / nb), k = (int)(w % nb);
+        const int lane = slotOf[j];
+        const int b = P.perm[k];
+        const size_t vi = vIdx<T>(lane / T, nb, k, lane % T);
+        const double2 xv = x0Per ? make_double2(x0Per[j * n + 2 * b], x0Per[j * n + 2 * b + 1])
+                                 : make_double2(x0Base[2 * b], x0Base[2 * b + 1]);
+        *reinterpret_cast<double2 *>(S.x + vi) = xv;
+        *reinterpret_cast<double2 *>(S.target + vi) = make_double2(targetBase[2 * b], targetBase[2 * b + 1]);
+        S.mode[vi] = (signed char)modeBase[2 * b];
+        S.mode[vi + 1] = (signed char)modeBase[2 * b + 1];
+    }
+}
+
+// the installed slots' counters and reactive-limit entries (laneInitKernel's work for slotOf[j], seed row j)
+template <int T>
+__global__ void bankLaneInitKernel(DevPlan P, DevState S, DevRL R, int nLocal, const int *slotOf,
+                                   const signed char *seed, int seedStride, const int *modeBase) {
+    const int nb = P.nb;
+    const size_t totalE = (size_t)nLocal * R.n;
+    for (size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x; w < (size_t)nLocal + totalE;
+         w += (size_t)gridDim.x * blockDim.x) {
+        if (w < (size_t)nLocal) {
+            const int lane = slotOf[w];
+            S.status[lane] = LANE_RUNNING;
+            S.it[lane] = 0;
+            S.solves[lane] = 0;
+            S.outer[lane] = 0;
+            S.total[lane] = 0;
+            S.clamps[lane] = 0;
+            S.norm[lane] = 0.0;
+            S.ratio[lane] = 0.0;
+            S.loopAt[lane] = 0;
+            S.sweepChanged[lane] = 0;
+            S.dsFailed[lane] = 0;
+            S.dsRetain[lane] = __builtin_nan("");
+            S.lsHas[lane] = 0;
+            S.slackP[lane] = 0.0;
+        } else {
+            const size_t q = w - nLocal;
+            const int j = (int)(q / R.n), e = (int)(q % R.n);
+            const int lane = slotOf[j];
+            const int tile = lane / T, t = lane % T;
+            const size_t ei = ((size_t)tile * R.n + e) * T + t;
+            const int k = R.node[e];
+            const int pRow = 2 * P.perm[k];
+            signed char lt = seed ? seed[(size_t)j * seedStride + e] : QLIM_NONE;
+            if (S.mode[vIdx<T>(tile, nb, k, t)] == MODE_FIXED && modeBase[pRow] != MODE_FIXED) lt = QLIM_FROZEN;
+            S.rlType[ei] = lt;
+            S.rlCount[ei] = 0;
+            S.rlPass[ei] = 0;
+        }
+    }
+}
+
+// every tile with a running lane runs (after a bank install, or to go on after a pause)
+template <int T>
+__global__ void tileRunningKernel(DevState S) {
+    for (int tile = blockIdx.x * blockDim.x + threadIdx.x; tile < S.nTiles; tile += gridDim.x * blockDim.x) {
+        int any = 0;
+        for (int t = 0; t < T; t++) any |= S.status[tile * T + t] == LANE_RUNNING;
+        S.tileRunning[tile] = any;
+    }
+}
+
+// a new context's slots: none solving
+__global__ void slotsEmptyKernel(int *status, int lanes, int *tileRunning, int nTiles) {
+    for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < lanes; w += gridDim.x * blockDim.x) {
+        status[w] = LANE_PADDING;
+        if (w < nTiles) tileRunning[w] = 0;
+    }
+}
+
 // states back in stable-row order, [lane][n]
 template <int T>
 __global__ void gatherStatesKernel(DevPlan P, DevState S, int nLanes, double *out) {
@@ -1929,7 +2032,8 @@ __global__ void gatherOuterStateKernel(DevPlan P, DevState S, DevRL R, int nLane
 }
 
 // running lanes after a step: sum over tiles into *count (one block)
-__global__ void countRunningKernel(const int *tileRunning, int nTiles, int *count) {
+// count[0]: tiles still running; count[1]: lanes that ended during the call (the pause test)
+__global__ void countRunningKernel(const int *tileRunning, int nTiles, const int *ended, int *count) {
     __shared__ int acc[256];
     int s = 0;
     for (int i = threadIdx.x; i < nTiles; i += blockDim.x) s += tileRunning[i] != 0;
@@ -1939,7 +2043,10 @@ __global__ void countRunningKernel(const int *tileRunning, int nTiles, int *coun
         if ((int)threadIdx.x < stride) acc[threadIdx.x] += acc[threadIdx.x + stride];
         __syncthreads();
     }
-    if (threadIdx.x == 0) *count = acc[0];
+    if (threadIdx.x == 0) {
+        count[0] = acc[0];
+        count[1] = *ended;
+    }
 }
 
 // standalone batched LU: factor + forward + backward per tile, rhs/solution in the node-ordered vector layout

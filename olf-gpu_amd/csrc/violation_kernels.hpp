@@ -141,16 +141,18 @@ __global__ void screenBusesKernel(DevPlan P, DevState S, DevLimits L, const int 
 // scenario gets flag 1 when a monitored quantity lies within band x |threshold| of one of its group's thresholds up to
 // the first one it exceeds beyond the band (a later threshold cannot be reported), or a bus voltage within band x
 // |limit| of its high or low limit, unless the bus holds its voltage (its V row an identity: PV, slack or islanded
-// buses, whose voltage is the setpoint whatever the start). One thread per (scenario, table branch or bus).
+// buses, whose voltage is the setpoint whatever the start). only: NULL, or per slot 1 for the scenarios the last solve
+// handed back (a bank solve, whose other slots hold scenarios ended earlier or still solving). One thread per
+// (scenario, table branch or bus).
 template <int T>
 __global__ void nearLimitsKernel(DevPlan P, DevState S, DevLimits L, const int *modeBase, int nScenarios, double band,
-                                 int *flag) {
+                                 const int *only, int *flag) {
     const int items = L.nBranches + L.nBuses;
     const size_t total = (size_t)nScenarios * items;
     for (size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x; w < total; w += (size_t)gridDim.x * blockDim.x) {
         const int item = (int)(w % items);
         const int s = (int)(w / items);
-        if (S.status[s] != ST_CONVERGED) continue;
+        if (S.status[s] != ST_CONVERGED || (only && !only[s])) continue;
         const int tile = s / T, t = s % T;
         bool near = false;
         if (item < L.nBranches) {
@@ -193,11 +195,11 @@ __global__ void gatherNodesKernel(DevPlan P, DevState S, const int *modeBase, in
     }
 }
 
-// Screen flags of the last solve's scenarios from their device statuses (1: converged) and their candidate counts
-// cleared, in one pass.
-__global__ void convergedLaneFlagsKernel(const int *status, int n, int *flag, int *counts) {
+// Screen flags of the last solve's scenarios from their device statuses (1: converged, and handed back by that solve
+// when `only` is not NULL) and their candidate counts cleared, in one pass.
+__global__ void convergedLaneFlagsKernel(const int *status, const int *only, int n, int *flag, int *counts) {
     for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < n; w += gridDim.x * blockDim.x) {
-        flag[w] = status[w] == ST_CONVERGED;
+        flag[w] = status[w] == ST_CONVERGED && (!only || only[w]);
         counts[w] = 0;
     }
 }

@@ -7,8 +7,8 @@
  * One JNI function per Java native, each one call into the library:
  *   StructureNative.createStructure / destroyStructure / recommendBatchSize
  *     (src/main/java/com/powsybl/openloadflow/ac/equations/vector/gpu/StructureNative.java:49-81)
- *   BatchSolveNative.createBatchContext / destroyBatchContext / solveBatchContext
- *     (src/main/java/com/powsybl/openloadflow/ac/solver/gpu/BatchSolveNative.java:41-125)
+ *   BatchSolveNative.createBatchContext / destroyBatchContext / beginBatchInstall / solveBatchContext /
+ *     readBatchHandedBackSlots (src/main/java/com/powsybl/openloadflow/ac/solver/gpu/BatchSolveNative.java:41-125)
  *
  * Java arrays are borrowed read-only (released with JNI_ABORT, never written back, as the reference's
  * jni_common.cuh:93-121); results go into fresh Java arrays. A library error becomes the exception class the reference's
@@ -342,16 +342,16 @@ JNIEXPORT jobjectArray JNICALL Java_com_powsybl_openloadflow_ac_solver_gpu_Batch
         refuseNonEmpty(env, zeroImpedanceBranchCloseScenarioElement, "zero-impedance branch switching") ||
         refuseNonEmpty(env, zeroImpedanceBranchOpenScenarioElement, "zero-impedance branch switching"))
         return NULL;
-    if (nBaseClosedZeroImpedanceBranches != 0 || pauseAtEnded != 0) {
+    if (nBaseClosedZeroImpedanceBranches != 0) {
         throwClass(env, "java/lang/UnsupportedOperationException",
-                   nBaseClosedZeroImpedanceBranches != 0 ? "zero-impedance branches: not modelled by the MI355X library"
-                                                         : "pauseAtEnded: the bank refill goes through the library's "
-                                                           "olfhip_solve_inputs.pause_at_ended");
+                   "zero-impedance branches: not modelled by the MI355X library");
         return NULL;
     }
     const int n = h->n;
+    /* the scenarios this call installs (into the slots beginBatchInstall chose, or the whole bank); it hands back
+     * at most the bank's capacity of rows */
     const int nS = (int)lengthOf(env, disabledElementMask);
-    if (nS <= 0 || nS > h->capacity || lengthOf(env, x0Base) != n || lengthOf(env, targetBase) != n ||
+    if (nS > h->capacity || lengthOf(env, x0Base) != n || lengthOf(env, targetBase) != n ||
         lengthOf(env, baseRowMode) != n) {
         throwClass(env, "java/lang/IllegalArgumentException", "solveBatchContext: array sizes do not match the context");
         return NULL;
@@ -362,12 +362,12 @@ JNIEXPORT jobjectArray JNICALL Java_com_powsybl_openloadflow_ac_solver_gpu_Batch
     int *ovIdx = ints(env, targetOverrideIndex, &ok);
     double *ovVal = doubles(env, targetOverrideValue, &ok), *x0Per = doubles(env, x0PerScenario, &ok);
     int *seed = ints(env, reactiveLimitsSeedState, &ok);
-    int *outaged = (int *)malloc(sizeof(int) * 4 * (size_t)nS);
+    int *outaged = (int *)malloc(sizeof(int) * 4 * (size_t)(nS > 0 ? nS : 1));
     const int nOv = (int)lengthOf(env, targetOverrideIndex);
     long long *ovIndex = nOv ? (long long *)malloc(sizeof(long long) * (size_t)nOv) : NULL;
     const int nSeed = (int)lengthOf(env, reactiveLimitsSeedState);
     signed char *seedType = nSeed ? (signed char *)malloc((size_t)nSeed) : NULL;
-    double *res = (double *)malloc(sizeof(double) * (size_t)nS * (size_t)(n + 3));
+    double *res = (double *)malloc(sizeof(double) * (size_t)h->capacity * (size_t)(n + 3));
     jobjectArray rows = NULL;
     int rc = OLFHIP_OK;
     if (ok && (!outaged || !res || (nOv && !ovIndex) || (nSeed && !seedType))) {
@@ -427,15 +427,18 @@ JNIEXPORT jobjectArray JNICALL Java_com_powsybl_openloadflow_ac_solver_gpu_Batch
         in.line_search_max_iteration = 0;       /* NewtonSettingsNative.setBatchLineSearch's defaults */
         in.line_search_step_fold = 0.0;
         in.conv_eps_per_eq = convEpsPerEq;
-        olfhip_solve_outputs out = {res, NULL, NULL, NULL, NULL};
+        in.pause_at_ended = pauseAtEnded;
+        int handedBack = 0;
+        olfhip_solve_outputs out = {res, NULL, NULL, NULL, NULL, &handedBack};
         rc = olfhip_solve_batch_context(h->context, &in, &out);
         if (rc != OLFHIP_OK) {
             throwFor(env, rc);
         } else {
-            /* double[nS][n + 3]: [x(n), iterations, ||F||inf, status] per scenario, in install order */
+            /* double[handed back][n + 3]: [x(n), iterations, ||F||inf, status] per scenario the solve handed back, in
+             * install order (every installed scenario when the solve ran to its end on the whole bank) */
             jclass rowClass = (*env)->FindClass(env, "[D");
-            rows = rowClass ? (*env)->NewObjectArray(env, nS, rowClass, NULL) : NULL;
-            for (int s = 0; rows && s < nS; s++) {
+            rows = rowClass ? (*env)->NewObjectArray(env, handedBack, rowClass, NULL) : NULL;
+            for (int s = 0; rows && s < handedBack; s++) {
                 jdoubleArray r = (*env)->NewDoubleArray(env, n + 3);
                 if (!r) { rows = NULL; break; }   /* OutOfMemoryError pending */
                 (*env)->SetDoubleArrayRegion(env, r, 0, n + 3, res + (size_t)s * (size_t)(n + 3));
@@ -447,4 +450,37 @@ JNIEXPORT jobjectArray JNICALL Java_com_powsybl_openloadflow_ac_solver_gpu_Batch
     free(x0); free(target); free(mode); free(entries); free(ovIdx); free(ovVal); free(x0Per); free(seed);
     free(outaged); free(ovIndex); free(seedType); free(res);
     return rows;
+}
+
+JNIEXPORT void JNICALL Java_com_powsybl_openloadflow_ac_solver_gpu_BatchSolveNative_beginBatchInstall(
+    JNIEnv *env, jclass cls, jlong handle, jintArray bankSlots) {
+    (void)cls;
+    jni_context *h = (jni_context *)(intptr_t)handle;
+    if (!h) return;
+    int ok = 1;
+    int *slots = ints(env, bankSlots, &ok);
+    if (!ok) return;
+    const int rc = olfhip_begin_batch_install(h->context, (int)lengthOf(env, bankSlots), slots);
+    free(slots);
+    if (rc != OLFHIP_OK) throwFor(env, rc);
+}
+
+JNIEXPORT jintArray JNICALL Java_com_powsybl_openloadflow_ac_solver_gpu_BatchSolveNative_readBatchHandedBackSlots(
+    JNIEnv *env, jclass cls, jlong handle) {
+    (void)cls;
+    jni_context *h = (jni_context *)(intptr_t)handle;
+    if (!h) return NULL;
+    int *slots = (int *)malloc(sizeof(int) * (size_t)h->capacity);
+    if (!slots) { throwClass(env, "java/lang/OutOfMemoryError", "olfgpu JNI"); return NULL; }
+    int count = 0;
+    const int rc = olfhip_read_handed_back_slots(h->context, &count, slots);
+    jintArray arr = NULL;
+    if (rc != OLFHIP_OK) {
+        throwFor(env, rc);
+    } else {
+        arr = (*env)->NewIntArray(env, count);
+        if (arr) (*env)->SetIntArrayRegion(env, arr, 0, count, (const jint *)slots);
+    }
+    free(slots);
+    return arr;
 }

@@ -31,7 +31,8 @@ EXPORTS = [
     "olfhip_create_limit_table", "olfhip_destroy_limit_table", "olfhip_screen_violations", "olfhip_detect_violations",
     "olfhip_set_distributed_slack",
     "olfhip_set_row_criteria", "olfhip_screen_violations_no_alloc", "olfhip_ambiguous_scenarios",
-    "olfhip_context_graph_statistics",
+    "olfhip_context_graph_statistics", "olfhip_set_x0_seeds", "olfhip_begin_batch_install",
+    "olfhip_read_handed_back_slots",
 ]
 NO_ALLOC_SCRATCH_BYTES = 32 << 20   # OLFHIP_NO_ALLOC_SCRATCH_BYTES
 
@@ -67,7 +68,7 @@ class SolveInputs(C.Structure):
                 ("reject_slack_distribution_failure", C.c_int), ("line_search_max_iteration", C.c_int),
                 ("line_search_step_fold", C.c_double), ("conv_eps_per_eq", C.c_double),
                 ("active_row_counts", C.c_void_p), ("reactive_limits_seed_shared", C.c_int),
-                ("linear_solve_fail_ratio", C.c_double)]
+                ("linear_solve_fail_ratio", C.c_double), ("pause_at_ended", C.c_int)]
 
 
 class DistributedSlackDesc(C.Structure):
@@ -78,7 +79,7 @@ class DistributedSlackDesc(C.Structure):
 
 class SolveOutputs(C.Structure):
     _fields_ = [("results", C.c_void_p), ("status", C.c_void_p), ("iterations", C.c_void_p),
-                ("total_iterations", C.c_void_p), ("norm", C.c_void_p)]
+                ("total_iterations", C.c_void_p), ("norm", C.c_void_p), ("n_handed_back", C.c_void_p)]
 
 
 _LIB = None
@@ -125,6 +126,9 @@ def lib():
         L.olfhip_read_states.argtypes = [vp, i, vp]
         L.olfhip_context_timing.argtypes = [vp, C.POINTER(d), C.POINTER(i)]
         L.olfhip_context_graph_statistics.argtypes = [vp, C.POINTER(i), C.POINTER(i)]
+        L.olfhip_set_x0_seeds.argtypes = [vp, i, vp, vp]
+        L.olfhip_begin_batch_install.argtypes = [vp, i, vp]
+        L.olfhip_read_handed_back_slots.argtypes = [vp, C.POINTER(i), vp]
         L.olfhip_set_distributed_slack.argtypes = [vp, C.c_void_p]
         L.olfhip_set_row_criteria.argtypes = [vp, C.c_void_p, C.c_void_p]
         L.olfhip_read_outer_loop_state.argtypes = [vp, i, vp, vp, vp]
@@ -299,7 +303,7 @@ class BatchContext:
                     states_to_host=True, x0_per_scenario_device=None, distributed_slack=False, slack_dist_tol=0.0,
                     reject_slack_distribution_failure=False, line_search_max_iteration=10,
                     line_search_step_fold=4.0 / 3.0, conv_eps_per_eq=0.0, active_row_counts=None,
-                    linear_solve_fail_ratio=0.0):
+                    linear_solve_fail_ratio=0.0, pause_at_ended=0):
         keep = []
 
         def arr(a, dt):
@@ -343,7 +347,7 @@ class BatchContext:
                          max_pq_pv_switch, int(skip_reactive_limits), int(states_to_host), x0_per_scenario_device,
                          int(distributed_slack), float(slack_dist_tol), int(reject_slack_distribution_failure),
                          int(line_search_max_iteration), float(line_search_step_fold), float(conv_eps_per_eq), ptr(ac),
-                         shared, float(linear_solve_fail_ratio))
+                         shared, float(linear_solve_fail_ratio), int(pause_at_ended))
         return si, keep
 
     def set_row_criteria(self, thresholds):
@@ -373,19 +377,50 @@ class BatchContext:
         check(lib().olfhip_set_distributed_slack(self.h, C.byref(d)))
         self._ds_keep = keep
 
+    def begin_install(self, bank_slots=None):
+        """olfhip_begin_batch_install: the next solve's scenario j goes into bank slot bank_slots[j], the other slots
+        keeping what they hold (None: the whole bank, scenario j into slot j)."""
+        if bank_slots is None or len(bank_slots) == 0:
+            check(lib().olfhip_begin_batch_install(self.h, 0, None))
+            self._bank = False
+            return
+        a = np.ascontiguousarray(bank_slots, np.int32)
+        check(lib().olfhip_begin_batch_install(self.h, a.size, ptr(a)))
+        self._bank = True
+
+    def read_handed_back_slots(self):
+        """olfhip_read_handed_back_slots: the bank slots of the last solve's rows."""
+        n = C.c_int()
+        out = np.zeros(self.capacity, np.int32)
+        check(lib().olfhip_read_handed_back_slots(self.h, C.byref(n), ptr(out)))
+        return out[:n.value].copy()
+
     def solve(self, inputs: SolveInputs, want_states=True):
+        """One solve. A bank solve (after begin_install with slots) or a paused one (pause_at_ended > 0) returns the
+        scenarios it handed back, in install order, with their bank slots under "slots"; the readbacks after it index
+        scenarios by slot (n_scenarios = capacity)."""
         k = inputs.n_scenarios
         n = self.structure.n
-        res = np.zeros((k, n + 3)) if want_states else None
-        st = np.zeros(k, np.int32)
-        it = np.zeros(k, np.int32)
-        tot = np.zeros(k, np.int32)
-        nm = np.zeros(k)
-        out = SolveOutputs(ptr(res), ptr(st), ptr(it), ptr(tot), ptr(nm))
-        check(lib().olfhip_solve_batch_context(self.h, C.byref(inputs), C.byref(out)))
-        self.last_n_scenarios = k
-        return dict(x=None if res is None else res[:, :n], results=res, status=st, iterations=it,
-                    total_iterations=tot, norm=nm)
+        by_slot = getattr(self, "_bank", False) or inputs.pause_at_ended > 0
+        rows = self.capacity if by_slot else k
+        res = np.zeros((rows, n + 3)) if want_states else None
+        st = np.zeros(rows, np.int32)
+        it = np.zeros(rows, np.int32)
+        tot = np.zeros(rows, np.int32)
+        nm = np.zeros(rows)
+        nh = np.zeros(1, np.int32)
+        out = SolveOutputs(ptr(res), ptr(st), ptr(it), ptr(tot), ptr(nm), ptr(nh))
+        try:
+            check(lib().olfhip_solve_batch_context(self.h, C.byref(inputs), C.byref(out)))
+        finally:
+            self._bank = False   # the chosen slots hold for one solve
+        h = int(nh[0])
+        self.last_n_scenarios = self.capacity if by_slot else k
+        r = dict(x=None if res is None else res[:h, :n], results=None if res is None else res[:h], status=st[:h],
+                 iterations=it[:h], total_iterations=tot[:h], norm=nm[:h])
+        if by_slot:
+            r["slots"] = self.read_handed_back_slots()
+        return r
 
     def timing(self):
         """(event-timed Newton-step milliseconds, step launches) of the last solve."""
@@ -393,6 +428,18 @@ class BatchContext:
         steps = C.c_int()
         check(lib().olfhip_context_timing(self.h, C.byref(ms), C.byref(steps)))
         return ms.value, steps.value
+
+    def set_x0_seeds(self, flat_index=None, value=None):
+        """olfhip_set_x0_seeds: start values scenario * n + row -> value for the next solves (None clears)."""
+        if flat_index is None or len(flat_index) == 0:
+            check(lib().olfhip_set_x0_seeds(self.h, 0, None, None))
+            self._seed_keep = None
+            return
+        keep = (np.ascontiguousarray(flat_index, np.int64), np.ascontiguousarray(value, np.float64))
+        if keep[0].shape != keep[1].shape:
+            raise ValueError("one value per seed index")
+        check(lib().olfhip_set_x0_seeds(self.h, keep[0].size, ptr(keep[0]), ptr(keep[1])))
+        self._seed_keep = keep
 
     def graph_statistics(self):
         """(instantiations, in-place updates) of the context's solve graphs so far."""

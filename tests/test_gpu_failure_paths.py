@@ -117,3 +117,47 @@ def test_autotune_picks_a_valid_shape(g2000):
     r = run(g2000, sa, ctg)
     assert np.array_equal(r.status, ref.status)
     assert np.abs(r.x - ref.x).max() < 1e-9
+
+
+def test_sparse_start_seeds_equal_full_starts(g2000):
+    """olfhip_set_x0_seeds (NewtonSettingsNative.setBatchX0Seeds): start values of chosen (scenario, row) pairs over
+    the tiled base start give bitwise the solve of the same starts passed in full (x0_per_scenario), and the oracle's
+    solve from those starts to 1e-6; clearing the seeds gives back the base-start solve."""
+    net, osys, st, (xb, mb, tb, ltb) = g2000
+    print("[seeds] grid2000 24 contingencies, sparse start seeds", flush=True)
+    ctg = gridgen.sample_contingencies(net, 24, 0.0, 8)
+    n = net.n
+    rng = np.random.default_rng(8)
+    idx, val = [], []
+    x0p = np.tile(xb, (len(ctg), 1))
+    for s_, e in enumerate(ctg):
+        for b in (net.bus1[e], net.bus2[e]):
+            for r, d in ((2 * b, rng.uniform(-0.01, 0.01)), (2 * b + 1, rng.uniform(-0.005, 0.005))):
+                idx.append(s_ * n + r)
+                val.append(xb[r] + d)
+                x0p[s_, r] = xb[r] + d
+    ctx = N.BatchContext(st, len(ctg), N.LU_FP64, net.reactive_limits())
+    inp, keep = N.BatchContext.make_inputs(len(ctg), xb, tb, mb, outaged_branches=ctg, reactive_limits_seed_state=ltb)
+    base = ctx.solve(inp)
+    ctx.set_x0_seeds(np.array(idx), np.array(val))
+    seeded = ctx.solve(inp)
+    ctx.set_x0_seeds(None)
+    again = ctx.solve(inp)
+    full, keep2 = N.BatchContext.make_inputs(len(ctg), xb, tb, mb, outaged_branches=ctg, reactive_limits_seed_state=ltb,
+                                             x0_per_scenario=x0p)
+    ref = ctx.solve(full)
+    np.testing.assert_array_equal(seeded["x"], ref["x"])
+    assert np.array_equal(seeded["total_iterations"], ref["total_iterations"])
+    np.testing.assert_array_equal(again["x"], base["x"])
+    assert np.abs(seeded["x"] - base["x"]).max() > 0
+    orc = osys.n1(xb, mb, tb, ltb, ctg, x0=x0p)
+    assert np.array_equal(orc["status"], seeded["status"])
+    assert np.abs(orc["x"] - seeded["x"]).max() <= 1e-6
+    # a seed of a scenario beyond the solve's count is refused
+    with pytest.raises(N.OlfHipError):
+        ctx.set_x0_seeds(np.array([len(ctg) * n]), np.array([1.0]))   # beyond the context's capacity
+    ctx.set_x0_seeds(np.array([(len(ctg) - 1) * n]), np.array([0.0]))
+    small, keep3 = N.BatchContext.make_inputs(len(ctg) - 1, xb, tb, mb, outaged_branches=ctg[:-1],
+                                              reactive_limits_seed_state=ltb)
+    with pytest.raises(N.OlfHipError):
+        ctx.solve(small)

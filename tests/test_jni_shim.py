@@ -18,6 +18,8 @@ EXPECTED = {
     "Java_com_powsybl_openloadflow_ac_solver_gpu_BatchSolveNative_createBatchContext",
     "Java_com_powsybl_openloadflow_ac_solver_gpu_BatchSolveNative_destroyBatchContext",
     "Java_com_powsybl_openloadflow_ac_solver_gpu_BatchSolveNative_solveBatchContext",
+    "Java_com_powsybl_openloadflow_ac_solver_gpu_BatchSolveNative_beginBatchInstall",
+    "Java_com_powsybl_openloadflow_ac_solver_gpu_BatchSolveNative_readBatchHandedBackSlots",
 }
 
 
