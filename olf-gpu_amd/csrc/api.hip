@@ -435,9 +435,12 @@ static ContextImpl *createContext(StructureImpl *st, int nScenarios, int mode, c
     c->mode = mode;
     c->T = mode == OLFHIP_LU_ROBUST ? 1
            : (tuning && tuning->lanes_per_tile > 0) ? tuning->lanes_per_tile : defaultLanesPerTile(nScenarios);
-    c->NT = (tuning && tuning->threads_per_tile > 0) ? tuning->threads_per_tile : 1024;
+    c->NT = (tuning && tuning->threads_per_tile > 0) ? tuning->threads_per_tile : MAX_THREADS;
     withT(c->T, [](auto) {});
     if (c->NT != 256 && c->NT != 512 && c->NT != 1024) throw ArgumentError("threads_per_tile must be 256, 512 or 1024");
+    // the step kernel's register budget is fixed for at most MAX_THREADS threads: a larger launch would not fit a CU
+    if (c->NT > MAX_THREADS)
+        throw ArgumentError("threads_per_tile above this build's maximum of " + std::to_string(MAX_THREADS));
     c->nTiles = (nScenarios + c->T - 1) / c->T;
     HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     HIP_OK(hipHostMalloc((void **)&c->pinnedCount, 2 * sizeof(int), hipHostMallocDefault));
@@ -1755,6 +1758,7 @@ int olfhip_autotune(olfhip_structure *structure, int mode, const olfhip_reactive
         for (int T : {1, 2, 4, 8, 16, 32}) {
             if (T > 1 && (T > K || mode == OLFHIP_LU_ROBUST)) break;   // the robust mode has one lane per tile
             for (int NT : {256, 512, 1024}) {
+                if (NT > MAX_THREADS) continue;
                 olfhip_tuning tu{T, NT};
                 std::unique_ptr<ContextImpl> c(createContext(st, K, mode, reactive_limits, &tu));
                 olfhip_solve_inputs in = *inputs;

@@ -57,55 +57,102 @@ __device__ __forceinline__ float fromBf16(unsigned short h) { return __uint_as_f
 // ---- GEMM: C[M x N] (+)= A[M x K] B[K x N], BF16 operands with arbitrary strides, FP32 accumulation on
 // v_mfma_f32_16x16x16bf16_1k. A(m,k) = A[m*sam + k*sak], B(k,n) = B[k*sbk + n*sbn], C row-major [M][ldc].
 // A 256-thread workgroup computes a 64x64 tile (each wave 32x32 = 2x2 MFMA tiles) over K range
-// [kz*kChunk, (kz+1)*kChunk); with several K chunks the partial tiles are added atomically (C zeroed first).
-constexpr int GT = 64, GK = 16;
+// [kz*kChunk, (kz+1)*kChunk), 64 of K per round; with several K chunks the partial tiles are added atomically (C
+// zeroed first). Both operands are staged k-contiguous in LDS ([m][k] and [n][k]) so that an MFMA operand (4
+// consecutive k of one row or column) is one 8-byte LDS read; a thread brings 8 consecutive elements along the
+// operand's contiguous dimension with one 16-byte global load where the run is aligned and inside the matrix, and the
+// next round's loads are in flight while the current round's MFMAs run (two LDS buffers, one barrier per round).
+constexpr int GT = 64, GK = 64, GKP = GK + 8;   // GKP: padded LDS row (144 bytes, 16-byte aligned)
+
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+
+// One operand's 64 x 64 slice, loaded as 2 runs of 8 per thread. The slice is X(r, k) for r in [r0, r0 + 64),
+// k in [k0, k0 + 64) with X(r, k) = X[r * sr + k * sk]; kc: the run direction is k (sk == 1) or r (sr == 1).
+struct Runs {
+    u16x8 v[2];
+};
+__device__ __forceinline__ void loadRuns(Runs &o, const unsigned short *__restrict__ X, long sr, long sk, int r0,
+                                         int rEnd, int k0, int kEnd, int tid) {
+    const bool kRun = sk == 1;   // else sr == 1, or neither (then element by element)
+    for (int h = 0; h < 2; h++) {
+        const int e = tid + 256 * h;   // 0..511
+        const int a = e >> 3, b = (e & 7) * 8;   // run a (of 64), offset b within it
+        const int r = kRun ? r0 + a : r0 + b, k = kRun ? k0 + b : k0 + a;
+        const long base = (long)r * sr + (long)k * sk;
+        const bool whole = kRun ? (r < rEnd && k + 7 < kEnd) : (sr == 1 && k < kEnd && r + 7 < rEnd);
+        if (whole && ((reinterpret_cast<uintptr_t>(X + base) & 15) == 0)) {
+            o.v[h] = *reinterpret_cast<const u16x8 *>(X + base);
+        } else {
+            for (int q = 0; q < 8; q++) {
+                const int rr = kRun ? r : r + q, kk = kRun ? k + q : k;
+                o.v[h][q] = (rr < rEnd && kk < kEnd) ? X[(long)rr * sr + (long)kk * sk] : (unsigned short)0;
+            }
+        }
+    }
+}
+// the runs into the LDS slice S[r][k]
+__device__ __forceinline__ void storeRuns(unsigned short (*S)[GKP], const Runs &o, bool kRun, int tid) {
+    for (int h = 0; h < 2; h++) {
+        const int e = tid + 256 * h;
+        const int a = e >> 3, b = (e & 7) * 8;
+        if (kRun) {
+            *reinterpret_cast<u16x8 *>(&S[a][b]) = o.v[h];
+        } else {
+            for (int q = 0; q < 8; q++) S[b + q][a] = o.v[h][q];
+        }
+    }
+}
 
 __global__ __launch_bounds__(256) void gemmBf16(int M, int N, int K, const unsigned short *__restrict__ A, long sam,
                                                 long sak, const unsigned short *__restrict__ B, long sbk, long sbn,
                                                 float *__restrict__ C, long ldc, int kChunk, int atomic) {
-    __shared__ unsigned short As[GT][GK + 4];
-    __shared__ unsigned short Bs[GK][GT + 4];
+    __shared__ unsigned short As[2][GT][GKP];   // [m][k]
+    __shared__ unsigned short Bs[2][GT][GKP];   // [n][k]
     const int tid = threadIdx.x;
     const int wave = tid / 64, lane = tid % 64;
     const int m0 = blockIdx.y * GT, n0 = blockIdx.x * GT;
     const int kBeg = blockIdx.z * kChunk;
     const int kEnd = min(K, kBeg + kChunk);
+    const bool aK = sak == 1, bK = sbk == 1;
     floatx4 acc[2][2];
     for (int i = 0; i < 2; i++)
         for (int j = 0; j < 2; j++) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
     const int wm = (wave / 2) * 32, wn = (wave % 2) * 32;
+    const int r = lane % 16, g = lane / 16;
+    Runs ra, rb;
+    // B as the [n][k] operand: X(n, k) = B[n * sbn + k * sbk]
+    loadRuns(ra, A, sam, sak, m0, M, kBeg, kEnd, tid);
+    loadRuns(rb, B, sbn, sbk, n0, N, kBeg, kEnd, tid);
+    storeRuns(As[0], ra, aK, tid);
+    storeRuns(Bs[0], rb, bK, tid);
+    __syncthreads();
+    int buf = 0;
     for (int k0 = kBeg; k0 < kEnd; k0 += GK) {
-        // stage: 64x16 of A and 16x64 of B, 4 elements per thread each, zero outside the matrix
-        for (int q = 0; q < 4; q++) {
-            const int e = tid * 4 + q;          // 0..1023
-            const int am = e / GK, ak = e % GK;
-            const int gm = m0 + am, gk = k0 + ak;
-            As[am][ak] = (gm < M && gk < kEnd) ? A[(long)gm * sam + (long)gk * sak] : (unsigned short)0;
-            const int bk = e / GT, bn = e % GT;
-            const int hk = k0 + bk, hn = n0 + bn;
-            Bs[bk][bn] = (hk < kEnd && hn < N) ? B[(long)hk * sbk + (long)hn * sbn] : (unsigned short)0;
+        const bool more = k0 + GK < kEnd;
+        if (more) {
+            loadRuns(ra, A, sam, sak, m0, M, k0 + GK, kEnd, tid);
+            loadRuns(rb, B, sbn, sbk, n0, N, k0 + GK, kEnd, tid);
+        }
+#pragma unroll
+        for (int kk = 0; kk < GK; kk += 16) {
+            bf16x4 a[2], b[2];
+            for (int i = 0; i < 2; i++)
+                a[i] = *reinterpret_cast<const bf16x4 *>(&As[buf][wm + i * 16 + r][kk + g * 4]);
+            for (int j = 0; j < 2; j++)
+                b[j] = *reinterpret_cast<const bf16x4 *>(&Bs[buf][wn + j * 16 + r][kk + g * 4]);
+            for (int i = 0; i < 2; i++)
+                for (int j = 0; j < 2; j++)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        if (more) {
+            // the other buffer was last read before the previous round's barrier
+            storeRuns(As[buf ^ 1], ra, aK, tid);
+            storeRuns(Bs[buf ^ 1], rb, bK, tid);
         }
         __syncthreads();
-        const int r = lane % 16, g = lane / 16;
-        for (int i = 0; i < 2; i++) {
-            bf16x4 a;
-            a[0] = (short)As[wm + i * 16 + r][g * 4 + 0];
-            a[1] = (short)As[wm + i * 16 + r][g * 4 + 1];
-            a[2] = (short)As[wm + i * 16 + r][g * 4 + 2];
-            a[3] = (short)As[wm + i * 16 + r][g * 4 + 3];
-            for (int j = 0; j < 2; j++) {
-                bf16x4 b;
-                b[0] = (short)Bs[g * 4 + 0][wn + j * 16 + r];
-                b[1] = (short)Bs[g * 4 + 1][wn + j * 16 + r];
-                b[2] = (short)Bs[g * 4 + 2][wn + j * 16 + r];
-                b[3] = (short)Bs[g * 4 + 3][wn + j * 16 + r];
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[i][j], 0, 0, 0);
-            }
-        }
-        __syncthreads();
+        buf ^= 1;
     }
     // lane holds D[4g + q][r] of each 16x16 tile
-    const int r = lane % 16, g = lane / 16;
     for (int i = 0; i < 2; i++)
         for (int j = 0; j < 2; j++)
             for (int q = 0; q < 4; q++) {
@@ -182,12 +229,29 @@ __global__ void geluBwd(long n, const float *dh, const float *z, float *dz, unsi
     }
 }
 
-// bias gradient: column sums of d [B x H] (one thread per column)
-__global__ void colSum(int B, int H, const float *d, float *g) {
-    for (int u = blockIdx.x * blockDim.x + threadIdx.x; u < H; u += gridDim.x * blockDim.x) {
+// bias gradient: column sums of d [B x H] in two passes with a fixed summation order (no atomics): colSumPart
+// writes the sums of chunks of COLSUM_ROWS rows, [chunks][H], each by a 256-thread block of 64 columns x 4 row
+// groups; colSumFinal adds the chunks in order.
+constexpr int COLSUM_ROWS = 64;
+__global__ __launch_bounds__(256) void colSumPart(int B, int H, const float *__restrict__ d, float *__restrict__ part) {
+    __shared__ float red[4][64];
+    const int c = blockIdx.x * 64 + (threadIdx.x & 63), rg = threadIdx.x >> 6;
+    const int r0 = blockIdx.y * COLSUM_ROWS, r1 = min(B, r0 + COLSUM_ROWS);
+    float acc = 0.f;
+    if (c < H)
+        for (int r = r0 + rg; r < r1; r += 4) acc += d[(long)r * H + c];
+    red[rg][threadIdx.x & 63] = acc;
+    __syncthreads();
+    if (rg == 0 && c < H) {
+        const int q = threadIdx.x & 63;
+        part[(long)blockIdx.y * H + c] = (red[0][q] + red[1][q]) + (red[2][q] + red[3][q]);
+    }
+}
+__global__ void colSumFinal(int chunks, int H, const float *__restrict__ part, float *__restrict__ g) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < H; c += gridDim.x * blockDim.x) {
         float s = 0.f;
-        for (int b = 0; b < B; b++) s += d[(long)b * H + u];
-        g[u] = s;
+        for (int q = 0; q < chunks; q++) s += part[(long)q * H + c];
+        g[c] = s;
     }
 }
 
@@ -200,17 +264,43 @@ __global__ void embScatter(int B, int H, const int *branch, const float *dz, flo
     }
 }
 
-// Adam on FP32 masters, refreshed BF16 copy (bf may be null for parameters only used in FP32)
+// Adam on FP32 masters, refreshed BF16 copy (bf may be null for parameters only used in FP32); four elements per
+// thread with 16-byte loads (the buffers come from hipMalloc: 256-byte aligned), the tail element by element
+__device__ __forceinline__ float adamOne(float &w, float gi, float &m, float &v, float lr, float b1, float b2, float c1,
+                                         float c2, float eps) {
+    m = b1 * m + (1.f - b1) * gi;
+    v = b2 * v + (1.f - b2) * gi * gi;
+    w = w - lr * (m / c1) / (sqrtf(v / c2) + eps);
+    return w;
+}
 __global__ void adam(long n, float *w, const float *g, float *m, float *v, unsigned short *bf, float lr, float b1,
                      float b2, float c1, float c2, float eps) {
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-        const float gi = g[i];
-        const float mi = b1 * m[i] + (1.f - b1) * gi;
-        const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    const long n4 = n / 4;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n4; i += (long)gridDim.x * blockDim.x) {
+        float4 wi = reinterpret_cast<float4 *>(w)[i];
+        const float4 gi = reinterpret_cast<const float4 *>(g)[i];
+        float4 mi = reinterpret_cast<float4 *>(m)[i];
+        float4 vi = reinterpret_cast<float4 *>(v)[i];
+        adamOne(wi.x, gi.x, mi.x, vi.x, lr, b1, b2, c1, c2, eps);
+        adamOne(wi.y, gi.y, mi.y, vi.y, lr, b1, b2, c1, c2, eps);
+        adamOne(wi.z, gi.z, mi.z, vi.z, lr, b1, b2, c1, c2, eps);
+        adamOne(wi.w, gi.w, mi.w, vi.w, lr, b1, b2, c1, c2, eps);
+        reinterpret_cast<float4 *>(w)[i] = wi;
+        reinterpret_cast<float4 *>(m)[i] = mi;
+        reinterpret_cast<float4 *>(v)[i] = vi;
+        if (bf) {
+            const unsigned long long p = (unsigned long long)toBf16(wi.x) | ((unsigned long long)toBf16(wi.y) << 16) |
+                                         ((unsigned long long)toBf16(wi.z) << 32) |
+                                         ((unsigned long long)toBf16(wi.w) << 48);
+            reinterpret_cast<unsigned long long *>(bf)[i] = p;
+        }
+    }
+    for (long i = 4 * n4 + blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        float wi = w[i], mi = m[i], vi = v[i];
+        adamOne(wi, g[i], mi, vi, lr, b1, b2, c1, c2, eps);
+        w[i] = wi;
         m[i] = mi;
         v[i] = vi;
-        const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
-        w[i] = wi;
         if (bf) bf[i] = toBf16(wi);
     }
 }
@@ -379,7 +469,7 @@ struct Mlp {
     // activations for a batch of up to cap
     int cap = 0;
     Buf X, c1, z1, h1, c2, z2, h2, y, dy, dyh, dh2, dz2, dz2h, dh1, dz1, dz1h, tgt, branch, baseRow, feat, scale, loss,
-        x0;
+        x0, colPart;
     long step = 0;
     hipStream_t s = nullptr;
 
@@ -438,6 +528,13 @@ struct Mlp {
         gemm(B, N, Hh, h2.as<unsigned short>(), Hh, 1, W3.bf.as<unsigned short>(), 1, Hh, y.as<float>(), N);
         biasAdd<<<gridOf((long)B * N), 256, 0, s>>>(B, N, y.as<float>(), b3.w.as<float>());
     }
+    // column sums of d [B x Hc] into g
+    void colSum(int B, int Hc, const float *d, float *g) {
+        const int chunks = (B + COLSUM_ROWS - 1) / COLSUM_ROWS;
+        colPart.alloc((size_t)chunks * Hc * 4);
+        colSumPart<<<dim3((Hc + 63) / 64, chunks), 256, 0, s>>>(B, Hc, d, colPart.as<float>());
+        colSumFinal<<<gridOf(Hc), 256, 0, s>>>(chunks, Hc, colPart.as<float>(), g);
+    }
     // gradients of the mean squared error into the .g buffers; returns the loss
     double backward(int B) {
         const int Hh = H;
@@ -446,18 +543,18 @@ struct Mlp {
                                                    dyh.as<unsigned short>(), loss.as<double>());
         // W3: g = dy^T h2 : [N x B][B x H]
         gemm(N, Hh, B, dyh.as<unsigned short>(), 1, N, h2.as<unsigned short>(), Hh, 1, W3.g.as<float>(), Hh);
-        colSum<<<gridOf(N), 256, 0, s>>>(B, N, dy.as<float>(), b3.g.as<float>());
+        colSum(B, N, dy.as<float>(), b3.g.as<float>());
         // dh2 = dy W3 : [B x N][N x H]
         gemm(B, Hh, N, dyh.as<unsigned short>(), N, 1, W3.bf.as<unsigned short>(), Hh, 1, dh2.as<float>(), Hh);
         geluBwd<<<gridOf((long)B * Hh), 256, 0, s>>>((long)B * Hh, dh2.as<float>(), z2.as<float>(), dz2.as<float>(),
                                                      dz2h.as<unsigned short>());
         gemm(Hh, Hh, B, dz2h.as<unsigned short>(), 1, Hh, h1.as<unsigned short>(), Hh, 1, W2.g.as<float>(), Hh);
-        colSum<<<gridOf(Hh), 256, 0, s>>>(B, Hh, dz2.as<float>(), b2.g.as<float>());
+        colSum(B, Hh, dz2.as<float>(), b2.g.as<float>());
         gemm(B, Hh, Hh, dz2h.as<unsigned short>(), Hh, 1, W2.bf.as<unsigned short>(), Hh, 1, dh1.as<float>(), Hh);
         geluBwd<<<gridOf((long)B * Hh), 256, 0, s>>>((long)B * Hh, dh1.as<float>(), z1.as<float>(), dz1.as<float>(),
                                                      dz1h.as<unsigned short>());
         gemm(Hh, (int)D(), B, dz1h.as<unsigned short>(), 1, Hh, X.as<unsigned short>(), D(), 1, Win.g.as<float>(), D());
-        colSum<<<gridOf(Hh), 256, 0, s>>>(B, Hh, dz1.as<float>(), b1.g.as<float>());
+        colSum(B, Hh, dz1.as<float>(), b1.g.as<float>());
         zeroF<<<gridOf(Emb.n), 256, 0, s>>>(Emb.g.as<float>(), Emb.n);
         embScatter<<<gridOf((long)B * Hh), 256, 0, s>>>(B, Hh, branch.as<int>(), dz1.as<float>(), Emb.g.as<float>());
         double l = 0.0;

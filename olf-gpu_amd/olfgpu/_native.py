@@ -10,7 +10,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(os.path.dirname(HERE), "libolfhip.so")
+# OLFHIP_LIBRARY: another build of the same library (a build with other compile-time settings)
+LIB_PATH = os.environ.get("OLFHIP_LIBRARY") or os.path.join(os.path.dirname(HERE), "libolfhip.so")
 
 OK, E_ARGUMENT, E_DEVICE, E_MEMORY, E_UNSUPPORTED = 0, -1, -2, -3, -4
 CONVERGED, MAX_ITERATION_REACHED, DIVERGED, REJECTED, LINEAR_SOLVE_FAILED = 0, 1, -1, -2, -3
