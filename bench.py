@@ -81,8 +81,10 @@ def main():
     ap.add_argument("--train-epochs", type=int, default=150)
     ap.add_argument("--dump-schedule", default=None, help="write the solve order's costs and the timed solve's "
                     "per-contingency iterations to this .npz")
-    ap.add_argument("--order", choices=("cost", "given"), default="cost",
-                    help="solve order: longest first by the training solves' iteration counts, or as sampled")
+    ap.add_argument("--order", choices=("warm", "cost", "given"), default="warm",
+                    help="solve order: longest first by the iterations the training set's outages take from the "
+                         "trained network's warm start (ties: from the base start, then base flow), by the base-start "
+                         "iterations alone, or as sampled")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: the N-1 batch shards by contingency across GPUs "
@@ -134,11 +136,21 @@ def main():
     # solve order: longest first by the Newton iterations each outage took in the training solves (at other load
     # scalings); results stay per contingency, the order only decides which scenarios share a tile and start first
     order = np.arange(len(ctg_given))
-    if data is not None and args.order == "cost":
+    warm_cost = None
+    if data is not None and args.order in ("cost", "warm"):
         from olfgpu import warmstart as W
         from olfgpu.solver import longest_first, outage_flow_magnitude
-        order = longest_first(W.iteration_costs(data, net.n_branch)[ctg_given],
-                              outage_flow_magnitude(net, xb, ctg_given))
+        base_cost = W.iteration_costs(data, net.n_branch)[ctg_given]
+        flow = outage_flow_magnitude(net, xb, ctg_given)
+        sched_cost = base_cost
+        if args.order == "warm":
+            tc = time.perf_counter()
+            warm_cost = W.warm_iteration_costs(net, structure, mlp, data)[ctg_given]
+            warm_info["warm_cost_s"] = round(time.perf_counter() - tc, 1)
+            # warm-start iterations (means of one or two solves: multiples of 0.5), the base-start ones only breaking
+            # ties, then the flow the outage redistributes
+            sched_cost = warm_cost + 1e-3 * base_cost
+        order = longest_first(sched_cost, flow)
     ctg = ctg_given[order]
     K = len(ctg)
     rme, tix, tvl, isl = sa.plan_islands(ctg[:, None])
@@ -256,9 +268,8 @@ def main():
         highVoltageAbsoluteThreshold=0.5, highVoltageProportionalThreshold=0.001)
     cost_kw = {}
     feats_given = None
-    if data is not None and args.order == "cost":
-        cost_kw = dict(cost=W.iteration_costs(data, net.n_branch)[ctg_given],
-                       cost_tie=outage_flow_magnitude(net, xb, ctg_given))
+    if data is not None and args.order in ("cost", "warm"):
+        cost_kw = dict(cost=sched_cost, cost_tie=flow)
     if mlp is not None:
         feats_given = W.endpoint_features(net, xb, ctg_given, 1.0, gridgen.bridges(net))
 
@@ -355,9 +366,12 @@ def main():
         "config": {"workload": "grid9241 x1.1 load, 2000 single-branch outages (5% bridges, seed 2), FP32 block LU + "
                                "1 FP64 refinement, reactive limits on, " +
                                ("base-case start" if mlp is None else "learned warm start (base-start fallback)") +
-                               (", solved longest first by the training solves' iteration counts (ties: larger base flow of the "
-                                "outaged branch first)"
-                                if data is not None and args.order == "cost" else ""),
+                               ({"cost": ", solved longest first by the training solves' iteration counts (ties: "
+                                         "larger base flow of the outaged branch first)",
+                                 "warm": ", solved longest first by the iterations the training set's outages take "
+                                         "from the trained warm start (ties: base-start iterations, then larger base "
+                                         "flow of the outaged branch)"}.get(args.order, "")
+                                if data is not None else ""),
                    "buses": net.n_bus, "branches": net.n_branch, "contingencies": K, "tol_pu": nrp.maxMismatch,
                    "tile": ctx_lanes(ctx), "lu_levels": stats["levels"], "lu_nnz": stats["lu_nnz"]},
         "statuses": {N.STATUS_NAMES[int(s)]: int((status == s).sum()) for s in np.unique(status)},

@@ -321,6 +321,7 @@ struct ContextImpl {
     DBuf prof;                               // opt-in section stamps of one tile (OLFHIP_PROFILE=1)
     int profCap = 0;
     int profTile = 0;                        // OLFHIP_PROFILE_TILE, default 0
+    DBuf tileClock;                          // opt-in tile timeline (OLFHIP_TILE_TIMELINE=file)
     TopPlan top;                             // the dense top block of the step kernel
     std::vector<int> lastStatus;             // statuses of the last solve's scenarios
     DBuf kryV, kryH;                         // robust mode: the GMRES basis and per-tile dense state
@@ -370,6 +371,7 @@ struct ContextImpl {
         S.prof = prof.as<unsigned long long>();
         S.profCap = profCap;
         S.profTile = profTile;
+        S.tileClock = tileClock.as<unsigned long long>();
         S.rlType = rlType.as<signed char>();
         S.rlCount = rlCount.as<signed char>();
         S.rlPass = rlPass.as<signed char>();
@@ -692,6 +694,13 @@ static void solve(ContextImpl *c, const olfhip_solve_inputs *in, olfhip_solve_ou
         c->profCap = 0;
         c->prof.release();
     }
+    const char *timeline = std::getenv("OLFHIP_TILE_TIMELINE");
+    if (timeline && *timeline) {
+        c->tileClock.alloc((size_t)c->nTiles * 3 * 8);
+        HIP_OK(hipMemsetAsync(c->tileClock.p, 0, (size_t)c->nTiles * 3 * 8, s));   // before any capture
+    } else {
+        c->tileClock.release();
+    }
     restore.committed = true;
     const DevPlan dp = c->st->dp;
     DevState S = c->devState();
@@ -822,19 +831,42 @@ static void solve(ContextImpl *c, const olfhip_solve_inputs *in, olfhip_solve_ou
         int rateKHz = 0;
         HIP_OK(hipDeviceGetAttribute(&rateKHz, hipDeviceAttributeWallClockRate, c->st->device));
         const double usPerTick = rateKHz > 0 ? 1e3 / rateKHz : 0.0;
-        const char *names[9] = {"start", "fill", "test+outer", "factor+fwd", "backward", "refine-J", "refine-solve",
-                                "guard", "update"};
+        // stamps in the order a step makes them: the end of each named section
+        const int order[13] = {1, 2, 9, 3, 10, 4, 5, 11, 12, 13, 6, 7, 8};
+        const char *names[13] = {"fill", "test+outer", "sparse-factor+fwd", "top-factor+fwd", "top-bwd", "sparse-bwd",
+                                 "refine-J", "refine-sparse-fwd", "refine-top-fwd", "refine-top-bwd",
+                                 "refine-sparse-bwd", "guard", "update"};
         const unsigned long long launches = std::min<unsigned long long>(h[0], (unsigned long long)maxSteps);
         for (unsigned long long k = 0; k < launches; k++) {
             const unsigned long long *r = h.data() + PROF_SECTIONS * (k + 1);
             std::fprintf(stderr, "OLFHIP_PROFILE T=%d NT=%d tile %d launch %llu:", c->T, c->NT, c->profTile, k);
             unsigned long long prev = r[0];
-            for (int i = 1; i <= 8; i++) {
-                if (r[i] == 0) continue;
-                std::fprintf(stderr, " %s=%.1fus", i < 9 ? names[i] : "end", (r[i] - prev) * usPerTick);
-                prev = r[i];
+            for (int q = 0; q < 13; q++) {
+                const unsigned long long v = r[order[q]];
+                if (v == 0) continue;
+                std::fprintf(stderr, " %s=%.1fus", names[q], (v - prev) * usPerTick);
+                prev = v;
             }
             std::fprintf(stderr, " total=%.1fus\n", (prev - r[0]) * usPerTick);
+        }
+    }
+    if (timeline && *timeline) {
+        // the tile timeline, appended to the file as a block per solve: tile, start and end (us from the earliest
+        // start), Newton steps
+        std::vector<unsigned long long> tc((size_t)c->nTiles * 3);
+        HIP_OK(hipMemcpy(tc.data(), c->tileClock.p, tc.size() * 8, hipMemcpyDeviceToHost));
+        int rateKHz = 0;
+        HIP_OK(hipDeviceGetAttribute(&rateKHz, hipDeviceAttributeWallClockRate, c->st->device));
+        unsigned long long t0c = ~0ull;
+        for (int i = 0; i < c->nTiles; i++)
+            if (tc[3 * i]) t0c = std::min(t0c, tc[3 * i]);
+        if (FILE *f = std::fopen(timeline, "a")) {
+            std::fprintf(f, "solve K=%d T=%d NT=%d tiles=%d\n", K, c->T, c->NT, c->nTiles);
+            for (int i = 0; i < c->nTiles; i++)
+                if (tc[3 * i])
+                    std::fprintf(f, "%d %.1f %.1f %llu\n", i, (tc[3 * i] - t0c) * 1e3 / rateKHz,
+                                 (tc[3 * i + 1] - t0c) * 1e3 / rateKHz, tc[3 * i + 2]);
+            std::fclose(f);
         }
     }
     // results: every slot whose scenario has ended and was not handed back yet, in the order the scenarios were

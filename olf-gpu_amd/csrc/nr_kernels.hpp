@@ -110,6 +110,8 @@ struct DevState {
     unsigned long long *prof;
     int profCap;
     int profTile;   // the tile whose sections are stamped
+    // opt-in tile timeline: per tile its first start and last end (wall clock) and its Newton steps in the solve
+    unsigned long long *tileClock;
 };
 constexpr int PROF_SECTIONS = 16;
 
@@ -910,27 +912,298 @@ __device__ void topBackward(const DevPlan &P, const DevTop &top, const TopLds<T,
     }
 }
 
+// ---- the top block by panels of TOP_PANEL nodes: a panel's own sequential work runs inside one wave (or one thread)
+// per lane with no workgroup barrier, and what the panel does to the rest of the block is one parallel pass, so a
+// panel costs two or three barriers instead of one per node.
+constexpr int TOP_PANEL = 8;
+// OLFHIP_TOP_BY_NODE=1 at build time: the node-by-node forms throughout (one barrier per node), for comparison
+#ifndef OLFHIP_TOP_BY_NODE
+#define OLFHIP_TOP_BY_NODE 0
+#endif
+
+// Orders the wave's LDS accesses before this point before those after it (the wave's lanes exchange values through
+// LDS between the steps of a panel).
+__device__ __forceinline__ void waveSync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+typedef float topF32x4 __attribute__((ext_vector_type(4)));
+
+// topFactor's elimination by panels (FP32 factors): for each panel [a0, a1)
+//   1. its columns, every row below its first node, eliminated node by node inside one wave per lane;
+//   2. its rows right of it (the right-hand side column included) by the panel's unit-lower solve, one thread per
+//      (column, lane), and z of its nodes;
+//   3. the trailing block, rows [a1, n) x columns [a1, n], minus L~(b, a) D(a)^-1 U(a, c) summed over the panel's
+//      nodes a: a product of inner size 2 x TOP_PANEL on the matrix cores (v_mfma_f32_16x16x4f32), one wave per
+//      (lane, 16-column strip) walking the strip's 16 x 16 tiles, the D^-1 U operand kept in registers.
+// The same updates as the node-by-node elimination in another summation order: the stored form of the factors
+// (L~ below the diagonal, U above, D^-1 apart) and z are those topFactor leaves, to FP32 rounding.
+template <int T>
+__device__ void topFactorBlocked(const DevPlan &P, const DevTop &top, const TopLds<T, float> &L, int tile, int tid,
+                                 int nt, const float *__restrict__ lu, const float *__restrict__ z,
+                                 const double *__restrict__ rhs, double sign, int *clampLane, const int *laneOn) {
+    const int nb = P.nb, n = top.n;
+    const Blk<float> zero = {0.f, 0.f, 0.f, 0.f};
+    for (int w = tid; w < n * (n + 1) * T; w += nt) stBlk<float>(L.base + (size_t)w * 4, zero);
+    __syncthreads();
+    for (int w = tid; w < (top.nSlots + n) * T; w += nt) {
+        const int t = w % T, q = w / T;
+        if (laneOn && !laneOn[t]) continue;
+        if (q < top.nSlots) {
+            const int4 sr = top.slots[q];
+            stBlk<float>(L.M(sr.y, sr.z, t), ldBlk<float>(lu + bIdx<T>(tile, P.nSlots, sr.x, t)));
+        } else {
+            const int a = q - top.nSlots;
+            const int k = P.levelNodes[top.start + a];
+            const double2 b = *reinterpret_cast<const double2 *>(rhs + vIdx<T>(tile, nb, k, t));
+            stBlk<float>(L.M(a, n, t), Blk<float>{(float)(sign * b.x), 0.f, (float)(sign * b.y), 0.f});
+        }
+    }
+    __syncthreads();
+    topFromBelow<T, float, true>(P, top, L, tile, tid, nt, lu, z, laneOn);
+    const int wave = tid >> 6, wl = tid & 63, nWaves = nt >> 6;
+    for (int a0 = 0; a0 < n; a0 += TOP_PANEL) {
+        const int a1 = min(n, a0 + TOP_PANEL), pw = a1 - a0;
+        // 1. the panel's columns
+        for (int t = wave; t < T; t += nWaves) {
+            if (laneOn && !laneOn[t]) continue;
+            for (int a = a0; a < a1; a++) {
+                bool cl;
+                const Blk<float> Di = inv2(ldBlk<float>(L.M(a, a, t)), cl, 0.f);
+                if (wl == 0) {
+                    stBlk<float>(L.Dinv(a, t), Di);
+                    if (cl && clampLane) atomicAdd(&clampLane[t], 1);
+                }
+                const int rows = n - 1 - a, cols = a1 - 1 - a;
+                for (int e = wl; e < rows * cols; e += 64) {
+                    const int b = a + 1 + e / cols, c = a + 1 + e % cols;
+                    float *mb = L.M(b, c, t);
+                    Blk<float> acc = ldBlk<float>(mb);
+                    mulSub(acc, ldBlk<float>(L.M(b, a, t)), mul(Di, ldBlk<float>(L.M(a, c, t))));
+                    stBlk<float>(mb, acc);
+                }
+                waveSync();
+            }
+        }
+        __syncthreads();
+        // 2. the panel's rows right of it, and z of its nodes (from the right-hand side column n), in place
+        for (int w = tid; w < (n + 1 - a1) * T; w += nt) {
+            const int t = w % T, c = a1 + w / T;
+            if (laneOn && !laneOn[t]) continue;
+            for (int q = 0; q < pw; q++) {
+                const Blk<float> u = mul(ldBlk<float>(L.Dinv(a0 + q, t)), ldBlk<float>(L.M(a0 + q, c, t)));
+                if (c == n) stV<float>(L.Z(a0 + q, t), V2<float>{u.a, u.c});
+                for (int r = q + 1; r < pw; r++) {
+                    float *mr = L.M(a0 + r, c, t);
+                    Blk<float> v = ldBlk<float>(mr);
+                    mulSub(v, ldBlk<float>(L.M(a0 + r, a0 + q, t)), u);
+                    stBlk<float>(mr, v);
+                }
+            }
+        }
+        __syncthreads();
+        // 3. the trailing block on the matrix cores
+        const int rowsS = 2 * (n - a1), colsS = 2 * (n - a1) + 1, kS = 2 * pw;
+        if (rowsS > 0) {
+            const int rT = (rowsS + 15) / 16, cT = (colsS + 15) / 16;
+            const int i16 = wl & 15, g = wl >> 4;
+            for (int job = wave; job < T * cT; job += nWaves) {
+                const int t = job % T, ct = job / T;
+                if (laneOn && !laneOn[t]) continue;   // uniform per wave
+                // D^-1 U operand of the strip: lane (g, i16) holds row k = 4 ks + g, column Cb, of each k step
+                const int Cb = 2 * a1 + ct * 16 + i16;
+                float bk[TOP_PANEL / 2];
+#pragma unroll
+                for (int ks = 0; ks < TOP_PANEL / 2; ks++) {
+                    const int k = 4 * ks + g;
+                    float v = 0.f;
+                    if (k < kS && Cb <= 2 * n) {
+                        const int a = a0 + (k >> 1), kc = k & 1;
+                        const float *d = L.Dinv(a, t);
+                        const float *m = L.M(a, Cb >> 1, t);
+                        v = d[2 * kc] * m[Cb & 1] + d[2 * kc + 1] * m[2 + (Cb & 1)];
+                    }
+                    bk[ks] = v;
+                }
+                for (int rt = 0; rt < rT; rt++) {
+                    const int Ra = 2 * a1 + rt * 16 + i16;
+                    topF32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                    for (int ks = 0; ks < TOP_PANEL / 2; ks++) {
+                        if (4 * ks >= kS) break;   // uniform
+                        const int k = 4 * ks + g;
+                        const float av = (k < kS && Ra < 2 * n)
+                                             ? L.M(Ra >> 1, a0 + (k >> 1), t)[2 * (Ra & 1) + (k & 1)]
+                                             : 0.f;
+                        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bk[ks], acc, 0, 0, 0);
+                    }
+                    // lane (g, i16) holds rows 4 g + q, column i16 of the tile
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        const int R = 2 * a1 + rt * 16 + 4 * g + q;
+                        if (R < 2 * n && Cb <= 2 * n) {
+                            float *p = L.M(R >> 1, Cb >> 1, t) + 2 * (R & 1) + (Cb & 1);
+                            *p -= acc[q];
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// topForward by panels: a panel's forward substitution in one thread per lane, then the rows below it minus
+// L~(b, a) z(a) over the panel's nodes, one thread per (row, lane). Two barriers per panel.
+template <int T, typename VT>
+__device__ void topForwardBlocked(const DevPlan &P, const DevTop &top, const TopLds<T, VT> &L, int tile, int tid,
+                                  int nt, const VT *__restrict__ lu, const VT *__restrict__ z,
+                                  const double *__restrict__ rhs, double sign, const int *laneOn) {
+    const int nb = P.nb, n = top.n;
+    for (int w = tid; w < n * T; w += nt) {
+        const int t = w % T, a = w / T;
+        if (laneOn && !laneOn[t]) continue;
+        const int k = P.levelNodes[top.start + a];
+        const double2 b = *reinterpret_cast<const double2 *>(rhs + vIdx<T>(tile, nb, k, t));
+        stBlk<VT>(L.M(a, n, t), Blk<VT>{(VT)(sign * b.x), (VT)0, (VT)(sign * b.y), (VT)0});
+    }
+    __syncthreads();
+    topFromBelow<T, VT, false>(P, top, L, tile, tid, nt, lu, z, laneOn);
+    for (int a0 = 0; a0 < n; a0 += TOP_PANEL) {
+        const int a1 = min(n, a0 + TOP_PANEL), pw = a1 - a0;
+        if (tid < T && (!laneOn || laneOn[tid])) {
+            // the panel's forward substitution, in place in the right-hand side column
+            const int t = tid;
+            for (int q = 0; q < pw; q++) {
+                const Blk<VT> Di = ldBlk<VT>(L.Dinv(a0 + q, t));
+                const Blk<VT> yq = ldBlk<VT>(L.M(a0 + q, n, t));
+                const V2<VT> zq = {Di.a * yq.a + Di.b * yq.c, Di.c * yq.a + Di.d * yq.c};
+                stV<VT>(L.Z(a0 + q, t), zq);
+                for (int r = q + 1; r < pw; r++) {
+                    VT *mr = L.M(a0 + r, n, t);
+                    Blk<VT> y = ldBlk<VT>(mr);
+                    const Blk<VT> Lrq = ldBlk<VT>(L.M(a0 + r, a0 + q, t));
+                    y.a -= Lrq.a * zq.p + Lrq.b * zq.q;
+                    y.c -= Lrq.c * zq.p + Lrq.d * zq.q;
+                    stBlk<VT>(mr, y);
+                }
+            }
+        }
+        __syncthreads();
+        if (a1 < n) {
+            for (int w = tid; w < (n - a1) * T; w += nt) {
+                const int t = w % T, b = a1 + w / T;
+                if (laneOn && !laneOn[t]) continue;
+                VT *mb = L.M(b, n, t);
+                Blk<VT> r = ldBlk<VT>(mb);
+                for (int a = a0; a < a1; a++) {
+                    const V2<VT> za = ldV<VT>(L.Z(a, t));
+                    const Blk<VT> Lba = ldBlk<VT>(L.M(b, a, t));
+                    r.a -= Lba.a * za.p + Lba.b * za.q;
+                    r.c -= Lba.c * za.p + Lba.d * za.q;
+                }
+                stBlk<VT>(mb, r);
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// topBackward by panels, from the last one down: a panel's back substitution in one thread per lane (its rows'
+// accumulators already hold the sum over the nodes after the panel), then the accumulators of the rows before it
+// plus U(b, a) x(a) over the panel's nodes, one thread per (row, lane). Two barriers per panel.
+template <int T, typename VT>
+__device__ void topBackwardBlocked(const DevPlan &P, const DevTop &top, const TopLds<T, VT> &L, int tile, int tid,
+                                   int nt, VT *__restrict__ xo, const int *laneOn, double *dx64 = nullptr,
+                                   bool accStep = false) {
+    const int nb = P.nb, n = top.n;
+    for (int w = tid; w < n * T; w += nt) stV<VT>(L.Acc(w / T, w % T), V2<VT>{(VT)0, (VT)0});
+    __syncthreads();
+    const int nPanels = (n + TOP_PANEL - 1) / TOP_PANEL;
+    for (int pi = nPanels - 1; pi >= 0; pi--) {
+        const int a0 = pi * TOP_PANEL, a1 = min(n, a0 + TOP_PANEL), pw = a1 - a0;
+        if (tid < T && (!laneOn || laneOn[tid])) {
+            // the panel's back substitution; x(a) replaces row a's accumulator once known
+            const int t = tid;
+            for (int a = a1 - 1; a >= a0; a--) {
+                V2<VT> ac = ldV<VT>(L.Acc(a, t));
+                for (int c = a + 1; c < a1; c++) {
+                    const Blk<VT> U = ldBlk<VT>(L.M(a, c, t));
+                    const V2<VT> xc = ldV<VT>(L.Acc(c, t));
+                    ac.p += U.a * xc.p + U.b * xc.q;
+                    ac.q += U.c * xc.p + U.d * xc.q;
+                }
+                const Blk<VT> Di = ldBlk<VT>(L.Dinv(a, t));
+                const V2<VT> za = ldV<VT>(L.Z(a, t));
+                const V2<VT> xa = {za.p - (Di.a * ac.p + Di.b * ac.q), za.q - (Di.c * ac.p + Di.d * ac.q)};
+                stV<VT>(L.Acc(a, t), xa);
+                const int k = P.levelNodes[top.start + a];
+                stV<VT>(xo + vIdx<T>(tile, nb, k, t), xa);
+                putStep<T, VT>(dx64, accStep, vIdx<T>(tile, nb, k, t), xa);
+            }
+        }
+        __syncthreads();
+        if (a0 > 0) {
+            for (int w = tid; w < a0 * T; w += nt) {
+                const int t = w % T, b = w / T;
+                if (laneOn && !laneOn[t]) continue;
+                V2<VT> ab = ldV<VT>(L.Acc(b, t));
+                for (int a = a0; a < a1; a++) {
+                    const V2<VT> xa = ldV<VT>(L.Acc(a, t));
+                    const Blk<VT> U = ldBlk<VT>(L.M(b, a, t));
+                    ab.p += U.a * xa.p + U.b * xa.q;
+                    ab.q += U.c * xa.p + U.d * xa.q;
+                }
+                stV<VT>(L.Acc(b, t), ab);
+            }
+            __syncthreads();
+        }
+    }
+}
+
 // The whole block LU: sparse levels below the top block, then the top block (n = 0: sparse only).
 template <int T, typename VT>
 __device__ __forceinline__ void luFactorForward(const DevPlan &P, const LevelMeta &M, const DevTop &top,
                                                 const TopLds<T, VT> &L, int tile, int tid, int nt, VT *lu, VT *z,
-                                                const double *rhs, double sign, int *clampLane, const int *laneOn) {
+                                                const double *rhs, double sign, int *clampLane, const int *laneOn,
+                                                unsigned long long *prof = nullptr, int mark = 0) {
     factorForward<T, VT>(P, M, top, tile, tid, nt, lu, z, rhs, sign, clampLane, top.l0, laneOn);
-    if (top.n > 0) topFactor<T, VT>(P, top, L, tile, tid, nt, lu, z, rhs, sign, clampLane, laneOn);
+    if (prof && tid == 0) prof[mark] = wall_clock64();   // opt-in section stamp between the two parts
+    if (top.n > 0) {
+        if constexpr (sizeof(VT) == 4) {
+            if constexpr (!OLFHIP_TOP_BY_NODE) {
+                topFactorBlocked<T>(P, top, L, tile, tid, nt, lu, z, rhs, sign, clampLane, laneOn);
+                return;
+            }
+        }
+        topFactor<T, VT>(P, top, L, tile, tid, nt, lu, z, rhs, sign, clampLane, laneOn);
+    }
 }
 template <int T, typename VT>
 __device__ __forceinline__ void luForward(const DevPlan &P, const LevelMeta &M, const DevTop &top,
                                           const TopLds<T, VT> &L, int tile, int tid, int nt, const VT *lu, VT *z,
-                                          const double *rhs, double sign, const int *laneOn) {
+                                          const double *rhs, double sign, const int *laneOn,
+                                          unsigned long long *prof = nullptr, int mark = 0) {
     forwardOnly<T, VT>(P, M, top, tile, tid, nt, lu, z, rhs, sign, top.l0, laneOn);
-    if (top.n > 0) topForward<T, VT>(P, top, L, tile, tid, nt, lu, z, rhs, sign, laneOn);
+    if (prof && tid == 0) prof[mark] = wall_clock64();
+    if (top.n > 0) {
+        if constexpr (!OLFHIP_TOP_BY_NODE) topForwardBlocked<T, VT>(P, top, L, tile, tid, nt, lu, z, rhs, sign, laneOn);
+        else topForward<T, VT>(P, top, L, tile, tid, nt, lu, z, rhs, sign, laneOn);
+    }
 }
 template <int T, typename VT>
 __device__ __forceinline__ void luBackward(const DevPlan &P, const LevelMeta &M, const DevTop &top,
                                            const TopLds<T, VT> &L, int tile, int tid, int nt, const VT *lu,
                                            const VT *z, VT *xo, const int *laneOn, double *dx64 = nullptr,
-                                           bool accStep = false) {
-    if (top.n > 0) topBackward<T, VT>(P, top, L, tile, tid, nt, xo, laneOn, dx64, accStep);
+                                           bool accStep = false, unsigned long long *prof = nullptr,
+                                           int mark = 0) {
+    if (top.n > 0) {
+        if constexpr (!OLFHIP_TOP_BY_NODE) topBackwardBlocked<T, VT>(P, top, L, tile, tid, nt, xo, laneOn, dx64, accStep);
+        else topBackward<T, VT>(P, top, L, tile, tid, nt, xo, laneOn, dx64, accStep);
+    }
+    if (prof && tid == 0) prof[mark] = wall_clock64();
     backward<T, VT>(P, M, top, tile, tid, nt, lu, z, xo, top.l0, laneOn, dx64, accStep);
 }
 
@@ -1319,6 +1592,8 @@ __global__ __launch_bounds__(MAX_THREADS, OLFHIP_STEP_MIN_BLOCKS *MAX_THREADS / 
     const int tid = threadIdx.x, nt = blockDim.x;
     const int myLane = tid % T;
     if (S.tileRunning[tile] == 0) return;   // uniform per block
+    const unsigned long long tileStart = (S.tileClock && tid == 0) ? wall_clock64() : 0ull;
+    int stepsRun = 0;
     extern __shared__ float4 olfhipTopLds[];
     const TopLds<T, VT> TL = {reinterpret_cast<VT *>(reinterpret_cast<unsigned char *>(olfhipTopLds) + top.metaBytes),
                               top.n};
@@ -1350,6 +1625,7 @@ __global__ __launch_bounds__(MAX_THREADS, OLFHIP_STEP_MIN_BLOCKS *MAX_THREADS / 
         __syncthreads();
         if (sPause) break;
     }
+    stepsRun++;
     if (tid == 0) {
         sProf = nullptr;
         if (S.prof && tile == S.profTile) {
@@ -1691,9 +1967,9 @@ __global__ __launch_bounds__(MAX_THREADS, OLFHIP_STEP_MIN_BLOCKS *MAX_THREADS / 
     if (sFlag[2]) {
         // ---- 4. factorization with the forward solve of -F, then the backward solve
         OLFHIP_MARK(2);
-        luFactorForward<T, VT>(P, LM, top, TL, tile, tid, nt, lu, y, S.F, -1.0, sClamp, sSolve);
+        luFactorForward<T, VT>(P, LM, top, TL, tile, tid, nt, lu, y, S.F, -1.0, sClamp, sSolve, sProf, 9);
         OLFHIP_MARK(3);
-        luBackward<T, VT>(P, LM, top, TL, tile, tid, nt, lu, y, dx1, sSolve, S.dx, false);   // and the FP64 step
+        luBackward<T, VT>(P, LM, top, TL, tile, tid, nt, lu, y, dx1, sSolve, S.dx, false, sProf, 10);   // FP64 step
         OLFHIP_MARK(4);
         // a lane whose factorization met a singular pivot block (clamped) was not factored: the unpivoted batch
         // factorization leaves it, REJECTED in a batch and LINEAR_SOLVE_FAILED alone (the reference routes such
@@ -1711,8 +1987,9 @@ __global__ __launch_bounds__(MAX_THREADS, OLFHIP_STEP_MIN_BLOCKS *MAX_THREADS / 
             applyJPass<T, VT>(P, S, tile, tid, nt, sSolve, S.dx, S.F, 1.0, S.r, pm);
             __syncthreads();
             OLFHIP_MARK(5);
-            luForward<T, VT>(P, LM, top, TL, tile, tid, nt, lu, y, S.r, -1.0, sSolve);
-            luBackward<T, VT>(P, LM, top, TL, tile, tid, nt, lu, y, dx1, sSolve, S.dx, true);   // dx += correction
+            luForward<T, VT>(P, LM, top, TL, tile, tid, nt, lu, y, S.r, -1.0, sSolve, sProf, 11);
+            OLFHIP_MARK(12);
+            luBackward<T, VT>(P, LM, top, TL, tile, tid, nt, lu, y, dx1, sSolve, S.dx, true, sProf, 13);   // dx +=
             OLFHIP_MARK(6);
         }
         if constexpr (ROBUST) {
@@ -1805,6 +2082,12 @@ __global__ __launch_bounds__(MAX_THREADS, OLFHIP_STEP_MIN_BLOCKS *MAX_THREADS / 
     }
     __syncthreads();
     if (!sRunning) break;
+    }
+    if (S.tileClock && tid == 0) {
+        unsigned long long *tc = S.tileClock + 3 * (size_t)tile;
+        if (tc[0] == 0) tc[0] = tileStart;
+        tc[1] = wall_clock64();
+        tc[2] += (unsigned long long)stepsRun;
     }
 }
 

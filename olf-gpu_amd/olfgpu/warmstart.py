@@ -257,7 +257,7 @@ def make_training_set(net, structure, hoods, n_samples=20000, levels=20, seed=3,
     per = int(np.ceil(n_samples / levels))
     order = rng.permutation(net.n_branch)
     bridges = gridgen.bridges(net)
-    xb_all, lvl, brs, feats, targets, iters = [], [], [], [], [], []
+    xb_all, lvl, brs, feats, targets, iters, lvl_scale = [], [], [], [], [], [], []
     t0 = time.perf_counter()
     pos = 0
     for k, s in enumerate(scales):
@@ -283,6 +283,7 @@ def make_training_set(net, structure, hoods, n_samples=20000, levels=20, seed=3,
                 d[j, 1::2][isl] = 0.0
         f = endpoint_features(net, base.x, c, s, bridges)
         xb_all.append(base.x)
+        lvl_scale.append(s)
         lvl.append(np.full(len(c), len(xb_all) - 1, np.int32))
         brs.append(c)
         feats.append(f)
@@ -293,7 +294,7 @@ def make_training_set(net, structure, hoods, n_samples=20000, levels=20, seed=3,
                   flush=True)
     return dict(x_base=np.array(xb_all), level=np.concatenate(lvl), branch=np.concatenate(brs).astype(np.int32),
                 features=np.concatenate(feats), target=np.concatenate(targets), iterations=np.concatenate(iters),
-                scales=scales, seconds=time.perf_counter() - t0)
+                scales=scales, level_scale=np.array(lvl_scale), seconds=time.perf_counter() - t0)
 
 
 def iteration_costs(data, n_branch):
@@ -302,6 +303,41 @@ def iteration_costs(data, n_branch):
     total = np.bincount(data["branch"], weights=data["iterations"], minlength=n_branch)
     count = np.bincount(data["branch"], minlength=n_branch)
     cost = np.full(n_branch, float(data["iterations"].mean()))
+    seen = count > 0
+    cost[seen] = total[seen] / count[seen]
+    return cost
+
+
+def warm_iteration_costs(net, structure, mlp, data):
+    """Expected Newton iterations of each branch's outage from the trained network's warm start: the training set's
+    outages solved again at their load scalings, each from its prediction (the iterations of the warm-started attempt,
+    all solves of its outer loop), averaged per branch; a branch the set did not cover gets the mean. Offline, once
+    per network and model, as the training is: a schedule cost for the solves that start from the same network."""
+    from . import gridgen
+    from .solver import GpuBatchedSecurityAnalysisSolver, GpuLoadFlowParameters, GpuNewtonRaphson
+
+    bridges = gridgen.bridges(net)
+    total = np.zeros(net.n_branch)
+    count = np.zeros(net.n_branch)
+    for k, s in enumerate(data["level_scale"]):
+        ctg = data["branch"][data["level"] == k]
+        if len(ctg) == 0:
+            continue
+        ns = net.scaled(float(s))
+        rl = ns.reactive_limits()
+        x0, tg, md, _ = ns.system()
+        base = GpuNewtonRaphson(structure, rl).run(x0, tg, md)
+        if base.status != "CONVERGED":
+            continue
+        sa = GpuBatchedSecurityAnalysisSolver(structure, rl,
+                                              GpuLoadFlowParameters(linearSolver="BATCHED_LU_FP32_REFINED"))
+        feats = endpoint_features(net, base.x, ctg, float(s), bridges)
+        res = sa.solve_batch_n1(base.x, base.target, base.row_mode, base.reactive_limit_types, ctg, warm_start=mlp,
+                                warm_start_features=feats, confirm_band=0)
+        its = res.total_iterations.astype(np.float64)
+        np.add.at(total, ctg, its)
+        np.add.at(count, ctg, 1.0)
+    cost = np.full(net.n_branch, float(total.sum() / max(count.sum(), 1.0)))
     seen = count > 0
     cost[seen] = total[seen] / count[seen]
     return cost
