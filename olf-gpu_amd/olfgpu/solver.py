@@ -326,6 +326,7 @@ class GpuBatchedSecurityAnalysisSolver:
         self.robust_retry = robust_retry
         self.force_mode = force_mode
         self._robust_ctx = None
+        self._redo_ctx = None
         self.distributed_slack = distributed_slack
         self.reactive_limits = reactive_limits
         self.gpu_parameters = gpu_parameters or GpuLoadFlowParameters()
@@ -383,6 +384,20 @@ class GpuBatchedSecurityAnalysisSolver:
                 self._robust_ctx.set_distributed_slack(self.distributed_slack)
             self.nr.install_stopping_criteria(self._robust_ctx)
         return self._robust_ctx
+
+    def redo_context(self, chunk: int) -> N.BatchContext:
+        """The context of the base-start solves of warm-started scenarios (fallbacks and confirmations): one
+        scenario per tile, so that each of these few scenarios runs its Newton steps on a whole workgroup (a step's
+        time is set by the levels' load chains, shorter with more threads per scenario)."""
+        if self._redo_ctx is None or self._redo_ctx.capacity < chunk:
+            if self._redo_ctx is not None:
+                self._redo_ctx.close()
+            self._redo_ctx = N.BatchContext(self.structure, chunk, self.lu_mode, self.reactive_limits, 1,
+                                            self.threads_per_tile)
+            if self.distributed_slack is not None:
+                self._redo_ctx.set_distributed_slack(self.distributed_slack)
+            self.nr.install_stopping_criteria(self._redo_ctx)
+        return self._redo_ctx
 
     def solve_batch_n1(self, x_base, target_base, row_mode_base, limit_types_base, contingencies,
                        x0_per_scenario=None, states_to_host=True, warm_start=None,
@@ -459,7 +474,7 @@ class GpuBatchedSecurityAnalysisSolver:
         robust_errors = []
         from_warm = np.zeros(k, bool)
 
-        def solve_part(part, x0_host=None, x0_dev=None, robust=False):
+        def solve_part(part, x0_host=None, x0_dev=None, robust=False, redo=False):
             nonlocal device_s, chunks, seed
             rme, tix, tvl, cnt = self.plan_islands(part)
             if lt is not None and lt.size:
@@ -476,7 +491,9 @@ class GpuBatchedSecurityAnalysisSolver:
                 line_search_max_iteration=nr.lineSearchStateVectorScalingMaxIteration,
                 line_search_step_fold=nr.lineSearchStateVectorScalingStepFold, conv_eps_per_eq=nr.convEpsPerEq,
                 active_row_counts=(n - 2 * cnt).astype(np.int32), linear_solve_fail_ratio=nr.linearSolveFailRatio)
-            ctx = self.robust_context(len(part)) if robust else self.context(len(part))
+            ctx = (self.robust_context(len(part)) if robust else
+                   self.redo_context(len(part)) if redo and self.lu_mode != N.LU_ROBUST else self.context(len(part)))
+            last_ctx[0] = ctx
             t0 = time.perf_counter()
             r = ctx.solve(inp, want_states=states_to_host)
             device_s += time.perf_counter() - t0
@@ -486,12 +503,14 @@ class GpuBatchedSecurityAnalysisSolver:
         viol = []
         confirmed = 0
 
+        last_ctx = [None]
+
         def detect(rows, ctx=None, keep=None):
             """violations of the last solve, scenario indices mapped through rows (those of scenarios not in keep
             dropped)"""
             if limit_table is None:
                 return
-            v = limit_table.detect(ctx or self._ctx, band=violation_band)
+            v = limit_table.detect(ctx or last_ctx[0], band=violation_band)
             if keep is not None:
                 v = v[keep[v["scenario"]]]
             v["scenario"] = rows[v["scenario"]]
@@ -527,7 +546,7 @@ class GpuBatchedSecurityAnalysisSolver:
                 from_warm[c0 + redo] = False
                 if redo.size:
                     fallback += int((r["status"] != 0).sum())
-                    r2, _ = solve_part(part[redo])
+                    r2, _ = solve_part(part[redo], redo=True)
                     detect(c0 + redo)
                     st[c0 + redo] = r2["status"]
                     it[c0 + redo] = r2["iterations"]

@@ -137,3 +137,65 @@ def test_bank_slot_choice_is_checked(g2000):
         i = holder[int(slot)]
         assert last["status"][row] == ref["status"][i] and last["total_iterations"][row] == tot[i]
         assert np.array_equal(last["x"][row], ref["x"][i])
+
+
+def _by_scenario(v, scenarios):
+    """violation records of the given scenarios, as comparable tuples"""
+    return sorted((int(r["scenario"]), int(r["kind"]), int(r["element"]), int(r["side"]), int(r["limit"]),
+                   float(r["value"])) for r in v if int(r["scenario"]) in scenarios)
+
+
+def test_readbacks_after_a_paused_solve_are_by_slot(g2000):
+    """After a bank or paused solve the readbacks index scenarios by slot: the violation detection, both screens and
+    the near-threshold flags report exactly the slots the solve handed back, with what one solve of the same
+    contingencies reports for them; a slot still solving, or one handed back by an earlier call, reports nothing."""
+    net, st, base = g2000
+    xb = base[0]
+    print("[bank] readbacks by slot", flush=True)
+    ctg = gridgen.sample_contingencies(net, 24, 0.1, 13)
+    lt = N.LimitTable(st, gridgen.limit_table(net, xb))
+    one = N.BatchContext(st, 24, N.LU_FP64, net.reactive_limits())
+    inp, keep = inputs(base, ctg)
+    ref = one.solve(inp)
+    ref_v = lt.detect(one)
+    ref_counts, ref_rec = lt.screen(one, 24)
+    ref_amb = lt.ambiguous(one)
+    bank = N.BatchContext(st, BANK, N.LU_FP64, net.reactive_limits())
+    inp, keep = inputs(base, ctg[:BANK], pause=3)
+    r = bank.solve(inp)
+    holder = {j: j for j in range(BANK)}   # the default install: contingency j in slot j
+
+    def check(handed):
+        v = lt.detect(bank)
+        assert {int(x["scenario"]) for x in v} <= set(handed)
+        got = [(holder[s], *rest) for s, *rest in _by_scenario(v, set(handed))]
+        want = _by_scenario(ref_v, {holder[s] for s in handed})
+        assert sorted(got) == want
+        for screen in (lt.screen, lt.screen_no_alloc):
+            counts, rec = screen(bank, BANK)
+            for s in range(BANK):
+                if s in handed:
+                    i = holder[s]
+                    assert counts[s] == ref_counts[i]
+                    n = min(counts[s], rec.shape[1])
+                    assert sorted(rec[s, :n]) == sorted(ref_rec[i, :n])
+                else:
+                    assert counts[s] == 0, s
+        amb = lt.ambiguous(bank)
+        for s in range(BANK):
+            assert amb[s] == (s in handed and ref_amb[holder[s]]), s
+
+    first = r["slots"].tolist()
+    assert len(first) >= 3
+    check(first)
+    assert np.array_equal(r["status"], ref["status"][[holder[s] for s in first]])
+    # refill two of the handed-back slots with the next contingencies and run the bank to its end
+    refill = first[:2]
+    bank.begin_install(refill)
+    for j, s in enumerate(refill):
+        holder[s] = BANK + j
+    inp, keep = inputs(base, ctg[BANK:BANK + 2])
+    r = bank.solve(inp)
+    second = r["slots"].tolist()
+    assert set(second) == (set(range(BANK)) - set(first)) | set(refill)
+    check(second)
