@@ -861,7 +861,8 @@ static void solve(ContextImpl *c, const olfhip_solve_inputs *in, olfhip_solve_ou
         for (int i = 0; i < c->nTiles; i++)
             if (tc[3 * i]) t0c = std::min(t0c, tc[3 * i]);
         if (FILE *f = std::fopen(timeline, "a")) {
-            std::fprintf(f, "solve K=%d T=%d NT=%d tiles=%d\n", K, c->T, c->NT, c->nTiles);
+            std::fprintf(f, "solve K=%d T=%d NT=%d tiles=%d t0=%.1f\n", K, c->T, c->NT, c->nTiles,
+                         t0c * 1e3 / rateKHz);   // t0: the earliest start, us on the device's wall clock
             for (int i = 0; i < c->nTiles; i++)
                 if (tc[3 * i])
                     std::fprintf(f, "%d %.1f %.1f %llu\n", i, (tc[3 * i] - t0c) * 1e3 / rateKHz,
