@@ -505,12 +505,13 @@ class GpuBatchedSecurityAnalysisSolver:
 
         last_ctx = [None]
 
-        def detect(rows, ctx=None, keep=None):
-            """violations of the last solve, scenario indices mapped through rows (those of scenarios not in keep
-            dropped)"""
+        def detect(rows, ctx=None, keep=None, v=None):
+            """violations of the last solve (or v), scenario indices mapped through rows (those of scenarios not in
+            keep dropped)"""
             if limit_table is None:
                 return
-            v = limit_table.detect(ctx or last_ctx[0], band=violation_band)
+            if v is None:
+                v = limit_table.detect(ctx or last_ctx[0], band=violation_band)
             if keep is not None:
                 v = v[keep[v["scenario"]]]
             v["scenario"] = rows[v["scenario"]]
@@ -536,12 +537,21 @@ class GpuBatchedSecurityAnalysisSolver:
                 from_warm[c0:c1] = True
                 # a warm start that did not converge, or whose answer has a limit decision within confirm_band of
                 # its threshold: solve again from the base-case start, whose answer stands
+                # (or of the increased-violation comparison with the base case's record of the same limit)
                 again = r["status"] != 0
+                v = None
                 if limit_table is not None and confirm_band > 0:
-                    near = limit_table.ambiguous(self._ctx, confirm_band) & (r["status"] == 0)
+                    near = limit_table.ambiguous(self._ctx, confirm_band)
+                    if base_violations is not None:
+                        from .violations import near_base_boundary
+                        v = limit_table.detect(self._ctx, band=violation_band)
+                        nb = near_base_boundary(v, limit_table.n_branches, base_violations, increased_violations,
+                                                limit_table.bus_nominal_kv, confirm_band)
+                        near[v["scenario"][nb]] = True
+                    near &= r["status"] == 0
                     confirmed += int(near.sum())
                     again |= near
-                detect(np.arange(c0, c1), keep=~again)
+                detect(np.arange(c0, c1), keep=~again, v=v)
                 redo = np.nonzero(again)[0]
                 from_warm[c0 + redo] = False
                 if redo.size:

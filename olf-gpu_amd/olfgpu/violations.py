@@ -98,6 +98,52 @@ def weakened_or_equivalent(ref: np.ndarray, rep: np.ndarray, params: IncreasedVi
     return same_type & (smaller | (same & within)) & (rep["kind"] <= KIND_LOW_VOLTAGE)
 
 
+def near_base_boundary(violations: np.ndarray, n_table_branches: int, base: np.ndarray,
+                       params: IncreasedViolationsParameters | None = None, bus_nominal_kv=None,
+                       band: float = 1e-4) -> np.ndarray:
+    """Per record of `violations`: whether the manager's comparison with the base case's record of the same key
+    (same limit type and threshold) is decided by a value within relative `band` of its boundary (the base value
+    raised by the increased-violation threshold), so that another iterate inside the solver's tolerance can decide
+    it the other way."""
+    v = np.asarray(violations, N.VIOLATION_DTYPE)
+    out = np.zeros(v.size, bool)
+    b = np.asarray(base, N.VIOLATION_DTYPE)
+    if v.size == 0 or b.size == 0:
+        return out
+    params = params or IncreasedViolationsParameters()
+    kv = None if bus_nominal_kv is None else np.asarray(bus_nominal_kv, np.float64)
+
+    def to_kv(r):
+        if kv is None:
+            return np.ones(r.size)
+        bus = (r["kind"] == KIND_HIGH_VOLTAGE) | (r["kind"] == KIND_LOW_VOLTAGE)
+        bus_kv = kv if kv.ndim == 0 else kv[np.where(bus, r["element"], 0)]
+        return np.where(bus, bus_kv, 1.0)
+
+    keys = _keys(v)
+    bkeys = _keys(b)
+    blast, _ = _last_per_key(bkeys)
+    bk = bkeys[blast]
+    order = np.argsort(bk)
+    bk, brec = bk[order], b[blast][order]
+    pos = np.minimum(np.searchsorted(bk, keys), bk.size - 1)
+    hit = np.nonzero(bk[pos] == keys)[0]
+    if hit.size == 0:
+        return out
+    ref, rep = brec[pos[hit]], v[hit]
+    v1 = ref["value"] * to_kv(ref)
+    v2 = rep["value"] * to_kv(rep)
+    kind = rep["kind"]
+    bound = np.where(kind == KIND_BRANCH, np.abs(v1) * (1.0 + params.flowProportionalThreshold),
+                     np.where(kind == KIND_HIGH_VOLTAGE,
+                              v1 + np.minimum(params.highVoltageAbsoluteThreshold, v1 * params.highVoltageProportionalThreshold),
+                              v1 - np.minimum(params.lowVoltageAbsoluteThreshold, v1 * params.lowVoltageProportionalThreshold)))
+    val = np.where(kind == KIND_BRANCH, np.abs(v2), v2)
+    decided = (_type(ref) == _type(rep)) & (rep["threshold"] == ref["threshold"]) & (kind <= KIND_LOW_VOLTAGE)
+    out[hit] = decided & (np.abs(val - bound) <= band * np.abs(bound))
+    return out
+
+
 def manage(violations: np.ndarray, n_table_branches: int, base: np.ndarray | None = None,
            params: IncreasedViolationsParameters | None = None, bus_nominal_kv=None,
            keep_voltage_angle_violations: bool = False) -> np.ndarray:

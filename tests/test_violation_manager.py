@@ -136,3 +136,22 @@ def test_voltage_angle_violations_have_a_key_per_limit_and_are_kept():
 def test_branch_element_beyond_the_table_is_refused():
     with pytest.raises(ValueError):
         manage(rec(cur(0, NB, 1, 1.0)), NB)
+
+
+def test_near_base_boundary_flags_values_at_the_increased_violation_boundary():
+    """A record whose comparison with the base case's record is decided within the relative band of the boundary
+    (flows: |v1| (1 + flowProportionalThreshold); high voltage: v1 + min(absolute, v1 proportional) in kV) is
+    flagged; one of another limit, a value off the boundary, or a key absent from the base is not."""
+    from olfgpu.violations import near_base_boundary
+    p = IncreasedViolationsParameters(flowProportionalThreshold=0.001, highVoltageAbsoluteThreshold=0.5,
+                                      highVoltageProportionalThreshold=0.001)
+    base = rec(cur(0, 2, 1, 1.2), high(0, 4, 1.06))
+    edge = 1.2 * 1.001
+    v_edge = (1.06 * 400.0 + min(0.5, 1.06 * 400.0 * 0.001)) / 400.0
+    post = rec(cur(7, 2, 1, edge * (1 + 5e-5)), cur(7, 2, 1, -edge * (1 - 5e-5)), cur(7, 2, 1, edge * 1.01),
+               cur(7, 2, 1, edge, limit=3, threshold=0.9), cur(7, 3, 1, edge), high(7, 4, v_edge * (1 + 2e-5)),
+               high(7, 4, 1.08), low(7, 4, v_edge))
+    got = near_base_boundary(post, NB, base, p, 400.0, band=1e-4)
+    assert got.tolist() == [True, True, False, False, False, True, False, False]
+    # a band of zero flags nothing off the boundary itself
+    assert not near_base_boundary(post, NB, base, p, 400.0, band=0.0)[:3].any()
