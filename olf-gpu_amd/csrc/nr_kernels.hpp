@@ -365,6 +365,24 @@ __device__ __forceinline__ void nodeTerms(const DevPlan &P, int tile, int k, int
     sumQ = sQ;
 }
 
+// The lanes a pass works for: every lane of the tile (tm = T lanes per item), or, when exactly one lane is on, that
+// lane alone with every thread on it (tm = 1, lane t0), so that a tile whose other lanes have finished runs its
+// passes in the work units, and rounds, of one lane. An item's S threads and their sums are the same either way.
+struct LaneMap {
+    int tm, t0;
+};
+template <int T>
+__device__ __forceinline__ LaneMap laneMapOf(const int *laneOn) {
+    if (T == 1 || !laneOn) return LaneMap{T, 0};
+    int n = 0, t0 = 0;
+    for (int t = 0; t < T; t++)
+        if (laneOn[t]) {
+            n++;
+            t0 = t;
+        }
+    // the flags are the workgroup's own: every thread reads the same values
+    return n == 1 ? LaneMap{1, __builtin_amdgcn_readfirstlane(t0)} : LaneMap{T, 0};
+}
 // Tree reduction over the item dimension, per lane: threads tid and tid + stride share a lane because strides are
 // multiples of T. Lane t's result ends in red[t]. Every thread of the block must call it.
 template <int T, bool IS_MAX>
@@ -383,19 +401,47 @@ __device__ __forceinline__ void laneReduceAnd(int *red, int tid, int nt) {
     }
     __syncthreads();
 }
+// laneReduce / laneReduceAnd after a pass run with lane map lm: for a lane alone (lm.tm = 1) every thread's partial
+// was that lane's, so the per-residue results combine into lane t0 and the other lanes get the neutral value their
+// threads would have left (0; 1 for the AND).
+template <int T, bool IS_MAX>
+__device__ __forceinline__ void laneReduceFor(double *red, int tid, int nt, LaneMap lm) {
+    laneReduce<T, IS_MAX>(red, tid, nt);
+    if (T > 1 && lm.tm == 1) {
+        if (tid == 0) {
+            double v = red[0];
+            for (int t = 1; t < T; t++) v = IS_MAX ? fmax(v, red[t]) : v + red[t];
+            for (int t = 0; t < T; t++) red[t] = t == lm.t0 ? v : 0.0;
+        }
+        __syncthreads();
+    }
+}
+template <int T>
+__device__ __forceinline__ void laneReduceAndFor(int *red, int tid, int nt, LaneMap lm) {
+    laneReduceAnd<T>(red, tid, nt);
+    if (T > 1 && lm.tm == 1) {
+        if (tid == 0) {
+            int v = red[0];
+            for (int t = 1; t < T; t++) v &= red[t];
+            for (int t = 0; t < T; t++) red[t] = t == lm.t0 ? v : 1;
+        }
+        __syncthreads();
+    }
+}
 
-// Cross-lane sum over the S threads that share an item (lanes t, t + T, ..., t + (S-1) T of one wave).
+// Cross-lane sum over the S threads that share an item (lanes t, t + tm, ..., t + (S-1) tm of one wave; tm = T, or 1
+// for a pass on one lane alone). The pairs summed are the same for every tm: thread s meets s ^ 1, s ^ 2, ...
 template <int T, typename VT>
-__device__ __forceinline__ VT groupSum(VT v, int S) {
-    for (int off = T; off < T * S; off <<= 1) v += __shfl_xor(v, off);
+__device__ __forceinline__ VT groupSum(VT v, int S, int tm = T) {
+    for (int off = tm; off < tm * S; off <<= 1) v += __shfl_xor(v, off);
     return v;
 }
 template <int T, typename VT>
-__device__ __forceinline__ void groupSumBlk(Blk<VT> &b, int S) {
-    b.a = groupSum<T>(b.a, S);
-    b.b = groupSum<T>(b.b, S);
-    b.c = groupSum<T>(b.c, S);
-    b.d = groupSum<T>(b.d, S);
+__device__ __forceinline__ void groupSumBlk(Blk<VT> &b, int S, int tm = T) {
+    b.a = groupSum<T>(b.a, S, tm);
+    b.b = groupSum<T>(b.b, S, tm);
+    b.c = groupSum<T>(b.c, S, tm);
+    b.d = groupSum<T>(b.d, S, tm);
 }
 
 // Work units of a run of buckets laid end to end: unit u is lane t and thread s of the S threads sharing record rec.
@@ -410,26 +456,27 @@ __device__ __forceinline__ int capSplit(int s) {
     return s < 64 / T ? s : 64 / T;
 }
 template <int T>
-__device__ __forceinline__ int bucketUnits(const int4 *bk, int b0, int b1, bool diagOnly = false) {
+__device__ __forceinline__ int bucketUnits(const int4 *bk, int b0, int b1, bool diagOnly = false, int tm = T) {
     int n = 0;
     for (int b = b0; b < b1; b++) {
         const int4 r = bk[b];
         if (diagOnly && r.w != 1) continue;
-        n += (r.y - r.x) * T * capSplit<T>(r.z);
+        n += (r.y - r.x) * tm * capSplit<T>(r.z);
     }
     return n;
 }
-// unit u of the run (with diagOnly, of its diagonal-item buckets alone)
+// unit u of the run (with diagOnly, of its diagonal-item buckets alone), for the lanes of lm
 template <int T>
-__device__ __forceinline__ WorkUnit unitOf(const int4 *bk, int b0, int b1, int u, bool diagOnly = false) {
+__device__ __forceinline__ WorkUnit unitOf(const int4 *bk, int b0, int b1, int u, bool diagOnly = false,
+                                           LaneMap lm = LaneMap{T, 0}) {
     for (int b = b0; b < b1; b++) {
         const int4 r = bk[b];
         if (diagOnly && r.w != 1) continue;
         const int S = capSplit<T>(r.z);
-        const int n = (r.y - r.x) * T * S;
+        const int n = (r.y - r.x) * lm.tm * S;
         if (u < n) {
-            const int sh = __builtin_ctz(T * S);
-            return WorkUnit{r.x + (u >> sh), u % T, (u / T) & (S - 1), S, b, u >> sh};
+            const int sh = __builtin_ctz(lm.tm * S), st = __builtin_ctz(lm.tm);
+            return WorkUnit{r.x + (u >> sh), lm.tm == 1 ? lm.t0 : u % T, (u >> st) & (S - 1), S, b, u >> sh};
         }
         u -= n;
     }
@@ -481,11 +528,12 @@ __device__ void factorForward(const DevPlan &P, const LevelMeta &M, const DevTop
                               VT *__restrict__ lu, VT *__restrict__ z, const double *__restrict__ rhs, double sign,
                               int *clampLane, int l0, const int *laneOn) {
     const int nb = P.nb;
+    const LaneMap lm = laneMapOf<T>(laneOn);
     for (int l = 0; l < l0; l++) {
         const int b0 = M.levelBucketPtr[l], b1 = M.levelBucketPtr[l + 1];
-        const int units = bucketUnits<T>(M.bucketRec, b0, b1);
+        const int units = bucketUnits<T>(M.bucketRec, b0, b1, false, lm.tm);
         for (int w = tid; w < units; w += nt) {
-            const WorkUnit un = unitOf<T>(M.bucketRec, b0, b1, w);
+            const WorkUnit un = unitOf<T>(M.bucketRec, b0, b1, w, false, lm);
             const int t = un.t, s = un.s, S = un.S;
             if (laneOn && !laneOn[t]) continue;
             // the item record and the unit's contribution records are independent loads
@@ -520,9 +568,9 @@ __device__ void factorForward(const DevPlan &P, const LevelMeta &M, const DevTop
                     mulSubV(yk, Lkj, zj);
                 }
                 if (S > 1) {
-                    groupSumBlk<T>(D, S);
-                    yk.p = groupSum<T>(yk.p, S);
-                    yk.q = groupSum<T>(yk.q, S);
+                    groupSumBlk<T>(D, S, lm.tm);
+                    yk.p = groupSum<T>(yk.p, S, lm.tm);
+                    yk.q = groupSum<T>(yk.q, S, lm.tm);
                 }
                 if (s == 0) {
                     const Blk<VT> Dk = {A.a + D.a, A.b + D.b, A.c + D.c, A.d + D.d};
@@ -558,8 +606,8 @@ __device__ void factorForward(const DevPlan &P, const LevelMeta &M, const DevTop
                     mulSub(U, Lkj, mul(Dj, Uji));
                 }
                 if (S > 1) {
-                    groupSumBlk<T>(L, S);
-                    groupSumBlk<T>(U, S);
+                    groupSumBlk<T>(L, S, lm.tm);
+                    groupSumBlk<T>(U, S, lm.tm);
                 }
                 if (s == 0) {
                     stBlk<VT>(own, Blk<VT>{A.a + L.a, A.b + L.b, A.c + L.c, A.d + L.d});
@@ -578,11 +626,12 @@ __device__ void forwardOnly(const DevPlan &P, const LevelMeta &M, const DevTop &
                             const VT *__restrict__ lu, VT *__restrict__ z, const double *__restrict__ rhs, double sign,
                             int l0, const int *laneOn) {
     const int nb = P.nb;
+    const LaneMap lm = laneMapOf<T>(laneOn);
     for (int l = 0; l < l0; l++) {
         const int b0 = M.levelBucketPtr[l], b1 = M.levelBucketPtr[l + 1];
-        const int units = bucketUnits<T>(M.bucketRec, b0, b1, true);
+        const int units = bucketUnits<T>(M.bucketRec, b0, b1, true, lm.tm);
         for (int w = tid; w < units; w += nt) {
-            const WorkUnit un = unitOf<T>(M.bucketRec, b0, b1, w, true);
+            const WorkUnit un = unitOf<T>(M.bucketRec, b0, b1, w, true, lm);
             const int t = un.t, s = un.s, S = un.S;
             if (laneOn && !laneOn[t]) continue;
             const int4 bk = M.bucketRec[un.b];
@@ -607,8 +656,8 @@ __device__ void forwardOnly(const DevPlan &P, const LevelMeta &M, const DevTop &
                 mulSubV(yk, Lkj, ldV<VT>(z + vIdx<T>(tile, nb, cr.y, t)));
             }
             if (S > 1) {
-                yk.p = groupSum<T>(yk.p, S);
-                yk.q = groupSum<T>(yk.q, S);
+                yk.p = groupSum<T>(yk.p, S, lm.tm);
+                yk.q = groupSum<T>(yk.q, S, lm.tm);
             }
             if (s == 0) {
                 const V2<VT> y = {(VT)(sign * b.x) + yk.p, (VT)(sign * b.y) + yk.q};
@@ -637,11 +686,12 @@ __device__ void backward(const DevPlan &P, const LevelMeta &M, const DevTop &top
                          const VT *__restrict__ lu, const VT *__restrict__ z, VT *__restrict__ xo, int l0,
                          const int *laneOn, double *dx64 = nullptr, bool accStep = false) {
     const int nb = P.nb;
+    const LaneMap lm = laneMapOf<T>(laneOn);
     for (int l = l0 - 1; l >= 0; l--) {
         const int b0 = M.levelBwdBucketPtr[l], b1 = M.levelBwdBucketPtr[l + 1];
-        const int units = bucketUnits<T>(M.bwdBucketRec, b0, b1);
+        const int units = bucketUnits<T>(M.bwdBucketRec, b0, b1, false, lm.tm);
         for (int w = tid; w < units; w += nt) {
-            const WorkUnit un = unitOf<T>(M.bwdBucketRec, b0, b1, w);
+            const WorkUnit un = unitOf<T>(M.bwdBucketRec, b0, b1, w, false, lm);
             const int t = un.t, s = un.s, S = un.S;
             if (laneOn && !laneOn[t]) continue;
             const int4 bk = M.bwdBucketRec[un.b];
@@ -666,8 +716,8 @@ __device__ void backward(const DevPlan &P, const LevelMeta &M, const DevTop &top
                 mulSubV(acc, U, ldV<VT>(xo + vIdx<T>(tile, nb, e.y, t)));
             }
             if (S > 1) {
-                acc.p = groupSum<T>(acc.p, S);
-                acc.q = groupSum<T>(acc.q, S);
+                acc.p = groupSum<T>(acc.p, S, lm.tm);
+                acc.q = groupSum<T>(acc.q, S, lm.tm);
             }
             if (s == 0) {
                 const V2<VT> r = {zk.p + Dinv.a * acc.p + Dinv.b * acc.q, zk.q + Dinv.c * acc.p + Dinv.d * acc.q};
@@ -709,13 +759,14 @@ template <int T, typename VT, bool FACTOR>
 __device__ void topFromBelow(const DevPlan &P, const DevTop &top, const TopLds<T, VT> &L, int tile, int tid, int nt,
                              const VT *__restrict__ lu, const VT *__restrict__ z, const int *laneOn) {
     const int nb = P.nb;
+    const LaneMap lm = laneMapOf<T>(laneOn);
     // without FACTOR only the diagonal items (one per top node) matter
     const int4 *bk = FACTOR ? top.buckets : top.diagBuckets;
     const int *list = FACTOR ? top.items : top.diagItems;
     const int nbk = FACTOR ? top.nBuckets : top.nDiagBuckets;
-    const int units = bucketUnits<T>(bk, 0, nbk);
+    const int units = bucketUnits<T>(bk, 0, nbk, false, lm.tm);
     for (int w = tid; w < units; w += nt) {
-        const WorkUnit un = unitOf<T>(bk, 0, nbk, w);
+        const WorkUnit un = unitOf<T>(bk, 0, nbk, w, false, lm);
         const int t = un.t, s = un.s, S = un.S;
         if (laneOn && !laneOn[t]) continue;
         const int item = list[un.rec];
@@ -737,9 +788,9 @@ __device__ void topFromBelow(const DevPlan &P, const DevTop &top, const TopLds<T
                 mulSubV(yk, Lkj, ldV<VT>(z + vIdx<T>(tile, nb, cr.y, t)));
             }
             if (S > 1) {
-                if (FACTOR) groupSumBlk<T>(D, S);
-                yk.p = groupSum<T>(yk.p, S);
-                yk.q = groupSum<T>(yk.q, S);
+                if (FACTOR) groupSumBlk<T>(D, S, lm.tm);
+                yk.p = groupSum<T>(yk.p, S, lm.tm);
+                yk.q = groupSum<T>(yk.q, S, lm.tm);
             }
             if (s == 0) {
                 if (FACTOR) {
@@ -767,8 +818,8 @@ __device__ void topFromBelow(const DevPlan &P, const DevTop &top, const TopLds<T
                 mulSub(Ux, Lkj, mul(Dj, Uji));
             }
             if (S > 1) {
-                groupSumBlk<T>(Lx, S);
-                groupSumBlk<T>(Ux, S);
+                groupSumBlk<T>(Lx, S, lm.tm);
+                groupSumBlk<T>(Ux, S, lm.tm);
             }
             if (s == 0) {
                 VT *ml = L.M(b, a, t);   // L~(i,k): row i, column k
@@ -1214,12 +1265,14 @@ __device__ void fillPass(const DevPlan &P, const DevState &S, int tile, int tid,
                          double &pmax, int &pfin, double &psq, int slackNode) {
     const int nb = P.nb;
     VT *lu = reinterpret_cast<VT *>(S.lu);
-    const int t = tid % T;   // fixed per thread: the stride nt is a multiple of T
+    // fixed per thread: the stride nt is a multiple of T (a lane alone: every thread on it)
+    const LaneMap lm = laneMapOf<T>(fillLane);
+    const int t = lm.tm == 1 ? lm.t0 : tid % T;
     if (!fillLane[t]) return;
     int o[MAX_OUTAGES] = {-1, -1, -1, -1};
     for (int q = 0; q < S.maxOut && q < MAX_OUTAGES; q++) o[q] = S.outages[(size_t)(tile * T + t) * S.maxOut + q];
-    for (int w = tid; w < nb * T; w += nt) {
-        const int k = w / T;
+    for (int w = tid; w < nb * lm.tm; w += nt) {
+        const int k = lm.tm == 1 ? w : w / T;
         const size_t vi = vIdx<T>(tile, nb, k, t);
         const signed char mP = S.mode[vi], mQ = S.mode[vi + 1];
         double sP, sQ, jP, jQ;
@@ -1243,10 +1296,11 @@ template <int T>
 __device__ void rowRatioPass(const DevPlan &P, const DevState &S, int tile, int tid, int nt, const int *laneOn,
                              double &pratio) {
     const int nb = P.nb;
-    const int t = tid % T;
+    const LaneMap lm = laneMapOf<T>(laneOn);   // the nodes fillPass gives the thread for the same flags
+    const int t = lm.tm == 1 ? lm.t0 : tid % T;
     if (!laneOn[t]) return;
-    for (int w = tid; w < nb * T; w += nt) {
-        const int k = w / T;
+    for (int w = tid; w < nb * lm.tm; w += nt) {
+        const int k = lm.tm == 1 ? w : w / T;
         const size_t vi = vIdx<T>(tile, nb, k, t);
         const double2 f = *reinterpret_cast<const double2 *>(S.F + vi);
         const double2 tf = *reinterpret_cast<const double2 *>(S.rowThrFixed + 2 * k);
@@ -1261,12 +1315,14 @@ template <int T, typename VT>
 __device__ void applyJPass(const DevPlan &P, const DevState &S, int tile, int tid, int nt, const int *laneOn,
                            const double *v, const double *add, double addSign, double *out, double &pmax) {
     const int nb = P.nb;
-    const int t = tid % T;   // fixed per thread: the stride nt is a multiple of T
+    // fixed per thread: the stride nt is a multiple of T (a lane alone: every thread on it)
+    const LaneMap lm = laneMapOf<T>(laneOn);
+    const int t = lm.tm == 1 ? lm.t0 : tid % T;
     if (!laneOn[t]) return;
     int o[MAX_OUTAGES] = {-1, -1, -1, -1};
     for (int q = 0; q < S.maxOut && q < MAX_OUTAGES; q++) o[q] = S.outages[(size_t)(tile * T + t) * S.maxOut + q];
-    for (int w = tid; w < nb * T; w += nt) {
-        const int k = w / T;
+    for (int w = tid; w < nb * lm.tm; w += nt) {
+        const int k = lm.tm == 1 ? w : w / T;
         const size_t vi = vIdx<T>(tile, nb, k, t);
         double sP, sQ, jP, jQ;
         nodeTerms<T, VT, false, true>(P, tile, k, t, S.x, o, nullptr, v, S.mode[vi], S.mode[vi + 1], sP, sQ, jP, jQ);
@@ -1662,11 +1718,12 @@ __global__ __launch_bounds__(MAX_THREADS, OLFHIP_STEP_MIN_BLOCKS *MAX_THREADS / 
     {
         double pmax = 0.0, psq = 0.0;
         int pfin = 1;
+        const LaneMap lmF = laneMapOf<T>(sFill);
         fillPass<T, VT>(P, S, tile, tid, nt, sFill, pmax, pfin, psq, slackNode);
         red[tid] = pmax;
         redI[tid] = pfin;
-        laneReduce<T, true>(red, tid, nt);
-        laneReduceAnd<T>(redI, tid, nt);
+        laneReduceFor<T, true>(red, tid, nt, lmF);
+        laneReduceAndFor<T>(redI, tid, nt, lmF);
         if (tid < T) {
             sNorm[tid] = red[tid];
             sFin[tid] = redI[tid];
@@ -1676,13 +1733,13 @@ __global__ __launch_bounds__(MAX_THREADS, OLFHIP_STEP_MIN_BLOCKS *MAX_THREADS / 
             rowRatioPass<T>(P, S, tile, tid, nt, sFill, pratio);   // the thread's own F entries: no barrier needed
             __syncthreads();
             red[tid] = pratio;
-            laneReduce<T, true>(red, tid, nt);
+            laneReduceFor<T, true>(red, tid, nt, lmF);
             if (tid < T) sRowR[tid] = red[tid];
         }
         if (prm.scaling == 1 || prm.convEps > 0.0) {
             __syncthreads();
             red[tid] = psq;
-            laneReduce<T, false>(red, tid, nt);
+            laneReduceFor<T, false>(red, tid, nt, lmF);
             if (tid < T) sSsq[tid] = red[tid];
         }
         if (prm.scaling == 1) {
@@ -1711,11 +1768,12 @@ __global__ __launch_bounds__(MAX_THREADS, OLFHIP_STEP_MIN_BLOCKS *MAX_THREADS / 
                 __syncthreads();
                 double bmax = 0.0, bsq = 0.0;
                 int bfin = 1;
+                const LaneMap lmB = laneMapOf<T>(sBt);
                 fillPass<T, VT>(P, S, tile, tid, nt, sBt, bmax, bfin, bsq, slackNode);
                 red[tid] = bmax;
                 redI[tid] = bfin;
-                laneReduce<T, true>(red, tid, nt);
-                laneReduceAnd<T>(redI, tid, nt);
+                laneReduceFor<T, true>(red, tid, nt, lmB);
+                laneReduceAndFor<T>(redI, tid, nt, lmB);
                 if (tid < T && sBt[tid]) {
                     sNorm[tid] = red[tid];
                     sFin[tid] = redI[tid];
@@ -1725,12 +1783,12 @@ __global__ __launch_bounds__(MAX_THREADS, OLFHIP_STEP_MIN_BLOCKS *MAX_THREADS / 
                     rowRatioPass<T>(P, S, tile, tid, nt, sBt, bratio);
                     __syncthreads();
                     red[tid] = bratio;
-                    laneReduce<T, true>(red, tid, nt);
+                    laneReduceFor<T, true>(red, tid, nt, lmB);
                     if (tid < T && sBt[tid]) sRowR[tid] = red[tid];
                 }
                 __syncthreads();
                 red[tid] = bsq;
-                laneReduce<T, false>(red, tid, nt);
+                laneReduceFor<T, false>(red, tid, nt, lmB);
                 if (tid < T && sBt[tid]) {
                     sSsq[tid] = red[tid];
                     if (sFin[tid] && sSsq[tid] < S.lsLast[tile * T + tid]) sBt[tid] = 0;
@@ -1945,11 +2003,12 @@ __global__ __launch_bounds__(MAX_THREADS, OLFHIP_STEP_MIN_BLOCKS *MAX_THREADS / 
         if (sFlag[1]) {
             double pmax = 0.0, psq = 0.0;
             int pfin = 1;
+            const LaneMap lmF = laneMapOf<T>(sFill);
             fillPass<T, VT>(P, S, tile, tid, nt, sFill, pmax, pfin, psq, slackNode);
             red[tid] = pmax;
             redI[tid] = pfin;
-            laneReduce<T, true>(red, tid, nt);
-            laneReduceAnd<T>(redI, tid, nt);
+            laneReduceFor<T, true>(red, tid, nt, lmF);
+            laneReduceAndFor<T>(redI, tid, nt, lmF);
             if (tid < T && sFill[tid]) {
                 sNorm[tid] = red[tid];
                 if (redI[tid] == 0) sStatus[tid] = ST_DIVERGED;
@@ -2007,9 +2066,10 @@ __global__ __launch_bounds__(MAX_THREADS, OLFHIP_STEP_MIN_BLOCKS *MAX_THREADS / 
         __syncthreads();
         if (sFlag[3]) {
             double pm = 0.0;
+            const LaneMap lmS = laneMapOf<T>(sSolve);
             applyJPass<T, VT>(P, S, tile, tid, nt, sSolve, S.dx, S.F, 1.0, nullptr, pm);
             red[tid] = pm;
-            laneReduce<T, true>(red, tid, nt);
+            laneReduceFor<T, true>(red, tid, nt, lmS);
             if (tid < T && sSolve[tid]) {
                 const double ratio = sNorm[tid] > 0.0 ? red[tid] / sNorm[tid] : 0.0;
                 if (sSolves[tid] == 0 && sIt[tid] == 1 && !(ratio <= prm.linFailRatio)) {

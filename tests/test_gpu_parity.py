@@ -166,6 +166,28 @@ def test_rerun_is_bitwise_identical(grid):
     assert np.array_equal(a.x, b.x) and np.array_equal(a.status, b.status)
 
 
+@pytest.mark.parametrize("lu", ["BATCHED_LU", "BATCHED_LU_FP32_REFINED"])
+def test_a_lane_answer_does_not_depend_on_its_tile_partner(grid, lu):
+    """Once one lane of a tile is left solving, the tile runs its passes in the work units of that lane alone (every
+    thread on it); its arithmetic is the same as beside a partner, so each scenario's answer is bitwise the one it
+    gets alone in a tile (the other lane padding, a lane alone from the first step)."""
+    net = grid("grid2000")
+    print(f"[partner] grid2000 {lu}", flush=True)
+    osys, xb, mb, tb, ltb = base_case(net, "grid2000")
+    ctg = gridgen.sample_contingencies(net, 12, 0.25, 9)
+    sa = GpuBatchedSecurityAnalysisSolver(N.Structure.from_network(net), net.reactive_limits(),
+                                          GpuLoadFlowParameters(linearSolver=lu), lanes_per_tile=2,
+                                          threads_per_tile=1024)
+    together = sa.solve_batch_n1(xb, tb, mb, ltb, ctg)
+    pairs = together.total_iterations.reshape(-1, 2)
+    assert (pairs[:, 0] != pairs[:, 1]).any(), "no tile ran a lane alone"
+    for j in range(len(ctg)):
+        alone = sa.solve_batch_n1(xb, tb, mb, ltb, ctg[j:j + 1])
+        assert alone.status[0] == together.status[j], j
+        assert alone.total_iterations[0] == together.total_iterations[j], j
+        assert np.array_equal(alone.x[0], together.x[j]), j
+
+
 def test_statuses_forced_from_the_host(grid):
     net = grid("grid118")
     print("[status] grid118", flush=True)
