@@ -845,8 +845,9 @@ __device__ void topFactor(const DevPlan &P, const DevTop &top, const TopLds<T, V
     const Blk<VT> zero = {(VT)0, (VT)0, (VT)0, (VT)0};
     for (int w = tid; w < n * (n + 1) * T; w += nt) stBlk<VT>(L.base + (size_t)w * 4, zero);
     __syncthreads();
-    for (int w = tid; w < (top.nSlots + n) * T; w += nt) {
-        const int t = w % T, q = w / T;
+    const LaneMap lm = laneMapOf<T>(laneOn);
+    for (int w = tid; w < (top.nSlots + n) * lm.tm; w += nt) {
+        const int t = lm.tm == 1 ? lm.t0 : w % T, q = lm.tm == 1 ? w : w / T;
         if (laneOn && !laneOn[t]) continue;
         if (q < top.nSlots) {
             const int4 sr = top.slots[q];
@@ -999,8 +1000,9 @@ __device__ void topFactorBlocked(const DevPlan &P, const DevTop &top, const TopL
     const Blk<float> zero = {0.f, 0.f, 0.f, 0.f};
     for (int w = tid; w < n * (n + 1) * T; w += nt) stBlk<float>(L.base + (size_t)w * 4, zero);
     __syncthreads();
-    for (int w = tid; w < (top.nSlots + n) * T; w += nt) {
-        const int t = w % T, q = w / T;
+    const LaneMap lm = laneMapOf<T>(laneOn);
+    for (int w = tid; w < (top.nSlots + n) * lm.tm; w += nt) {
+        const int t = lm.tm == 1 ? lm.t0 : w % T, q = lm.tm == 1 ? w : w / T;
         if (laneOn && !laneOn[t]) continue;
         if (q < top.nSlots) {
             const int4 sr = top.slots[q];
@@ -2099,8 +2101,9 @@ __global__ __launch_bounds__(MAX_THREADS, OLFHIP_STEP_MIN_BLOCKS *MAX_THREADS / 
             if (tid < T && red[tid] > prm.maxDphi) sFactor[tid] = fmin(sFactor[tid], prm.maxDphi / red[tid]);
             __syncthreads();
         }
-        for (int w = tid; w < nb * T; w += nt) {
-            const int k = w / T, t = w % T;
+        const LaneMap lmU = laneMapOf<T>(sSolve);
+        for (int w = tid; w < nb * lmU.tm; w += nt) {
+            const int k = lmU.tm == 1 ? w : w / T, t = lmU.tm == 1 ? lmU.t0 : w % T;
             if (!sSolve[t]) continue;
             const double f = sFactor[t];
             const size_t vi = vIdx<T>(tile, nb, k, t);
