@@ -2,7 +2,7 @@
  * olfhip — C ABI of the MI355X AC Newton-Raphson / batched N-1 library (libolfhip.so).
  *
  * The drop-in boundary for the reference's JNI natives: one JNI function per Java native marshals its arrays into
- * these entry points (the binding is shown in INTEGRATION.md). Plain pointers and sizes only. Every entry returns 0
+ * these entry points (olf-gpu_amd/jni/olfhip_jni.c). Plain pointers and sizes only. Every entry returns 0
  * on success or a negative OLFHIP_E_* code; olfhip_last_error() then holds the message (per thread). No exception
  * crosses this boundary. Caller-owned input arrays are borrowed read-only for the duration of the call; outputs go
  * into caller-provided buffers.

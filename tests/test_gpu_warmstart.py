@@ -1,6 +1,6 @@
 """GPU tests of the learned warm start: the matrix-core GEMM against numpy (lane layout and rounding of
 v_mfma_f32_16x16x16bf16_1k), the network's gradients against the CPU oracle, and what the trained network gains on
-N-1 solves while the Newton mismatch stays the judge of every answer (same statuses as the base-case start)."""
+N-1 solves while the Newton mismatch stays the sole criterion of every answer (same statuses as the base-case start)."""
 import numpy as np
 import pytest
 
