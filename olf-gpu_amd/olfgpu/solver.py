@@ -341,22 +341,32 @@ class GpuBatchedSecurityAnalysisSolver:
     def plan_islands(self, contingencies: np.ndarray):
         """(row_mode_entries, target_override_index, target_override_value, islanded_count) for the chunk."""
         n = self.structure.n
-        rme, tix, tvl = [], [], []
-        counts = np.zeros(len(contingencies), np.int32)
-        for s, outs in enumerate(contingencies):
-            key = tuple(sorted(int(o) for o in np.atleast_1d(outs) if o >= 0))
+        k = len(contingencies)
+        c = np.asarray(contingencies, dtype=np.int64).reshape(k, -1)
+        # one island analysis per distinct outage set (sorted, unused places -1 first)
+        uniq, inv = np.unique(np.sort(c, axis=1), axis=0, return_inverse=True)
+        inv = np.asarray(inv).reshape(-1)
+        per_set = []
+        for row in uniq:
+            key = tuple(int(o) for o in row if o >= 0)
             buses = self._islands.get(key)
             if buses is None:
                 buses = np.nonzero(self.structure.islanded_buses(list(key), self.slack_bus))[0]
                 self._islands[key] = buses
-            counts[s] = buses.size
-            for b in buses:
-                for r, val in ((2 * b, 0.0), (2 * b + 1, 1.0)):
-                    rme.append((s, r, 0))   # FIXED_VARIABLE
-                    tix.append(s * n + r)
-                    tvl.append(val)
-        return (np.asarray(rme, np.int32).reshape(-1, 3), np.asarray(tix, np.int64), np.asarray(tvl, np.float64),
-                counts)
+            per_set.append(buses)
+        counts = np.array([b.size for b in per_set], np.int32)[inv] if k else np.zeros(0, np.int32)
+        hit = np.nonzero(counts)[0]
+        if hit.size == 0:
+            return np.zeros((0, 3), np.int32), np.zeros(0, np.int64), np.zeros(0, np.float64), counts
+        # per scenario with islanded buses, in scenario order: for each bus its angle row (0) then its voltage row (1)
+        buses = np.concatenate([per_set[inv[s]] for s in hit]).astype(np.int64)
+        scen = np.repeat(hit, counts[hit]).astype(np.int64)
+        rows = np.stack([2 * buses, 2 * buses + 1], axis=1).reshape(-1)
+        scen2 = np.repeat(scen, 2)
+        rme = np.stack([scen2, rows, np.zeros_like(rows)], axis=1).astype(np.int32)   # FIXED_VARIABLE
+        tix = scen2 * n + rows
+        tvl = np.tile(np.array([0.0, 1.0]), buses.size)
+        return rme, tix.astype(np.int64), tvl, counts
 
     def context(self, chunk: int) -> N.BatchContext:
         if self._ctx is None or self._ctx.capacity < chunk:
