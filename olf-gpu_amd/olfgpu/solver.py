@@ -434,6 +434,7 @@ class GpuBatchedSecurityAnalysisSolver:
         relative confirm_band of a threshold (voltages held at a setpoint excluded; LimitTable.ambiguous) is solved
         again from the base-case start, whose answer and violations stand (result.confirmed counts them; 0 turns the
         rule off)."""
+        import threading
         import time
 
         if cost is not None:
@@ -511,7 +512,13 @@ class GpuBatchedSecurityAnalysisSolver:
             return r, cnt
 
         viol = []
+        managed = []   # lists already through OLF's manager (scenarios disjoint from those in viol)
         confirmed = 0
+
+        def manage_kept(v):
+            from .violations import manage
+            return manage(v, limit_table.n_branches, base_violations, increased_violations,
+                          limit_table.bus_nominal_kv, keep_voltage_angle_violations)
 
         last_ctx = [None]
 
@@ -566,7 +573,28 @@ class GpuBatchedSecurityAnalysisSolver:
                 from_warm[c0 + redo] = False
                 if redo.size:
                     fallback += int((r["status"] != 0).sum())
-                    r2, _ = solve_part(part[redo], redo=True)
+                    if limit_table is not None and base_violations is not None:
+                        # the manager's pass over the warm-started answers kept (scenarios of their own) runs on
+                        # the host while the device solves the others again
+                        kept_warm = viol.pop()
+                        out = {}
+
+                        def resolve():
+                            try:
+                                out["r"] = solve_part(part[redo], redo=True)
+                            except BaseException as e:   # re-raised on the caller's thread
+                                out["e"] = e
+                        th = threading.Thread(target=resolve)
+                        th.start()
+                        try:
+                            managed.append(manage_kept(kept_warm))
+                        finally:
+                            th.join()
+                        if "e" in out:
+                            raise out["e"]
+                        r2, _ = out["r"]
+                    else:
+                        r2, _ = solve_part(part[redo], redo=True)
                     detect(c0 + redo)
                     st[c0 + redo] = r2["status"]
                     it[c0 + redo] = r2["iterations"]
@@ -602,9 +630,11 @@ class GpuBatchedSecurityAnalysisSolver:
             violations = np.concatenate(viol) if viol else np.zeros(0, N.VIOLATION_DTYPE)
             violations = violations[np.argsort(violations["scenario"], kind="stable")]
             if base_violations is not None:
-                from .violations import manage
-                violations = manage(violations, limit_table.n_branches, base_violations, increased_violations,
-                                    limit_table.bus_nominal_kv, keep_voltage_angle_violations)
+                violations = manage_kept(violations)
+                if managed:
+                    # the manager works scenario by scenario: lists of disjoint scenarios merge by scenario
+                    violations = np.concatenate([violations] + managed)
+                    violations = violations[np.argsort(violations["scenario"], kind="stable")]
         return BatchScenarioResult(st, it, tot, nm, xs, islanded, chunks, device_s, fallback, violations,
                                    robust_attempted, robust_recovered, robust_errors,
                                    from_warm if warm_start is not None else None, confirmed)
