@@ -1,0 +1,219 @@
+"""The JNI shim (olf-gpu_amd/jni/olfhip_jni.c) run without a JVM: compiled against a test double of <jni.h> and a
+fake JVM (tests/native/jni_test), its natives are called with the argument lists of the reference's
+StructureNative.java and BatchSolveNative.java.
+
+* CPU: createStructure marshals the closed-branch and shunt packs into a structure (its statistics equal those of the
+  same packs handed to olfhip_create_structure); a non-empty family the library does not model, or phase control on
+  a closed branch, throws UnsupportedOperationException and returns 0; the linear-solve modes map as the reference's
+  GpuLinearSolveMode codes (UBATCH, a cuDSS mode, throws IllegalArgumentException); entry flags and voltage-control
+  slopes are refused; handles 0 are ignored by the destroys; no access of the shim leaves a Java array's range, and
+  every pinned array is released.
+* GPU: solveBatchContext's rows (x, iterations, ||F||inf, status) equal bitwise the rows olfhip_solve_batch_context
+  writes for the same inputs given through the Python bindings (FP64 and FP32-refined LU, reactive limits, seeded
+  limit types); a disabled element that is not a closed branch, or more than four, throws; the bank calls
+  (beginBatchInstall, pauseAtEnded, readBatchHandedBackSlots) hand back the same rows and slots as the bindings."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import jni_harness as J
+import pyoracle as po
+from olfgpu import _native as N
+from olfgpu import gridgen
+
+
+@pytest.fixture(scope="module")
+def jvm():
+    return J.Jvm()
+
+
+@pytest.fixture(scope="module")
+def net118(grid):
+    return grid("grid118")
+
+
+def test_create_structure_marshals_the_packs(jvm, net118):
+    net = net118.with_shunts()
+    sp, si = net.shunts()
+    h = jvm.create_structure(net.n, net.pi_model(), net.branch_indices(), sp, si)
+    assert jvm.exception() is None and h != 0
+    st = N.Structure.from_network(net)
+    inner = N.Structure.__new__(N.Structure)   # a view of the library structure the shim's handle holds
+    try:
+        # the shim's handle points at {olfhip_structure *, n, ...}
+        inner.n, inner.h = net.n, ctypes.c_void_p(ctypes.c_void_p.from_address(h).value)
+        assert inner.statistics() == st.statistics()
+        assert np.array_equal(inner.order(), st.order())
+    finally:
+        inner.h = None
+        st.close()
+        jvm.destroy_structure(h)
+    jvm.destroy_structure(0)
+    assert jvm.exception() is None
+    assert jvm.violations() == 0
+
+
+@pytest.mark.parametrize("family", ["open_branch_indices", "hvdc_indices", "load_model_indices", "identity_rows"])
+def test_unmodelled_families_are_refused(jvm, net118, family):
+    h = jvm.create_structure(net118.n, net118.pi_model(), net118.branch_indices(), **{family: [0, 1, 2]})
+    assert h == 0
+    cls, msg = jvm.exception()
+    assert cls == "java/lang/UnsupportedOperationException" and "not modelled" in msg
+    assert jvm.violations() == 0
+
+
+def test_phase_control_is_refused_and_its_absence_accepted(jvm, net118):
+    m = net118.n_branch
+    none = -np.ones(m, np.int32)
+    h = jvm.create_structure(net118.n, net118.pi_model(), net118.branch_indices(), phase_control_side=none)
+    assert h != 0 and jvm.exception() is None
+    jvm.destroy_structure(h)
+    one = none.copy()
+    one[5] = 1
+    assert jvm.create_structure(net118.n, net118.pi_model(), net118.branch_indices(), phase_control_side=one) == 0
+    cls, msg = jvm.exception()
+    assert cls == "java/lang/UnsupportedOperationException" and "phase control" in msg
+    assert jvm.violations() == 0
+
+
+def test_modes_and_context_refusals(jvm, net118):
+    h = jvm.create_structure(net118.n, net118.pi_model(), net118.branch_indices())
+    rl = net118.reactive_limits()
+    try:
+        for call in (lambda: jvm.recommend_batch_size(h, 64, J.MODE_UBATCH),
+                     lambda: jvm.create_batch_context(h, 8, J.MODE_UBATCH, rl),
+                     lambda: jvm.create_batch_context(h, 8, 99, rl)):
+            assert call() == 0
+            cls, msg = jvm.exception()
+            assert cls == "java/lang/IllegalArgumentException", msg
+        flags = np.zeros(len(rl["voltage_row"]), np.int8)
+        flags[0] = 1
+        assert jvm.create_batch_context(h, 8, J.MODE_BATCHED_LU, rl, flags=flags) == 0
+        cls, msg = jvm.exception()
+        assert cls == "java/lang/UnsupportedOperationException" and "entry flags" in msg
+        assert jvm.recommend_batch_size(0, 64, J.MODE_BATCHED_LU) == 0
+        assert jvm.exception()[0] == "java/lang/IllegalArgumentException"
+    finally:
+        jvm.destroy_structure(h)
+    jvm.destroy_batch_context(0)
+    assert jvm.exception() is None
+    assert jvm.violations() == 0
+
+
+# ---------------------------------------------------------------------------------------------------------- GPU
+
+
+@pytest.fixture(scope="module")
+def g2000(grid):
+    if N.device_count() == 0:
+        pytest.fail("no GPU visible: the gpu-marked tests need an MI355X")
+    net = grid("grid2000")
+    osys = po.OracleSystem(net)
+    x0, tg, md, _ = net.system()
+    xb, mb, tb, ltb, res = osys.solve(x0, md, tg)
+    assert po.STATUS[res.status] == "CONVERGED"
+    ctg = gridgen.sample_contingencies(net, 40, 0.1, seed=5)
+    return net, (xb, mb, tb, ltb), ctg
+
+
+def mask_of(ctg, n_elements):
+    ctg = np.asarray(ctg).reshape(len(ctg), -1)
+    m = np.zeros((len(ctg), n_elements), bool)
+    for s, row in enumerate(ctg):
+        m[s, row[row >= 0]] = True
+    return m
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode,jmode", [(N.LU_FP64, J.MODE_BATCHED_LU),
+                                        (N.LU_FP32_REFINED, J.MODE_BATCHED_LU_FP32_REFINED)])
+def test_solve_rows_equal_the_c_abi(jvm, g2000, mode, jmode):
+    net, (xb, mb, tb, ltb), ctg = g2000
+    rl = net.reactive_limits()
+    k = len(ctg)
+    sh = jvm.create_structure(net.n, net.pi_model(), net.branch_indices())
+    ch = jvm.create_batch_context(sh, k, jmode, rl)
+    assert jvm.exception() is None and ch != 0
+    st = N.Structure.from_network(net)
+    ctx = N.BatchContext(st, k, mode, reactive_limits=rl)
+    try:
+        mask = mask_of(ctg, net.n_branch)
+        rows = jvm.solve(ch, xb, tb, mb, mask, seed_state=np.asarray(ltb, np.int32))
+        assert jvm.exception() is None
+        inp, keep = N.BatchContext.make_inputs(k, xb, tb, mb, outaged_branches=ctg, reactive_limits_seed_state=ltb,
+                                               line_search_max_iteration=0, line_search_step_fold=0.0)
+        ref = ctx.solve(inp)["results"]
+        assert rows.shape == (k, net.n + 3)
+        assert np.array_equal(rows, ref)
+        assert (rows[:, net.n + 2] == N.CONVERGED).sum() >= 30   # the others split the grid (bridges)
+    finally:
+        ctx.close()
+        st.close()
+        jvm.destroy_batch_context(ch)
+        jvm.destroy_structure(sh)
+    assert jvm.violations() == 0
+
+
+@pytest.mark.gpu
+def test_solve_refuses_other_disabled_elements(jvm, g2000):
+    net, (xb, mb, tb, ltb), ctg = g2000
+    sh = jvm.create_structure(net.n, net.pi_model(), net.branch_indices())
+    ch = jvm.create_batch_context(sh, 4, J.MODE_BATCHED_LU, net.reactive_limits())
+    try:
+        for bad in (np.ones((1, net.n_branch), bool), np.zeros((1, net.n_branch + 1), bool)):
+            if bad.shape[1] > net.n_branch:
+                bad[0, net.n_branch] = True
+            assert jvm.solve(ch, xb, tb, mb, bad) is None
+            cls, msg = jvm.exception()
+            assert cls == "java/lang/UnsupportedOperationException" and "closed branch" in msg
+        assert jvm.solve(ch, xb, tb, mb, np.zeros((5, net.n_branch), bool)) is None
+        assert jvm.exception()[0] == "java/lang/IllegalArgumentException"
+    finally:
+        jvm.destroy_batch_context(ch)
+        jvm.destroy_structure(sh)
+    assert jvm.violations() == 0
+
+
+@pytest.mark.gpu
+def test_bank_calls_equal_the_c_abi(jvm, g2000):
+    net, (xb, mb, tb, ltb), ctg = g2000
+    rl = net.reactive_limits()
+    bank = 16
+    sh = jvm.create_structure(net.n, net.pi_model(), net.branch_indices())
+    ch = jvm.create_batch_context(sh, bank, J.MODE_BATCHED_LU, rl)
+    st = N.Structure.from_network(net)
+    ctx = N.BatchContext(st, bank, N.LU_FP64, reactive_limits=rl)
+    try:
+        # a full bank solve pausing once 4 scenarios ended, then 6 new contingencies into slots it handed back
+        first, nxt = ctg[:bank], ctg[bank:bank + 6]
+        jvm.begin_install(ch, np.arange(bank))
+        rows = jvm.solve(ch, xb, tb, mb, mask_of(first, net.n_branch), seed_state=np.asarray(ltb, np.int32),
+                         pause_at_ended=4)
+        slots = jvm.handed_back_slots(ch)
+        ctx.begin_install(np.arange(bank))
+        inp, keep = N.BatchContext.make_inputs(bank, xb, tb, mb, outaged_branches=first, reactive_limits_seed_state=ltb,
+                                               line_search_max_iteration=0, line_search_step_fold=0.0, pause_at_ended=4)
+        ref = ctx.solve(inp)
+        assert jvm.exception() is None
+        assert len(slots) >= 4 and np.array_equal(slots, ref["slots"])
+        assert np.array_equal(rows, ref["results"])
+        take = slots[:len(nxt)] if len(slots) >= len(nxt) else slots
+        nxt = nxt[:len(take)]
+        jvm.begin_install(ch, take)
+        rows2 = jvm.solve(ch, xb, tb, mb, mask_of(nxt, net.n_branch), seed_state=np.asarray(ltb, np.int32))
+        slots2 = jvm.handed_back_slots(ch)
+        ctx.begin_install(take)
+        inp2, keep2 = N.BatchContext.make_inputs(len(take), xb, tb, mb, outaged_branches=nxt,
+                                                 reactive_limits_seed_state=ltb, line_search_max_iteration=0,
+                                                 line_search_step_fold=0.0)
+        ref2 = ctx.solve(inp2)
+        assert jvm.exception() is None
+        assert np.array_equal(slots2, ref2["slots"]) and np.array_equal(rows2, ref2["results"])
+        assert len(slots2) == bank - len(slots) + len(take)
+    finally:
+        ctx.close()
+        st.close()
+        jvm.destroy_batch_context(ch)
+        jvm.destroy_structure(sh)
+    assert jvm.violations() == 0
