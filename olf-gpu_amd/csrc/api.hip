@@ -970,7 +970,7 @@ struct LimitTableImpl {
     std::vector<int> angleFrom, angleTo;
     std::vector<double> angleHigh, angleLow;
     DBuf dBranch, dGroupStart, dGroupSide, dGroupQty, dLimitStart, dThreshold, dBusHigh, dBusLow, dCoef, dNode;
-    DBuf records, counts, screenLane, req, vals, isl, near;
+    DBuf records, counts, screenLane, req, vals, isl, near, packed, offsets;
     DevLimits dev() const {
         return DevLimits{nBranches, nGroups, nLimits, nBuses, dBranch.as<int>(), dGroupStart.as<int>(),
                          dGroupSide.as<int>(), dGroupQty.as<int>(), dLimitStart.as<int>(), dThreshold.as<double>(),
@@ -1056,8 +1056,11 @@ static LimitTableImpl *createLimitTable(StructureImpl *st, const olfhip_limit_ta
 }
 
 // Screens the last solve's converged scenarios on the device: counts [K], records [K][capacity] on the host.
+// Device screen of the last solve. Without `packed`: counts [K] and records [K][capacity] (a row's first counts[k]
+// places hold scenario k's records, all of them when counts[k] <= capacity). With `packed`: records holds the rows'
+// used places one after another, scenario k's from (*packed)[k] to (*packed)[k + 1].
 static void screenViolations(ContextImpl *c, LimitTableImpl *t, double band, int capacity, std::vector<int> &counts,
-                             std::vector<int> &records) {
+                             std::vector<int> &records, std::vector<int> *packed = nullptr) {
     if (c->st != t->st) throw ArgumentError("the limit table belongs to another structure");
     const int K = c->lastInstalled;
     if (K <= 0 || (int)c->lastStatus.size() != K) throw ArgumentError("no solve to screen");
@@ -1086,10 +1089,32 @@ static void screenViolations(ContextImpl *c, LimitTableImpl *t, double band, int
     });
     HIP_OK(hipGetLastError());
     counts.resize(K);
-    records.resize((size_t)K * capacity);
+    if (!packed) {
+        records.resize((size_t)K * capacity);
+        HIP_OK(hipMemcpyAsync(counts.data(), t->counts.p, (size_t)K * 4, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipMemcpyAsync(records.data(), t->records.p, records.size() * 4, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+        return;
+    }
+    // the scenarios' records packed on the device, so that only the records found cross to the host
+    packed->assign((size_t)K + 1, 0);
+    t->offsets.alloc(((size_t)K + 1) * 4);
+    t->packed.alloc((size_t)K * capacity * 4);
+    packOffsetsKernel<<<1, PACK_THREADS, 0, s>>>(t->counts.as<int>(), K, capacity, t->offsets.as<int>());
+    packRecordsKernel<<<K, 256, 0, s>>>(t->records.as<int>(), t->counts.as<int>(), t->offsets.as<int>(), capacity,
+                                        t->packed.as<int>());
+    HIP_OK(hipGetLastError());
     HIP_OK(hipMemcpyAsync(counts.data(), t->counts.p, (size_t)K * 4, hipMemcpyDeviceToHost, s));
-    HIP_OK(hipMemcpyAsync(records.data(), t->records.p, records.size() * 4, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(packed->data(), t->offsets.p, ((size_t)K + 1) * 4, hipMemcpyDeviceToHost, s));
     HIP_OK(hipStreamSynchronize(s));
+    const int total = (*packed)[K];
+    if (total < 0 || (long long)total > (long long)K * capacity)
+        throw DeviceError("screened record count out of range");
+    records.resize((size_t)total);
+    if (total > 0) {
+        HIP_OK(hipMemcpyAsync(records.data(), t->packed.p, (size_t)total * 4, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+    }
 }
 
 // Which scenarios of the last solve a state difference of relative size `band` could change the violations of
@@ -1209,8 +1234,8 @@ static void branchFlows(const Plan &P, int e, double2 x1, double2 x2, double f[6
 // converged scenario and decided here (GpuLimitScanner.java:239-267).
 static void detectViolations(ContextImpl *c, LimitTableImpl *t, double band, int capacity,
                              std::vector<olfhip_violation> &out) {
-    std::vector<int> counts, records;
-    screenViolations(c, t, band, capacity, counts, records);
+    std::vector<int> counts, records, offsets;
+    screenViolations(c, t, band, capacity, counts, records, &offsets);
     const Plan &P = c->st->plan;
     const int K = c->lastInstalled;
     // per scenario, its sorted candidate records (all records when it overflowed)
@@ -1228,7 +1253,7 @@ static void detectViolations(ContextImpl *c, LimitTableImpl *t, double band, int
             }
             cand[k] = allRecords;
         } else {
-            cand[k].assign(records.begin() + (size_t)k * capacity, records.begin() + (size_t)k * capacity + counts[k]);
+            cand[k].assign(records.begin() + offsets[k], records.begin() + offsets[k + 1]);
             std::sort(cand[k].begin(), cand[k].end());
         }
     }

@@ -208,4 +208,40 @@ __global__ void zeroIntKernel(int *p, int n) {
     for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < n; w += gridDim.x * blockDim.x) p[w] = 0;
 }
 
+// Where each scenario's screened records start in one packed list: off[k] = the sum over j < k of min(counts[j],
+// capacity), off[K] the total. One workgroup of PACK_THREADS threads: each sums a run of consecutive scenarios, the
+// run sums are scanned in LDS (Hillis-Steele), then each thread writes its run's offsets.
+constexpr int PACK_THREADS = 1024;
+__global__ __launch_bounds__(PACK_THREADS) void packOffsetsKernel(const int *counts, int K, int capacity, int *off) {
+    __shared__ int run[PACK_THREADS];
+    const int t = threadIdx.x;
+    const int per = (K + PACK_THREADS - 1) / PACK_THREADS;
+    const int k0 = min(K, t * per), k1 = min(K, k0 + per);
+    int sum = 0;
+    for (int k = k0; k < k1; k++) sum += min(counts[k], capacity);
+    run[t] = sum;
+    __syncthreads();
+    for (int d = 1; d < PACK_THREADS; d <<= 1) {
+        const int add = t >= d ? run[t - d] : 0;
+        __syncthreads();
+        run[t] += add;
+        __syncthreads();
+    }
+    int at = run[t] - sum;   // exclusive
+    for (int k = k0; k < k1; k++) {
+        off[k] = at;
+        at += min(counts[k], capacity);
+    }
+    if (t == PACK_THREADS - 1) off[K] = run[t];
+}
+
+// Scenario k's records (block k) copied from its row of `capacity` places to its place in the packed list.
+__global__ void packRecordsKernel(const int *records, const int *counts, const int *off, int capacity, int *packed) {
+    const int k = blockIdx.x;
+    const int c = min(counts[k], capacity);
+    const int *src = records + (size_t)k * capacity;
+    int *dst = packed + off[k];
+    for (int q = threadIdx.x; q < c; q += blockDim.x) dst[q] = src[q];
+}
+
 }  // namespace olfhip

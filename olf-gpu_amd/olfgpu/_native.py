@@ -535,12 +535,13 @@ class LimitTable:
         return counts, records
 
     def detect(self, context: "BatchContext", band=1e-7, capacity=1024, max_violations=1 << 20):
-        """Screen + exact host decisions: a VIOLATION_DTYPE array ordered by scenario, branches, buses."""
-        out = np.zeros(max_violations, VIOLATION_DTYPE)
+        """Screen + exact host decisions: a VIOLATION_DTYPE array ordered by scenario, branches, buses (a view of
+        the first records of a buffer of max_violations: only the pages the records fill are touched)."""
+        out = np.empty(max_violations, VIOLATION_DTYPE)
         n = C.c_int()
         check(lib().olfhip_detect_violations(context.h, self.h, band, capacity, max_violations, ptr(out),
                                              C.byref(n)))
-        return out[:n.value].copy()
+        return out[:n.value]
 
 
 def autotune(structure: Structure, inputs: SolveInputs, mode=LU_FP64, reactive_limits=None):

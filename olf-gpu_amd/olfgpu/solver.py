@@ -326,7 +326,7 @@ class GpuBatchedSecurityAnalysisSolver:
         self.robust_retry = robust_retry
         self.force_mode = force_mode
         self._robust_ctx = None
-        self._redo_ctx = None
+        self._redo_ctx = [None, None]
         self.distributed_slack = distributed_slack
         self.reactive_limits = reactive_limits
         self.gpu_parameters = gpu_parameters or GpuLoadFlowParameters()
@@ -395,19 +395,21 @@ class GpuBatchedSecurityAnalysisSolver:
             self.nr.install_stopping_criteria(self._robust_ctx)
         return self._robust_ctx
 
-    def redo_context(self, chunk: int) -> N.BatchContext:
-        """The context of the base-start solves of warm-started scenarios (fallbacks and confirmations): one
+    def redo_context(self, chunk: int, which: int = 0) -> N.BatchContext:
+        """The contexts of the base-start solves of warm-started scenarios (fallbacks and confirmations): one
         scenario per tile, so that each of these few scenarios runs its Newton steps on a whole workgroup (a step's
-        time is set by the levels' load chains, shorter with more threads per scenario)."""
-        if self._redo_ctx is None or self._redo_ctx.capacity < chunk:
-            if self._redo_ctx is not None:
-                self._redo_ctx.close()
-            self._redo_ctx = N.BatchContext(self.structure, chunk, self.lu_mode, self.reactive_limits, 1,
-                                            self.threads_per_tile)
+        time is set by the levels' load chains, shorter with more threads per scenario). `which` picks one of two
+        such contexts (each with its own stream), so that two sets of them can be solved at once."""
+        ctx = self._redo_ctx[which]
+        if ctx is None or ctx.capacity < chunk:
+            if ctx is not None:
+                ctx.close()
+            ctx = N.BatchContext(self.structure, chunk, self.lu_mode, self.reactive_limits, 1, self.threads_per_tile)
             if self.distributed_slack is not None:
-                self._redo_ctx.set_distributed_slack(self.distributed_slack)
-            self.nr.install_stopping_criteria(self._redo_ctx)
-        return self._redo_ctx
+                ctx.set_distributed_slack(self.distributed_slack)
+            self.nr.install_stopping_criteria(ctx)
+            self._redo_ctx[which] = ctx
+        return ctx
 
     def solve_batch_n1(self, x_base, target_base, row_mode_base, limit_types_base, contingencies,
                        x0_per_scenario=None, states_to_host=True, warm_start=None,
@@ -436,6 +438,8 @@ class GpuBatchedSecurityAnalysisSolver:
         rule off)."""
         import threading
         import time
+
+        from .violations import near_base_boundary
 
         if cost is not None:
             order = longest_first(cost, cost_tie)
@@ -485,7 +489,7 @@ class GpuBatchedSecurityAnalysisSolver:
         robust_errors = []
         from_warm = np.zeros(k, bool)
 
-        def solve_part(part, x0_host=None, x0_dev=None, robust=False, redo=False):
+        def solve_part(part, x0_host=None, x0_dev=None, robust=False, redo=False, which=0):
             nonlocal device_s, chunks, seed
             rme, tix, tvl, cnt = self.plan_islands(part)
             if lt is not None and lt.size:
@@ -503,14 +507,18 @@ class GpuBatchedSecurityAnalysisSolver:
                 line_search_step_fold=nr.lineSearchStateVectorScalingStepFold, conv_eps_per_eq=nr.convEpsPerEq,
                 active_row_counts=(n - 2 * cnt).astype(np.int32), linear_solve_fail_ratio=nr.linearSolveFailRatio)
             ctx = (self.robust_context(len(part)) if robust else
-                   self.redo_context(len(part)) if redo and self.lu_mode != N.LU_ROBUST else self.context(len(part)))
-            last_ctx[0] = ctx
+                   self.redo_context(len(part), which) if redo and self.lu_mode != N.LU_ROBUST else
+                   self.context(len(part)))
+            if not redo:
+                last_ctx[0] = ctx
             t0 = time.perf_counter()
             r = ctx.solve(inp, want_states=states_to_host)
-            device_s += time.perf_counter() - t0
-            chunks += 1
-            return r, cnt
+            with counters:
+                device_s += time.perf_counter() - t0
+                chunks += 1
+            return r, cnt, ctx
 
+        counters = threading.Lock()
         viol = []
         managed = []   # lists already through OLF's manager (scenarios disjoint from those in viol)
         confirmed = 0
@@ -540,7 +548,7 @@ class GpuBatchedSecurityAnalysisSolver:
             x0_dev = None
             if warm_start is not None:
                 x0_dev = warm_start.predict_warm_start(x_base, part[:, 0], warm_start_features[c0:c1])
-            r, cnt = solve_part(part, None if x0_per_scenario is None else x0_per_scenario[c0:c1], x0_dev)
+            r, cnt, _ = solve_part(part, None if x0_per_scenario is None else x0_per_scenario[c0:c1], x0_dev)
             islanded[c0:c1] = cnt
             st[c0:c1] = r["status"]
             it[c0:c1] = r["iterations"]
@@ -553,62 +561,78 @@ class GpuBatchedSecurityAnalysisSolver:
             else:
                 from_warm[c0:c1] = True
                 # a warm start that did not converge, or whose answer has a limit decision within confirm_band of
-                # its threshold: solve again from the base-case start, whose answer stands
-                # (or of the increased-violation comparison with the base case's record of the same limit)
+                # its threshold (or of the increased-violation comparison with the base case's record of the same
+                # limit): solve again from the base-case start, whose answer stands
                 again = r["status"] != 0
+                redo_runs = []   # (rows of the chunk, thread, its outcome) of the base-start solves under way
+
+                def start_redo(rows, which):
+                    out = {}
+
+                    def run():
+                        try:
+                            out["r"] = solve_part(part[rows], redo=True, which=which)
+                        except BaseException as e:   # re-raised on the caller's thread
+                            out["e"] = e
+                    th = threading.Thread(target=run)
+                    th.start()
+                    redo_runs.append((rows, th, out))
+
                 v = None
                 if limit_table is not None and confirm_band > 0:
-                    near = limit_table.ambiguous(self._ctx, confirm_band)
+                    near = limit_table.ambiguous(self._ctx, confirm_band) & (r["status"] == 0)
+                    again |= near
                     if base_violations is not None:
-                        from .violations import near_base_boundary
+                        # these solves start now, on a context of their own; the increased-violation boundary is
+                        # checked on the detected records meanwhile, and the few scenarios it adds start on a
+                        # second context
+                        first = np.nonzero(again)[0]
+                        if first.size:
+                            start_redo(first, 0)
                         v = limit_table.detect(self._ctx, band=violation_band)
                         nb = near_base_boundary(v, limit_table.n_branches, base_violations, increased_violations,
                                                 limit_table.bus_nominal_kv, confirm_band)
-                        near[v["scenario"][nb]] = True
-                    near &= r["status"] == 0
+                        extra = np.zeros(len(part), bool)
+                        extra[v["scenario"][nb]] = True
+                        extra &= (r["status"] == 0) & ~again
+                        near |= extra
+                        again |= extra
+                        if extra.any():
+                            start_redo(np.nonzero(extra)[0], 1)
                     confirmed += int(near.sum())
-                    again |= near
-                detect(np.arange(c0, c1), keep=~again, v=v)
                 redo = np.nonzero(again)[0]
                 from_warm[c0 + redo] = False
-                if redo.size:
-                    fallback += int((r["status"] != 0).sum())
-                    if limit_table is not None and base_violations is not None:
+                fallback += int((r["status"] != 0).sum()) if redo.size else 0
+                try:
+                    detect(np.arange(c0, c1), keep=~again, v=v)
+                    if redo_runs and limit_table is not None and base_violations is not None:
                         # the manager's pass over the warm-started answers kept (scenarios of their own) runs on
                         # the host while the device solves the others again
-                        kept_warm = viol.pop()
-                        out = {}
-
-                        def resolve():
-                            try:
-                                out["r"] = solve_part(part[redo], redo=True)
-                            except BaseException as e:   # re-raised on the caller's thread
-                                out["e"] = e
-                        th = threading.Thread(target=resolve)
-                        th.start()
-                        try:
-                            managed.append(manage_kept(kept_warm))
-                        finally:
-                            th.join()
-                        if "e" in out:
-                            raise out["e"]
-                        r2, _ = out["r"]
-                    else:
-                        r2, _ = solve_part(part[redo], redo=True)
-                    detect(c0 + redo)
-                    st[c0 + redo] = r2["status"]
-                    it[c0 + redo] = r2["iterations"]
-                    tot[c0 + redo] += r2["total_iterations"]
-                    nm[c0 + redo] = r2["norm"]
+                        managed.append(manage_kept(viol.pop()))
+                finally:
+                    for _, th, _ in redo_runs:
+                        th.join()
+                for _, _, out in redo_runs:
+                    if "e" in out:
+                        raise out["e"]
+                if redo.size and not redo_runs:
+                    redo_runs.append((redo, None, {"r": solve_part(part[redo], redo=True)}))
+                for rows, _, out in redo_runs:
+                    r2, _, ctx2 = out["r"]
+                    detect(c0 + rows, ctx2)
+                    st[c0 + rows] = r2["status"]
+                    it[c0 + rows] = r2["iterations"]
+                    tot[c0 + rows] += r2["total_iterations"]
+                    nm[c0 + rows] = r2["norm"]
                     if states_to_host:
-                        xs[c0 + redo] = r2["x"]
+                        xs[c0 + rows] = r2["x"]
             if self.robust_retry and self.lu_mode != N.LU_ROBUST:
                 # the scenarios the batch rejected, once more in the robust mode from the base-case start
                 redo = np.nonzero((st[c0:c1] == N.REJECTED) | (st[c0:c1] == N.LINEAR_SOLVE_FAILED))[0]
                 if redo.size:
                     robust_attempted += redo.size
                     try:
-                        r3, _ = solve_part(part[redo], robust=True)
+                        r3, _, _ = solve_part(part[redo], robust=True)
                     except N.OlfHipError as e:
                         # an optional pass: its failure (e.g. no device memory for its FP64 Krylov basis) keeps the
                         # first pass's answers (GpuBatchPrecomputer.java:780-786)

@@ -120,26 +120,47 @@ def near_base_boundary(violations: np.ndarray, n_table_branches: int, base: np.n
         bus_kv = kv if kv.ndim == 0 else kv[np.where(bus, r["element"], 0)]
         return np.where(bus, bus_kv, 1.0)
 
-    keys = _keys(v)
-    bkeys = _keys(b)
-    blast, _ = _last_per_key(bkeys)
-    bk = bkeys[blast]
-    order = np.argsort(bk)
-    bk, brec = bk[order], b[blast][order]
-    pos = np.minimum(np.searchsorted(bk, keys), bk.size - 1)
-    hit = np.nonzero(bk[pos] == keys)[0]
+    # the base case's last record of each key, found for every record through a lookup table per key class (branch
+    # sides, buses): a gather per record instead of a search
+    blast, _ = _last_per_key(_keys(b))
+    brec = b[blast]
+    kind, el = v["kind"], v["element"]
+    size = int(max(el.max(), brec["element"].max())) + 1
+    lut_branch = np.full(2 * size, -1, np.int64)
+    lut_bus = np.full(size, -1, np.int64)
+    bb = brec["kind"] == KIND_BRANCH
+    bv = (brec["kind"] == KIND_HIGH_VOLTAGE) | (brec["kind"] == KIND_LOW_VOLTAGE)
+    lut_branch[2 * brec["element"][bb] + brec["side"][bb] - 1] = np.nonzero(bb)[0]
+    lut_bus[brec["element"][bv]] = np.nonzero(bv)[0]
+    pos = np.where(kind == KIND_BRANCH, lut_branch[np.clip(2 * el + v["side"] - 1, 0, 2 * size - 1)],
+                   np.where(kind <= KIND_LOW_VOLTAGE, lut_bus[el], -1))
+    hit = np.nonzero(pos >= 0)[0]
     if hit.size == 0:
         return out
-    ref, rep = brec[pos[hit]], v[hit]
-    v1 = ref["value"] * to_kv(ref)
-    v2 = rep["value"] * to_kv(rep)
-    kind = rep["kind"]
+    p = pos[hit]
+    kind = kind[hit]
+    bkind = brec["kind"][p]
+    is_bus = (kind == KIND_HIGH_VOLTAGE) | (kind == KIND_LOW_VOLTAGE)
+    if kv is None:
+        rep_kv = ref_kv = 1.0
+    elif kv.ndim == 0:
+        rep_kv = np.where(is_bus, kv, 1.0)
+        ref_kv = np.where((bkind == KIND_HIGH_VOLTAGE) | (bkind == KIND_LOW_VOLTAGE), kv, 1.0)
+    else:
+        rep_kv = np.where(is_bus, kv[np.where(is_bus, el[hit], 0)], 1.0)
+        bbus = (bkind == KIND_HIGH_VOLTAGE) | (bkind == KIND_LOW_VOLTAGE)
+        ref_kv = np.where(bbus, kv[np.where(bbus, brec["element"][p], 0)], 1.0)
+    v1 = brec["value"][p] * ref_kv
+    v2 = v["value"][hit] * rep_kv
     bound = np.where(kind == KIND_BRANCH, np.abs(v1) * (1.0 + params.flowProportionalThreshold),
                      np.where(kind == KIND_HIGH_VOLTAGE,
                               v1 + np.minimum(params.highVoltageAbsoluteThreshold, v1 * params.highVoltageProportionalThreshold),
                               v1 - np.minimum(params.lowVoltageAbsoluteThreshold, v1 * params.lowVoltageProportionalThreshold)))
     val = np.where(kind == KIND_BRANCH, np.abs(v2), v2)
-    decided = (_type(ref) == _type(rep)) & (rep["threshold"] == ref["threshold"]) & (kind <= KIND_LOW_VOLTAGE)
+    # same limit type (LimitViolationType: branch quantity, or the voltage kind) and the same threshold
+    same_type = np.where(kind == KIND_BRANCH, (bkind == KIND_BRANCH) & (brec["quantity"][p] == v["quantity"][hit]),
+                         bkind == kind)
+    decided = same_type & (v["threshold"][hit] == brec["threshold"][p])
     out[hit] = decided & (np.abs(val - bound) <= band * np.abs(bound))
     return out
 
