@@ -404,7 +404,10 @@ class GpuBatchedSecurityAnalysisSolver:
         if ctx is None or ctx.capacity < chunk:
             if ctx is not None:
                 ctx.close()
-            ctx = N.BatchContext(self.structure, chunk, self.lu_mode, self.reactive_limits, 1, self.threads_per_tile)
+            # capacities in powers of two (at least 16), so that a context is rarely made again: making one waits
+            # for the device, including a solve running on the other context
+            cap = max(16, 1 << (int(chunk) - 1).bit_length())
+            ctx = N.BatchContext(self.structure, cap, self.lu_mode, self.reactive_limits, 1, self.threads_per_tile)
             if self.distributed_slack is not None:
                 ctx.set_distributed_slack(self.distributed_slack)
             self.nr.install_stopping_criteria(ctx)
