@@ -8,7 +8,9 @@
  * into caller-provided buffers.
  *
  * Threading: a context is not re-entrant (one context per thread, as the reference serialises the device under one
- * lock, GpuBatchPrecomputer.java:65); structures may be shared by contexts of the same thread.
+ * lock, GpuBatchPrecomputer.java:65). Contexts and limit tables made on one structure may be used on different
+ * threads at the same time (the structure is read-only once on the device; its reference count is atomic); a limit
+ * table is used by one thread at a time.
  */
 #ifndef OLFHIP_H
 #define OLFHIP_H

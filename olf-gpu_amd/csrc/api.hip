@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -75,8 +76,9 @@ struct StructureImpl {
     std::vector<int> bus1, bus2;
     std::vector<int> incPtr, inc;    // branches incident to each bus (island search)
     double jacobianScale = 1.0;      // twice the largest bus admittance sum: the order of ||J||inf near 1 pu
-    int refs = 1;
+    std::atomic<int> refs{1};   // the caller's handle and every context and limit table made on it
     bool onDevice = false;
+    std::mutex deviceLock;      // the plan's first upload, by whichever thread first needs it
     int device = -1;
     std::vector<std::unique_ptr<DBuf>> bufs;
     DevPlan dp{};
@@ -88,6 +90,7 @@ struct StructureImpl {
         return bufs.back()->as<X>();
     }
     void ensureDevice() {
+        std::lock_guard<std::mutex> lock(deviceLock);
         if (onDevice) return;
         int count = 0;
         if (hipGetDeviceCount(&count) != hipSuccess || count == 0) throw DeviceError("no HIP device is visible");

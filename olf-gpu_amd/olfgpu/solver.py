@@ -570,6 +570,7 @@ class GpuBatchedSecurityAnalysisSolver:
                 redo_runs = []   # (rows of the chunk, thread, its outcome) of the base-start solves under way
 
                 def start_redo(rows, which):
+                    self.redo_context(len(rows), which)   # made (or grown) on this thread, before its solve starts
                     out = {}
 
                     def run():
