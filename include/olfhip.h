@@ -416,7 +416,7 @@ int olfhip_detect_violations(olfhip_context *context, olfhip_limit_table *table,
  * first, -1 padded. Input: the neighbourhood's base-case angles relative to endpoint 1 and voltages minus 1
  * (2 n_local rows, normalised), the outaged branch (an embedding row, the one-hot input of the first layer) and
  * n_features endpoint features; output: 2 n_local rows [angle changes ; voltage changes], normalised. The Newton solve
- * it seeds stays the sole judge of the answer. */
+ * it seeds alone decides the answer. */
 typedef struct olfhip_mlp olfhip_mlp;
 
 int olfhip_mlp_create(int n_bus, int n_branch, const int *bus1, const int *bus2, const double *weight, int n_local,
