@@ -1,7 +1,7 @@
 // Learned warm start: a 3-layer MLP trained and served by hand-written gfx950 kernels (BF16 operands on the matrix
 // cores, FP32 accumulation, FP32 master weights with Adam). The prediction seeds the Newton solve through the
-// per-scenario x0 hook (BatchSolveNative x0PerScenario, BatchSolveNative.java:119); the Newton mismatch stays the
-// sole judge of the answer.
+// per-scenario x0 hook (BatchSolveNative x0PerScenario, BatchSolveNative.java:119); the Newton mismatch alone
+// decides the answer.
 //
 // The network works on the outage's neighbourhood: the K buses electrically nearest the outaged branch (multi-source
 // Dijkstra over |x| from both endpoints, endpoints first, -1 padded), computed once per branch at creation. Its N = 2K

@@ -1475,7 +1475,7 @@ constexpr double P_RESIDUE_EPS = 1e-5;   // OLF ActivePowerDistribution.P_RESIDU
 // on[]; every thread of the block calls it. Per lane: the slack's retained share of the total from the anchor
 // (native/cuda/outerloop/distributed_slack.cu:131-195), the gate on the participants' share (:215-305), then, when
 // it fires, every participant reset to its anchor and the whole total water-filled from there, the factors
-// renormalised over the unsaturated participants each round (:395-420, :530-566), and OLF's verdict: unstable when
+// renormalised over the unsaturated participants each pass (:395-420, :530-566), and OLF's outcome: unstable when
 // the targets and the slack's applied share moved by more than 0.9 P_RESIDUE_EPS, failed when more than
 // P_RESIDUE_EPS stayed unplaced (:568-600). A participant whose P row is not an equation (islanded) takes no part.
 // Sums over the participants are per-lane block reductions; participant i of lane t is always handled by the thread
@@ -1605,7 +1605,7 @@ __device__ void dsCheck(const DevPlan &P, const DevState &S, const DevRL &R, con
         if (tid < T && sGo[tid]) sRem[tid] -= red[tid];
         __syncthreads();
     }
-    // (5) the verdict
+    // (5) the outcome
     double mv = 0.0;
     if (sFire[t]) {
         for (int w = tid; w < D.n * T; w += nt) {

@@ -11,7 +11,7 @@ start's branch-wise differences, and a step there would cost more than the predi
 
 Training samples are post-contingency states the batched solver computes under random load scalings; every branch of
 the grid is covered before any is repeated. A warm-started scenario that does not converge is solved again from the
-base-case start, so the converged Newton mismatch stays the sole judge of every answer.
+base-case start, so the converged Newton mismatch alone decides every answer.
 """
 from __future__ import annotations
 

@@ -667,7 +667,7 @@ double oracle_bus_p(const oracle_sys *s, const double *x, const int *out_br, int
 /* One DistributedSlackOuterLoop check, the reference's aggregate (per-bus) path: the slack's retained share
  * (distributed_slack.cu:131-195, computeSlackRetain), the gate on the participants' share (:215-305), the reset of
  * every participant to its anchor and the water-fill of the whole total from there, factors renormalised over the
- * unsaturated participants each round (:395-420, 530-566), and OLF's verdict: UNSTABLE when the targets (and the
+ * unsaturated participants each pass (:395-420, 530-566), and OLF's outcome: UNSTABLE when the targets (and the
  * slack's applied share) moved by more than 0.9 P_RESIDUE_EPS (:568-600). A participant whose P row is not an
  * equation (islanded) takes no part. *retain_applied is NaN before the first distribution. Returns 1 when unstable;
  * *failed when more than P_RESIDUE_EPS stayed unplaced. */
