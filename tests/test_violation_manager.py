@@ -155,3 +155,60 @@ def test_near_base_boundary_flags_values_at_the_increased_violation_boundary():
     assert got.tolist() == [True, True, False, False, False, True, False, False]
     # a band of zero flags nothing off the boundary itself
     assert not near_base_boundary(post, NB, base, p, 400.0, band=0.0)[:3].any()
+
+
+def test_near_base_boundary_equals_a_record_by_record_restatement():
+    """On random feeds (keys shared between branch sides and buses of the same element number, repeated base keys, a
+    kilovolt table), near_base_boundary equals a plain loop over the records that looks up the base case's last
+    record of the same key (a branch side, or a bus for high and low alike), requires the same limit type and
+    threshold, and tests |value - boundary| <= band |boundary|."""
+    from olfgpu.violations import near_base_boundary
+    rng = np.random.default_rng(5)
+    for trial in range(40):
+        n, m = int(rng.integers(1, 300)), int(rng.integers(1, 12))
+        v = np.zeros(n, N.VIOLATION_DTYPE)
+        v["scenario"] = np.sort(rng.integers(0, 6, n))
+        v["kind"] = rng.integers(0, 3, n)
+        v["element"] = rng.integers(0, 8, n)
+        v["side"] = np.where(v["kind"] == 0, rng.integers(1, 3, n), 0)
+        v["quantity"] = np.where(v["kind"] == 0, rng.integers(0, 3, n), 0)
+        v["threshold"] = rng.integers(1, 3, n) * 0.5
+        v["value"] = rng.uniform(0.45, 1.2, n) * np.where(rng.random(n) < 0.2, -1.0, 1.0)
+        b = v[rng.integers(0, n, m)].copy()
+        b["scenario"] = 0
+        kv = rng.uniform(100.0, 400.0, 8)
+        p = IncreasedViolationsParameters(0.05, 0.01, 2.0, 0.02, 1.0)
+        # some records placed on their base record's boundary
+        for i in rng.integers(0, n, n // 3):
+            j = int(rng.integers(0, m))
+            v[i] = b[j]
+            v1 = b["value"][j] * (1.0 if b["kind"][j] == 0 else kv[b["element"][j]])
+            bound = (abs(v1) * 1.05 if b["kind"][j] == 0 else
+                     v1 + min(1.0, v1 * 0.02) if b["kind"][j] == 1 else v1 - min(2.0, v1 * 0.01))
+            scale = 1.0 if b["kind"][j] == 0 else kv[b["element"][j]]
+            v["value"][i] = bound * (1.0 + rng.uniform(-2e-4, 2e-4)) / scale
+        band = 1e-4
+        want = []
+        for r in v:
+            key = (0, r["element"], r["side"]) if r["kind"] == 0 else (1, r["element"], 0)
+            ref = None
+            for q in b:
+                if ((0, q["element"], q["side"]) if q["kind"] == 0 else (1, q["element"], 0)) == key:
+                    ref = q   # the last one fed
+            if ref is None or r["threshold"] != ref["threshold"]:
+                want.append(False)
+                continue
+            if r["kind"] == 0:
+                if ref["kind"] != 0 or ref["quantity"] != r["quantity"]:
+                    want.append(False)
+                    continue
+                bound, val = abs(ref["value"]) * 1.05, abs(r["value"])
+            else:
+                if ref["kind"] != r["kind"]:
+                    want.append(False)
+                    continue
+                v1, val = ref["value"] * kv[ref["element"]], r["value"] * kv[r["element"]]
+                bound = v1 + min(1.0, v1 * 0.02) if r["kind"] == 1 else v1 - min(2.0, v1 * 0.01)
+            want.append(abs(val - bound) <= band * abs(bound))
+        got = near_base_boundary(v, NB, b, p, kv, band=band)
+        assert got.tolist() == want, trial
