@@ -217,3 +217,17 @@ def test_bank_calls_equal_the_c_abi(jvm, g2000):
         jvm.destroy_batch_context(ch)
         jvm.destroy_structure(sh)
     assert jvm.violations() == 0
+
+
+def test_null_context_handles(jvm, net118):
+    """solveBatchContext on handle 0 throws IllegalArgumentException and returns null; beginBatchInstall and
+    readBatchHandedBackSlots on handle 0 do nothing and return null (no device is touched)."""
+    x = np.zeros(net118.n)
+    assert jvm.solve(0, x, x, np.ones(net118.n, np.int32), np.zeros((1, net118.n_branch), bool)) is None
+    cls, msg = jvm.exception()
+    assert cls == "java/lang/IllegalArgumentException" and "null context" in msg
+    jvm.begin_install(0, [0, 1])
+    assert jvm.exception() is None
+    assert getattr(jvm.lib, J.BATCH + "readBatchHandedBackSlots")(jvm.env, None, 0) is None
+    assert jvm.exception() is None
+    assert jvm.violations() == 0
