@@ -1134,12 +1134,26 @@ static void ambiguousScenarios(ContextImpl *c, LimitTableImpl *t, double band, i
     const DevPlan dp = c->st->dp;
     const DevState S = c->devState();
     const DevLimits L = t->dev();
+    // one workgroup per scenario with its node states in LDS where they fit (OLFHIP_NEAR_GATHER=1: the gathering
+    // kernel, one thread per scenario and item)
+    const size_t stagedBytes = (size_t)dp.nb * 17;
+    int perBlock = 0;
+    HIP_OK(hipDeviceGetAttribute(&perBlock, hipDeviceAttributeMaxSharedMemoryPerBlock, c->st->device));
+    const char *gather = std::getenv("OLFHIP_NEAR_GATHER");
+    const bool staged = stagedBytes <= (size_t)perBlock && !(gather && std::strcmp(gather, "0") != 0);
+    const int *only = c->bankReadback ? c->handedNow.as<int>() : nullptr;
     if (t->nBranches + t->nBuses > 0)
         withT(c->T, [&](auto TC) {
             constexpr int T = decltype(TC)::value;
-            nearLimitsKernel<T><<<gridFor((size_t)K * (t->nBranches + t->nBuses), 256), 256, 0, s>>>(
-                dp, S, L, c->modeBase.as<int>(), K, band, c->bankReadback ? c->handedNow.as<int>() : nullptr,
-                t->near.as<int>());
+            if (staged) {
+                HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void *>(&nearLimitsStagedKernel<T>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)stagedBytes));
+                nearLimitsStagedKernel<T><<<K, 1024, stagedBytes, s>>>(dp, S, L, c->modeBase.as<int>(), band, only,
+                                                                      t->near.as<int>());
+            } else {
+                nearLimitsKernel<T><<<gridFor((size_t)K * (t->nBranches + t->nBuses), 256), 256, 0, s>>>(
+                    dp, S, L, c->modeBase.as<int>(), K, band, only, t->near.as<int>());
+            }
         });
     HIP_OK(hipGetLastError());
     HIP_OK(hipMemcpyAsync(flags, t->near.p, (size_t)K * 4, hipMemcpyDeviceToHost, s));

@@ -55,12 +55,7 @@ __device__ __forceinline__ bool islandedNode(const DevPlan &P, const DevState &S
 
 // Flows at both ends of closed branch e in lane t of a tile, from the own/far coefficients of the fill: f = p1 q1 v1
 // p2 q2 v2.
-template <int T>
-__device__ __forceinline__ void branchEndFlows(const DevPlan &P, const DevState &S, const DevLimits &L, int tile, int t,
-                                               int e, double f[6]) {
-    const int k1 = L.branchNode[2 * e], k2 = L.branchNode[2 * e + 1];
-    const double2 x1 = *reinterpret_cast<const double2 *>(S.x + vIdx<T>(tile, P.nb, k1, t));
-    const double2 x2 = *reinterpret_cast<const double2 *>(S.x + vIdx<T>(tile, P.nb, k2, t));
+__device__ __forceinline__ void branchEndFlowsOf(const DevLimits &L, int e, double2 x1, double2 x2, double f[6]) {
     const double *c = L.coef + 8 * (size_t)e;
     double s1, c1, s2, c2;
     sincosReduced(c[3] - x1.x + x2.x, &s1, &c1);
@@ -72,6 +67,15 @@ __device__ __forceinline__ void branchEndFlows(const DevPlan &P, const DevState 
     f[3] = c[4] * x2.y * x2.y - bvv * s2;
     f[4] = c[5] * x2.y * x2.y - bvv * c2;
     f[5] = x2.y;
+}
+
+template <int T>
+__device__ __forceinline__ void branchEndFlows(const DevPlan &P, const DevState &S, const DevLimits &L, int tile, int t,
+                                               int e, double f[6]) {
+    const int k1 = L.branchNode[2 * e], k2 = L.branchNode[2 * e + 1];
+    const double2 x1 = *reinterpret_cast<const double2 *>(S.x + vIdx<T>(tile, P.nb, k1, t));
+    const double2 x2 = *reinterpret_cast<const double2 *>(S.x + vIdx<T>(tile, P.nb, k2, t));
+    branchEndFlowsOf(L, e, x1, x2, f);
 }
 
 // Whether closed branch e is out of service in scenario s: outaged, or an end cut off from the slack.
@@ -181,6 +185,64 @@ __global__ void nearLimitsKernel(DevPlan P, DevState S, DevLimits L, const int *
         }
         if (near) flag[s] = 1;
     }
+}
+
+// nearLimitsKernel with one workgroup per scenario: the lane's node states (16 B a node) and two flags per node (bit 0:
+// cut off from the slack, bit 1: voltage held at its setpoint) staged in LDS (17 B a node: 157 KB at 9,241 buses), so
+// that the branches and buses read them there instead of gathering them across the lane's states in global memory.
+// The same comparisons on the same values: the flags equal nearLimitsKernel's.
+template <int T>
+__global__ __launch_bounds__(1024) void nearLimitsStagedKernel(DevPlan P, DevState S, DevLimits L, const int *modeBase,
+                                                               double band, const int *only, int *flag) {
+    extern __shared__ double2 nearX[];
+    __shared__ int sNear;
+    const int s = blockIdx.x;
+    if (S.status[s] != ST_CONVERGED || (only && !only[s])) return;   // uniform over the workgroup
+    const int tile = s / T, t = s % T;
+    unsigned char *nodeFlags = reinterpret_cast<unsigned char *>(nearX + P.nb);
+    if (threadIdx.x == 0) sNear = 0;
+    for (int k = threadIdx.x; k < P.nb; k += blockDim.x) {
+        const size_t vi = vIdx<T>(tile, P.nb, k, t);
+        nearX[k] = *reinterpret_cast<const double2 *>(S.x + vi);
+        nodeFlags[k] = (unsigned char)((S.mode[vi] == MODE_FIXED && modeBase[2 * P.perm[k]] != MODE_FIXED ? 1 : 0) |
+                                       (S.mode[vi + 1] == MODE_FIXED ? 2 : 0));
+    }
+    int outs[MAX_OUTAGES];
+    for (int q = 0; q < MAX_OUTAGES; q++) outs[q] = q < S.maxOut ? S.outages[(size_t)s * S.maxOut + q] : -1;
+    __syncthreads();
+    bool near = false;
+#pragma unroll 4
+    for (int b = threadIdx.x; b < L.nBranches; b += blockDim.x) {
+        const int g0 = L.groupStart[b], g1 = L.groupStart[b + 1];
+        if (g0 == g1) continue;
+        const int e = L.branch[b];
+        const int k1 = L.branchNode[2 * e], k2 = L.branchNode[2 * e + 1];
+        bool out = (nodeFlags[k1] & 1) || (nodeFlags[k2] & 1);
+        for (int q = 0; q < MAX_OUTAGES; q++) out |= outs[q] == e;
+        if (out) continue;
+        double f[6];
+        branchEndFlowsOf(L, e, nearX[k1], nearX[k2], f);
+        for (int g = g0; g < g1 && !near; g++) {
+            const double *end = L.groupSide[g] == 1 ? f : f + 3;
+            const double value = comparedQuantity(L.groupQty[g], end[0], end[1], end[2]);
+            for (int l = L.limitStart[g]; l < L.limitStart[g + 1]; l++) {
+                const double margin = fabs(L.threshold[l]) * band;
+                near |= fabs(value - L.threshold[l]) <= margin;
+                if (near || value > L.threshold[l] + margin) break;
+            }
+        }
+    }
+#pragma unroll 4
+    for (int i = threadIdx.x; i < L.nBuses; i += blockDim.x) {
+        const int k = P.pos[i];
+        if (nodeFlags[k] & 2) continue;   // voltage held at its setpoint (or frozen)
+        const double v = nearX[k].y;
+        near |= fabs(v - L.busHigh[i]) <= fabs(L.busHigh[i]) * band ||
+                fabs(v - L.busLow[i]) <= fabs(L.busLow[i]) * band;   // NaN limits compare false
+    }
+    if (near) sNear = 1;
+    __syncthreads();
+    if (threadIdx.x == 0 && sNear) flag[s] = 1;
 }
 
 // Gathers the (phi, V) pairs of the requested (scenario, node) pairs and whether each node is islanded.

@@ -179,3 +179,29 @@ def test_voltage_angle_limits_match_the_oracle_scan(grid):
         n_angle += sum(1 for q in ref if q[0] >= 3)
     print(f"[viol] {n_angle} voltage-angle violations over {len(ctg)} contingencies", flush=True)
     assert n_angle > 0
+
+
+@pytest.mark.parametrize("name, lanes", [("grid9241", 2), ("grid2000", 1)])
+def test_staged_ambiguity_flags_equal_the_gathering_kernel(grid, name, lanes, monkeypatch):
+    """The ambiguity check with a workgroup per scenario and its states in LDS (nearLimitsStagedKernel) flags exactly
+    the scenarios the one-thread-per-item kernel (nearLimitsKernel, OLFHIP_NEAR_GATHER=1) flags, with islanded buses
+    (bridges) and PV buses in the batch, at bands from 1e-6 to 0.1."""
+    net = grid(name)
+    st, base, table, ctg, _ = _setup(net, "BATCHED_LU_FP32_REFINED", 200, 0.1, 3)
+    sa = GpuBatchedSecurityAnalysisSolver(st, net.reactive_limits(),
+                                          GpuLoadFlowParameters(linearSolver="BATCHED_LU_FP32_REFINED"),
+                                          lanes_per_tile=lanes)
+    r = sa.solve_batch_n1(base.x, base.target, base.row_mode, base.reactive_limit_types, ctg, states_to_host=False)
+    assert (r.status == N.CONVERGED).sum() > 150
+    lt = N.LimitTable(st, table)
+    counts = []
+    for band in (1e-6, 1e-4, 1e-2, 0.1):
+        monkeypatch.delenv("OLFHIP_NEAR_GATHER", raising=False)
+        staged = lt.ambiguous(sa._ctx, band)
+        monkeypatch.setenv("OLFHIP_NEAR_GATHER", "1")
+        gathered = lt.ambiguous(sa._ctx, band)
+        monkeypatch.delenv("OLFHIP_NEAR_GATHER", raising=False)
+        assert np.array_equal(staged, gathered), band
+        counts.append(int(staged.sum()))
+    print(f"[viol] {name}: ambiguous scenarios at bands 1e-6..0.1: {counts}")
+    assert counts == sorted(counts) and counts[3] > 0
